@@ -1,0 +1,827 @@
+"""Host-side mirror of gplite's R API (gp_init / gp_fit / gp_energy / gp_pred /
+gp_optim with cf_* / lik_* / prior_* / approx_* objects) over the C ABI.
+
+The R package keeps the model as a list and dispatches on S3 classes; here the
+objects are small Python classes with the same constructor arguments and
+defaults.  Everything O(n^2) or worse runs in libgplite_amd.so on the GPU;
+this module only does what the reference does in R at O(n*d): input column
+selection and normalisation (R/misc.R:17-31), parameter packing on the log
+scale (R/cf.R:327-382), hyperprior densities (R/priors.R:96-112), quantiles and
+response transforms of the predictive distribution (R/gp_pred.R:102-120).
+"""
+
+from __future__ import annotations
+
+import copy
+import math
+import warnings
+import weakref
+
+import numpy as np
+
+from . import _lib as L
+
+# ---------------------------------------------------------------------------
+# priors (R/priors.R:56-112)
+# ---------------------------------------------------------------------------
+
+
+class Prior:
+    kind = "prior"
+
+    def lpdf(self, p):
+        return 0.0
+
+    def dlpdf(self, p):
+        return 0.0
+
+    def spec(self):
+        return {"kind": self.kind, **{k: v for k, v in self.__dict__.items()}}
+
+
+class prior_fixed(Prior):
+    kind = "fixed"
+
+
+class prior_logunif(Prior):
+    kind = "logunif"
+
+
+class prior_lognormal(Prior):
+    kind = "lognormal"
+
+    def __init__(self, loc=0.0, scale=1.0):
+        self.loc, self.scale = loc, scale
+
+    def lpdf(self, p):  # dnorm on the log-parameter (R/priors.R:110-112)
+        return -0.5 * math.log(2 * math.pi) - math.log(self.scale) - 0.5 * ((p - self.loc) / self.scale) ** 2
+
+    def dlpdf(self, p):
+        return -(p - self.loc) / self.scale ** 2
+
+
+class prior_half_t(Prior):
+    kind = "half_t"
+
+    def __init__(self, df=1.0, scale=1.0):
+        self.df, self.scale = df, scale
+
+    def lpdf(self, p):  # R/priors.R:104-108
+        t = math.exp(p) / self.scale
+        df = self.df
+        ldt = (math.lgamma((df + 1) / 2) - math.lgamma(df / 2) - 0.5 * math.log(df * math.pi)
+               - (df + 1) / 2 * math.log1p(t * t / df))
+        return math.log(2) - math.log(self.scale) + ldt + p
+
+    def dlpdf(self, p):
+        t = math.exp(p) / self.scale
+        return 1.0 - (self.df + 1) * t * t / (self.df + t * t)
+
+
+def _fixed(pr):
+    return isinstance(pr, prior_fixed)
+
+
+# ---------------------------------------------------------------------------
+# covariance functions (R/cf.R:92-213)
+# ---------------------------------------------------------------------------
+
+class Cf:
+    kind = "cf"
+    param_names = ()
+
+    def __init__(self):
+        self.vars = None
+        self.normalize = False
+        self.means = None
+        self.scales = None
+        self.priors = {}
+
+    # R/misc.R:17-31
+    def prepare(self, x):
+        x = _as_matrix(x)
+        if self.vars is not None:
+            x = x[:, _col_index(x, self.vars)]
+        if self.normalize and self.means is not None and self.scales is not None:
+            x = (x - self.means) / self.scales
+        return x
+
+    def learn_scales(self, x):  # R/cf.R:836-845
+        x = _as_matrix(x)
+        if self.vars is not None:
+            x = x[:, _col_index(x, self.vars)]
+        self.means = x.mean(axis=0)
+        self.scales = x.std(axis=0, ddof=1) if x.shape[0] > 1 else np.ones(x.shape[1])
+
+    def __mul__(self, other):  # "*.cf" (R/cf.R:193-209)
+        a = self.cfs if isinstance(self, cf_prod) else [self]
+        b = other.cfs if isinstance(other, cf_prod) else [other]
+        return cf_prod(*a, *b)
+
+    # parameter slots: (object, name, index or None) in get_param order
+    def slots(self):
+        out = []
+        for nm in self.param_names:
+            if _fixed(self.priors.get(nm)):
+                continue
+            v = np.atleast_1d(getattr(self, nm))
+            if v.size == 1:
+                out.append((self, nm, None))
+            else:
+                out.extend((self, nm, i) for i in range(v.size))
+        return out
+
+    def all_slots(self):
+        out = []
+        for nm in self.param_names:
+            v = np.atleast_1d(getattr(self, nm))
+            if v.size == 1:
+                out.append((self, nm, None))
+            else:
+                out.extend((self, nm, i) for i in range(v.size))
+        return out
+
+    def label(self):
+        return "cf_" + self.kind
+
+    def spec(self):
+        d = {"kind": self.kind, "vars": None if self.vars is None else list(self.vars),
+             "normalize": self.normalize, "means": self.means, "scales": self.scales,
+             "priors": {k: v.spec() for k, v in self.priors.items()}}
+        for nm in self.param_names:
+            d[nm] = copy.deepcopy(getattr(self, nm))
+        return d
+
+
+class cf_const(Cf):
+    kind = "const"
+    param_names = ("magn",)
+
+    def __init__(self, magn=1.0, prior_magn=None):
+        super().__init__()
+        self.magn = magn
+        self.priors = {"magn": prior_magn or prior_logunif()}
+
+    def prepare(self, x):
+        return _as_matrix(x)[:, :0]
+
+    def learn_scales(self, x):
+        pass
+
+
+class cf_lin(Cf):
+    kind = "lin"
+    param_names = ("magn",)
+
+    def __init__(self, vars=None, magn=1.0, prior_magn=None, normalize=False):
+        super().__init__()
+        self.vars, self.magn, self.normalize = vars, magn, normalize
+        self.priors = {"magn": prior_magn or prior_logunif()}
+
+
+class _Stationary(Cf):
+    param_names = ("lscale", "magn")
+
+    def __init__(self, vars=None, lscale=0.3, magn=1.0, prior_lscale=None, prior_magn=None, normalize=False):
+        super().__init__()
+        self.vars, self.magn, self.normalize = vars, magn, normalize
+        ls = np.atleast_1d(np.asarray(lscale, dtype=float))
+        self.lscale = float(ls[0]) if ls.size == 1 else ls
+        self.priors = {"lscale": prior_lscale or prior_logunif(), "magn": prior_magn or prior_logunif()}
+
+
+class cf_sexp(_Stationary):
+    kind = "sexp"
+
+
+class cf_matern32(_Stationary):
+    kind = "matern32"
+
+
+class cf_matern52(_Stationary):
+    kind = "matern52"
+
+
+class cf_nn(Cf):
+    kind = "nn"
+    param_names = ("sigma0", "sigma", "magn")
+
+    def __init__(self, vars=None, sigma0=1.0, sigma=3.0, magn=1.0, prior_sigma0=None, prior_sigma=None,
+                 prior_magn=None, normalize=True):
+        super().__init__()
+        self.vars, self.sigma0, self.sigma, self.magn, self.normalize = vars, sigma0, sigma, magn, normalize
+        self.priors = {"sigma0": prior_sigma0 or prior_half_t(), "sigma": prior_sigma or prior_half_t(),
+                       "magn": prior_magn or prior_logunif()}
+
+
+class cf_periodic(Cf):
+    kind = "periodic"
+    param_names = ("period",)
+
+    def __init__(self, vars=None, period=1.0, cf_base=None, prior_period=None):
+        super().__init__()
+        self.vars, self.period = vars, period
+        base = copy.deepcopy(cf_base) if cf_base is not None else cf_sexp()
+        if not isinstance(base, (cf_sexp, cf_matern32, cf_matern52)):
+            raise ValueError("cf_periodic: base kernel must be cf_sexp, cf_matern32 or cf_matern52")
+        base.normalize = False  # R/cf.R:172-173
+        base.vars = None
+        self.base = base
+        self.priors = {"period": prior_period or prior_logunif()}
+
+    def learn_scales(self, x):  # periodic does not normalise (R/cf.R:852-855)
+        pass
+
+    def prepare(self, x):
+        x = _as_matrix(x)
+        if self.vars is not None:
+            x = x[:, _col_index(x, self.vars)]
+        return x
+
+    def slots(self):
+        out = [] if _fixed(self.priors.get("period")) else [(self, "period", None)]
+        return out + self.base.slots()
+
+    def all_slots(self):
+        return [(self, "period", None)] + self.base.all_slots()
+
+    def spec(self):
+        d = super().spec()
+        d["base"] = self.base.spec()
+        return d
+
+
+class cf_prod(Cf):
+    kind = "prod"
+
+    def __init__(self, *cfs):
+        super().__init__()
+        self.cfs = list(cfs)
+        nfree = sum(1 for c in self.cfs if "magn" in c.param_names and not _fixed(c.priors.get("magn")))
+        if nfree > 1:  # R/misc.R:58-81
+            warnings.warn("Detected a product of covariance functions with more than one non-fixed magnitude "
+                          "parameters. Consider setting prior_magn = prior_fixed() for all except one of the "
+                          "kernels in the product.")
+
+    def learn_scales(self, x):
+        for c in self.cfs:
+            c.learn_scales(x)
+
+    def slots(self):
+        return [s for c in self.cfs for s in c.slots()]
+
+    def all_slots(self):
+        return [s for c in self.cfs for s in c.all_slots()]
+
+    def spec(self):
+        return {"kind": "prod", "cfs": [c.spec() for c in self.cfs]}
+
+
+def _as_matrix(x):
+    if hasattr(x, "to_numpy"):
+        x = x.to_numpy()
+    x = np.asarray(x, dtype=np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    return x
+
+
+def _col_index(x, vars):
+    return [int(v) for v in vars]
+
+
+# ---------------------------------------------------------------------------
+# likelihoods (R/lik.R:46-90), approximations (R/approx.R:60-75), method
+# ---------------------------------------------------------------------------
+
+class Lik:
+    kind = "lik"
+    param_names = ()
+
+    def __init__(self):
+        self.priors = {}
+        self.link = "identity"
+
+    def slots(self):
+        return [(self, nm, None) for nm in self.param_names if not _fixed(self.priors.get(nm))]
+
+    def all_slots(self):
+        return [(self, nm, None) for nm in self.param_names]
+
+    def label(self):
+        return "lik_" + self.kind
+
+    def spec(self):
+        d = {"kind": self.kind, "link": self.link, "priors": {k: v.spec() for k, v in self.priors.items()}}
+        for nm in self.param_names:
+            d[nm] = getattr(self, nm)
+        return d
+
+    def response(self, f):
+        f = np.asarray(f, dtype=float)
+        if self.link == "logit":
+            return 1.0 / (1.0 + np.exp(-f))
+        if self.link == "probit":
+            from scipy.special import ndtr
+            return ndtr(f)
+        if self.link == "log":
+            return np.exp(f)
+        return f
+
+
+class lik_gaussian(Lik):
+    kind = "gaussian"
+    param_names = ("sigma",)
+
+    def __init__(self, sigma=0.5, prior_sigma=None):
+        super().__init__()
+        self.sigma = sigma
+        self.priors = {"sigma": prior_sigma or prior_logunif()}
+
+
+class lik_bernoulli(Lik):
+    kind = "bernoulli"
+
+    def __init__(self, link="logit"):
+        super().__init__()
+        if link not in ("logit", "probit"):
+            raise ValueError("Unknown link: " + str(link))
+        self.link = link
+
+
+class lik_binomial(lik_bernoulli):
+    kind = "binomial"
+
+
+class lik_poisson(Lik):
+    kind = "poisson"
+
+    def __init__(self, link="log"):
+        super().__init__()
+        if link != "log":
+            raise ValueError("Unknown link: " + str(link))
+        self.link = link
+
+
+class approx_laplace:
+    name = "laplace"
+
+    def __init__(self, maxiter=30, tol=1e-4):
+        self.maxiter, self.tol = maxiter, tol
+
+
+class approx_ep:
+    name = "ep"
+
+    def __init__(self, damping=0.9, quad_order=11, maxiter=100):
+        if damping <= 0 or damping >= 1:
+            raise ValueError("Damping should be between 0 and 1 (exclusive)")
+        self.damping, self.quad_order, self.maxiter = damping, quad_order, maxiter
+
+
+class method_full:
+    name = "full"
+
+
+# ---------------------------------------------------------------------------
+# the GP object (R/gp.R)
+# ---------------------------------------------------------------------------
+
+class _Fit:
+    """Device-resident fit (gp$fit, R/laplace.R:119): freed with the handle."""
+
+    def __init__(self, handle, n, log_evidence, iters, approx):
+        self.handle = handle
+        self.n = n
+        self.log_evidence = log_evidence
+        self.iters = iters
+        self.type = "analytic"
+        self.approx = approx
+        self._fin = weakref.finalize(self, _free_model, handle.value)
+
+    def vector(self, which):
+        out = np.empty(self.n)
+        L.check(L.lib().gpl_model_vector(self.handle, which, L.dptr(out)))
+        return out
+
+    @property
+    def fmean(self):
+        return self.vector(L.VEC_FMEAN)
+
+    @property
+    def alpha(self):
+        return self.vector(L.VEC_ALPHA)
+
+    @property
+    def z(self):
+        return self.vector(L.VEC_Z)
+
+    @property
+    def V(self):
+        return self.vector(L.VEC_V)
+
+    @property
+    def fvar(self):
+        return self.vector(L.VEC_FVAR)
+
+    @property
+    def C_chol(self):
+        out = np.empty((self.n, self.n), order="F")
+        L.check(L.lib().gpl_model_chol(self.handle, out.ctypes.data_as(L._dp)))
+        return out
+
+
+def _free_model(ptr):
+    if ptr and L._lib is not None:
+        L._lib.gpl_model_free(ptr)
+
+
+class GP:
+    def __init__(self, cfs, lik, method, approx):
+        self.cfs = cfs
+        self.lik = lik
+        self.method = method
+        self.approx = approx
+        self.fitted = False
+        self.fit = None
+        self.x = None
+        self.precision = "fp64"
+
+    def copy(self):
+        g = GP(copy.deepcopy(self.cfs), copy.deepcopy(self.lik), self.method, self.approx)
+        g.fitted, g.fit, g.x, g.precision = self.fitted, self.fit, self.x, self.precision
+        return g
+
+
+def gp_init(cfs=None, lik=None, method=None, approx=None):
+    """R/gp.R:52-71."""
+    cfs = cf_sexp() if cfs is None else cfs
+    lik = lik_gaussian() if lik is None else lik
+    method = method_full() if method is None else method
+    approx = approx_laplace() if approx is None else approx
+    if isinstance(lik, lik_gaussian) and isinstance(approx, approx_ep):
+        raise ValueError("approx_ep disabled for lik_gaussian: use approx_laplace instead, because it gives exact "
+                         "results")
+    if not isinstance(method, method_full):
+        raise NotImplementedError("only method_full (the dense core) is provided")
+    if isinstance(cfs, Cf):
+        cfs = [cfs]
+    return GP(list(cfs), lik, method, approx)
+
+
+# ---- parameters (R/cf.R:327-382, R/gp.R:150-165) -------------------------
+
+def _slots(gp, free=True):
+    out = []
+    for cf in gp.cfs:
+        out.extend(cf.slots() if free else cf.all_slots())
+    out.extend(gp.lik.slots() if free else gp.lik.all_slots())
+    return out
+
+
+def _slot_label(obj, nm, i, owner_label):
+    return f"{owner_label}.{nm}" if i is None else f"{owner_label}.{nm}[{i}]"
+
+
+def _labels(gp):
+    labs = []
+
+    def visit(cf):
+        if isinstance(cf, cf_prod):
+            for c in cf.cfs:
+                visit(c)
+            return
+        lab = cf.label()
+        for (o, nm, i) in cf.slots():
+            labs.append(_slot_label(o, nm, i, lab))
+
+    for cf in gp.cfs:
+        visit(cf)
+    for (o, nm, i) in gp.lik.slots():
+        labs.append(_slot_label(o, nm, i, gp.lik.label()))
+    return labs
+
+
+def get_param(gp):
+    """log of every free hyperparameter, as an ordered {name: value} dict."""
+    vals = []
+    for (o, nm, i) in _slots(gp):
+        v = getattr(o, nm)
+        vals.append(math.log(float(v if i is None else np.atleast_1d(v)[i])))
+    return dict(zip(_labels(gp), vals))
+
+
+def set_param(gp, param):
+    """Returns a copy with exp(param) set and the fit invalidated (R/gp.R:158-165)."""
+    g = gp.copy()
+    vals = list(param.values()) if isinstance(param, dict) else list(np.asarray(param, dtype=float))
+    sl = _slots(g)
+    if len(vals) != len(sl):
+        raise ValueError(f"expected {len(sl)} parameters, got {len(vals)}")
+    for (o, nm, i), v in zip(sl, vals):
+        if i is None:
+            setattr(o, nm, math.exp(v))
+        else:
+            arr = np.array(np.atleast_1d(getattr(o, nm)), dtype=float)
+            arr[i] = math.exp(v)
+            setattr(o, nm, arr)
+    g.fitted = False
+    return g
+
+
+def lpdf_prior(gp):
+    """R/gp.R:167-169."""
+    lp = 0.0
+    for (o, nm, i) in _slots(gp):
+        v = getattr(o, nm)
+        p = math.log(float(v if i is None else np.atleast_1d(v)[i]))
+        lp += o.priors.get(nm, prior_logunif()).lpdf(p)
+    return lp
+
+
+def _lpdf_prior_grad(gp):
+    out = []
+    for (o, nm, i) in _slots(gp):
+        v = getattr(o, nm)
+        p = math.log(float(v if i is None else np.atleast_1d(v)[i]))
+        out.append(o.priors.get(nm, prior_logunif()).dlpdf(p))
+    return np.array(out)
+
+
+def is_fitted(gp):
+    """R/gp.R:191-205."""
+    if gp.fit is not None:
+        if not gp.fitted:
+            raise RuntimeError("The GP object seems to contain a posterior fit, but is not refitted after setting "
+                               "new hyperparameter values. Please refit using gp_fit after calling set_param.")
+        return True
+    return False
+
+
+# ---- covariance program for the device ----------------------------------
+
+_KIND = {"const": L.CF_CONST, "lin": L.CF_LIN, "sexp": L.CF_SEXP, "matern32": L.CF_MATERN32,
+         "matern52": L.CF_MATERN52, "nn": L.CF_NN, "periodic": L.CF_PERIODIC}
+
+
+def _flatten(cfs):
+    """[(cf, op)] with products flattened; op = SUM starts a summand."""
+    out = []
+    for cf in cfs:
+        if isinstance(cf, cf_prod):
+            for k, c in enumerate(cf.cfs):
+                if isinstance(c, cf_prod):
+                    raise ValueError("nested cf_prod")
+                out.append((c, L.OP_SUM if k == 0 else L.OP_PROD))
+        else:
+            out.append((cf, L.OP_SUM))
+    return out
+
+
+def _program(cfs, x):
+    """Build the gpl_cf_term array and the prepared input matrix (column-major)."""
+    terms = _flatten(cfs)
+    if len(terms) > L.MAX_TERMS:
+        raise ValueError("at most 8 covariance terms")
+    arr = (L.CfTerm * len(terms))()
+    blocks = []
+    col = 0
+    free_mask = []
+    for k, (cf, op) in enumerate(terms):
+        t = arr[k]
+        t.kind = _KIND[cf.kind]
+        t.op = op
+        xc = cf.prepare(x)
+        if cf.kind == "const":
+            t.ncols = 0
+            t.col0 = 0
+        else:
+            t.col0 = col
+            t.ncols = xc.shape[1]
+            if t.ncols > L.MAX_COLS:
+                raise ValueError("a covariance term reads at most 16 input columns")
+            blocks.append(xc)
+            col += xc.shape[1]
+        if cf.kind in ("sexp", "matern32", "matern52"):
+            ls = np.atleast_1d(cf.lscale)
+            t.n_lscale = ls.size
+            t.lscale[:ls.size] = ls
+            t.magn = cf.magn
+        elif cf.kind == "periodic":
+            b = cf.base
+            t.base_kind = _KIND[b.kind]
+            ls = np.atleast_1d(b.lscale)
+            t.n_lscale = ls.size
+            t.lscale[:ls.size] = ls
+            t.magn = b.magn
+            t.period = cf.period
+        elif cf.kind == "nn":
+            t.sigma0, t.sigma, t.magn = cf.sigma0, cf.sigma, cf.magn
+        else:
+            t.magn = cf.magn
+        free_mask.extend(0 if _fixed(o.priors.get(nm)) else 1 for (o, nm, _i) in cf.all_slots())
+    xp = np.hstack(blocks) if blocks else np.zeros((_as_matrix(x).shape[0], 0))
+    return arr, len(terms), np.asfortranarray(xp), free_mask
+
+
+def eval_cf(cfs, x1, x2, diag_only=False):
+    """K between two input sets on the device (R/cf.R:402-411)."""
+    if isinstance(cfs, Cf):
+        cfs = [cfs]
+    t1, nt, x1p, _ = _program(cfs, x1)
+    _, _, x2p, _ = _program(cfs, x2)
+    n1, n2 = x1p.shape[0], x2p.shape[0]
+    out = np.empty(min(n1, n2)) if diag_only else np.empty((n1, n2), order="F")
+    L.check(L.lib().gpl_eval_cf(L.ctx(), t1, nt, L.dptr(x1p), n1, L.dptr(x2p), n2, x1p.shape[1], int(diag_only),
+                                out.ctypes.data_as(L._dp)))
+    return out
+
+
+# ---- gp_fit / gp_energy / gp_pred ---------------------------------------
+
+_LIKMAP = {"gaussian": L.LIK_GAUSSIAN, "bernoulli": L.LIK_BERNOULLI, "binomial": L.LIK_BINOMIAL,
+           "poisson": L.LIK_POISSON}
+_LINKMAP = {"identity": L.LINK_IDENTITY, "logit": L.LINK_LOGIT, "probit": L.LINK_PROBIT, "log": L.LINK_LOG}
+
+
+def _lik_struct(lik):
+    if lik.kind not in _LIKMAP:
+        raise NotImplementedError(f"lik_{lik.kind} is not on the device path")
+    s = L.Lik()
+    s.kind = _LIKMAP[lik.kind]
+    s.link = _LINKMAP[lik.link]
+    s.sigma = getattr(lik, "sigma", 0.0)
+    return s
+
+
+def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None):
+    """R/gp_fit.R:54-60: learn input scales, build K + jitter I on the device and
+    run the Laplace Newton loop or EP; returns a new fitted GP."""
+    g = gp.copy()
+    x = _as_matrix(x)
+    y = np.ascontiguousarray(np.asarray(y, dtype=np.float64).ravel())
+    if x.shape[0] != y.shape[0]:
+        raise ValueError("x and y have different numbers of observations")
+    for cf in g.cfs:
+        cf.learn_scales(x)
+    terms, nt, xp, _ = _program(g.cfs, x)
+    if isinstance(g.lik, lik_binomial) and trials is None:
+        raise ValueError("trials must be provided for the binomial likelihood.")
+    if isinstance(g.lik, lik_bernoulli) and not isinstance(g.lik, lik_binomial):
+        trials = None
+    tr = None if trials is None else np.ascontiguousarray(np.asarray(trials, dtype=np.float64).ravel())
+    off = None if offset is None else np.ascontiguousarray(np.broadcast_to(np.asarray(offset, float), y.shape))
+    o = L.Opts()
+    L.lib().gpl_opts_default(C_byref(o))
+    if isinstance(g.approx, approx_ep):
+        o.approx = L.APPROX_EP
+        o.maxiter = g.approx.maxiter
+        o.damping = g.approx.damping
+        o.quad_order = g.approx.quad_order
+    else:
+        o.approx = L.APPROX_LAPLACE
+        o.maxiter = g.approx.maxiter
+        o.tol = g.approx.tol
+    o.jitter = 1e-6 if jitter is None else float(jitter)
+    prec = precision or g.precision
+    o.precision = L.PREC_MIXED if prec == "mixed" else L.PREC_FP64
+    g.precision = prec
+    lk = _lik_struct(g.lik)
+    handle = L._vp()
+    logz = C_double()
+    n = y.shape[0]
+    rc = L.lib().gpl_fit(L.ctx(), terms, nt, L.dptr(xp), n, xp.shape[1], L.dptr(y), L.dptr(tr), L.dptr(off),
+                         C_byref(lk), C_byref(o), C_byref(handle), C_byref(logz))
+    if rc != L.GPL_OK and not handle.value:
+        L.check(rc)
+    iters = C_int32()
+    L.lib().gpl_model_info(handle, None, C_byref(iters), None)
+    fit = _Fit(handle, n, float(logz.value), int(iters.value), o.approx)
+    if rc == L.GPL_ERR_NOCONV:
+        warnings.warn(L.lib().gpl_last_error().decode())
+    elif rc == L.GPL_ERR_NAN:
+        fit.log_evidence = -math.inf
+    elif rc != L.GPL_OK:
+        L.check(rc)
+    g.fit = fit
+    g.x = x
+    g.fitted = True
+    g._trials, g._offset = tr, off
+    return g
+
+
+def gp_energy(gp, include_prior=True):
+    """-log evidence (- log prior): R/gp.R:138-147."""
+    if not is_fitted(gp):
+        raise RuntimeError("The GP must be fitted. Call gp_fit first.")
+    e = -gp.fit.log_evidence
+    if include_prior:
+        e -= lpdf_prior(gp)
+    return e
+
+
+def gp_energy_grad(gp, include_prior=True):
+    """d gp_energy / d get_param(gp) (Gaussian likelihood), computed on the device."""
+    if not is_fitted(gp):
+        raise RuntimeError("The GP must be fitted. Call gp_fit first.")
+    terms, nt, _, mask = _program(gp.cfs, gp.x)
+    ng = L.lib().gpl_grad_len(terms, nt, _LIKMAP[gp.lik.kind])
+    full = np.empty(ng)
+    L.check(L.lib().gpl_energy_grad(gp.fit.handle, L.dptr(full)))
+    mask = mask + [0 if _fixed(gp.lik.priors.get(nm)) else 1 for nm in gp.lik.param_names]
+    g = full[np.asarray(mask, dtype=bool)]
+    if include_prior:
+        g = g - _lpdf_prior_grad(gp)
+    return g
+
+
+def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, jitter=None, quad_order=15,
+            offset=None):
+    """R/gp_pred.R:88-122 for method_full: posterior if fitted, prior otherwise."""
+    if cfind is not None:
+        raise NotImplementedError("cfind is not supported by the device path")
+    if quantiles is not None or transform:
+        var = True
+    xt = _as_matrix(xnew)
+    m = xt.shape[0]
+    if not is_fitted(gp):  # R/gp_pred.R:142-156
+        mean = np.zeros(m)
+        v = eval_cf(gp.cfs, xt, xt, diag_only=True) if var else None
+    else:
+        _, _, xtp, _ = _program(gp.cfs, xt)
+        mean = np.empty(m)
+        v = np.empty(m) if var else None
+        L.check(L.lib().gpl_pred(gp.fit.handle, L.dptr(xtp), m, int(bool(var)), L.dptr(mean), L.dptr(v)))
+    if offset is not None:
+        mean = mean + offset
+    out = {"mean": mean}
+    if var:
+        out["var"] = v
+    if quantiles is not None:
+        from scipy.stats import norm
+        q = np.column_stack([norm.ppf(qq, loc=mean, scale=np.sqrt(v)) for qq in np.atleast_1d(quantiles)])
+        if transform:
+            q = gp.lik.response(q)
+        out["quantiles"] = q
+    if transform:
+        xg, wg = _gh(quad_order)
+        fg = mean[:, None] + np.sqrt(v)[:, None] * xg[None, :]
+        ft = gp.lik.response(fg)
+        mt = ft @ wg
+        out["mean"] = mt
+        out["var"] = ((ft - mt[:, None]) ** 2) @ wg
+    return out
+
+
+def _gh(q):
+    x, w = np.polynomial.hermite_e.hermegauss(q)
+    return x, w / w.sum()
+
+
+def gp_optim(gp, x, y, tol=1e-4, tol_param=1e-1, maxiter=500, restarts=1, verbose=True, warnings_=True,
+             method="Nelder-Mead", **fit_args):
+    """R/gp_optim.R:63-125: minimise gp_energy over log-parameters.  Every
+    evaluation is a device refit.  method="L-BFGS-B" uses the device gradient
+    (Gaussian likelihood)."""
+    from scipy.optimize import minimize
+    names = list(get_param(gp).keys())
+    p0 = np.array(list(get_param(gp).values()))
+    n = len(np.asarray(y).ravel())
+    state = {"it": 0}
+
+    def energy(p):
+        g2 = gp_fit(set_param(gp, p), x, y, **fit_args)
+        e = gp_energy(g2)
+        state["it"] += 1
+        if verbose:
+            print(" ".join(f"{v:8.2f}" for v in p), f"{e:10.2f}", f"{state['it']:9d}")
+        return e / n, g2
+
+    if method == "L-BFGS-B":
+        def fg(p):
+            e, g2 = energy(p)
+            return e, gp_energy_grad(g2) / n
+        res = minimize(fg, p0, jac=True, method="L-BFGS-B", options={"maxiter": maxiter, "ftol": tol})
+    else:
+        res = minimize(lambda p: energy(p)[0], p0, method="Nelder-Mead",
+                       options={"maxiter": maxiter, "fatol": tol, "xatol": 1e-8})
+    popt = res.x
+    g = gp_fit(set_param(gp, popt), x, y, **fit_args)
+    if warnings_:
+        e_opt = gp_energy(g)
+        for k in range(len(popt)):  # check_convergence, R/gp_optim.R:127-160
+            d = np.zeros_like(popt); d[k] = tol_param
+            e1 = gp_energy(gp_fit(set_param(gp, popt + d), x, y, **fit_args))
+            e2 = gp_energy(gp_fit(set_param(gp, popt - d), x, y, **fit_args))
+            if e_opt - min(e1, e2) > 1e-2:
+                if restarts > 0:
+                    return gp_optim(g, x, y, tol, tol_param, maxiter, restarts - 1, verbose, warnings_, method,
+                                    **fit_args)
+                warnings.warn(f"Not all hyperparameters have reached convergence within tolerance tol_param = "
+                              f"{tol_param}. Try increasing the number of restarts, or reducing tol.")
+                break
+    _ = names
+    return g
+
+
+from ctypes import byref as C_byref, c_double as C_double, c_int32 as C_int32  # noqa: E402

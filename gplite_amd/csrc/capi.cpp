@@ -1,0 +1,1035 @@
+// C-ABI implementation: device context, model handles and the orchestration of
+// the dense GP core (fit / energy+gradient / prediction) on one gfx950 GPU.
+//
+// Reference call chains restated here:
+//   gp_fit -> fit_laplace.method_full  R/gp_fit.R:54-60,91-99
+//          -> laplace.method_full      R/laplace.R:274-297 (Newton loop)
+//          -> laplace_iter.method_full R/laplace.R:13-39,119 (B-form Cholesky chain)
+//   gp_fit -> fit_ep.method_full       R/gp_fit.R:150-161 -> ep.method_full R/ep.R:286-330
+//   gp_pred -> gp_pred_post.method_full R/gp_pred.R:202-232
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpl_internal.h"
+
+using namespace gpl;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static thread_local int64_t g_minor = 0;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(GPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+extern "C" const char* gpl_last_error(void) { return g_err.c_str(); }
+extern "C" int64_t gpl_last_error_minor(void) { return g_minor; }
+
+static inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
+
+// ---------------------------------------------------------------------------
+// context, profiling
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+
+struct ProfClass {
+    std::string name;
+    double ms = 0, flops = 0, bytes = 0;
+    int64_t launches = 0;
+};
+
+struct gpl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool prof = false;
+    std::vector<ProfRec> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<ProfClass> classes;
+    std::map<std::string, int> cls_index;
+    std::vector<void*> scratch;
+    std::vector<size_t> scratch_sz;
+    std::mutex mu;
+
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    void resolve() {
+        if (pending.empty()) return;
+        hipEventSynchronize(pending.back().b);
+        for (auto& r : pending) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, r.a, r.b);
+            auto& c = classes[r.cls];
+            c.ms += ms;
+            c.launches += 1;
+            c.flops += r.flops;
+            c.bytes += r.bytes;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    int cls(const char* name) {
+        auto it = cls_index.find(name);
+        if (it != cls_index.end()) return it->second;
+        int id = (int)classes.size();
+        classes.push_back(ProfClass{name});
+        cls_index[name] = id;
+        return id;
+    }
+    void* buf(int slot, size_t bytes) {
+        if ((int)scratch.size() <= slot) {
+            scratch.resize(slot + 1, nullptr);
+            scratch_sz.resize(slot + 1, 0);
+        }
+        if (scratch_sz[slot] < bytes) {
+            if (scratch[slot]) hipFree(scratch[slot]);
+            scratch[slot] = nullptr;
+            scratch_sz[slot] = 0;
+            if (hipMalloc(&scratch[slot], bytes) != hipSuccess) return nullptr;
+            scratch_sz[slot] = bytes;
+        }
+        return scratch[slot];
+    }
+};
+
+// Times one launch group on the library stream when profiling is on.
+struct Prof {
+    gpl_ctx* c;
+    ProfRec r;
+    bool on;
+    Prof(gpl_ctx* ctx, const char* name, double flops, double bytes) : c(ctx), on(ctx->prof) {
+        if (!on) return;
+        r.cls = c->cls(name);
+        r.a = c->ev();
+        r.b = c->ev();
+        r.flops = flops;
+        r.bytes = bytes;
+        hipEventRecord(r.a, c->stream);
+    }
+    ~Prof() {
+        if (!on) return;
+        hipEventRecord(r.b, c->stream);
+        c->pending.push_back(r);
+        if (c->pending.size() > 8192) c->resolve();
+    }
+};
+
+extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
+    if (!out) return fail(GPL_ERR_ARG, "out is NULL");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(GPL_ERR_ARG, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    gpl_ctx* c = new gpl_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(GPL_ERR_HIP, hipGetErrorString(e));
+    }
+    *out = c;
+    return GPL_OK;
+}
+
+extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    c->resolve();
+    for (auto e : c->pool) hipEventDestroy(e);
+    for (auto p : c->scratch)
+        if (p) hipFree(p);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int gpl_device_name(gpl_ctx* c, char* buf, int32_t len) {
+    if (!c || !buf || len <= 0) return fail(GPL_ERR_ARG, "bad args");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, c->device));
+    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return GPL_OK;
+}
+
+extern "C" int gpl_profile_enable(gpl_ctx* c, int32_t on) {
+    if (!c) return fail(GPL_ERR_ARG, "ctx NULL");
+    c->prof = on != 0;
+    return GPL_OK;
+}
+
+extern "C" int gpl_profile_reset(gpl_ctx* c) {
+    if (!c) return fail(GPL_ERR_ARG, "ctx NULL");
+    hipStreamSynchronize(c->stream);
+    c->resolve();
+    for (auto& k : c->classes) k.ms = k.flops = k.bytes = 0, k.launches = 0;
+    return GPL_OK;
+}
+
+extern "C" int gpl_profile_read(gpl_ctx* c, int32_t cap, char* names, double* total_ms, int64_t* launches,
+                                double* flops, double* bytes) {
+    if (!c) return fail(GPL_ERR_ARG, "ctx NULL");
+    hipStreamSynchronize(c->stream);
+    c->resolve();
+    int n = (int)c->classes.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        const auto& k = c->classes[i];
+        if (names) {
+            strncpy(names + 32 * i, k.name.c_str(), 31);
+            names[32 * i + 31] = 0;
+        }
+        if (total_ms) total_ms[i] = k.ms;
+        if (launches) launches[i] = k.launches;
+        if (flops) flops[i] = k.flops;
+        if (bytes) bytes[i] = k.bytes;
+    }
+    return n;
+}
+
+extern "C" void gpl_opts_default(gpl_opts* o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->approx = GPL_APPROX_LAPLACE;
+    o->maxiter = 30;
+    o->tol = 1e-4;
+    o->damping = 0.9;
+    o->damping_min = 0.1;
+    o->quad_order = 11;
+    o->precision = GPL_PREC_FP64;
+    o->jitter = 1e-6;
+    o->refine_iters = 8;
+}
+
+// ---------------------------------------------------------------------------
+// covariance program preparation
+// ---------------------------------------------------------------------------
+static int term_ngrad(const gpl_cf_term& t) {
+    switch (t.kind) {
+        case GPL_CF_CONST:
+        case GPL_CF_LIN: return 1;
+        case GPL_CF_NN: return 3;
+        case GPL_CF_PERIODIC: return 2 + t.n_lscale;
+        default: return 1 + t.n_lscale;
+    }
+}
+
+extern "C" int32_t gpl_grad_len(const gpl_cf_term* terms, int32_t nterms, int32_t lik_kind) {
+    int32_t g = 0;
+    for (int k = 0; k < nterms; ++k) g += term_ngrad(terms[k]);
+    if (lik_kind == GPL_LIK_GAUSSIAN) g += 1;
+    return g;
+}
+
+static int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P) {
+    memset(&P, 0, sizeof(P));
+    if (!terms || nterms < 1 || nterms > GPL_MAX_TERMS) return fail(GPL_ERR_ARG, "nterms must be in [1, 8]");
+    int f = 0, g = 0;
+    // periodic derivative features are placed after all value features
+    int nval = 0;
+    for (int k = 0; k < nterms; ++k) {
+        const gpl_cf_term& t = terms[k];
+        if (t.kind == GPL_CF_CONST) continue;
+        nval += (t.kind == GPL_CF_PERIODIC) ? 2 * t.ncols : t.ncols;
+    }
+    int fd = nval;
+    for (int k = 0; k < nterms; ++k) {
+        const gpl_cf_term& t = terms[k];
+        DevTerm& d = P.t[k];
+        if (t.kind < GPL_CF_CONST || t.kind > GPL_CF_PERIODIC) return fail(GPL_ERR_ARG, "unknown cf kind");
+        d.kind = t.kind;
+        d.op = (k == 0) ? GPL_OP_SUM : t.op;
+        d.m2 = t.magn * t.magn;
+        if (!(t.magn > 0) && t.kind != GPL_CF_CONST && t.kind != GPL_CF_LIN)
+            return fail(GPL_ERR_ARG, "magn must be positive");
+        if (t.kind == GPL_CF_CONST) {
+            d.nc = d.nf = 0;
+        } else {
+            if (t.ncols < 1 || t.ncols > GPL_MAX_COLS) return fail(GPL_ERR_ARG, "term ncols must be in [1, 16]");
+            if (t.col0 < 0 || t.col0 + t.ncols > ncols) return fail(GPL_ERR_ARG, "term columns out of range");
+            d.c0 = t.col0;
+            d.nc = t.ncols;
+            d.f0 = f;
+            d.nf = (t.kind == GPL_CF_PERIODIC) ? 2 * t.ncols : t.ncols;
+            f += d.nf;
+        }
+        if (t.kind == GPL_CF_SEXP || t.kind == GPL_CF_MATERN32 || t.kind == GPL_CF_MATERN52 ||
+            t.kind == GPL_CF_PERIODIC) {
+            if (t.kind == GPL_CF_PERIODIC) {
+                if (t.base_kind != GPL_CF_SEXP && t.base_kind != GPL_CF_MATERN32 && t.base_kind != GPL_CF_MATERN52)
+                    return fail(GPL_ERR_ARG, "periodic base must be sexp/matern32/matern52");
+                if (!(t.period > 0)) return fail(GPL_ERR_ARG, "period must be positive");
+                d.base_kind = t.base_kind;
+                d.w = 2.0 * M_PI / t.period;
+                d.df0 = fd;
+                fd += d.nf;
+            }
+            if (t.n_lscale != 1 && t.n_lscale != d.nf)
+                return fail(GPL_ERR_ARG, "n_lscale must be 1 or the number of (feature) columns");
+            d.nl = t.n_lscale;
+            for (int c = 0; c < d.nf; ++c) {
+                const double l = t.lscale[t.n_lscale == 1 ? 0 : c];
+                if (!(l > 0)) return fail(GPL_ERR_ARG, "lscale must be positive");
+                d.il2[c] = 1.0 / (l * l);
+            }
+        }
+        if (t.kind == GPL_CF_NN) {
+            d.s2 = t.sigma * t.sigma;
+            d.b0 = t.sigma0 * t.sigma0 * d.s2;
+        }
+        d.g0 = g;
+        d.ng = term_ngrad(t);
+        g += d.ng;
+    }
+    if (fd > MAX_FEAT) return fail(GPL_ERR_ARG, "too many feature columns (max 64)");
+    if (g > GRAD_MAX) return fail(GPL_ERR_ARG, "too many hyperparameters");
+    P.nterms = nterms;
+    P.nfeat = fd;
+    P.ngrad = g;
+    return GPL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// model
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t sz = 0;
+    void free_() {
+        if (p) hipFree(p);
+        p = nullptr;
+        sz = 0;
+    }
+    bool ensure(size_t bytes) {
+        if (sz >= bytes && p) return true;
+        free_();
+        if (bytes == 0) bytes = 8;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        sz = bytes;
+        return true;
+    }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+enum {
+    B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
+    B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
+    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO,
+    B_COUNT
+};
+
+struct gpl_model {
+    gpl_ctx* ctx = nullptr;
+    int64_t n = 0, N = 0, ncols = 0;
+    DevProg prog{};
+    gpl_lik lik{};
+    gpl_opts opts{};
+    bool has_trials = false, has_offset = false;
+    bool have_xinv = false, have_binv = false;
+    double logZ = 0;
+    int32_t iters = 0;
+    DevBuf b[B_COUNT];
+    ~gpl_model() {
+        for (auto& x : b) x.free_();
+    }
+    double* d(int i) const { return b[i].as<double>(); }
+};
+
+#define ENSURE(m, idx, bytes)                                                             \
+    do {                                                                                  \
+        if (!(m)->b[idx].ensure(bytes)) return fail(GPL_ERR_OOM, "device allocation failed"); \
+    } while (0)
+
+// Blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
+static void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+    const int64_t T = N / 64;
+    hipStream_t s = c->stream;
+    for (int64_t k = 0; k < T; ++k) {
+        {
+            Prof p(c, "potrf_diag", 64.0 * 64 * 64 / 3 + 64.0 * 64 * 64 / 3, 2 * 64.0 * 64 * 8);
+            potrf_diag_f64(s, L, N, k, Dinv, ldiag, info);
+        }
+        if (k + 1 < T) {
+            const int64_t M = (T - 1 - k) * 64;
+            double* L21 = L + (k + 1) * 64 + k * 64 * N;
+            {
+                Prof p(c, "trsm_panel", 2.0 * M * 64 * 64, 2.0 * M * 64 * 8);
+                gemm_f64(s, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0);
+            }
+            {
+                Prof p(c, "syrk_update", (double)M * M * 64, 8.0 * M * M + 8.0 * M * 64);
+                gemm_f64(s, 0, 1, M, M, 64, -1.0, L21, N, L21, N, 1.0, L + (k + 1) * 64 * (N + 1), N, 1);
+            }
+        }
+    }
+}
+
+// X = L^-1 by blocked forward substitution against the identity.
+static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
+    const int64_t T = N / 64;
+    hipStream_t s = c->stream;
+    set_identity_lower_f64(s, X, N, N);
+    for (int64_t i = 0; i < T; ++i) {
+        const int64_t W = (i + 1) * 64;
+        {
+            Prof p(c, "trtri_diag", 2.0 * 64 * 64 * W, 16.0 * 64 * W);
+            gemm_f64(s, 0, 0, 64, W, 64, 1.0, Dinv + i * 64 * 64, 64, X + i * 64, N, 0.0, X + i * 64, N, 0);
+        }
+        if (i + 1 < T) {
+            const int64_t M = (T - 1 - i) * 64;
+            Prof p(c, "trtri_update", 2.0 * M * W * 64, 16.0 * M * W);
+            gemm_f64(s, 0, 0, M, W, 64, -1.0, L + (i + 1) * 64 + i * 64 * N, N, X + i * 64, N, 1.0, X + (i + 1) * 64,
+                     N, 0);
+        }
+    }
+}
+
+// alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)
+static void solve_alpha(gpl_ctx* c, gpl_model* m, const double* z, double* alpha) {
+    hipStream_t s = c->stream;
+    const int64_t N = m->N;
+    double* w1 = m->d(B_W1);
+    double* w2 = m->d(B_W2);
+    Prof p(c, "trsv", 2.0 * N * N, 8.0 * N * N);
+    fill_f64(s, w1, N, 0.0);
+    vec_mul(s, m->n, m->d(B_SV), z, w1);
+    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w1, w2, 0);
+    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w2, w1, 1);
+    fill_f64(s, alpha, N, 0.0);
+    vec_mul(s, m->n, m->d(B_SV), w1, alpha);
+}
+
+static int check_info(gpl_model* m) {
+    int info = 0;
+    HIP_TRY(hipMemcpyAsync(&info, m->b[B_INFO].p, sizeof(int), hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    if (info > 0) {
+        g_minor = info;
+        return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(info) + " is not positive");
+    }
+    return GPL_OK;
+}
+
+static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k) {
+    const int64_t N = pad64(n);
+    m->n = n;
+    m->N = N;
+    m->ncols = ncols;
+    const size_t vec = (size_t)N * 8;
+    const size_t mat = (size_t)N * N * 8;
+    ENSURE(m, B_X, (size_t)n * ncols * 8);
+    ENSURE(m, B_F, (size_t)N * MAX_FEAT * 8);
+    ENSURE(m, B_Y, vec);
+    ENSURE(m, B_TR, vec);
+    ENSURE(m, B_OFF, vec);
+    ENSURE(m, B_L, mat);
+    ENSURE(m, B_DINV, (size_t)N * 64 * 8);
+    ENSURE(m, B_LDIAG, vec);
+    if (need_k) ENSURE(m, B_K, mat);
+    ENSURE(m, B_PROG, sizeof(DevProg));
+    for (int i : {B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM, B_FOLD, B_FVOLD, B_ZOLD,
+                  B_POLD, B_W1, B_W2, B_W3})
+        ENSURE(m, i, vec);
+    ENSURE(m, B_SCAL, 256 * 8);
+    ENSURE(m, B_KMV, (size_t)N * kmatvec_chunks(N) * 8);
+    ENSURE(m, B_GH, 64 * 8);
+    ENSURE(m, B_INFO, 64);
+    return GPL_OK;
+}
+
+static int upload_vec(gpl_model* m, int idx, const double* h, int64_t n) {
+    hipStream_t s = m->ctx->stream;
+    fill_f64(s, m->d(idx), m->N, 0.0);
+    if (h) HIP_TRY(hipMemcpyAsync(m->d(idx), h, n * 8, hipMemcpyHostToDevice, s));
+    return GPL_OK;
+}
+
+// Read small device scalars (one synchronisation).
+static int read_scalars(gpl_model* m, double* out, int k) {
+    HIP_TRY(hipMemcpyAsync(out, m->d(B_SCAL), k * 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return GPL_OK;
+}
+
+// One Gaussian "Newton step" (exactly one: R/laplace.R:281-283).
+static int fit_gaussian(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const double sigma = m->lik.sigma;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    {
+        Prof p(c, "build_B", 0, 4.0 * N * (N + 64));
+        launch_build_sym_f64(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1, 1.0 / sigma, nullptr, m->d(B_L), N);
+    }
+    lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
+                      m->d(B_SV), m->b[B_INFO].as<int>() + 1);
+    potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
+    solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+    {
+        Prof p(c, "kmatvec", 0, 0);
+        launch_kmatvec(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV),
+                       kmatvec_chunks(N));
+    }
+    lik_loglik(s, GPL_LIK_GAUSSIAN, 0, sigma, n, m->d(B_FMEAN), m->d(B_Y), nullptr,
+               m->has_offset ? m->d(B_OFF) : nullptr, m->d(B_W3));
+    double* sc = m->d(B_SCAL);
+    reduce_sum(s, m->d(B_W3), n, sc + 0);
+    reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
+    reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+    int r = check_info(m);
+    if (r) return r;
+    double h[3];
+    r = read_scalars(m, h, 3);
+    if (r) return r;
+    // log_lik - 1/2 f^T alpha - 1/2 (logdet C - logdet V), logdet C - logdet V = 2 sum log L_ii
+    m->logZ = h[0] - 0.5 * h[1] - h[2];
+    m->iters = 1;
+    return GPL_OK;
+}
+
+// Laplace Newton loop for non-Gaussian likelihoods (R/laplace.R:274-297).
+static int fit_laplace(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
+    const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+    {
+        Prof p(c, "build_K", 0, 8.0 * N * (N + 64) / 2);
+        launch_build_sym_f64(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 0, 1.0, nullptr, m->d(B_K), N);
+    }
+    fill_f64(s, m->d(B_FOLD), N, 0.0);  // fhat = 0
+    const int maxiter = m->opts.maxiter > 0 ? m->opts.maxiter : 30;
+    double diff = INFINITY;
+    int it = 0;
+    for (it = 1; it <= maxiter; ++it) {
+        int* flags = m->b[B_INFO].as<int>();
+        hipMemsetAsync(flags, 0, 8, s);
+        lik_pseudodata_la(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
+                          m->d(B_V), m->d(B_SV), flags + 1);
+        {
+            Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
+            form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
+        }
+        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags);
+        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+        {
+            Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
+            symv_lower_f64(s, m->d(B_K), N, N, m->d(B_ALPHA), m->d(B_FMEAN), nullptr);
+        }
+        lik_loglik(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
+        double* sc = m->d(B_SCAL);
+        reduce_sum(s, m->d(B_W3), n, sc + 0);
+        reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
+        reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+        reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+        int fl[2];
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        double h[4];
+        int r = read_scalars(m, h, 4);
+        if (r) return r;
+        if (fl[1] & 6) {
+            m->logZ = -INFINITY;
+            return fail((fl[1] & 2) ? GPL_ERR_NAN : GPL_ERR_UNSUPPORTED,
+                        (fl[1] & 2) ? "NaN in Laplace pseudo-data"
+                                    : "non-log-concave pseudo-variance (V <= 0): host fallback branch "
+                                      "R/laplace.R:40-117 is not on the device path");
+        }
+        if (fl[0] > 0) {
+            g_minor = fl[0];
+            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+        }
+        m->logZ = h[0] - 0.5 * h[1] - h[2];
+        diff = h[3];
+        HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
+        if (diff < m->opts.tol) break;
+    }
+    if (it > maxiter) it = maxiter;
+    m->iters = it;
+    if (maxiter > 1 && it == maxiter && !(diff < m->opts.tol)) {
+        // the reference only warns here (R/laplace.R:293-295): keep the fit, report NOCONV
+        g_err = "Maximum number of iterations in Laplace reached, max delta f = " + std::to_string(diff);
+        return GPL_ERR_NOCONV;
+    }
+    return GPL_OK;
+}
+
+// probabilists' Gauss-Hermite rule: nodes = eigenvalues of the Jacobi matrix of He_k
+// (Golub-Welsch), weights normalised to sum 1 -- the same rule as R/quadrature.R:28-34.
+static void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w) {
+    // symmetric tridiagonal: diag 0, off-diagonal sqrt(k), k = 1..q-1; Jacobi eigen-iteration
+    std::vector<double> A(q * q, 0.0), V(q * q, 0.0);
+    for (int k = 1; k < q; ++k) A[k * q + k - 1] = A[(k - 1) * q + k] = std::sqrt((double)k);
+    for (int i = 0; i < q; ++i) V[i * q + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int i = 0; i < q; ++i)
+            for (int j = i + 1; j < q; ++j) off += A[i * q + j] * A[i * q + j];
+        if (off < 1e-30) break;
+        for (int p = 0; p < q; ++p)
+            for (int r = p + 1; r < q; ++r) {
+                const double apr = A[p * q + r];
+                if (std::fabs(apr) < 1e-300) continue;
+                const double th = (A[r * q + r] - A[p * q + p]) / (2 * apr);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                const double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < q; ++k) {
+                    const double akp = A[k * q + p], akr = A[k * q + r];
+                    A[k * q + p] = cs * akp - sn * akr;
+                    A[k * q + r] = sn * akp + cs * akr;
+                }
+                for (int k = 0; k < q; ++k) {
+                    const double apk = A[p * q + k], ark = A[r * q + k];
+                    A[p * q + k] = cs * apk - sn * ark;
+                    A[r * q + k] = sn * apk + cs * ark;
+                }
+                for (int k = 0; k < q; ++k) {
+                    const double vkp = V[k * q + p], vkr = V[k * q + r];
+                    V[k * q + p] = cs * vkp - sn * vkr;
+                    V[k * q + r] = sn * vkp + cs * vkr;
+                }
+            }
+    }
+    std::vector<std::pair<double, double>> nw(q);
+    for (int i = 0; i < q; ++i) nw[i] = {A[i * q + i], V[0 * q + i] * V[0 * q + i]};
+    std::sort(nw.begin(), nw.end());
+    x.resize(q);
+    w.resize(q);
+    double sw = 0;
+    for (int i = 0; i < q; ++i) sw += nw[i].second;
+    for (int i = 0; i < q; ++i) {
+        x[i] = nw[i].first;
+        w[i] = nw[i].second / sw;
+    }
+}
+
+// Parallel EP (R/ep.R:286-330, ep_iter R/ep.R:16-49,140-146).
+static int fit_ep(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
+    const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+    const int q = m->opts.quad_order > 0 ? m->opts.quad_order : 11;
+    if (q > 32) return fail(GPL_ERR_ARG, "quad_order must be <= 32");
+    std::vector<double> gx, gw;
+    gauss_hermite(q, gx, gw);
+    HIP_TRY(hipMemcpyAsync(m->d(B_GH), gx.data(), q * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(m->d(B_GH) + 32, gw.data(), q * 8, hipMemcpyHostToDevice, s));
+    ENSURE(m, B_XINV, (size_t)N * N * 8);
+    {
+        Prof p(c, "build_K", 0, 8.0 * N * (N + 64) / 2);
+        launch_build_sym_f64(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 0, 1.0, nullptr, m->d(B_K), N);
+    }
+    // P_old = 0, z_old = 0, fmean_old = 0, fvar_old = diag(K)
+    fill_f64(s, m->d(B_POLD), N, 0.0);
+    fill_f64(s, m->d(B_ZOLD), N, 0.0);
+    fill_f64(s, m->d(B_FOLD), N, 0.0);
+    launch_build_diag(s, pd, m->prog, m->d(B_F), N, m->d(B_F), N, n, m->d(B_FVOLD));
+    // fvar_old = diag(K) with K = eval_cf + jitter I (R/gp_fit.R:154, R/ep.R:298)
+    fill_f64(s, m->d(B_W3), N, m->opts.jitter);
+    vec_axpby(s, n, 1.0, m->d(B_FVOLD), 1.0, m->d(B_W3), m->d(B_FVOLD));
+
+    const int maxiter = m->opts.maxiter > 0 ? m->opts.maxiter : 100;
+    const double tol = 1e-4, damping = m->opts.damping, dmin = m->opts.damping_min > 0 ? m->opts.damping_min : 0.1;
+    double damp = damping, diff_old = INFINITY, diff_new = INFINITY;
+    bool quad_ok = true;
+    int it;
+    for (it = 1; it <= maxiter; ++it) {
+        int* flags = m->b[B_INFO].as<int>();
+        hipMemsetAsync(flags, 0, 8, s);
+        {
+            Prof p(c, "ep_sites", 0, 0);
+            ep_sites(s, m->lik.kind, m->lik.link, n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
+                     m->d(B_Y), tr, off, q, m->d(B_GH), m->d(B_GH) + 32, damp, m->d(B_Z), m->d(B_V), m->d(B_SV),
+                     m->d(B_LOGZS), m->d(B_CAVM), m->d(B_CAVV), flags + 1);
+        }
+        int fl[2];
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (fl[1] & 2) {
+            m->logZ = -INFINITY;
+            m->iters = it;
+            return fail(GPL_ERR_NAN, "NaN in EP site parameters (R/ep.R:30-32 returns log_evidence = -Inf)");
+        }
+        if (fl[1] & 4) {
+            m->logZ = -INFINITY;
+            return fail(GPL_ERR_UNSUPPORTED,
+                        "EP site variance <= 0: host fallback branch R/ep.R:50-138 is not on the device path");
+        }
+        if (fl[1] & 1) quad_ok = false;
+        {
+            Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
+            form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
+        }
+        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags);
+        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+        {
+            Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
+            symv_lower_f64(s, m->d(B_K), N, N, m->d(B_ALPHA), m->d(B_FMEAN), nullptr);
+        }
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);
+        ep_post(s, n, m->d(B_V), m->d(B_Z), m->d(B_W3), m->d(B_CAVM), m->d(B_CAVV), m->d(B_LOGZS), m->d(B_FVAR),
+                m->d(B_TERM));
+        double* sc = m->d(B_SCAL);
+        reduce_sum(s, m->d(B_TERM), n, sc + 0);
+        reduce_dot(s, m->d(B_Z), m->d(B_ALPHA), n, sc + 1);
+        reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+        reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        double h[4];
+        int r = read_scalars(m, h, 4);
+        if (r) return r;
+        if (fl[0] > 0) {
+            g_minor = fl[0];
+            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+        }
+        m->logZ = h[0] - h[2] - 0.5 * h[1];
+        diff_new = h[3];
+        if (diff_new > diff_old) {
+            damp = damp * damping;
+            continue;
+        }
+        damp = std::max(damp, dmin);
+        HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(m->d(B_FVOLD), m->d(B_FVAR), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(m->d(B_ZOLD), m->d(B_Z), N * 8, hipMemcpyDeviceToDevice, s));
+        vec_recip(s, n, m->d(B_V), m->d(B_POLD));  // P_old = fit$P = 1 / V
+        diff_old = diff_new;
+        if (diff_new < tol) break;
+    }
+    if (it > maxiter) it = maxiter;
+    m->iters = it;
+    if (maxiter > 1 && it == maxiter && !(diff_new < tol)) {
+        m->logZ = -INFINITY;
+        g_err = "Maximum number of iterations in EP reached, max delta f = " + std::to_string(diff_new);
+        return GPL_ERR_NOCONV;
+    }
+    if (!quad_ok) m->logZ = -INFINITY;
+    m->have_xinv = true;
+    return GPL_OK;
+}
+
+extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n,
+                       int64_t ncols, const double* y, const double* trials, const double* offset, const gpl_lik* lik,
+                       const gpl_opts* opts, gpl_model** inout, double* log_evidence) {
+    if (!c || !x || !y || !lik || !inout) return fail(GPL_ERR_ARG, "NULL argument");
+    if (n < 1 || ncols < 0) return fail(GPL_ERR_ARG, "n must be >= 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    gpl_opts o;
+    if (opts) o = *opts;
+    else gpl_opts_default(&o);
+    if (lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_EP)
+        return fail(GPL_ERR_ARG, "approx_ep disabled for lik_gaussian: use approx_laplace instead, because it gives "
+                                 "exact results");  // R/gp.R:54-56
+    if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
+    if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
+    DevProg P;
+    int r = prepare_prog(terms, nterms, ncols, P);
+    if (r) return r;
+    gpl_model* m = *inout;
+    bool fresh = false;
+    if (!m || m->ctx != c) {
+        m = new gpl_model();
+        m->ctx = c;
+        fresh = true;
+    }
+    const bool need_k = !(lik->kind == GPL_LIK_GAUSSIAN);
+    r = alloc_model(m, n, ncols, need_k);
+    if (r) {
+        if (fresh) delete m;
+        return r;
+    }
+    m->prog = P;
+    m->lik = *lik;
+    m->opts = o;
+    m->has_trials = trials != nullptr;
+    m->has_offset = offset != nullptr;
+    m->have_xinv = m->have_binv = false;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemcpyAsync(m->b[B_PROG].p, &m->prog, sizeof(DevProg), hipMemcpyHostToDevice, s));
+    if (n * ncols > 0) HIP_TRY(hipMemcpyAsync(m->d(B_X), x, n * ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, m->d(B_X), n, n, (int)ncols, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), m->N);
+    upload_vec(m, B_Y, y, n);
+    upload_vec(m, B_TR, trials, n);
+    upload_vec(m, B_OFF, offset, n);
+    hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
+    if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
+    else if (lik->kind == GPL_LIK_GAUSSIAN) r = fit_gaussian(m);
+    else r = fit_laplace(m);
+    *inout = m;
+    if (log_evidence) *log_evidence = m->logZ;
+    if (r && r != GPL_ERR_NOCONV && r != GPL_ERR_NAN) return r;
+    return r;
+}
+
+extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
+    if (!m || !grad) return fail(GPL_ERR_ARG, "NULL argument");
+    if (m->lik.kind != GPL_LIK_GAUSSIAN)
+        return fail(GPL_ERR_UNSUPPORTED, "the energy gradient is provided for the Gaussian likelihood");
+    gpl_ctx* c = m->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    ENSURE(m, B_XINV, (size_t)N * N * 8);
+    ENSURE(m, B_BINV, (size_t)N * N * 8);
+    const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
+    const int ng = m->prog.ngrad;
+    ENSURE(m, B_PART, (size_t)ntiles * (ng > 0 ? ng : 1) * 8);
+    if (!m->have_xinv) {
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        m->have_xinv = true;
+    }
+    {
+        Prof p(c, "lauum", (double)N * N * N / 3, 8.0 * N * N);
+        gemm_f64(s, 1, 0, N, N, N, 1.0, m->d(B_XINV), N, m->d(B_XINV), N, 0.0, m->d(B_BINV), N, 2);
+    }
+    m->have_binv = true;
+    const double sigma = m->lik.sigma;
+    const double s2 = sigma * sigma;
+    {
+        Prof p(c, "grad", 0, 8.0 * N * (N + 64) / 2);
+        launch_grad_tiles(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, m->d(B_BINV), N, 1.0 / s2,
+                          m->d(B_ALPHA), m->d(B_PART));
+    }
+    double* sc = m->d(B_SCAL);
+    reduce_cols(s, m->d(B_PART), ntiles, ng, ng, sc);
+    reduce_dot(s, m->d(B_ALPHA), m->d(B_ALPHA), n, sc + ng);
+    colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);
+    reduce_sum(s, m->d(B_W3), n, sc + ng + 1);
+    std::vector<double> h(ng + 2);
+    HIP_TRY(hipMemcpyAsync(h.data(), sc, (ng + 2) * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int p = 0; p < ng; ++p) grad[p] = -0.5 * h[p];
+    // sigma: -1/2 tr((a a^T - Ky^-1) 2 sigma^2 I) = -sigma^2 a^T a + tr(B^-1)
+    grad[ng] = -s2 * h[ng] + h[ng + 1];
+    return GPL_OK;
+}
+
+extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var) {
+    if (!m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
+    if (mt <= 0) return GPL_OK;
+    gpl_ctx* c = m->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N, M = pad64(mt);
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
+    double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
+    double* KtT = (double*)c->buf(2, (size_t)N * M * 8);
+    double* vbuf = (double*)c->buf(3, (size_t)M * 3 * 8);
+    if (!xtd || !Ft || !KtT || !vbuf) return fail(GPL_ERR_OOM, "device allocation failed");
+    if (mt * m->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, xtd, mt, mt, (int)m->ncols, pd, m->prog, Ft, M);
+    {
+        Prof p(c, "pred_cross", 0, 8.0 * N * M);
+        launch_build_cross(s, pd, m->prog, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
+    }
+    gemv_t_f64(s, KtT, N, N, mt, m->d(B_ALPHA), vbuf);
+    if (want_var) {
+        // aux = C^-1 K_t^T = L^-1 (sv .* K_t^T)  (R/gp_pred.R:218)
+        scale_rows_f64(s, KtT, N, n, M, m->d(B_SV));
+        const int64_t T = N / 64;
+        if (m->have_xinv) {
+            double* A2 = (double*)c->buf(4, (size_t)N * M * 8);
+            if (!A2) return fail(GPL_ERR_OOM, "device allocation failed");
+            Prof p(c, "pred_trmm", (double)N * N * M, 8.0 * N * M * 2);
+            gemm_f64(s, 0, 0, N, M, N, 1.0, m->d(B_XINV), N, KtT, N, 0.0, A2, N, 3);
+            KtT = A2;
+        } else {
+            for (int64_t i = 0; i < T; ++i) {
+                {
+                    Prof p(c, "pred_trsm_diag", 2.0 * 64 * 64 * M, 16.0 * 64 * M);
+                    gemm_f64(s, 0, 0, 64, M, 64, 1.0, m->d(B_DINV) + i * 64 * 64, 64, KtT + i * 64, N, 0.0,
+                             KtT + i * 64, N, 0);
+                }
+                if (i + 1 < T) {
+                    const int64_t R = (T - 1 - i) * 64;
+                    Prof p(c, "pred_trsm_update", 2.0 * R * M * 64, 16.0 * R * M);
+                    gemm_f64(s, 0, 0, R, M, 64, -1.0, m->d(B_L) + (i + 1) * 64 + i * 64 * N, N, KtT + i * 64, N, 1.0,
+                             KtT + (i + 1) * 64, N, 0);
+                }
+            }
+        }
+        colsumsq_f64(s, KtT, N, N, mt, vbuf + M, 0);
+        launch_build_diag(s, pd, m->prog, Ft, M, Ft, M, mt, vbuf + 2 * M);
+        vec_axpby(s, mt, 1.0, vbuf + 2 * M, -1.0, vbuf + M, vbuf + M);
+        HIP_TRY(hipMemcpyAsync(var, vbuf + M, mt * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipMemcpyAsync(mean, vbuf, mt * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GPL_OK;
+}
+
+extern "C" int gpl_model_vector(gpl_model* m, int32_t which, double* out) {
+    if (!m || !out) return fail(GPL_ERR_ARG, "NULL argument");
+    int idx;
+    switch (which) {
+        case GPL_VEC_FMEAN: idx = B_FMEAN; break;
+        case GPL_VEC_ALPHA: idx = B_ALPHA; break;
+        case GPL_VEC_Z: idx = B_Z; break;
+        case GPL_VEC_V: idx = B_V; break;
+        case GPL_VEC_FVAR: idx = B_FVAR; break;
+        case GPL_VEC_CAVITY_MEAN: idx = B_CAVM; break;
+        case GPL_VEC_CAVITY_VAR: idx = B_CAVV; break;
+        default: return fail(GPL_ERR_ARG, "unknown vector");
+    }
+    std::lock_guard<std::mutex> lk(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    HIP_TRY(hipMemcpyAsync(out, m->d(idx), m->n * 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return GPL_OK;
+}
+
+extern "C" int gpl_model_chol(gpl_model* m, double* out) {
+    if (!m || !out) return fail(GPL_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    const int64_t n = m->n, N = m->N;
+    std::vector<double> V(n);
+    HIP_TRY(hipMemcpy2DAsync(out, n * 8, m->d(B_L), N * 8, n * 8, n, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(V.data(), m->d(B_V), n * 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) out[i + j * n] = (i >= j) ? std::sqrt(V[i]) * out[i + j * n] : 0.0;
+    return GPL_OK;
+}
+
+extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence) {
+    if (!m) return fail(GPL_ERR_ARG, "NULL model");
+    if (n) *n = m->n;
+    if (iters) *iters = m->iters;
+    if (log_evidence) *log_evidence = m->logZ;
+    return GPL_OK;
+}
+
+extern "C" void gpl_model_free(gpl_model* m) {
+    if (!m) return;
+    hipSetDevice(m->ctx->device);
+    hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+// ---------------------------------------------------------------------------
+// Level 0: eval_cf between two point sets (host in, host out)
+// ---------------------------------------------------------------------------
+extern "C" int gpl_eval_cf(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, const double* x1, int64_t n1,
+                           const double* x2, int64_t n2, int64_t ncols, int32_t diag_only, double* out) {
+    if (!c || !out || (!x1 && n1 > 0) || (!x2 && n2 > 0)) return fail(GPL_ERR_ARG, "NULL argument");
+    if (n1 < 0 || n2 < 0) return fail(GPL_ERR_ARG, "negative size");
+    DevProg P;
+    int r = prepare_prog(terms, nterms, ncols, P);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t N1 = pad64(std::max<int64_t>(n1, 1)), N2 = pad64(std::max<int64_t>(n2, 1));
+    const int64_t nc = std::max<int64_t>(ncols, 1);
+    DevProg* pd = (DevProg*)c->buf(10, sizeof(DevProg));
+    double* X1 = (double*)c->buf(11, (size_t)std::max<int64_t>(n1, 1) * nc * 8);
+    double* X2 = (double*)c->buf(12, (size_t)std::max<int64_t>(n2, 1) * nc * 8);
+    double* F1 = (double*)c->buf(13, (size_t)N1 * MAX_FEAT * 8);
+    double* F2 = (double*)c->buf(14, (size_t)N2 * MAX_FEAT * 8);
+    if (!pd || !X1 || !X2 || !F1 || !F2) return fail(GPL_ERR_OOM, "device allocation failed");
+    HIP_TRY(hipMemcpyAsync(pd, &P, sizeof(DevProg), hipMemcpyHostToDevice, s));
+    if (n1 * ncols > 0) HIP_TRY(hipMemcpyAsync(X1, x1, n1 * ncols * 8, hipMemcpyHostToDevice, s));
+    if (n2 * ncols > 0) HIP_TRY(hipMemcpyAsync(X2, x2, n2 * ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, X1, n1, n1, (int)ncols, pd, P, F1, N1);
+    launch_features(s, X2, n2, n2, (int)ncols, pd, P, F2, N2);
+    if (diag_only) {
+        const int64_t nd = std::min(n1, n2);
+        double* o = (double*)c->buf(15, (size_t)std::max<int64_t>(nd, 1) * 8);
+        if (!o) return fail(GPL_ERR_OOM, "device allocation failed");
+        launch_build_diag(s, pd, P, F1, N1, F2, N2, nd, o);
+        if (nd > 0) HIP_TRY(hipMemcpyAsync(out, o, nd * 8, hipMemcpyDeviceToHost, s));
+    } else {
+        double* o = (double*)c->buf(15, (size_t)N1 * N2 * 8);
+        if (!o) return fail(GPL_ERR_OOM, "device allocation failed");
+        {
+            Prof p(c, "build_cross", 0, 8.0 * N1 * N2);
+            launch_build_cross(s, pd, P, F1, N1, n1, N1, F2, N2, n2, N2, o, N1);
+        }
+        if (n1 > 0 && n2 > 0)
+            HIP_TRY(hipMemcpy2DAsync(out, n1 * 8, o, N1 * 8, n1 * 8, n2, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return GPL_OK;
+}
+
+static int cf_simple(gpl_ctx* c, int kind, const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t d,
+                     const double* lscale, int64_t nl, double magn, double sigma0, double sigma, int32_t diag_only,
+                     double* out) {
+    if (d < 1 || d > GPL_MAX_COLS) return fail(GPL_ERR_ARG, "d must be in [1, 16]");
+    gpl_cf_term t;
+    memset(&t, 0, sizeof(t));
+    t.kind = kind;
+    t.op = GPL_OP_SUM;
+    t.col0 = 0;
+    t.ncols = (int32_t)d;
+    t.magn = magn;
+    t.sigma0 = sigma0;
+    t.sigma = sigma;
+    if (kind != GPL_CF_NN) {
+        if (!lscale || (nl != 1 && nl != d)) return fail(GPL_ERR_ARG, "lscale must have length 1 or d");
+        t.n_lscale = (int32_t)nl;
+        for (int64_t k = 0; k < nl; ++k) t.lscale[k] = lscale[k];
+    }
+    return gpl_eval_cf(c, &t, 1, x1, n1, x2, n2, d, diag_only, out);
+}
+
+extern "C" int gpl_cf_sexp(gpl_ctx* c, const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t d,
+                           const double* lscale, int64_t n_lscale, double magn, int32_t diag_only, double* out) {
+    return cf_simple(c, GPL_CF_SEXP, x1, n1, x2, n2, d, lscale, n_lscale, magn, 0, 0, diag_only, out);
+}
+extern "C" int gpl_cf_matern32(gpl_ctx* c, const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t d,
+                               const double* lscale, int64_t n_lscale, double magn, int32_t diag_only, double* out) {
+    return cf_simple(c, GPL_CF_MATERN32, x1, n1, x2, n2, d, lscale, n_lscale, magn, 0, 0, diag_only, out);
+}
+extern "C" int gpl_cf_matern52(gpl_ctx* c, const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t d,
+                               const double* lscale, int64_t n_lscale, double magn, int32_t diag_only, double* out) {
+    return cf_simple(c, GPL_CF_MATERN52, x1, n1, x2, n2, d, lscale, n_lscale, magn, 0, 0, diag_only, out);
+}
+extern "C" int gpl_cf_nn(gpl_ctx* c, const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t d,
+                         double sigma0, double sigma, double magn, int32_t diag_only, double* out) {
+    return cf_simple(c, GPL_CF_NN, x1, n1, x2, n2, d, nullptr, 1, magn, sigma0, sigma, diag_only, out);
+}

@@ -1,0 +1,136 @@
+// Internal declarations shared by the gfx950 kernels and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gplite_amd.h"
+
+namespace gpl {
+
+constexpr int TB = 64;              // tile edge: every dense matrix is padded to a multiple of 64
+constexpr int MAX_FEAT = 64;        // device feature columns per model
+constexpr int GRAD_MAX = 96;        // gradient entries computed on the device
+
+// A covariance term after host preparation: features instead of raw columns.
+//   stationary / lin / nn: features = the term's input columns
+//   periodic: features = [sin(w x), cos(w x)] (2*ncols), plus 2*ncols derivative
+//             features d/dlog(period) stored at df0
+struct DevTerm {
+    int kind, op;
+    int c0, nc;        // raw input columns
+    int f0, nf;        // feature columns
+    int nl;            // number of length-scales (1 or nf)
+    int base_kind;     // periodic base kernel
+    int df0;           // periodic: derivative features
+    int g0, ng;        // gradient slots [g0, g0+ng)
+    int pad;
+    double m2;   // magn^2
+    double b0;   // nn: sigma0^2 sigma^2
+    double s2;   // nn: sigma^2
+    double w;    // periodic: 2 pi / period
+    double il2[2 * GPL_MAX_COLS];  // 1/lscale^2 per feature column
+};
+
+struct DevProg {
+    int nterms;
+    int nfeat;     // feature columns in use
+    int ngrad;     // gradient entries of the cf part
+    int pad;
+    DevTerm t[GPL_MAX_TERMS];
+};
+
+// ---- launchers (defined in the .hip files) -------------------------------
+// Covariance (cov.hip)
+void launch_features(hipStream_t s, const double* x, int64_t n, int64_t ldx, int ncols,
+                     const DevProg* prog_dev, const DevProg& prog, double* F, int64_t ldf);
+// out[i + j*ld] for the lower tiles of an N x N matrix built from F (N rows).
+// mode 0: K + jitter*I; mode 1: B = I + s_i (K + jitter I)_ij s_j with s scalar
+// (svec == null) or per-point; padded rows/cols (>= n) become identity.
+void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
+                          int64_t n, int64_t N, double jitter, int mode, double sscal,
+                          const double* svec, double* out, int64_t ld);
+void launch_build_sym_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
+                          int64_t n, int64_t N, double jitter, int mode, double sscal,
+                          const double* svec, float* out, int64_t ld);
+// out (Na x Nb col-major, ld) = K(Fa, Fb); zero outside [na) x [nb).
+void launch_build_cross(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* Fa,
+                        int64_t lda_f, int64_t na, int64_t Na, const double* Fb, int64_t ldb_f,
+                        int64_t nb, int64_t Nb, double* out, int64_t ld);
+// out[i] = k(Fa_i, Fb_i), i < n
+void launch_build_diag(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* Fa,
+                       int64_t lda_f, const double* Fb, int64_t ldb_f, int64_t n, double* out);
+// y = (K + jitter I) v  (matrix-free FP64, recomputes K from F). part: N*nchunk scratch.
+void launch_kmatvec(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
+                    int64_t n, int64_t N, double jitter, const double* v, double* y, double* part,
+                    int nchunk);
+int kmatvec_chunks(int64_t N);
+// gradient partials: part[tile * ng + p]; W is (K+V)^-1 given as c * Winv (lower tiles).
+void launch_grad_tiles(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
+                       int64_t n, int64_t N, const double* W, int64_t ldw, double wscale,
+                       const double* alpha, double* part);
+void launch_grad_tiles_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
+                           int64_t n, int64_t N, const float* W, int64_t ldw, double wscale,
+                           const double* alpha, double* part);
+
+// Dense FP64 (dense.hip)
+// C[M x N] = beta*C + alpha * op(A) op(B); all dims multiples of 64.
+// ta/tb: operand stored transposed.  tiles: 0 = full grid, 1 = lower tiles of a square C,
+// 2 = lower tiles and k starts at the row tile (C = X^T X with X lower triangular),
+// 3 = full grid and k stops after the row tile (C = X B with X lower triangular).
+void gemm_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+              int64_t ldc, int tiles);
+// Factor the 64x64 diagonal block at A + k*64*(ld+1) in place; write its inverse to
+// Dinv + k*64*64 and log(L_ii) to ldiag + k*64; info = first failing row + 1.
+void potrf_diag_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag,
+                    int* info);
+// triangular solve with the blocked factor (L lower, Dinv diagonal inverses), in place on b.
+void trsv_lower_f64(hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N,
+                    double* b, double* x, int trans);
+// y = L^T? x with a dense lower-triangular X (GEMV on lower tiles): trans=0: y = X b; 1: y = X^T b
+void trmv_lower_f64(hipStream_t s, const double* X, int64_t ld, int64_t N, const double* b,
+                    double* y, int trans, double* part);
+// y = K v for a symmetric K stored in its lower tiles.
+void symv_lower_f64(hipStream_t s, const double* K, int64_t ld, int64_t N, const double* v,
+                    double* y, double* part);
+// out[j] = sum_i A[i + j*ld] * v[i] for j < ncol (one wavefront per column)
+void gemv_t_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* v,
+                double* out);
+// out[j] = sum_i A[i + j*ld]^2 for j < ncol (deterministic column sums of squares)
+void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
+                  int lower);
+void set_identity_lower_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
+void fill_f64(hipStream_t s, double* p, int64_t n, double v);
+void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
+// B (lower tiles) = I + s_i K_ij s_j for i,j < n; identity on padding
+void form_B_f64(hipStream_t s, const double* K, double* B, int64_t ld, int64_t n, int64_t N,
+                const double* sv);
+
+// Deterministic reductions (lik.hip)
+void reduce_sum(hipStream_t s, const double* v, int64_t n, double* out);          // out[0] = sum v
+void reduce_dot(hipStream_t s, const double* a, const double* b, int64_t n, double* out);
+void reduce_maxabsdiff(hipStream_t s, const double* a, const double* b, int64_t n, double* out);
+void reduce_cols(hipStream_t s, const double* part, int64_t rows, int64_t ld, int64_t ncols,
+                 double* out);  // out[c] = sum_r part[r*ld + c]
+
+// Likelihood kernels (lik.hip)
+void lik_pseudodata_la(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f,
+                       const double* y, const double* trials, const double* offset, double* z,
+                       double* V, double* sv, int* bad);
+void lik_loglik(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f,
+                const double* y, const double* trials, const double* offset, double* ll);
+void ep_sites(hipStream_t s, int lik, int link, int64_t n, const double* fmean, const double* fvar,
+              const double* z_old, const double* P_old, const double* y, const double* trials,
+              const double* offset, int q, const double* gh_x, const double* gh_w, double damping,
+              double* z, double* V, double* sv, double* logZs, double* cav_m, double* cav_v,
+              int* nan_flag);
+void ep_post(hipStream_t s, int64_t n, const double* V, const double* z, const double* binv_diag,
+             const double* cav_m, const double* cav_v, const double* logZs, double* fvar,
+             double* term);
+void vec_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y,
+               double* out);
+void vec_mul(hipStream_t s, int64_t n, const double* a, const double* b, double* out);
+void vec_recip(hipStream_t s, int64_t n, const double* a, double* out);
+
+}  // namespace gpl

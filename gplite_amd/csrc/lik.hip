@@ -1,0 +1,277 @@
+// Elementwise likelihood work and deterministic reductions for gfx950.
+// Restates R/lik.R: pseudo-data for Laplace (:188-201) with the reference's
+// forward-difference curvature (:449-454), log densities (:297-366), first
+// derivatives (:376-443), responses (:460-498), and the EP site update with
+// Gauss-Hermite tilted moments (:207-270, R/quadrature.R, logsumexp
+// R/gp_loo.R:136-144).
+#include "gpl_internal.h"
+
+#include <cmath>
+
+namespace gpl {
+
+constexpr double kLog2Pi = 1.8378770664093454836;
+constexpr double kInvSqrt2Pi = 0.39894228040143267794;
+
+__device__ __forceinline__ double d_response(int lik, int link, double f) {
+    if (lik == GPL_LIK_POISSON) return exp(f);
+    if (link == GPL_LINK_PROBIT) return 0.5 * erfc(-f * 0.70710678118654752440);
+    return 1.0 / (1.0 + exp(-f));
+}
+
+// d loglik / d f at f (offset already added)
+__device__ __forceinline__ double d_loglik_d(int lik, int link, double f, double y, double tr) {
+    const double mu = d_response(lik, link, f);
+    if (lik == GPL_LIK_POISSON) return (y / mu - 1.0) * mu;
+    const double dmu = (link == GPL_LINK_PROBIT) ? kInvSqrt2Pi * exp(-0.5 * f * f) : mu * (1.0 - mu);
+    return dmu * (y / mu - (tr - y) / (1.0 - mu));
+}
+
+__device__ __forceinline__ double d_loglik(int lik, int link, double sigma, double f, double y, double tr) {
+    if (lik == GPL_LIK_GAUSSIAN) {
+        const double u = (y - f) / sigma;
+        return -0.5 * kLog2Pi - log(sigma) - 0.5 * u * u;
+    }
+    const double mu = d_response(lik, link, f);
+    if (lik == GPL_LIK_POISSON) return (y > 0.0 ? y * log(mu) : 0.0) - mu - lgamma(y + 1.0);
+    const double lc = lgamma(tr + 1.0) - lgamma(y + 1.0) - lgamma(tr - y + 1.0);
+    return lc + (y > 0.0 ? y * log(mu) : 0.0) + (tr - y > 0.0 ? (tr - y) * log1p(-mu) : 0.0);
+}
+
+__global__ void k_pseudo_la(int lik, int link, double sigma, int64_t n, const double* __restrict__ f,
+                            const double* __restrict__ y, const double* __restrict__ trials,
+                            const double* __restrict__ offset, double* __restrict__ z, double* __restrict__ V,
+                            double* __restrict__ sv, int* __restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (lik == GPL_LIK_GAUSSIAN) {  // R/lik.R:188-192: z = y (offset ignored), V = sigma^2
+        z[i] = y[i];
+        V[i] = sigma * sigma;
+        sv[i] = sqrt(1.0 / (sigma * sigma));
+        return;
+    }
+    const double off = offset ? offset[i] : 0.0;
+    const double tr = trials ? trials[i] : 1.0;
+    const double eps = 1e-6;
+    const double fi = f[i];
+    const double g = d_loglik_d(lik, link, fi + off, y[i], tr);
+    const double ge = d_loglik_d(lik, link, (fi + eps) + off, y[i], tr);
+    double g2 = (ge - g) / eps;
+    const double sg = (g2 > 0.0) ? 1.0 : ((g2 < 0.0) ? -1.0 : 0.0);
+    g2 = sg * fmax(fabs(g2), 1e-12);
+    const double Vi = -1.0 / g2;
+    z[i] = fi - g / g2;
+    V[i] = Vi;
+    sv[i] = sqrt(1.0 / Vi);
+    if (isnan(Vi) || isnan(z[i])) atomicOr(bad, 2);
+    else if (!(Vi > 0.0)) atomicOr(bad, 4);
+}
+
+void lik_pseudodata_la(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f, const double* y,
+                       const double* trials, const double* offset, double* z, double* V, double* sv, int* bad) {
+    hipLaunchKernelGGL(k_pseudo_la, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lik, link, sigma, n, f, y,
+                       trials, offset, z, V, sv, bad);
+}
+
+__global__ void k_loglik(int lik, int link, double sigma, int64_t n, const double* __restrict__ f,
+                         const double* __restrict__ y, const double* __restrict__ trials,
+                         const double* __restrict__ offset, double* __restrict__ ll) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double off = offset ? offset[i] : 0.0;
+    const double tr = trials ? trials[i] : 1.0;
+    ll[i] = d_loglik(lik, link, sigma, f[i] + off, y[i], tr);
+}
+
+void lik_loglik(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f, const double* y,
+                const double* trials, const double* offset, double* ll) {
+    hipLaunchKernelGGL(k_loglik, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lik, link, sigma, n, f, y,
+                       trials, offset, ll);
+}
+
+// EP site update (one thread per site, q <= 32 quadrature nodes)
+__global__ void k_ep_sites(int lik, int link, int64_t n, const double* __restrict__ fmean,
+                           const double* __restrict__ fvar, const double* __restrict__ z_old,
+                           const double* __restrict__ P_old, const double* __restrict__ y,
+                           const double* __restrict__ trials, const double* __restrict__ offset, int q,
+                           const double* __restrict__ gx, const double* __restrict__ gw, double damping,
+                           double* __restrict__ z, double* __restrict__ V, double* __restrict__ sv,
+                           double* __restrict__ logZs, double* __restrict__ cav_m, double* __restrict__ cav_v,
+                           int* __restrict__ nan_flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double post_prec = 1.0 / fvar[i];
+    const double Po = P_old[i], zo = z_old[i];
+    const double cprec = post_prec - Po;
+    const double cmean = 1.0 / cprec * (fmean[i] * post_prec - zo * Po);
+    const double scale = sqrt(1.0 / cprec);
+    const double off = offset ? offset[i] : 0.0;
+    const double tr = trials ? trials[i] : 1.0;
+    const double yi = y[i];
+    double ll[32], fg[32];
+    bool ok = true;
+    double m = -INFINITY;
+    for (int k = 0; k < q; ++k) {
+        fg[k] = cmean + scale * gx[k];
+        double v = d_loglik(lik, link, 0.0, fg[k] + off, yi, tr);
+        if (isnan(v)) { v = -INFINITY; ok = false; }
+        ll[k] = v;
+        m = fmax(m, v);
+    }
+    double sw = 0.0;
+    for (int k = 0; k < q; ++k) sw += gw[k] * exp(ll[k] - m);
+    const double logC = m + log(sw);
+    const double C = exp(logC);
+    double mt = 0.0;
+    for (int k = 0; k < q; ++k) mt += fg[k] * exp(ll[k]) * gw[k];
+    mt = (1.0 / C) * mt;
+    double vt = 0.0;
+    for (int k = 0; k < q; ++k) {
+        const double d = fg[k] - mt;
+        vt += d * d * exp(ll[k]) * gw[k];
+    }
+    vt = (1.0 / C) * vt;
+    const double cvar = 1.0 / cprec;
+    double Pn = 1.0 / vt - 1.0 / cvar;
+    double nun = mt / vt - cmean / cvar;
+    const double nuo = zo * Po;
+    nun = damping * nun + (1.0 - damping) * nuo;
+    Pn = damping * Pn + (1.0 - damping) * Po;
+    const double Vi = 1.0 / Pn;
+    const double zi = (1.0 / Pn) * nun;
+    z[i] = zi;
+    V[i] = Vi;
+    sv[i] = sqrt(1.0 / Vi);
+    logZs[i] = logC;
+    cav_m[i] = cmean;
+    cav_v[i] = cvar;
+    if (!ok) atomicOr(nan_flag, 1);
+    if (isnan(Vi) || isnan(zi)) atomicOr(nan_flag, 2);
+    else if (!(Vi > 0.0)) atomicOr(nan_flag, 4);
+}
+
+void ep_sites(hipStream_t s, int lik, int link, int64_t n, const double* fmean, const double* fvar,
+              const double* z_old, const double* P_old, const double* y, const double* trials, const double* offset,
+              int q, const double* gh_x, const double* gh_w, double damping, double* z, double* V, double* sv,
+              double* logZs, double* cav_m, double* cav_v, int* nan_flag) {
+    hipLaunchKernelGGL(k_ep_sites, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, lik, link, n, fmean, fvar,
+                       z_old, P_old, y, trials, offset, q, gh_x, gh_w, damping, z, V, sv, logZs, cav_m, cav_v,
+                       nan_flag);
+}
+
+// fvar_i = V_i (1 - [B^-1]_ii); per-site evidence term
+__global__ void k_ep_post(int64_t n, const double* __restrict__ V, const double* __restrict__ z,
+                          const double* __restrict__ binv_diag, const double* __restrict__ cav_m,
+                          const double* __restrict__ cav_v, const double* __restrict__ logZs,
+                          double* __restrict__ fvar, double* __restrict__ term) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double Vi = V[i];
+    fvar[i] = Vi * (1.0 - binv_diag[i]);
+    const double cv = cav_v[i] + Vi;
+    const double dz = cav_m[i] - z[i];
+    term[i] = logZs[i] + 0.5 * log(cv) + 0.5 * dz * dz / cv - 0.5 * log(Vi);
+}
+
+void ep_post(hipStream_t s, int64_t n, const double* V, const double* z, const double* binv_diag, const double* cav_m,
+             const double* cav_v, const double* logZs, double* fvar, double* term) {
+    hipLaunchKernelGGL(k_ep_post, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, V, z, binv_diag, cav_m,
+                       cav_v, logZs, fvar, term);
+}
+
+__global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                        double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + (y ? b * y[i] : 0.0);
+}
+
+void vec_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y, double* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, x, b, y, out);
+}
+
+__global__ void k_mul(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] * b[i];
+}
+
+void vec_mul(hipStream_t s, int64_t n, const double* a, const double* b, double* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mul, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, b, out);
+}
+
+__global__ void k_recip(int64_t n, const double* __restrict__ a, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 1.0 / a[i];
+}
+
+void vec_recip(hipStream_t s, int64_t n, const double* a, double* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_recip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, out);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: one workgroup, fixed per-thread order, fixed tree.
+// ---------------------------------------------------------------------------
+template <int OP>  // 0 sum, 1 dot, 2 max |a-b|
+__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ a, const double* __restrict__ b,
+                                                 int64_t n, double* __restrict__ out) {
+    __shared__ double red[1024];
+    const int t = threadIdx.x;
+    double s = (OP == 2) ? 0.0 : 0.0;
+    for (int64_t i = t; i < n; i += 1024) {
+        if (OP == 0) s += a[i];
+        else if (OP == 1) s += a[i] * b[i];
+        else {
+            const double d = fabs(a[i] - b[i]);
+            s = (d > s || isnan(d)) ? d : s;
+        }
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) {
+            if (OP == 2) {
+                const double u = red[t + o];
+                red[t] = (u > red[t] || isnan(u)) ? u : red[t];
+            } else {
+                red[t] += red[t + o];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[0] = red[0];
+}
+
+void reduce_sum(hipStream_t s, const double* v, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_reduce<0>, dim3(1), dim3(1024), 0, s, v, v, n, out);
+}
+void reduce_dot(hipStream_t s, const double* a, const double* b, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_reduce<1>, dim3(1), dim3(1024), 0, s, a, b, n, out);
+}
+void reduce_maxabsdiff(hipStream_t s, const double* a, const double* b, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_reduce<2>, dim3(1), dim3(1024), 0, s, a, b, n, out);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_cols(const double* __restrict__ part, int64_t rows, int64_t ld,
+                                                     double* __restrict__ out) {
+    __shared__ double red[256];
+    const int64_t c = blockIdx.x;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int64_t r = t; r < rows; r += 256) s += part[r * ld + c];
+    red[t] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) out[c] = red[0];
+}
+
+void reduce_cols(hipStream_t s, const double* part, int64_t rows, int64_t ld, int64_t ncols, double* out) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(256), 0, s, part, rows, ld, out);
+}
+
+}  // namespace gpl

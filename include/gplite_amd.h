@@ -1,0 +1,179 @@
+/*
+ * gplite_amd -- MI355X (gfx950) dense Gaussian-process core behind a C ABI.
+ *
+ * Every entry point takes plain host pointers and sizes.  Matrices are
+ * column-major FP64 exactly as R hands them to `.Call` (x[i + j*n]).
+ * Inputs are borrowed for the duration of the call; outputs are written
+ * into caller-owned buffers.  Device-resident state (factors, alpha, the
+ * inverse) lives in a gpl_model handle until gpl_model_free().
+ *
+ * All functions return a gpl_status; on failure gpl_last_error() gives a
+ * message and gpl_last_error_minor() the failing leading minor (1-based) for
+ * GPL_ERR_NOT_PD, mirroring R's "leading minor of order k is not positive".
+ */
+#ifndef GPLITE_AMD_H
+#define GPLITE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GPL_OK = 0,
+    GPL_ERR_ARG = 1,        /* invalid argument (R: stop()) */
+    GPL_ERR_NOT_PD = 2,     /* Cholesky found a non-positive pivot (R: chol() error) */
+    GPL_ERR_OOM = 3,        /* device allocation failed / admission refused */
+    GPL_ERR_NOCONV = 4,     /* Newton / EP did not converge (R: warning or -Inf) */
+    GPL_ERR_HIP = 5,        /* HIP runtime error */
+    GPL_ERR_NAN = 6,        /* NaN in pseudo-data / quadrature (R: log_evidence = -Inf) */
+    GPL_ERR_UNSUPPORTED = 7 /* a branch the reference handles on the host (non-log-concave sites) */
+} gpl_status;
+
+/* ---- covariance program ------------------------------------------------ */
+/* Kinds follow the reference constructors cf_const ... cf_periodic
+ * (R/cf.R:92-181).  A model's covariance is a list of terms: a term with
+ * op == GPL_OP_SUM starts a new summand (R/cf.R:402-411, eval_cf.list), a term
+ * with op == GPL_OP_PROD multiplies into the current summand (R/cf.R:491-497,
+ * eval_cf.cf_prod).  The host applies `vars` selection and `normalize`
+ * (R/misc.R:17-31) before the call; each term reads columns
+ * [col0, col0+ncols) of the input matrix. */
+enum {
+    GPL_CF_CONST = 0, GPL_CF_LIN = 1, GPL_CF_SEXP = 2, GPL_CF_MATERN32 = 3,
+    GPL_CF_MATERN52 = 4, GPL_CF_NN = 5, GPL_CF_PERIODIC = 6
+};
+enum { GPL_OP_SUM = 0, GPL_OP_PROD = 1 };
+
+#define GPL_MAX_TERMS 8
+#define GPL_MAX_COLS 16
+
+typedef struct gpl_cf_term {
+    int32_t kind;       /* GPL_CF_* */
+    int32_t op;         /* GPL_OP_SUM / GPL_OP_PROD (ignored for the first term) */
+    int32_t col0;       /* first input column read by this term */
+    int32_t ncols;      /* number of input columns read (<= GPL_MAX_COLS) */
+    int32_t base_kind;  /* GPL_CF_PERIODIC only: SEXP, MATERN32 or MATERN52 */
+    int32_t n_lscale;   /* 1 (isotropic) or ncols (ARD); periodic: 1 or 2*ncols */
+    double magn;
+    double sigma0;      /* GPL_CF_NN */
+    double sigma;       /* GPL_CF_NN */
+    double period;      /* GPL_CF_PERIODIC */
+    double lscale[2 * GPL_MAX_COLS];
+} gpl_cf_term;
+
+/* Gradient layout of one term (d energy / d log theta), in this order:
+ *   CONST, LIN: magn
+ *   SEXP, MATERN32, MATERN52: lscale[0..n_lscale-1], magn
+ *   NN: sigma0, sigma, magn
+ *   PERIODIC: period, lscale[0..n_lscale-1], magn
+ * followed by the likelihood's parameters (Gaussian: sigma). */
+int32_t gpl_grad_len(const gpl_cf_term* terms, int32_t nterms, int32_t lik_kind);
+
+/* ---- likelihood and options -------------------------------------------- */
+enum { GPL_LIK_GAUSSIAN = 0, GPL_LIK_BERNOULLI = 1, GPL_LIK_BINOMIAL = 2, GPL_LIK_POISSON = 3 };
+enum { GPL_LINK_IDENTITY = 0, GPL_LINK_LOGIT = 1, GPL_LINK_PROBIT = 2, GPL_LINK_LOG = 3 };
+
+typedef struct gpl_lik {
+    int32_t kind;   /* GPL_LIK_* (R/lik.R:46-90) */
+    int32_t link;   /* GPL_LINK_* */
+    double sigma;   /* Gaussian noise sd */
+} gpl_lik;
+
+enum { GPL_APPROX_LAPLACE = 0, GPL_APPROX_EP = 1 };
+enum { GPL_PREC_FP64 = 0, GPL_PREC_MIXED = 1 };
+
+typedef struct gpl_opts {
+    int32_t approx;       /* GPL_APPROX_LAPLACE (approx_laplace) / GPL_APPROX_EP (approx_ep) */
+    int32_t maxiter;      /* approx_laplace(maxiter=30) / approx_ep(maxiter=100) */
+    double tol;           /* approx_laplace(tol=1e-4); EP always uses 1e-4 (R/ep.R:286) */
+    double damping;       /* approx_ep(damping=0.9) */
+    double damping_min;   /* 0.1 (R/gp_fit.R:158) */
+    int32_t quad_order;   /* approx_ep(quad_order=11) */
+    int32_t precision;    /* GPL_PREC_FP64 (plain) or GPL_PREC_MIXED (FP32 MFMA + FP64 refinement) */
+    double jitter;        /* 1e-6 (R/gp.R:276-281) */
+    int32_t refine_iters; /* mixed precision: max iterative-refinement sweeps (default 8) */
+    int32_t flags;        /* reserved, 0 */
+} gpl_opts;
+
+void gpl_opts_default(gpl_opts* o);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct gpl_ctx gpl_ctx;
+typedef struct gpl_model gpl_model;
+
+int gpl_ctx_create(int32_t device, gpl_ctx** out);
+void gpl_ctx_destroy(gpl_ctx* ctx);
+const char* gpl_last_error(void);
+int64_t gpl_last_error_minor(void);
+int gpl_device_name(gpl_ctx* ctx, char* buf, int32_t len);
+
+/* Per-kernel-class device timing with HIP events on the library's stream. */
+int gpl_profile_enable(gpl_ctx* ctx, int32_t on);
+int gpl_profile_reset(gpl_ctx* ctx);
+/* Fills up to `cap` entries; returns the number of kernel classes. */
+int gpl_profile_read(gpl_ctx* ctx, int32_t cap, char* names /* cap*32 bytes */,
+                     double* total_ms, int64_t* launches, double* flops, double* bytes);
+
+/* ---- Level 0: the four .Call routines of src/cf.cpp --------------------- */
+/* Replace _gplite_cf_sexp_c / _matern32_c / _matern52_c (src/RcppExports.cpp:16-58,
+ * src/cf.cpp:9-119) and _gplite_cf_nn_c (src/RcppExports.cpp:61-73,
+ * src/cf.cpp:125-162).  x1 is n1 x d, x2 is n2 x d, column-major.  out is
+ * n1 x n2 column-major, or min(n1,n2) x 1 when diag_only.  ARD: n_lscale == d. */
+int gpl_cf_sexp(gpl_ctx* ctx, const double* x1, int64_t n1, const double* x2, int64_t n2,
+                int64_t d, const double* lscale, int64_t n_lscale, double magn,
+                int32_t diag_only, double* out);
+int gpl_cf_matern32(gpl_ctx* ctx, const double* x1, int64_t n1, const double* x2, int64_t n2,
+                    int64_t d, const double* lscale, int64_t n_lscale, double magn,
+                    int32_t diag_only, double* out);
+int gpl_cf_matern52(gpl_ctx* ctx, const double* x1, int64_t n1, const double* x2, int64_t n2,
+                    int64_t d, const double* lscale, int64_t n_lscale, double magn,
+                    int32_t diag_only, double* out);
+int gpl_cf_nn(gpl_ctx* ctx, const double* x1, int64_t n1, const double* x2, int64_t n2,
+              int64_t d, double sigma0, double sigma, double magn, int32_t diag_only,
+              double* out);
+
+/* eval_cf over a whole covariance program (R/cf.R:402-497). */
+int gpl_eval_cf(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms,
+                const double* x1, int64_t n1, const double* x2, int64_t n2, int64_t ncols,
+                int32_t diag_only, double* out);
+
+/* ---- Level 1: fused fit / energy / gradient / prediction --------------- */
+/* gp_fit for method_full (R/gp_fit.R:54-60,91-99,150-161): builds
+ * K = eval_cf(x,x) + jitter*I on the device, runs the Laplace Newton loop
+ * (R/laplace.R:274-297; one step for a Gaussian likelihood) or parallel EP
+ * (R/ep.R:286-330), and leaves the factor of B = I + S K S, alpha and the
+ * posterior vectors resident in *inout.  If *inout is an existing model of
+ * the same n it is reused (its buffers are recycled), otherwise a new one is
+ * allocated.  trials/offset may be NULL. */
+int gpl_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms,
+            const double* x, int64_t n, int64_t ncols, const double* y,
+            const double* trials, const double* offset, const gpl_lik* lik,
+            const gpl_opts* opts, gpl_model** inout, double* log_evidence);
+
+/* d(-log evidence)/d(log theta) for a Gaussian-likelihood fit (new: the
+ * reference optimises without gradients, R/gp_optim.R:94).  grad has
+ * gpl_grad_len() entries (full layout, fixed parameters included). */
+int gpl_energy_grad(gpl_model* model, double* grad);
+
+/* gp_pred posterior for method_full (R/gp_pred.R:202-232): latent mean
+ * K_t alpha and (want_var) variance diag(K_tt) - colsum((C^-1 K_t^T)^2).
+ * xt is m x ncols column-major. */
+int gpl_pred(gpl_model* model, const double* xt, int64_t m, int32_t want_var,
+             double* mean, double* var);
+
+enum {
+    GPL_VEC_FMEAN = 0, GPL_VEC_ALPHA = 1, GPL_VEC_Z = 2, GPL_VEC_V = 3, GPL_VEC_FVAR = 4,
+    GPL_VEC_CAVITY_MEAN = 5, GPL_VEC_CAVITY_VAR = 6
+};
+/* gp$fit fields (R/laplace.R:119, R/ep.R:140-145). */
+int gpl_model_vector(gpl_model* model, int32_t which, double* out /* n */);
+/* C_chol = diag(sqrt V) chol(B)^T, n x n column-major lower (for gp_save). */
+int gpl_model_chol(gpl_model* model, double* out);
+int gpl_model_info(gpl_model* model, int64_t* n, int32_t* iters, double* log_evidence);
+void gpl_model_free(gpl_model* model);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPLITE_AMD_H */

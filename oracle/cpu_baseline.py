@@ -1,0 +1,71 @@
+"""CPU baseline for the energy+gradient workload (test/benchmark infrastructure).
+
+The reference's CPU path is R + host LAPACK.  R is not available, so the
+baseline is this FP64 restatement of the same operation sequence on host
+LAPACK through scipy (OpenBLAS threads = the cores given):
+K build (src/cf.cpp:86-119 with ARD), B = I + K/sigma^2 and dpotrf
+(R/laplace.R:27-28), alpha by two triangular solves (:29), f = K alpha (:30),
+log evidence (:33-38), then the gradient the reference lacks: dpotri for
+B^-1 and -1/2 sum (a a^T - K_y^-1) o dK/dlog(theta) (Rasmussen & Williams eq. 5.9).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.linalg import lapack
+
+
+def _m52_parts(x1, x2, w, magn):
+    d = x1[:, None, :] - x2[None, :, :]
+    t2 = d * d * w
+    rho = t2.sum(-1)
+    r = np.sqrt(rho)
+    e = np.exp(-math.sqrt(5.0) * r)
+    m2 = magn * magn
+    g = m2 * (1 + math.sqrt(5.0) * r + 5.0 * rho / 3.0) * e
+    dg = -(5.0 / 6.0) * m2 * (1 + math.sqrt(5.0) * r) * e
+    return g, dg, t2
+
+
+def energy_grad_matern52_ard(x, y, lscale, magn, sigma, jitter=1e-6, chunk=256):
+    """Returns (log evidence, gradient of the energy wrt [log l_1..l_d, log magn, log sigma], seconds)."""
+    t0 = time.perf_counter()
+    n, d = x.shape
+    w = 1.0 / np.asarray(lscale, float) ** 2
+    K = np.empty((n, n))
+    for s in range(0, n, chunk):
+        K[s:s + chunk] = _m52_parts(x[s:s + chunk], x, w, magn)[0]
+    K[np.diag_indices(n)] += jitter
+    s2 = sigma * sigma
+    B = K / s2
+    B[np.diag_indices(n)] += 1.0
+    Lb, info = lapack.dpotrf(B, lower=1, overwrite_a=1, clean=1)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"dpotrf info={info}")
+    sv = 1.0 / sigma
+    u = sla.solve_triangular(Lb, sv * y, lower=True, check_finite=False)
+    alpha = sv * sla.solve_triangular(Lb, u, lower=True, trans="T", check_finite=False)
+    f = K @ alpha
+    ll = float(np.sum(-0.5 * math.log(2 * math.pi) - math.log(sigma) - 0.5 * ((y - f) / sigma) ** 2))
+    logz = ll - 0.5 * float(f @ alpha) - float(np.sum(np.log(np.diag(Lb))))
+    del K
+    Binv, info = lapack.dpotri(Lb, lower=1, overwrite_c=1)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"dpotri info={info}")
+    # Binv holds the lower triangle; symmetrise
+    Binv = np.tril(Binv) + np.tril(Binv, -1).T
+    g = np.zeros(d + 1)
+    for s in range(0, n, chunk):
+        gk, dg, t2 = _m52_parts(x[s:s + chunk], x, w, magn)
+        coef = alpha[s:s + chunk, None] * alpha[None, :] - Binv[s:s + chunk] / s2
+        cd = -2.0 * coef * dg
+        g[:d] += np.einsum("ij,ijk->k", cd, t2)
+        g[d] += float(np.sum(2.0 * coef * gk))
+    grad = np.empty(d + 2)
+    grad[:d + 1] = -0.5 * g
+    grad[d + 1] = -s2 * float(alpha @ alpha) + float(np.trace(Binv))
+    return logz, grad, time.perf_counter() - t0

@@ -1,0 +1,822 @@
+"""FP64 restatement of gplite's dense GP core (test infrastructure only).
+
+Covariance functions are plain dicts (the reference's cf objects are R lists):
+
+    {"kind": "const", "magn": m}
+    {"kind": "lin", "magn": m, "vars": [...]|None, "normalize": bool}
+    {"kind": "sexp"|"matern32"|"matern52", "lscale": float|array, "magn": m, ...}
+    {"kind": "nn", "sigma0": s0, "sigma": s, "magn": m, ...}   (normalize defaults True)
+    {"kind": "periodic", "period": p, "base": <stationary cf>, "vars": [...]|None}
+    {"kind": "prod", "cfs": [cf, cf, ...]}
+
+A model's covariance is a list of cfs (summed).  ``vars`` are 0-based column
+indices.  ``means``/``scales`` are filled by :func:`learn_scales`.
+Likelihoods: {"kind": "gaussian", "sigma"}, {"kind": "bernoulli"|"binomial",
+"link": "logit"|"probit"}, {"kind": "poisson", "link": "log"}.
+Priors: {"kind": "logunif"|"fixed"|"lognormal"(loc, scale)|"half_t"(df, scale)},
+given per parameter name in the cf/lik dict's "priors" entry.
+
+Parity status: every public function here is pinned by a check in
+tests/test_oracle.py (closed forms, scipy LAPACK, numpy hermegauss,
+brute-force loops, finite differences, fixed-point invariants).
+"""
+
+from __future__ import annotations
+
+import copy
+import math
+import warnings
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.special import gammaln, ndtr
+
+__all__ = [
+    "JITTER", "cf_sexp_c", "cf_matern32_c", "cf_matern52_c", "cf_nn_c",
+    "learn_scales", "prepare_inputmat", "eval_cf", "eval_cf_grads",
+    "get_param", "set_param", "param_names", "lpdf_prior", "lpdf_prior_grad",
+    "loglik", "loglik_d", "loglik_d2", "response", "pseudodata_la",
+    "hermite_polynomials", "gauss_hermite_points", "logsumexp_w",
+    "tilted_moments", "pseudodata_ep",
+    "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
+    "gp_energy_grad", "gp_pred", "gaussian_logz_closed_form",
+]
+
+JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
+
+
+# ---------------------------------------------------------------------------
+# Native covariance builders: src/cf.cpp
+# ---------------------------------------------------------------------------
+
+def _inv_l2(lscale, d):
+    ls = np.atleast_1d(np.asarray(lscale, dtype=np.float64))
+    if ls.size == 1:
+        return np.full(d, 1.0 / (ls[0] * ls[0]))
+    if ls.size != d:
+        raise ValueError("lscale must have length 1 or ncol(x)")
+    return 1.0 / (ls * ls)
+
+
+def _scaled_r2(x1, x2, lscale, chunk=1024):
+    """r^2 = sum_k (x1_ik - x2_jk)^2 / l_k^2, difference-then-square as
+    src/cf.cpp:37 (`sum(square(x1.row(i)-x2.row(j)))`), chunked over rows."""
+    x1 = np.asarray(x1, dtype=np.float64)
+    x2 = np.asarray(x2, dtype=np.float64)
+    d = x1.shape[1]
+    w = _inv_l2(lscale, d)
+    out = np.empty((x1.shape[0], x2.shape[0]))
+    for s in range(0, x1.shape[0], chunk):
+        diff = x1[s:s + chunk, None, :] - x2[None, :, :]
+        out[s:s + chunk] = np.einsum("ijk,ijk,k->ij", diff, diff, w)
+    return out
+
+
+def _diag_pairs(x1, x2):
+    n = min(x1.shape[0], x2.shape[0])
+    return x1[:n], x2[:n]
+
+
+def cf_sexp_c(x1, x2, lscale, magn, diag_only=False):
+    """magn^2 exp(-0.5 r^2 / l^2): src/cf.cpp:9-42 (formula :37-38).
+    ARD extension: lscale of length ncol(x)."""
+    x1 = np.atleast_2d(np.asarray(x1, float)); x2 = np.atleast_2d(np.asarray(x2, float))
+    if diag_only:  # src/cf.cpp:17-25 pairs row i with row i
+        a, b = _diag_pairs(x1, x2)
+        r2 = np.sum((a - b) ** 2 * _inv_l2(lscale, a.shape[1]), axis=1)
+        return magn * magn * np.exp(-0.5 * r2)
+    r2 = _scaled_r2(x1, x2, lscale)
+    return magn * magn * np.exp(-0.5 * r2)
+
+
+def cf_matern32_c(x1, x2, lscale, magn, diag_only=False):
+    """magn^2 (1 + sqrt3 r/l) exp(-sqrt3 r/l): src/cf.cpp:48-80 (:75-76)."""
+    x1 = np.atleast_2d(np.asarray(x1, float)); x2 = np.atleast_2d(np.asarray(x2, float))
+    if diag_only:
+        a, b = _diag_pairs(x1, x2)
+        r = np.sqrt(np.sum((a - b) ** 2 * _inv_l2(lscale, a.shape[1]), axis=1))
+    else:
+        r = np.sqrt(_scaled_r2(x1, x2, lscale))
+    s3 = math.sqrt(3.0)
+    return magn * magn * (1.0 + s3 * r) * np.exp(-s3 * r)
+
+
+def cf_matern52_c(x1, x2, lscale, magn, diag_only=False):
+    """magn^2 (1 + sqrt5 r/l + 5 r^2/(3 l^2)) exp(-sqrt5 r/l): src/cf.cpp:86-119."""
+    x1 = np.atleast_2d(np.asarray(x1, float)); x2 = np.atleast_2d(np.asarray(x2, float))
+    if diag_only:
+        a, b = _diag_pairs(x1, x2)
+        r = np.sqrt(np.sum((a - b) ** 2 * _inv_l2(lscale, a.shape[1]), axis=1))
+    else:
+        r = np.sqrt(_scaled_r2(x1, x2, lscale))
+    s5 = math.sqrt(5.0)
+    return magn * magn * (1.0 + s5 * r + 5.0 * r * r / 3.0) * np.exp(-s5 * r)
+
+
+def cf_nn_c(x1, x2, sigma0, sigma, magn, diag_only=False):
+    """Neural-network (arcsine) kernel: src/cf.cpp:125-162 (:148-158)."""
+    x1 = np.atleast_2d(np.asarray(x1, float)); x2 = np.atleast_2d(np.asarray(x2, float))
+    s2 = sigma * sigma
+    b0 = sigma0 * sigma0 * s2
+    if diag_only:
+        a_, b_ = _diag_pairs(x1, x2)
+        a = b0 + np.sum(a_ * b_, axis=1) * s2
+        b = b0 + np.sum(a_ * a_, axis=1) * s2
+        c = b0 + np.sum(b_ * b_, axis=1) * s2
+    else:
+        a = b0 + (x1 @ x2.T) * s2
+        b = (b0 + np.sum(x1 * x1, axis=1) * s2)[:, None]
+        c = (b0 + np.sum(x2 * x2, axis=1) * s2)[None, :]
+    u = 2.0 * a / np.sqrt((1.0 + 2.0 * b) * (1.0 + 2.0 * c))
+    return magn * magn * (2.0 / math.pi) * np.arcsin(u)
+
+
+# ---------------------------------------------------------------------------
+# R-side composition: R/cf.R:402-497, R/misc.R:17-31, learn_scales R/cf.R:824-855
+# ---------------------------------------------------------------------------
+
+_STATIONARY = ("sexp", "matern32", "matern52")
+
+
+def learn_scales(cfs, x):
+    """Column means / sd (n-1) per cf: R/cf.R:836-845; periodic skips (:852-855)."""
+    x = np.atleast_2d(np.asarray(x, float))
+    out = []
+    for cf in cfs:
+        cf = copy.deepcopy(cf)
+        if cf["kind"] == "prod":
+            cf["cfs"] = learn_scales(cf["cfs"], x)
+        elif cf["kind"] != "periodic":
+            xs = x if cf.get("vars") is None else x[:, cf["vars"]]
+            cf["means"] = xs.mean(axis=0)
+            cf["scales"] = xs.std(axis=0, ddof=1)
+        out.append(cf)
+    return out
+
+
+def _normalize_default(cf):
+    return cf.get("normalize", cf["kind"] == "nn")
+
+
+def prepare_inputmat(cf, x):
+    """Column selection then (x - means)/scales if normalize: R/misc.R:17-31."""
+    x = np.atleast_2d(np.asarray(x, float))
+    if cf.get("vars") is not None:
+        x = x[:, cf["vars"]]
+    if cf["kind"] not in ("periodic", "const") and _normalize_default(cf):
+        if cf.get("means") is not None and cf.get("scales") is not None:
+            x = (x - cf["means"]) / cf["scales"]
+    return x
+
+
+def _periodic_map(cf, x):
+    w = 2.0 * math.pi / cf["period"]
+    return np.hstack([np.sin(w * x), np.cos(w * x)])  # R/cf.R:483-484
+
+
+def _eval_one(cf, x1, x2, diag_only=False):
+    kind = cf["kind"]
+    if kind == "prod":  # R/cf.R:491-497
+        K = 1.0
+        for c in cf["cfs"]:
+            K = K * _eval_one(c, x1, x2, diag_only)
+        return K
+    a = prepare_inputmat(cf, x1)
+    b = prepare_inputmat(cf, x2)
+    if kind == "const":  # R/cf.R:413-424
+        n = min(a.shape[0], b.shape[0])
+        if diag_only:
+            return np.full(n, cf["magn"] ** 2)
+        return np.full((a.shape[0], b.shape[0]), cf["magn"] ** 2)
+    if kind == "lin":  # R/cf.R:426-435
+        K = cf["magn"] ** 2 * (a @ b.T)
+        return np.diag(K).copy() if diag_only else K
+    if kind == "sexp":
+        return cf_sexp_c(a, b, cf["lscale"], cf["magn"], diag_only)
+    if kind == "matern32":
+        return cf_matern32_c(a, b, cf["lscale"], cf["magn"], diag_only)
+    if kind == "matern52":
+        return cf_matern52_c(a, b, cf["lscale"], cf["magn"], diag_only)
+    if kind == "nn":
+        return cf_nn_c(a, b, cf["sigma0"], cf["sigma"], cf["magn"], diag_only)
+    if kind == "periodic":  # R/cf.R:478-489
+        base = dict(cf["base"]); base["vars"] = None; base["normalize"] = False
+        return _eval_one(base, _periodic_map(cf, a), _periodic_map(cf, b), diag_only)
+    raise ValueError(kind)
+
+
+def eval_cf(cfs, x1, x2, diag_only=False, cfind=None):
+    """Sum over the cf list: R/cf.R:402-411."""
+    idx = range(len(cfs)) if cfind is None else cfind
+    K = 0.0
+    for k in idx:
+        K = K + _eval_one(cfs[k], x1, x2, diag_only)
+    return K
+
+
+# ---------------------------------------------------------------------------
+# Parameters (log scale): R/cf.R:327-382, R/lik.R:145-166, R/gp.R:150-165
+# ARD extension: a vector lscale contributes one log-parameter per column.
+# ---------------------------------------------------------------------------
+
+def _is_fixed(obj, name):
+    pr = obj.get("priors", {}).get(name)
+    return pr is not None and pr.get("kind") == "fixed"
+
+
+def _own_names(cf):
+    kind = cf["kind"]
+    if kind in ("const", "lin"):
+        return ["magn"]
+    if kind in _STATIONARY:
+        return ["lscale", "magn"]
+    if kind == "nn":
+        return ["sigma0", "sigma", "magn"]
+    if kind == "periodic":
+        return ["period"]
+    if kind == "gaussian":
+        return ["sigma"]
+    return []
+
+
+def param_names(cfs, lik):
+    """Names in get_param.gp order (cf list order, then lik)."""
+    out = []
+
+    def visit(cf, label):
+        if cf["kind"] == "prod":
+            for c in cf["cfs"]:
+                visit(c, "cf_" + c["kind"])
+            return
+        if cf["kind"] == "periodic":
+            if not _is_fixed(cf, "period"):
+                out.append(f"{label}.period")
+            visit_base(cf["base"], label)
+            return
+        visit_base(cf, label)
+
+    def visit_base(cf, label):
+        for nm in _own_names(cf):
+            if _is_fixed(cf, nm):
+                continue
+            v = np.atleast_1d(cf[nm])
+            if v.size == 1:
+                out.append(f"{label}.{nm}")
+            else:
+                out.extend(f"{label}.{nm}[{i}]" for i in range(v.size))
+
+    for cf in cfs:
+        visit(cf, "cf_" + cf["kind"])
+    for nm in _own_names(lik):
+        if not _is_fixed(lik, nm):
+            out.append(f"lik_{lik['kind']}.{nm}")
+    return out
+
+
+def _param_slots(cfs, lik):
+    """List of (object, name, index-or-None) in get_param order."""
+    slots = []
+
+    def visit(cf):
+        if cf["kind"] == "prod":
+            for c in cf["cfs"]:
+                visit(c)
+            return
+        if cf["kind"] == "periodic":
+            if not _is_fixed(cf, "period"):
+                slots.append((cf, "period", None))
+            visit(cf["base"])
+            return
+        for nm in _own_names(cf):
+            if _is_fixed(cf, nm):
+                continue
+            v = np.atleast_1d(cf[nm])
+            if v.size == 1:
+                slots.append((cf, nm, None))
+            else:
+                slots.extend((cf, nm, i) for i in range(v.size))
+
+    for cf in cfs:
+        visit(cf)
+    for nm in _own_names(lik):
+        if not _is_fixed(lik, nm):
+            slots.append((lik, nm, None))
+    return slots
+
+
+def get_param(cfs, lik):
+    """log of every free parameter: R/cf.R:334 (`param <- log(param)`)."""
+    vals = []
+    for obj, nm, i in _param_slots(cfs, lik):
+        v = obj[nm] if i is None else np.atleast_1d(obj[nm])[i]
+        vals.append(math.log(float(np.atleast_1d(v)[0] if i is None else v)))
+    return np.array(vals)
+
+
+def set_param(cfs, lik, p):
+    """exp() back into the objects: R/cf.R:369-382, R/lik.R:160-166."""
+    cfs = copy.deepcopy(cfs); lik = copy.deepcopy(lik)
+    for (obj, nm, i), v in zip(_param_slots(cfs, lik), p):
+        if i is None:
+            obj[nm] = math.exp(v)
+        else:
+            arr = np.array(np.atleast_1d(obj[nm]), dtype=float)
+            arr[i] = math.exp(v)
+            obj[nm] = arr
+    return cfs, lik
+
+
+def _lpdf_one(prior, p):
+    kind = (prior or {"kind": "logunif"})["kind"]
+    if kind in ("logunif", "fixed"):  # R/priors.R:96-102
+        return 0.0
+    if kind == "lognormal":  # R/priors.R:110-112 (dnorm on the log-parameter)
+        loc, sc = prior.get("loc", 0.0), prior.get("scale", 1.0)
+        return -0.5 * math.log(2 * math.pi) - math.log(sc) - 0.5 * ((p - loc) / sc) ** 2
+    if kind == "half_t":  # R/priors.R:104-108
+        df, sc = prior.get("df", 1.0), prior.get("scale", 1.0)
+        t = math.exp(p) / sc
+        ldt = (math.lgamma((df + 1) / 2) - math.lgamma(df / 2) - 0.5 * math.log(df * math.pi)
+               - (df + 1) / 2 * math.log1p(t * t / df))
+        return math.log(2) - math.log(sc) + ldt + p
+    raise ValueError(kind)
+
+
+def _lpdf_grad_one(prior, p):
+    kind = (prior or {"kind": "logunif"})["kind"]
+    if kind in ("logunif", "fixed"):
+        return 0.0
+    if kind == "lognormal":
+        loc, sc = prior.get("loc", 0.0), prior.get("scale", 1.0)
+        return -(p - loc) / (sc * sc)
+    if kind == "half_t":
+        df, sc = prior.get("df", 1.0), prior.get("scale", 1.0)
+        t = math.exp(p) / sc
+        return 1.0 - (df + 1) * t * t / (df + t * t)
+    raise ValueError(kind)
+
+
+def _default_prior(obj, nm):
+    if obj["kind"] == "nn" and nm in ("sigma0", "sigma"):
+        return {"kind": "half_t"}  # R/cf.R:156-158 defaults
+    return {"kind": "logunif"}
+
+
+def lpdf_prior(cfs, lik):
+    """sum of hyperprior log densities on log-params: R/gp.R:167-169, R/cf.R:503-528."""
+    p = get_param(cfs, lik)
+    lp = 0.0
+    for (obj, nm, _i), v in zip(_param_slots(cfs, lik), p):
+        lp += _lpdf_one(obj.get("priors", {}).get(nm, _default_prior(obj, nm)), v)
+    return lp
+
+
+def lpdf_prior_grad(cfs, lik):
+    p = get_param(cfs, lik)
+    return np.array([_lpdf_grad_one(obj.get("priors", {}).get(nm, _default_prior(obj, nm)), v)
+                     for (obj, nm, _i), v in zip(_param_slots(cfs, lik), p)])
+
+
+# ---------------------------------------------------------------------------
+# dK/dlog(theta) (new work: the reference has no gradient)
+# ---------------------------------------------------------------------------
+
+def _stationary_dg(kind, rho, magn):
+    """k = g(rho), rho = sum_k D_k^2/l_k^2; returns (g, dg/drho)."""
+    m2 = magn * magn
+    if kind == "sexp":
+        g = m2 * np.exp(-0.5 * rho)
+        return g, -0.5 * g
+    r = np.sqrt(rho)
+    if kind == "matern32":
+        e = np.exp(-math.sqrt(3.0) * r)
+        return m2 * (1 + math.sqrt(3.0) * r) * e, -1.5 * m2 * e
+    e = np.exp(-math.sqrt(5.0) * r)
+    return (m2 * (1 + math.sqrt(5.0) * r + 5.0 * rho / 3.0) * e,
+            -(5.0 / 6.0) * m2 * (1 + math.sqrt(5.0) * r) * e)
+
+
+def _stationary_grads(cf, a, b, dfeat_a=None, dfeat_b=None):
+    """K and the list of dK/dlog(param) for a stationary kernel on features a,b.
+    If dfeat_* given (periodic), also returns dK/dlog(period) first."""
+    d = a.shape[1]
+    w = _inv_l2(cf["lscale"], d)
+    diff = a[:, None, :] - b[None, :, :]
+    rho = np.einsum("ijk,ijk,k->ij", diff, diff, w)
+    g, dg = _stationary_dg(cf["kind"], rho, cf["magn"])
+    grads = []
+    if dfeat_a is not None:
+        ddiff = dfeat_a[:, None, :] - dfeat_b[None, :, :]
+        grads.append(dg * 2.0 * np.einsum("ijk,ijk,k->ij", diff, ddiff, w))
+    if not _is_fixed(cf, "lscale"):
+        ls = np.atleast_1d(cf["lscale"])
+        if ls.size == 1:
+            grads.append(dg * (-2.0 * rho))
+        else:
+            for k in range(d):
+                grads.append(dg * (-2.0 * diff[:, :, k] ** 2 * w[k]))
+    if not _is_fixed(cf, "magn"):
+        grads.append(2.0 * g)
+    return g, grads
+
+
+def _one_grads(cf, x1, x2):
+    kind = cf["kind"]
+    if kind == "prod":
+        parts = [_one_grads(c, x1, x2) for c in cf["cfs"]]
+        Ks = [p[0] for p in parts]
+        K = np.prod(Ks, axis=0)
+        grads = []
+        for i, (_, gl) in enumerate(parts):
+            others = np.prod([Ks[j] for j in range(len(Ks)) if j != i], axis=0) if len(Ks) > 1 else 1.0
+            grads.extend(g * others for g in gl)
+        return K, grads
+    a = prepare_inputmat(cf, x1)
+    b = prepare_inputmat(cf, x2)
+    if kind == "const":
+        K = np.full((a.shape[0], b.shape[0]), cf["magn"] ** 2)
+        return K, ([] if _is_fixed(cf, "magn") else [2.0 * K])
+    if kind == "lin":
+        K = cf["magn"] ** 2 * (a @ b.T)
+        return K, ([] if _is_fixed(cf, "magn") else [2.0 * K])
+    if kind in _STATIONARY:
+        return _stationary_grads(cf, a, b)
+    if kind == "nn":
+        s2 = cf["sigma"] ** 2
+        b0 = cf["sigma0"] ** 2 * s2
+        xx = a @ b.T
+        aa = np.sum(a * a, 1)[:, None]
+        bb = np.sum(b * b, 1)[None, :]
+        A = b0 + xx * s2; Bv = b0 + aa * s2; Cv = b0 + bb * s2
+        D = (1 + 2 * Bv) * (1 + 2 * Cv)
+        u = 2 * A / np.sqrt(D)
+        m2 = cf["magn"] ** 2
+        K = m2 * (2 / math.pi) * np.arcsin(u)
+        dk = m2 * (2 / math.pi) / np.sqrt(1 - u * u)
+        grads = []
+
+        def du(dA, dB, dC):
+            return 2 * dA / np.sqrt(D) - u * (dB / (1 + 2 * Bv) + dC / (1 + 2 * Cv))
+        if not _is_fixed(cf, "sigma0"):
+            grads.append(dk * du(2 * b0, 2 * b0, 2 * b0))
+        if not _is_fixed(cf, "sigma"):
+            grads.append(dk * du(2 * A, 2 * Bv, 2 * Cv))
+        if not _is_fixed(cf, "magn"):
+            grads.append(2 * K)
+        return K, grads
+    if kind == "periodic":
+        wp = 2 * math.pi / cf["period"]
+        fa = _periodic_map(cf, a); fb = _periodic_map(cf, b)
+        # d phi / d log(period) = -(w x) [cos(wx), -sin(wx)]
+        da = np.hstack([-(wp * a) * np.cos(wp * a), (wp * a) * np.sin(wp * a)])
+        db = np.hstack([-(wp * b) * np.cos(wp * b), (wp * b) * np.sin(wp * b)])
+        base = dict(cf["base"])
+        if base["kind"] not in _STATIONARY:
+            raise NotImplementedError("periodic gradient needs a stationary base kernel")
+        K, grads = _stationary_grads(base, fa, fb, da, db)
+        if _is_fixed(cf, "period"):
+            grads = grads[1:]
+        return K, grads
+    raise ValueError(kind)
+
+
+def eval_cf_grads(cfs, x1, x2):
+    """(K, [dK/dlog theta_j]) over the cf list, in get_param order."""
+    K = 0.0
+    grads = []
+    for cf in cfs:
+        Ki, gi = _one_grads(cf, x1, x2)
+        K = K + Ki
+        grads.extend(gi)
+    return K, grads
+
+
+# ---------------------------------------------------------------------------
+# Likelihoods: R/lik.R
+# ---------------------------------------------------------------------------
+
+def response(lik, f):
+    """R/lik.R:460-498."""
+    if lik["kind"] == "gaussian":
+        return f
+    if lik["kind"] in ("bernoulli", "binomial"):
+        if lik.get("link", "logit") == "probit":
+            return ndtr(f)
+        return 1.0 / (1.0 + np.exp(-f))
+    if lik["kind"] == "poisson":
+        return np.exp(f)
+    raise ValueError(lik["kind"])
+
+
+def _dnorm_log(x):
+    return -0.5 * math.log(2 * math.pi) - 0.5 * x * x
+
+
+def loglik(lik, f, y, trials=None, offset=None, total=True):
+    """Observation log density: R/lik.R:297-366 (dnorm/dbinom/dpois)."""
+    f = np.asarray(f, float)
+    if offset is not None:
+        f = f + offset
+    y = np.asarray(y, float)
+    kind = lik["kind"]
+    if kind == "gaussian":
+        s = lik["sigma"]
+        ll = _dnorm_log((y - f) / s) - math.log(s)
+    elif kind in ("bernoulli", "binomial"):
+        t = np.ones_like(y) if kind == "bernoulli" or trials is None else np.asarray(trials, float)
+        mu = response(lik, f)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            lc = gammaln(t + 1) - gammaln(y + 1) - gammaln(t - y + 1)
+            ll = lc + np.where(y > 0, y * np.log(mu), 0.0) + np.where(t - y > 0, (t - y) * np.log1p(-mu), 0.0)
+    elif kind == "poisson":
+        mu = response(lik, f)
+        ll = np.where(y > 0, y * np.log(mu), 0.0) - mu - gammaln(y + 1)
+    else:
+        raise ValueError(kind)
+    return float(np.sum(ll)) if total else ll
+
+
+def loglik_d(lik, f, y, trials=None, offset=None):
+    """d loglik / d f: R/lik.R:376-397 (binomial), :427-443 (poisson)."""
+    f = np.asarray(f, float)
+    if offset is not None:
+        f = f + offset
+    y = np.asarray(y, float)
+    kind = lik["kind"]
+    if kind in ("bernoulli", "binomial"):
+        t = np.ones_like(y) if kind == "bernoulli" or trials is None else np.asarray(trials, float)
+        mu = response(lik, f)
+        if lik.get("link", "logit") == "probit":
+            dmu = np.exp(_dnorm_log(f))
+        else:
+            dmu = mu * (1 - mu)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return dmu * (y / mu - (t - y) / (1 - mu))
+    if kind == "poisson":
+        mu = response(lik, f)
+        return (y / mu - 1) * mu
+    raise ValueError(kind)
+
+
+def loglik_d2(lik, f, y, eps=1e-6, **kw):
+    """Forward finite difference of the gradient: R/lik.R:449-454."""
+    g = loglik_d(lik, f, y, **kw)
+    ge = loglik_d(lik, np.asarray(f, float) + eps, y, **kw)
+    return g, (ge - g) / eps
+
+
+def pseudodata_la(lik, f, y, min_abs_curvature=1e-12, **kw):
+    """Laplace pseudo-data: R/lik.R:188-201."""
+    if lik["kind"] == "gaussian":
+        n = len(y)
+        return np.asarray(y, float).copy(), lik["sigma"] ** 2 * np.ones(n)
+    g, g2 = loglik_d2(lik, f, y, **kw)
+    g2 = np.sign(g2) * np.maximum(np.abs(g2), min_abs_curvature)
+    return f - g / g2, -1.0 / g2
+
+
+# ---------------------------------------------------------------------------
+# Quadrature: R/quadrature.R, logsumexp R/gp_loo.R:136-144
+# ---------------------------------------------------------------------------
+
+def hermite_polynomials(n):
+    """Coefficients of probabilists' Hermite polynomials: R/quadrature.R:1-16."""
+    c = np.zeros((n + 1, n + 1))
+    c[0, 0] = 1
+    if n == 0:
+        return c
+    c[1, 1] = 1
+    for i in range(2, n + 1):
+        c[i, :] = np.concatenate([[0.0], c[i - 1, :n]]) - (i - 1) * c[i - 2, :]
+    return c
+
+
+def gauss_hermite_points(p):
+    """Nodes = roots of He_p, weights p!/(p^2 He_{p-1}(x)^2): R/quadrature.R:28-34."""
+    c = hermite_polynomials(p)
+    x = np.sort(np.real(np.roots(c[p, ::-1])))
+    w = math.factorial(p) / (p * p * np.polyval(c[p - 1, ::-1], x) ** 2)
+    return x, w
+
+
+def logsumexp_w(x, w):
+    m = np.max(x, axis=-1, keepdims=True)
+    return (m + np.log(np.sum(w * np.exp(x - m), axis=-1, keepdims=True)))[..., 0]
+
+
+def tilted_moments(lik, post_mean, post_prec, z_old, P_old, y, quad_order=11, **kw):
+    """Cavity + tilted moments by GH quadrature: R/lik.R:237-270."""
+    cav_prec = post_prec - P_old
+    cav_mean = 1.0 / cav_prec * (post_mean * post_prec - z_old * P_old)
+    gx, gw = gauss_hermite_points(quad_order)
+    fgrid = cav_mean[:, None] + np.sqrt(1.0 / cav_prec)[:, None] * gx[None, :]
+    n, q = fgrid.shape
+    kw_rep = {k: (None if v is None else np.repeat(np.asarray(v, float), q)) for k, v in kw.items()}
+    ll = loglik(lik, fgrid.ravel(), np.repeat(y, q), total=False, **kw_rep).reshape(n, q)
+    lik_v = np.exp(ll)
+    quad_ok = True
+    if np.any(np.isnan(lik_v)):
+        lik_v[np.isnan(lik_v)] = 0
+        ll[np.isnan(ll)] = -np.inf
+        quad_ok = False
+    logC = logsumexp_w(ll, gw[None, :])
+    C = np.exp(logC)
+    mean_t = (1.0 / C) * ((fgrid * lik_v) @ gw)
+    var_t = (1.0 / C) * (((fgrid - mean_t[:, None]) ** 2 * lik_v) @ gw)
+    return mean_t, var_t, logC, cav_mean, 1.0 / cav_prec, quad_ok
+
+
+def pseudodata_ep(lik, post_mean, post_prec, z_old, P_old, y, quad_order=11, damping=0.5, **kw):
+    """Site update with damping: R/lik.R:207-232."""
+    mt, vt, logC, cm, cv, ok = tilted_moments(lik, post_mean, post_prec, z_old, P_old, y, quad_order, **kw)
+    P_new = 1 / vt - 1 / cv
+    nu_new = mt / vt - cm / cv
+    nu_old = z_old * P_old
+    nu_new = damping * nu_new + (1 - damping) * nu_old
+    P_new = damping * P_new + (1 - damping) * P_old
+    return (1 / P_new) * nu_new, 1 / P_new, logC, cm, cv, ok
+
+
+# ---------------------------------------------------------------------------
+# Engines: R/laplace.R, R/ep.R
+# ---------------------------------------------------------------------------
+
+def _chol_B(K, V):
+    """B = I + S K S, C = diag(sqrt V) chol(B)^T: R/laplace.R:27-28."""
+    s = np.sqrt(1.0 / V)
+    B = np.eye(len(V)) + s[:, None] * K * s[None, :]
+    LB = sla.cholesky(B, lower=True)
+    return np.sqrt(V)[:, None] * LB
+
+
+def laplace_iter(lik, K, y, fhat_old, **kw):
+    """One Newton step, log-concave branch: R/laplace.R:13-39,119."""
+    z, V = pseudodata_la(lik, fhat_old, y, **kw)
+    if not np.all(V > 0):
+        raise NotImplementedError("non-log-concave branch (R/laplace.R:40-117) is a host fallback")
+    C = _chol_B(K, V)
+    alpha = sla.solve_triangular(C, sla.solve_triangular(C, z, lower=True), lower=True, trans="T")
+    fnew = K @ alpha
+    C_logdet = 2 * np.sum(np.log(np.diag(C)))
+    V_logdet = np.sum(np.log(V))
+    ll = loglik(lik, fnew, y, **kw)
+    logZ = ll - 0.5 * float(fnew @ alpha) - 0.5 * (C_logdet - V_logdet)
+    return dict(fmean=fnew, z=z, V=V, C_chol=C, alpha=alpha, log_evidence=logZ)
+
+
+def laplace(lik, K, y, maxiter=30, tol=1e-4, **kw):
+    """Newton loop: R/laplace.R:274-297 (Gaussian forced to one step)."""
+    n = len(y)
+    fhat = np.zeros(n)
+    if lik["kind"] == "gaussian":
+        maxiter = 1
+    it = 0
+    diff = np.inf
+    fit = None
+    for it in range(1, maxiter + 1):
+        fit = laplace_iter(lik, K, y, fhat, **kw)
+        diff = float(np.max(np.abs(fhat - fit["fmean"])))
+        fhat = fit["fmean"]
+        if diff < tol:
+            break
+    if maxiter > 1 and it == maxiter:
+        warnings.warn(f"Maximum number of iterations in Laplace reached, max delta f = {diff}")
+    fit["iters"] = it
+    return fit
+
+
+def ep_iter(lik, K, y, fmean_old, fvar_old, z_old, P_old, quad_order=11, damping=0.9, **kw):
+    """Parallel EP sweep, log-concave branch: R/ep.R:16-49,140-146."""
+    z, V, logZ_site, cm, cv, ok = pseudodata_ep(lik, fmean_old, 1 / fvar_old, z_old, P_old, y,
+                                                quad_order, damping, **kw)
+    if np.any(np.isnan(V)) or np.any(np.isnan(z)):
+        return dict(fmean=fmean_old, log_evidence=-np.inf, nan=True)
+    if not np.all(V > 0):
+        raise NotImplementedError("non-log-concave EP branch (R/ep.R:50-138) is a host fallback")
+    C = _chol_B(K, V)
+    alpha = sla.solve_triangular(C, sla.solve_triangular(C, z, lower=True), lower=True, trans="T")
+    fmean = K @ alpha
+    W = sla.solve_triangular(C, sla.solve_triangular(C, K, lower=True), lower=True, trans="T")
+    fvar = V * np.diag(W)
+    C_logdet = 2 * np.sum(np.log(np.diag(C)))
+    logZ = (-0.5 * C_logdet - 0.5 * float(z @ alpha) + np.sum(logZ_site)
+            + 0.5 * np.sum(np.log(cv + V)) + 0.5 * np.sum((cm - z) ** 2 / (cv + V)))
+    return dict(fmean=fmean, fvar=fvar, cavity_mean=cm, cavity_var=cv, z=z, P=1 / V,
+                C_chol=C, alpha=alpha, log_evidence=float(logZ), quad_ok=ok)
+
+
+def ep(lik, K, y, maxiter=100, tol=1e-4, damping=0.9, damping_min=0.1, quad_order=11, **kw):
+    """Damped parallel-EP loop: R/ep.R:286-330."""
+    n = len(y)
+    P_old = np.zeros(n); z_old = np.zeros(n)
+    fmean_old = np.zeros(n); fvar_old = np.diag(K).copy()
+    diff_old = np.inf
+    damp = damping
+    it = 0
+    fit = None
+    diff_new = np.inf
+    for it in range(1, maxiter + 1):
+        fit = ep_iter(lik, K, y, fmean_old, fvar_old, z_old, P_old, quad_order, damp, **kw)
+        diff_new = float(np.max(np.abs(fmean_old - fit["fmean"])))
+        if diff_new > diff_old:
+            damp = damp * damping
+            continue
+        damp = max(damp, damping_min)
+        fmean_old = fit["fmean"]; fvar_old = fit["fvar"]; z_old = fit["z"]; P_old = fit["P"]
+        diff_old = diff_new
+        if diff_new < tol:
+            break
+    if maxiter > 1 and it == maxiter:
+        fit["log_evidence"] = -np.inf
+        warnings.warn(f"Maximum number of iterations in EP reached, max delta f = {diff_new}")
+    if fit.get("quad_ok") is False:
+        fit["log_evidence"] = -np.inf
+    fit["iters"] = it
+    return fit
+
+
+# ---------------------------------------------------------------------------
+# User-level: R/gp_fit.R, R/gp.R, R/gp_pred.R
+# ---------------------------------------------------------------------------
+
+def gp_fit(cfs, lik, x, y, approx="laplace", maxiter=None, tol=1e-4, jitter=None,
+           damping=0.9, quad_order=11, normalize_learn=True, **kw):
+    """fit_laplace/fit_ep.method_full: R/gp_fit.R:54-60,91-99,150-161."""
+    x = np.atleast_2d(np.asarray(x, float))
+    if x.shape[0] == 1 and len(y) != 1:
+        x = x.T
+    if normalize_learn:
+        cfs = learn_scales(cfs, x)
+    jitter = JITTER if jitter is None else jitter
+    n = len(y)
+    K = eval_cf(cfs, x, x) + jitter * np.eye(n)
+    if approx == "laplace":
+        fit = laplace(lik, K, y, maxiter=30 if maxiter is None else maxiter, tol=tol, **kw)
+    elif approx == "ep":
+        if lik["kind"] == "gaussian":
+            raise ValueError("approx_ep disabled for lik_gaussian")  # R/gp.R:54-56
+        fit = ep(lik, K, y, maxiter=100 if maxiter is None else maxiter, tol=1e-4,
+                 damping=damping, damping_min=0.1, quad_order=quad_order, **kw)
+    else:
+        raise ValueError(approx)
+    fit["cfs"] = cfs; fit["lik"] = lik; fit["x"] = x; fit["K"] = K; fit["jitter"] = jitter
+    return fit
+
+
+def gp_energy(fit, include_prior=True):
+    """-log evidence - log prior: R/gp.R:138-147."""
+    e = -fit["log_evidence"]
+    if include_prior:
+        e -= lpdf_prior(fit["cfs"], fit["lik"])
+    return e
+
+
+def gp_energy_grad(fit, include_prior=True):
+    """d energy / d log(theta) for the Gaussian likelihood (new work, the
+    reference has none): -1/2 tr((a a^T - Ky^-1) dKy/dtheta), Rasmussen &
+    Williams (2006) eq. 5.9, with Ky = K_cf + (jitter + sigma^2) I."""
+    lik = fit["lik"]
+    if lik["kind"] != "gaussian":
+        raise NotImplementedError("gradient is defined for the Gaussian likelihood")
+    x = fit["x"]; n = x.shape[0]
+    _, dK = eval_cf_grads(fit["cfs"], x, x)
+    Ky = fit["K"] + lik["sigma"] ** 2 * np.eye(n)
+    L = sla.cholesky(Ky, lower=True)
+    Kinv = sla.cho_solve((L, True), np.eye(n))
+    a = fit["alpha"]
+    g = [-0.5 * (float(a @ (D @ a)) - float(np.sum(Kinv * D))) for D in dK]
+    if not _is_fixed(lik, "sigma"):
+        s2 = 2 * lik["sigma"] ** 2
+        g.append(-0.5 * s2 * (float(a @ a) - float(np.trace(Kinv))))
+    g = np.array(g)
+    if include_prior:
+        g = g - lpdf_prior_grad(fit["cfs"], lik)
+    return g
+
+
+def gp_pred(fit, xt, var=False, jitter=None, prior=False):
+    """Latent predictive mean / variance: R/gp_pred.R:142-156 (prior),
+    :202-232 (posterior; forms K_tt and aux^T aux as the reference does)."""
+    cfs = fit["cfs"]
+    xt = np.atleast_2d(np.asarray(xt, float))
+    if prior:
+        mean = np.zeros(xt.shape[0])
+        if var:
+            return mean, eval_cf(cfs, xt, xt, diag_only=True)
+        return mean, None
+    Kt = eval_cf(cfs, xt, fit["x"])
+    mean = Kt @ fit["alpha"]
+    if not var:
+        return mean, None
+    aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
+    v = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux * aux, axis=0)
+    return mean, v
+
+
+def gaussian_logz_closed_form(K_cf, y, sigma, jitter=JITTER):
+    """log N(y | 0, K + (sigma^2 + jitter) I) by a direct Cholesky of K_y
+    (the closed form tests/testthat/test_basic.R:85-89 checks against)."""
+    n = len(y)
+    L = sla.cholesky(K_cf + (sigma ** 2 + jitter) * np.eye(n), lower=True)
+    a = sla.solve_triangular(L, y, lower=True)
+    return -0.5 * float(a @ a) - float(np.sum(np.log(np.diag(L)))) - 0.5 * n * math.log(2 * math.pi)

@@ -1,0 +1,336 @@
+"""Pins the FP64 oracle against mathematics that is independent of it:
+library routines (scipy, sklearn, numpy), closed forms, Monte Carlo,
+numerical integration and finite differences."""
+
+import math
+import warnings
+
+import numpy as np
+import pytest
+import scipy.integrate as si
+import scipy.linalg as sla
+import scipy.stats as st
+from sklearn.gaussian_process import kernels as skk
+
+import oracle as O
+
+
+def _rng(s=0):
+    return np.random.default_rng(s)
+
+
+# ---------------------------------------------------------------- covariances
+
+@pytest.mark.parametrize("d", [1, 3, 8])
+def test_stationary_vs_sklearn_ard(d):
+    r = _rng(d)
+    x1 = r.uniform(-1, 1, (23, d)); x2 = r.uniform(-1, 1, (17, d))
+    ls = r.uniform(0.2, 1.0, d)
+    m = 1.7
+    np.testing.assert_allclose(O.cf_sexp_c(x1, x2, ls, m),
+                               m * m * skk.RBF(ls)(x1, x2), rtol=1e-13)
+    np.testing.assert_allclose(O.cf_matern32_c(x1, x2, ls, m),
+                               m * m * skk.Matern(ls, nu=1.5)(x1, x2), rtol=1e-12)
+    np.testing.assert_allclose(O.cf_matern52_c(x1, x2, ls, m),
+                               m * m * skk.Matern(ls, nu=2.5)(x1, x2), rtol=1e-12)
+    # isotropic scalar lscale == ARD with equal entries
+    np.testing.assert_allclose(O.cf_matern52_c(x1, x2, 0.4, m),
+                               O.cf_matern52_c(x1, x2, np.full(d, 0.4), m), rtol=1e-14)
+
+
+def test_stationary_special_values():
+    x = np.array([[0.0]]); y = np.array([[0.3]])
+    assert O.cf_sexp_c(x, x, 0.3, 2.0)[0, 0] == 4.0
+    assert O.cf_sexp_c(x, y, 0.3, 1.0)[0, 0] == pytest.approx(math.exp(-0.5), rel=1e-15)
+    assert O.cf_matern32_c(x, y, 0.3, 1.0)[0, 0] == pytest.approx((1 + math.sqrt(3)) * math.exp(-math.sqrt(3)), rel=1e-14)
+    assert O.cf_matern52_c(x, y, 0.3, 1.0)[0, 0] == pytest.approx((1 + math.sqrt(5) + 5 / 3) * math.exp(-math.sqrt(5)), rel=1e-14)
+
+
+def test_nn_kernel_monte_carlo():
+    """Williams (1998): k = E_u[erf(u.xt) erf(u.xt')], u ~ N(0, s^2 diag(s0^2, 1..1))."""
+    from scipy.special import erf
+    r = _rng(5)
+    d = 2
+    x = r.normal(size=(3, d))
+    s0, s, m = 1.0, 0.8, 1.0
+    K = O.cf_nn_c(x, x, s0, s, m)
+    N = 2_000_000
+    u0 = r.normal(size=N) * s0 * s
+    u = r.normal(size=(N, d)) * s
+    h = erf(u0[:, None] + u @ x.T)
+    Kmc = (h.T @ h) / N
+    np.testing.assert_allclose(K, Kmc, atol=4e-3)
+
+
+def test_periodic_closed_form():
+    """cos/sin feature map + sexp == MacKay's exp(-2 sin^2(pi dx/p)/l^2)."""
+    r = _rng(2)
+    x1 = r.uniform(-3, 3, (11, 1)); x2 = r.uniform(-3, 3, (9, 1))
+    p, l, m = 1.3, 0.7, 1.2
+    cf = {"kind": "periodic", "period": p, "base": {"kind": "sexp", "lscale": l, "magn": m}}
+    K = O.eval_cf([cf], x1, x2)
+    dx = x1 - x2.T
+    Kc = m * m * np.exp(-2 * np.sin(math.pi * dx / p) ** 2 / l ** 2)
+    np.testing.assert_allclose(K, Kc, rtol=1e-12)
+
+
+def test_composition_and_diag():
+    r = _rng(3)
+    x = r.normal(size=(12, 2))
+    a = {"kind": "sexp", "lscale": 0.5, "magn": 1.1}
+    b = {"kind": "lin", "magn": 0.7}
+    c = {"kind": "const", "magn": 0.4}
+    Ka = O.eval_cf([a], x, x); Kb = O.eval_cf([b], x, x); Kc = O.eval_cf([c], x, x)
+    np.testing.assert_allclose(O.eval_cf([a, b, c], x, x), Ka + Kb + Kc, rtol=1e-14)
+    np.testing.assert_allclose(O.eval_cf([{"kind": "prod", "cfs": [a, b, c]}], x, x), Ka * Kb * Kc, rtol=1e-14)
+    np.testing.assert_allclose(Kb, 0.49 * x @ x.T, rtol=1e-14)
+    for cf in (a, b, c, {"kind": "nn", "sigma0": 1.0, "sigma": 3.0, "magn": 1.0},
+               {"kind": "matern32", "lscale": 0.3, "magn": 1.0},
+               {"kind": "periodic", "period": 1.0, "base": {"kind": "matern52", "lscale": 0.3, "magn": 1.0}}):
+        K = O.eval_cf([cf], x, x)
+        np.testing.assert_allclose(O.eval_cf([cf], x, x, diag_only=True), np.diag(K), rtol=1e-14)
+        np.testing.assert_allclose(K, K.T, rtol=1e-14)
+
+
+def test_normalization_invariance():
+    """R tests/testthat/test_normalization.R:81-139: normalize=TRUE on raw x
+    equals the plain kernel on pre-scaled x."""
+    r = _rng(4)
+    x = 5 * (r.normal(size=(20, 1)) - 1)
+    xs = (x - x.mean(0)) / x.std(0, ddof=1)
+    for kind in ("sexp", "matern32", "matern52", "lin", "nn"):
+        cf = {"kind": kind, "lscale": 0.3, "magn": 1.0, "sigma0": 1.0, "sigma": 3.0}
+        cfn = dict(cf, normalize=True)
+        cfp = dict(cf, normalize=False)
+        K1 = O.eval_cf(O.learn_scales([cfn], x), x, x)
+        K2 = O.eval_cf([cfp], xs, xs)
+        np.testing.assert_allclose(K1, K2, rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------- likelihoods
+
+def test_loglik_vs_scipy():
+    r = _rng(6)
+    f = r.normal(size=50)
+    y = r.integers(0, 2, 50).astype(float)
+    ll = O.loglik({"kind": "bernoulli", "link": "logit"}, f, y, total=False)
+    np.testing.assert_allclose(ll, st.bernoulli.logpmf(y, 1 / (1 + np.exp(-f))), rtol=1e-12)
+    ll = O.loglik({"kind": "bernoulli", "link": "probit"}, f, y, total=False)
+    np.testing.assert_allclose(ll, st.bernoulli.logpmf(y, st.norm.cdf(f)), rtol=1e-12)
+    t = r.integers(1, 10, 50).astype(float)
+    yb = np.minimum(r.integers(0, 10, 50), t)
+    ll = O.loglik({"kind": "binomial", "link": "logit"}, f, yb, trials=t, total=False)
+    np.testing.assert_allclose(ll, st.binom.logpmf(yb, t, 1 / (1 + np.exp(-f))), rtol=1e-11)
+    yp = r.poisson(2.0, 50).astype(float)
+    ll = O.loglik({"kind": "poisson", "link": "log"}, f, yp, total=False)
+    np.testing.assert_allclose(ll, st.poisson.logpmf(yp, np.exp(f)), rtol=1e-12)
+    ll = O.loglik({"kind": "gaussian", "sigma": 0.3}, f, y, total=False)
+    np.testing.assert_allclose(ll, st.norm.logpdf(y, f, 0.3), rtol=1e-13)
+
+
+@pytest.mark.parametrize("lik", [{"kind": "bernoulli", "link": "logit"}, {"kind": "bernoulli", "link": "probit"},
+                                 {"kind": "poisson", "link": "log"}])
+def test_loglik_derivative_fd(lik):
+    r = _rng(7)
+    f = r.normal(size=20)
+    y = (r.integers(0, 2, 20) if lik["kind"] == "bernoulli" else r.poisson(2.0, 20)).astype(float)
+    h = 1e-6
+    fd = (O.loglik(lik, f + h, y, total=False) - O.loglik(lik, f - h, y, total=False)) / (2 * h)
+    np.testing.assert_allclose(O.loglik_d(lik, f, y), fd, rtol=1e-6, atol=1e-8)
+    g, g2 = O.loglik_d2(lik, f, y)
+    fd2 = (O.loglik_d(lik, f + h, y) - O.loglik_d(lik, f - h, y)) / (2 * h)
+    np.testing.assert_allclose(g2, fd2, rtol=1e-4, atol=1e-6)
+
+
+# ---------------------------------------------------------------- quadrature
+
+@pytest.mark.parametrize("p", [3, 7, 11, 15])
+def test_gauss_hermite_vs_numpy(p):
+    x, w = O.gauss_hermite_points(p)
+    xr, wr = np.polynomial.hermite_e.hermegauss(p)
+    np.testing.assert_allclose(x, xr, atol=1e-10)
+    np.testing.assert_allclose(w, wr / math.sqrt(2 * math.pi), rtol=1e-8)
+    assert abs(w.sum() - 1) < 1e-12
+
+
+# ---------------------------------------------------------------- Gaussian fit
+
+def _toy(n=40, d=2, seed=8):
+    r = _rng(seed)
+    x = r.normal(size=(n, d)); y = np.sin(x.sum(1)) + 0.1 * r.normal(size=n)
+    return x, y
+
+
+def test_gaussian_logz_closed_form_and_scipy():
+    x, y = _toy()
+    cfs = [{"kind": "matern52", "lscale": [0.7, 1.3], "magn": 1.2}]
+    lik = {"kind": "gaussian", "sigma": 0.2}
+    fit = O.gp_fit(cfs, lik, x, y)
+    Kcf = O.eval_cf(cfs, x, x)
+    ref = st.multivariate_normal(np.zeros(len(y)), Kcf + (0.04 + O.JITTER) * np.eye(len(y))).logpdf(y)
+    assert fit["log_evidence"] == pytest.approx(ref, rel=1e-12)
+    assert O.gaussian_logz_closed_form(Kcf, y, 0.2) == pytest.approx(ref, rel=1e-12)
+    Ky = Kcf + (0.04 + O.JITTER) * np.eye(len(y))
+    np.testing.assert_allclose(fit["alpha"], np.linalg.solve(Ky, y), rtol=1e-10)
+    np.testing.assert_allclose(fit["C_chol"] @ fit["C_chol"].T, Ky, rtol=1e-12, atol=1e-12)
+
+
+def test_pred_closed_form():
+    """R tests/testthat/test_basic.R:99-132 restated at 1e-10."""
+    x, y = _toy()
+    xt = _rng(9).normal(size=(15, 2))
+    cfs = [{"kind": "sexp", "lscale": 0.8, "magn": 1.0}, {"kind": "lin", "magn": 0.3}]
+    lik = {"kind": "gaussian", "sigma": 0.3}
+    fit = O.gp_fit(cfs, lik, x, y)
+    mean, var = O.gp_pred(fit, xt, var=True)
+    K = O.eval_cf(cfs, x, x) + O.JITTER * np.eye(len(y)); Kt = O.eval_cf(cfs, xt, x); Ktt = O.eval_cf(cfs, xt, xt)
+    C = K + 0.09 * np.eye(len(y))
+    np.testing.assert_allclose(mean, Kt @ np.linalg.solve(C, y), rtol=1e-10)
+    np.testing.assert_allclose(var, np.diag(Ktt - Kt @ np.linalg.solve(C, Kt.T)), rtol=1e-9)
+    m0, v0 = O.gp_pred(fit, xt, var=True, prior=True)
+    assert np.all(m0 == 0); np.testing.assert_allclose(v0, np.diag(Ktt), rtol=1e-14)
+
+
+ALL_CFS = [
+    [{"kind": "sexp", "lscale": [0.7, 1.3], "magn": 1.2}],
+    [{"kind": "matern32", "lscale": 0.9, "magn": 0.8}],
+    [{"kind": "matern52", "lscale": [0.5, 0.9], "magn": 1.1}, {"kind": "const", "magn": 0.5}],
+    [{"kind": "nn", "sigma0": 1.0, "sigma": 1.5, "magn": 1.0}],
+    [{"kind": "lin", "magn": 0.6}, {"kind": "sexp", "lscale": 1.0, "magn": 1.0}],
+    [{"kind": "periodic", "period": 2.0, "vars": [0], "base": {"kind": "sexp", "lscale": 0.8, "magn": 1.0}}],
+    [{"kind": "periodic", "period": 2.0, "vars": [1], "base": {"kind": "matern52", "lscale": 0.8, "magn": 1.0}}],
+    [{"kind": "prod", "cfs": [{"kind": "sexp", "lscale": 0.9, "magn": 1.0},
+                               {"kind": "matern32", "lscale": [1.1, 0.6], "magn": 0.7, "priors": {"magn": {"kind": "fixed"}}}]}],
+]
+
+
+@pytest.mark.parametrize("ci", range(len(ALL_CFS)))
+def test_energy_gradient_finite_difference(ci):
+    """Analytic gradient (R&W eq 5.9) vs central differences of the energy."""
+    x, y = _toy(n=30)
+    cfs = ALL_CFS[ci]
+    lik = {"kind": "gaussian", "sigma": 0.25, "priors": {"sigma": {"kind": "lognormal", "loc": -1.0, "scale": 0.5}}}
+    fit = O.gp_fit(cfs, lik, x, y)
+    g = O.gp_energy_grad(fit)
+    p0 = O.get_param(cfs, lik)
+    assert len(g) == len(p0) == len(O.param_names(cfs, lik))
+    h = 1e-5
+    for j in range(len(p0)):
+        e = []
+        for s in (+1, -1):
+            p = p0.copy(); p[j] += s * h
+            c2, l2 = O.set_param(cfs, lik, p)
+            e.append(O.gp_energy(O.gp_fit(c2, l2, x, y)))
+        fd = (e[0] - e[1]) / (2 * h)
+        assert g[j] == pytest.approx(fd, rel=2e-6, abs=2e-7), (ci, j)
+
+
+def test_param_roundtrip():
+    """R tests/testthat/test_param.R:73-84: get_param(set_param(p)) == p."""
+    cfs = ALL_CFS[7] + ALL_CFS[5]
+    lik = {"kind": "gaussian", "sigma": 0.5}
+    p = _rng(1).normal(size=len(O.get_param(cfs, lik)))
+    c2, l2 = O.set_param(cfs, lik, p)
+    np.testing.assert_allclose(O.get_param(c2, l2), p, rtol=1e-14)
+
+
+def test_priors_vs_scipy():
+    p = 0.37
+    cf = {"kind": "nn", "sigma0": math.exp(p), "sigma": 1.0, "magn": 1.0,
+          "priors": {"sigma": {"kind": "fixed"}, "magn": {"kind": "fixed"},
+                     "sigma0": {"kind": "half_t", "df": 3.0, "scale": 2.0}}}
+    lp = O.lpdf_prior([cf], {"kind": "bernoulli"})
+    th = math.exp(p)
+    ref = math.log(2) - math.log(2.0) + st.t.logpdf(th / 2.0, 3.0) + p
+    assert lp == pytest.approx(ref, rel=1e-12)
+    lik = {"kind": "gaussian", "sigma": math.exp(p), "priors": {"sigma": {"kind": "lognormal", "loc": 0.2, "scale": 0.7}}}
+    assert O.lpdf_prior([], lik) == pytest.approx(st.norm.logpdf(p, 0.2, 0.7), rel=1e-12)
+    for obj in ([cf], []):
+        lk = lik if obj == [] else {"kind": "bernoulli"}
+        g = O.lpdf_prior_grad(obj, lk)[0]
+        h = 1e-6
+        pp = O.get_param(obj, lk)
+        fd = (O.lpdf_prior(*O.set_param(obj, lk, pp + h)) - O.lpdf_prior(*O.set_param(obj, lk, pp - h))) / (2 * h)
+        assert g == pytest.approx(fd, rel=1e-6)
+
+
+# ---------------------------------------------------------------- Laplace / EP
+
+def _class_data(n=60, seed=10, kind="bernoulli"):
+    r = _rng(seed)
+    x = r.normal(size=(n, 2))
+    K = O.eval_cf([{"kind": "sexp", "lscale": 0.8, "magn": 2.0}], x, x) + 1e-8 * np.eye(n)
+    f = sla.cholesky(K, lower=True) @ r.normal(size=n)
+    if kind == "bernoulli":
+        y = (r.uniform(size=n) < 1 / (1 + np.exp(-f))).astype(float)
+    else:
+        y = r.poisson(np.exp(0.5 * f)).astype(float)
+    return x, y
+
+
+@pytest.mark.parametrize("lik", [{"kind": "bernoulli", "link": "logit"}, {"kind": "bernoulli", "link": "probit"},
+                                 {"kind": "poisson", "link": "log"}])
+def test_laplace_mode_fixed_point(lik):
+    """At the mode f = K grad log p(y|f) (R&W eq 3.17), independent of W."""
+    x, y = _class_data(kind="poisson" if lik["kind"] == "poisson" else "bernoulli")
+    cfs = [{"kind": "sexp", "lscale": 0.8, "magn": 2.0 if lik["kind"] != "poisson" else 0.7}]
+    fit = O.gp_fit(cfs, lik, x, y, maxiter=100, tol=1e-10)
+    f = fit["fmean"]
+    np.testing.assert_allclose(f, fit["K"] @ O.loglik_d(lik, f, y), atol=1e-8)
+    # logZ vs R&W Alg 3.2 with the analytic W at the mode (FD curvature differs ~1e-6)
+    K = fit["K"]
+    h = 1e-5
+    W = -(O.loglik_d(lik, f + h, y) - O.loglik_d(lik, f - h, y)) / (2 * h)
+    sW = np.sqrt(W)
+    L = sla.cholesky(np.eye(len(y)) + sW[:, None] * K * sW[None, :], lower=True)
+    a = O.loglik_d(lik, f, y)
+    lz = -0.5 * a @ f + O.loglik(lik, f, y) - np.sum(np.log(np.diag(L)))
+    assert fit["log_evidence"] == pytest.approx(lz, rel=1e-6)
+
+
+def test_ep_single_site_is_exact():
+    """n=1: the EP evidence equals log int N(f|0,k) p(y|f) df."""
+    for lik, y in (({"kind": "bernoulli", "link": "probit"}, 1.0), ({"kind": "bernoulli", "link": "logit"}, 0.0),
+                   ({"kind": "poisson", "link": "log"}, 3.0)):
+        k = 1.7
+        # with 11 nodes the evidence is the 11-point probabilists' GH rule (numpy's nodes)
+        fit = O.ep(lik, np.array([[k]]), np.array([y]), maxiter=200, quad_order=11)
+        xg, wg = np.polynomial.hermite_e.hermegauss(11)
+        gh = np.sum(wg / math.sqrt(2 * math.pi) * np.exp(O.loglik(lik, math.sqrt(k) * xg, np.full(11, y), total=False)))
+        assert fit["log_evidence"] == pytest.approx(math.log(gh), abs=1e-10)
+        # with many nodes it converges to the exact integral
+        fit = O.ep(lik, np.array([[k]]), np.array([y]), maxiter=200, quad_order=61)
+        integ = si.quad(lambda f: st.norm.pdf(f, 0, math.sqrt(k)) * math.exp(O.loglik(lik, np.array([f]), np.array([y]))),
+                        -30, 30, epsabs=1e-13)[0]
+        assert fit["log_evidence"] == pytest.approx(math.log(integ), abs=1e-5)
+
+
+def test_ep_fixed_point_moment_matching():
+    x, y = _class_data(n=40)
+    lik = {"kind": "bernoulli", "link": "probit"}
+    cfs = [{"kind": "sexp", "lscale": 0.8, "magn": 2.0}]
+    x = x; K = O.eval_cf(cfs, x, x) + 1e-6 * np.eye(len(y))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        fit = O.ep(lik, K, y, maxiter=500, tol=1e-10, damping=0.9)
+    # posterior marginals = tilted moments of the cavity x likelihood
+    mt, vt, *_ = O.tilted_moments(lik, fit["fmean"], 1 / fit["fvar"], fit["z"], fit["P"], y, 11)
+    np.testing.assert_allclose(fit["fmean"], mt, atol=1e-7)
+    np.testing.assert_allclose(fit["fvar"], vt, rtol=1e-6)
+    # fvar is the diagonal of (K^-1 + diag(P))^-1
+    S = np.linalg.inv(np.linalg.inv(K) + np.diag(fit["P"]))
+    np.testing.assert_allclose(fit["fvar"], np.diag(S), rtol=1e-8)
+    np.testing.assert_allclose(fit["fmean"], S @ (fit["P"] * fit["z"]), rtol=1e-7, atol=1e-9)
+
+
+def test_cpu_baseline_matches_oracle():
+    """The LAPACK-based CPU baseline computes the same evidence and gradient."""
+    from oracle.cpu_baseline import energy_grad_matern52_ard
+    r = _rng(21)
+    x = r.uniform(0, 1, (90, 3)); y = np.sin(x.sum(1)) + 0.05 * r.normal(size=90)
+    ls = np.array([0.3, 0.5, 0.8])
+    lz, g, _ = energy_grad_matern52_ard(x, y, ls, 1.1, 0.07)
+    cfs = [{"kind": "matern52", "lscale": ls, "magn": 1.1}]
+    fit = O.gp_fit(cfs, {"kind": "gaussian", "sigma": 0.07}, x, y)
+    assert lz == pytest.approx(fit["log_evidence"], rel=1e-10)
+    np.testing.assert_allclose(g, O.gp_energy_grad(fit), rtol=1e-8)
