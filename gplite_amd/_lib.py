@@ -85,6 +85,16 @@ def _sig(lib):
         "gpl_model_chol": (C.c_int, [_vp, _dp]),
         "gpl_model_info": (C.c_int, [_vp, C.POINTER(i64), C.POINTER(i32), _dp]),
         "gpl_model_free": (None, [_vp]),
+        "gpl_batch_create": (C.c_int, [_vp, i32, C.POINTER(CfTerm), i32, C.POINTER(_dp), C.POINTER(i64),
+                                       C.POINTER(i64), C.POINTER(_dp), _dp, C.POINTER(Opts), C.POINTER(_vp)]),
+        "gpl_batch_set_params": (C.c_int, [_vp, C.POINTER(CfTerm), _dp]),
+        "gpl_batch_energy_grad": (C.c_int, [_vp, _dp, _dp]),
+        "gpl_batch_grad_len": (i32, [_vp]),
+        "gpl_batch_info": (C.c_int, [_vp, C.POINTER(i32), _dp]),
+        "gpl_batch_tune": (C.c_int, [_vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), _dp,
+                                     C.POINTER(i32)]),
+        "gpl_batch_set_tuning": (C.c_int, [_vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "gpl_batch_free": (None, [_vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)

@@ -654,9 +654,10 @@ def _lik_struct(lik):
     return s
 
 
-def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None):
+def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want_grad=False):
     """R/gp_fit.R:54-60: learn input scales, build K + jitter I on the device and
-    run the Laplace Newton loop or EP; returns a new fitted GP."""
+    run the Laplace Newton loop or EP; returns a new fitted GP.  want_grad forms
+    L^-1 during the fit (the gradient needs it anyway)."""
     g = gp.copy()
     x = _as_matrix(x)
     y = np.ascontiguousarray(np.asarray(y, dtype=np.float64).ravel())
@@ -683,6 +684,7 @@ def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None):
         o.maxiter = g.approx.maxiter
         o.tol = g.approx.tol
     o.jitter = 1e-6 if jitter is None else float(jitter)
+    o.flags = 1 if want_grad else 0
     prec = precision or g.precision
     o.precision = L.PREC_MIXED if prec == "mixed" else L.PREC_FP64
     g.precision = prec
@@ -798,6 +800,8 @@ def gp_optim(gp, x, y, tol=1e-4, tol_param=1e-1, maxiter=500, restarts=1, verbos
         return e / n, g2
 
     if method == "L-BFGS-B":
+        fit_args = dict(fit_args, want_grad=True)
+
         def fg(p):
             e, g2 = energy(p)
             return e, gp_energy_grad(g2) / n

@@ -1,0 +1,506 @@
+// Batched form: energy + gradient of many independent Gaussian-likelihood GPs
+// of uneven size in one device pass.  Problems are grouped into buckets by
+// their padded size N (multiple of 64; the padding is an identity block, so
+// every result is exactly that of the unpadded problem).  Each bucket runs the
+// mixed-precision pipeline -- FP32 B build, FP32-MFMA blocked Cholesky with
+// FP64 64x64 diagonal blocks and panel solves, FP32 L^-1 and B^-1, FP64
+// iterative refinement of alpha with a matrix-free FP64 K*v, fused gradient --
+// as batched launches (problem = grid row).  The whole multi-bucket launch
+// sequence is captured once into a hipGraph and replayed.  The GEMM tile edge
+// and the outer panel width per bucket come from gpl_batch_tune (timed
+// candidates) or gpl_batch_set_tuning (a persisted table).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "host_common.h"
+
+using namespace gpl;
+
+namespace {
+
+enum {
+    BB_X, BB_XOFF, BB_N, BB_PROG, BB_F, BB_Y, BB_SV, BB_SCAL, BB_SIG, BB_WSC, BB_IDX, BB_RHS, BB_U, BB_R, BB_D, BB_T1,
+    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_COUNT
+};
+
+struct Bucket {
+    int64_t N = 0;
+    std::vector<int> idx;  // problem indices
+    int nfeat = 1;
+    int bm = 0;       // GEMM tile edge (0 = heuristic)
+    int nbo = 256;    // outer panel width
+    int nchunk = 1;
+    DevBuf b[BB_COUNT];
+    double* d(int i) const { return b[i].as<double>(); }
+    float* f(int i) const { return b[i].as<float>(); }
+    ~Bucket() {
+        for (auto& x : b) x.free_();
+    }
+};
+
+}  // namespace
+
+struct gpl_batch {
+    gpl_ctx* ctx = nullptr;
+    int nprob = 0, nterms = 0, ng = 0;
+    gpl_opts opts{};
+    DevProg hp{};  // term structure shared by all problems (host copy of problem 0)
+    std::vector<Bucket*> buckets;
+    std::vector<int> bucket_of;  // problem -> bucket index
+    std::vector<int> pos_in;     // problem -> position in its bucket
+    DevBuf res_logz, res_grad, relmax;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~gpl_batch() {
+        for (auto* b : buckets) delete b;
+        res_logz.free_();
+        res_grad.free_();
+        relmax.free_();
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }
+};
+
+// --------------------------------------------------------------------------
+// the per-bucket pipeline (enqueued on one stream; captured into the graph)
+// --------------------------------------------------------------------------
+static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
+    gpl_ctx* c = B->ctx;
+    const int nb = (int)k.idx.size();
+    const int64_t N = k.N, N2 = N * N, T = N / 64;
+    const int64_t sF = N * MAX_FEAT;
+    const DevProg* progs = k.b[BB_PROG].as<DevProg>();
+    const int64_t* nvec = k.b[BB_N].as<int64_t>();
+    float* L = k.f(BB_L);
+    float* X = k.f(BB_X32);
+    float* Bi = k.f(BB_BI);
+    double* Dinv = k.d(BB_DINV);
+    double* ldiag = k.d(BB_LDIAG);
+    int* info = k.b[BB_INFO].as<int>();
+    const int bm = k.bm;
+    const int64_t NBo = k.nbo;
+    {
+        Prof p(c, "batch_build_B_f32", 0, 4.0 * nb * N * (N + 64) / 2, s);
+        launch_build_B_f32_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), L, N2, nb);
+    }
+    // blocked Cholesky (FP32 storage, FP64 diagonal blocks and panel solves)
+    for (int64_t K0 = 0; K0 < N; K0 += NBo) {
+        const int64_t W = std::min<int64_t>(NBo, N - K0);
+        for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
+            const int64_t kk = j0 / 64;
+            {
+                Prof p(c, "batch_potrf_diag", 0, 0, s);
+                potrf_diag3_f32_batched(s, L, N, kk, Dinv, ldiag, info, nb, N2, N * 64, N);
+            }
+            const int64_t M = N - (j0 + 64);
+            if (M <= 0) continue;
+            float* L21 = L + (j0 + 64) + j0 * N;
+            {
+                Prof p(c, "batch_trsm_f64", 2.0 * nb * M * 64 * 64, 0, s);
+                gemm_mixed_f32f64_batched(s, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + kk * 64 * 64, 64, 0.0, L21, N, 0, nb,
+                                          N2, N * 64, N2);
+            }
+            const int64_t Wr = K0 + W - (j0 + 64);
+            if (Wr > 0) {
+                Prof p(c, "batch_panel_f32", 2.0 * nb * M * Wr * 64, 0, s);
+                gemm_f32_batched(s, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, bm, nb, N2,
+                                 N2, N2);
+            }
+        }
+        const int64_t R = N - (K0 + W);
+        if (R > 0) {
+            const float* P = L + (K0 + W) + K0 * N;
+            Prof p(c, "batch_syrk_f32", (double)nb * R * R * W, 0, s);
+            gemm_f32_batched(s, 0, 1, R, R, W, -1.f, P, N, P, N, 1.f, L + (K0 + W) * (N + 1), N, 1, bm, nb, N2, N2, N2);
+        }
+    }
+    // X = L^-1 (FP32, FP64 diagonal-block applications)
+    set_identity_lower_f32_batched(s, X, N, N, nb, N2);
+    for (int64_t I0 = 0; I0 < N; I0 += NBo) {
+        const int64_t H = std::min<int64_t>(NBo, N - I0);
+        for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
+            const int64_t i = r0 / 64, Wc = r0 + 64;
+            {
+                Prof p(c, "batch_trtri_diag_f64", 2.0 * nb * 64 * 64 * Wc, 0, s);
+                gemm_mixed_f64f32_batched(s, 0, 0, 64, Wc, 64, 1.0, Dinv + i * 64 * 64, 64, X + r0, N, 0.0, X + r0, N, 0,
+                                          nb, N * 64, N2, N2);
+            }
+            const int64_t Mr = I0 + H - (r0 + 64);
+            if (Mr > 0) {
+                Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * Mr * Wc * 64, 0, s);
+                gemm_f32_batched(s, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0,
+                                 bm, nb, N2, N2, N2);
+            }
+        }
+        const int64_t R = N - (I0 + H);
+        if (R > 0) {
+            const int64_t Wc = I0 + H;
+            Prof p(c, "batch_trtri_f32", 2.0 * nb * R * Wc * H, 0, s);
+            gemm_f32_batched(s, 0, 0, R, Wc, H, -1.f, L + (I0 + H) + I0 * N, N, X + I0, N, 1.f, X + I0 + H, N, 0, bm, nb,
+                             N2, N2, N2);
+        }
+    }
+    // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is static)
+    const int64_t NV = (int64_t)nb * N;
+    double* rhs = k.d(BB_RHS);
+    double* u = k.d(BB_U);
+    double* r = k.d(BB_R);
+    double* dd = k.d(BB_D);
+    double* t1 = k.d(BB_T1);
+    double* Ku = k.d(BB_KU);
+    double* sv = k.d(BB_SV);
+    seg_scale(s, nullptr, k.d(BB_SCAL), nvec, N, sv, nb);       // S = 1/sigma on valid rows, 0 on padding
+    vec_mul(s, NV, sv, k.d(BB_Y), rhs);                         // S y
+    fill_f64(s, u, NV, 0.0);
+    vec_axpby(s, NV, 1.0, rhs, 0.0, nullptr, r);
+    const int sweeps = B->opts.refine_iters > 0 ? B->opts.refine_iters : 4;
+    for (int it = 0; it < sweeps; ++it) {
+        {
+            Prof p(c, "batch_refine_f32", 0, 0, s);
+            trmv_lower_f32_batched(s, X, N, N, r, t1, 0, nb, N2, N);
+            trmv_lower_f32_batched(s, X, N, N, t1, dd, 1, nb, N2, N);
+        }
+        vec_axpby(s, NV, 1.0, dd, 1.0, u, u);
+        if (it == sweeps - 1) relmax_accumulate(s, dd, u, NV, B->relmax.as<double>());
+        vec_mul(s, NV, sv, u, t1);
+        {
+            Prof p(c, "batch_kmatvec_f64", 0, 0, s);
+            launch_kmatvec_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, t1, Ku, N, k.d(BB_KMV),
+                                   k.nchunk, nb);
+        }
+        vec_mul(s, NV, sv, Ku, r);
+        vec_axpby(s, NV, -1.0, r, -1.0, u, r);
+        vec_axpby(s, NV, 1.0, r, 1.0, rhs, r);
+    }
+    double* alpha = k.d(BB_ALPHA);
+    vec_mul(s, NV, sv, u, alpha);
+    // B^-1 = X^T X and the gradient
+    {
+        Prof p(c, "batch_lauum_f32", (double)nb * N * N * N / 3, 0, s);
+        gemm_f32_batched(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, bm, nb, N2, N2, N2);
+    }
+    const int64_t ntiles = T * (T + 1) / 2;
+    {
+        Prof p(c, "batch_grad", 0, 0, s);
+        launch_grad_tiles_f32_batched(s, progs, B->hp, k.d(BB_F), sF, nvec, N, Bi, N2, k.d(BB_WSC), alpha, N,
+                                      k.d(BB_PART), nb);
+    }
+    double* red = k.d(BB_RED);
+    seg_dot(s, k.d(BB_Y), alpha, nvec, N, red, 8, 0, nb);
+    seg_dot(s, ldiag, nullptr, nvec, N, red, 8, 1, nb);
+    seg_dot(s, alpha, alpha, nvec, N, red, 8, 2, nb);
+    colsumsq_f32_batched(s, X, N, N, N, t1, 1, nb, N2, N);
+    seg_dot(s, t1, nullptr, nvec, N, red, 8, 3, nb);
+    seg_cols(s, k.d(BB_PART), ntiles, B->hp.ngrad, red, 8, 4, nb);
+    batch_finish(s, red, nvec, k.d(BB_SIG), k.b[BB_IDX].as<int32_t>(), B->hp.ngrad, nb, B->res_logz.as<double>(),
+                 B->res_grad.as<double>(), B->ng);
+}
+
+static int upload_params(gpl_batch* B, const gpl_cf_term* terms, const double* sigma) {
+    hipStream_t s = B->ctx->stream;
+    for (auto* k : B->buckets) {
+        const int nb = (int)k->idx.size();
+        std::vector<DevProg> progs(nb);
+        std::vector<double> scal(nb), sig(nb), wsc(nb);
+        for (int q = 0; q < nb; ++q) {
+            const int p = k->idx[q];
+            int ncols = 0;
+            for (int t = 0; t < B->nterms; ++t)
+                ncols = std::max(ncols, terms[(size_t)p * B->nterms + t].col0 + terms[(size_t)p * B->nterms + t].ncols);
+            int r = prepare_prog(terms + (size_t)p * B->nterms, B->nterms, ncols, progs[q]);
+            if (r) return r;
+            if (progs[q].ngrad != B->hp.ngrad || progs[q].t[0].kind != B->hp.t[0].kind ||
+                progs[q].t[0].nl != B->hp.t[0].nl)
+                return fail(GPL_ERR_ARG, "all problems of a batch need the same covariance structure");
+            if (!(sigma[p] > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
+            scal[q] = 1.0 / sigma[p];
+            sig[q] = sigma[p];
+            wsc[q] = 1.0 / (sigma[p] * sigma[p]);
+        }
+        HIP_TRY(hipMemcpyAsync(k->b[BB_PROG].p, progs.data(), nb * sizeof(DevProg), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_SCAL].p, scal.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_SIG].p, sig.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_WSC].p, wsc.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return GPL_OK;
+}
+
+static void drop_graph(gpl_batch* B) {
+    if (B->exec) hipGraphExecDestroy(B->exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+}
+
+extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
+                                const double* const* x, const int64_t* n, const int64_t* ncols,
+                                const double* const* y, const double* sigma, const gpl_opts* opts, gpl_batch** out) {
+    if (!c || !terms || !x || !n || !ncols || !y || !sigma || !out || nprob < 1)
+        return fail(GPL_ERR_ARG, "NULL argument or empty batch");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    gpl_batch* B = new gpl_batch();
+    B->ctx = c;
+    B->nprob = nprob;
+    B->nterms = nterms;
+    if (opts) B->opts = *opts;
+    else gpl_opts_default(&B->opts);
+    if (!opts) B->opts.refine_iters = 4;
+    int r = prepare_prog(terms, nterms, ncols[0], B->hp);
+    if (r) {
+        delete B;
+        return r;
+    }
+    B->ng = B->hp.ngrad + 1;
+    {
+        // kernels size their feature staging and the gradient fast path by the widest problem
+        int maxc = 0;
+        for (int p = 0; p < nprob; ++p) maxc = std::max<int64_t>(maxc, ncols[p]);
+        DevProg widest;
+        std::vector<gpl_cf_term> tw(terms + (size_t)0, terms + nterms);
+        for (int p = 0; p < nprob; ++p)
+            if (ncols[p] == maxc) {
+                for (int t = 0; t < nterms; ++t) tw[t] = terms[(size_t)p * nterms + t];
+                break;
+            }
+        r = prepare_prog(tw.data(), nterms, maxc, widest);
+        if (r) {
+            delete B;
+            return r;
+        }
+        B->hp = widest;
+    }
+    // buckets by padded size
+    std::map<int64_t, int> bidx;
+    B->bucket_of.resize(nprob);
+    B->pos_in.resize(nprob);
+    for (int p = 0; p < nprob; ++p) {
+        if (n[p] < 1) {
+            delete B;
+            return fail(GPL_ERR_ARG, "every problem needs n >= 1");
+        }
+        const int64_t Np = pad64(n[p]);
+        auto it = bidx.find(Np);
+        if (it == bidx.end()) {
+            bidx[Np] = (int)B->buckets.size();
+            Bucket* k = new Bucket();
+            k->N = Np;
+            B->buckets.push_back(k);
+            it = bidx.find(Np);
+        }
+        Bucket* k = B->buckets[it->second];
+        B->bucket_of[p] = it->second;
+        B->pos_in[p] = (int)k->idx.size();
+        k->idx.push_back(p);
+    }
+    hipStream_t s = c->stream;
+    for (auto* k : B->buckets) {
+        const int nb = (int)k->idx.size();
+        const int64_t N = k->N, T = N / 64;
+        k->nchunk = kmatvec_chunks(N);
+        k->nfeat = std::max(1, B->hp.nfeat);
+        std::vector<int64_t> xoff(nb), nv(nb);
+        std::vector<int32_t> idx(nb);
+        int64_t tot = 0;
+        for (int q = 0; q < nb; ++q) {
+            const int p = k->idx[q];
+            xoff[q] = tot;
+            nv[q] = n[p];
+            idx[q] = p;
+            tot += n[p] * ncols[p];
+        }
+        const size_t vec = (size_t)nb * N * 8, mat32 = (size_t)nb * N * N * 4;
+        bool ok = k->b[BB_X].ensure((size_t)std::max<int64_t>(tot, 1) * 8) && k->b[BB_XOFF].ensure(nb * 8) &&
+                  k->b[BB_N].ensure(nb * 8) && k->b[BB_PROG].ensure(nb * sizeof(DevProg)) &&
+                  k->b[BB_F].ensure((size_t)nb * N * MAX_FEAT * 8) && k->b[BB_SCAL].ensure(nb * 8) &&
+                  k->b[BB_SIG].ensure(nb * 8) && k->b[BB_WSC].ensure(nb * 8) && k->b[BB_IDX].ensure(nb * 4) &&
+                  k->b[BB_L].ensure(mat32) && k->b[BB_X32].ensure(mat32) && k->b[BB_BI].ensure(mat32) &&
+                  k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_LDIAG].ensure(vec) &&
+                  k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&
+                  k->b[BB_KMV].ensure((size_t)nb * k->nchunk * N * 8) && k->b[BB_RED].ensure((size_t)nb * 8 * 8);
+        for (int i : {BB_Y, BB_SV, BB_RHS, BB_U, BB_R, BB_D, BB_T1, BB_KU, BB_ALPHA}) ok = ok && k->b[i].ensure(vec);
+        if (!ok) {
+            delete B;
+            return fail(GPL_ERR_OOM, "device allocation failed for the batch");
+        }
+        // concatenated raw inputs and padded targets
+        std::vector<double> xs((size_t)std::max<int64_t>(tot, 1)), ys((size_t)nb * N, 0.0);
+        for (int q = 0; q < nb; ++q) {
+            const int p = k->idx[q];
+            memcpy(xs.data() + xoff[q], x[p], n[p] * ncols[p] * 8);
+            memcpy(ys.data() + (size_t)q * N, y[p], n[p] * 8);
+        }
+        HIP_TRY(hipMemcpyAsync(k->b[BB_X].p, xs.data(), xs.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_XOFF].p, xoff.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_N].p, nv.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_IDX].p, idx.data(), nb * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(k->b[BB_Y].p, ys.data(), ys.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (!B->res_logz.ensure((size_t)nprob * 8) || !B->res_grad.ensure((size_t)nprob * B->ng * 8) ||
+        !B->relmax.ensure(64)) {
+        delete B;
+        return fail(GPL_ERR_OOM, "device allocation failed for the batch");
+    }
+    r = upload_params(B, terms, sigma);
+    if (r) {
+        delete B;
+        return r;
+    }
+    // features (O(n d), once; hyperparameter updates do not change them for stationary terms)
+    for (auto* k : B->buckets) {
+        fill_f64(s, k->d(BB_F), (int64_t)k->idx.size() * k->N * MAX_FEAT, 0.0);
+        launch_features_batched(s, k->d(BB_X), k->b[BB_XOFF].as<int64_t>(), k->b[BB_N].as<int64_t>(),
+                                k->b[BB_PROG].as<DevProg>(), k->d(BB_F), k->N, k->N * MAX_FEAT, (int)k->idx.size());
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = B;
+    return GPL_OK;
+}
+
+extern "C" int gpl_batch_set_params(gpl_batch* B, const gpl_cf_term* terms, const double* sigma) {
+    if (!B || !terms || !sigma) return fail(GPL_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(B->ctx->mu);
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    for (int p = 0; p < B->nprob; ++p)
+        for (int t = 0; t < B->nterms; ++t)
+            if (terms[(size_t)p * B->nterms + t].kind == GPL_CF_PERIODIC)
+                return fail(GPL_ERR_UNSUPPORTED, "batched periodic terms need a feature rebuild");
+    return upload_params(B, terms, sigma);
+}
+
+static void reset_flags(gpl_batch* B, hipStream_t s) {
+    // outside any graph: status words are cleared with plain memsets before a pass
+    hipMemsetAsync(B->relmax.p, 0, 8, s);
+    for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_INFO].p, 0, k->idx.size() * 4, s);
+}
+
+static int run_once(gpl_batch* B) {
+    hipStream_t s = B->ctx->stream;
+    reset_flags(B, s);
+    if (!B->exec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const bool prof = B->ctx->prof;
+        B->ctx->prof = false;  // no event records inside the graph
+        for (auto* k : B->buckets) bucket_pipeline(B, *k, s);
+        B->ctx->prof = prof;
+        hipError_t e = hipStreamEndCapture(s, &B->graph);
+        if (e != hipSuccess) return fail(GPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(B->exec, s));
+    return GPL_OK;
+}
+
+extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double* grad) {
+    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
+    std::lock_guard<std::mutex> lk(B->ctx->mu);
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    hipStream_t s = B->ctx->stream;
+    if (B->ctx->prof) {
+        // profiled pass: eager launches with per-class events (no graph)
+        reset_flags(B, s);
+        for (auto* k : B->buckets) bucket_pipeline(B, *k, s);
+    } else {
+        int r = run_once(B);
+        if (r) return r;
+    }
+    if (log_evidence)
+        HIP_TRY(hipMemcpyAsync(log_evidence, B->res_logz.p, (size_t)B->nprob * 8, hipMemcpyDeviceToHost, s));
+    if (grad) HIP_TRY(hipMemcpyAsync(grad, B->res_grad.p, (size_t)B->nprob * B->ng * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // non-PD pivots in any problem
+    for (auto* k : B->buckets) {
+        std::vector<int> info(k->idx.size());
+        HIP_TRY(hipMemcpy(info.data(), k->b[BB_INFO].p, info.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < info.size(); ++q)
+            if (info[q] > 0) {
+                set_minor(info[q]);
+                return fail(GPL_ERR_NOT_PD, "problem " + std::to_string(k->idx[q]) + ": leading minor of order " +
+                                                std::to_string(info[q]) + " is not positive");
+            }
+    }
+    return GPL_OK;
+}
+
+extern "C" int32_t gpl_batch_grad_len(gpl_batch* B) { return B ? B->ng : 0; }
+
+extern "C" int gpl_batch_info(gpl_batch* B, int32_t* nbuckets, double* refine_rel) {
+    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
+    if (nbuckets) *nbuckets = (int32_t)B->buckets.size();
+    if (refine_rel) HIP_TRY(hipMemcpy(refine_rel, B->relmax.p, 8, hipMemcpyDeviceToHost));
+    return GPL_OK;
+}
+
+extern "C" int gpl_batch_tune(gpl_batch* B, int32_t cap, int32_t* bucket_n, int32_t* bm, int32_t* nbo,
+                              double* best_ms, int32_t* nbuckets) {
+    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
+    std::lock_guard<std::mutex> lk(B->ctx->mu);
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    hipStream_t s = B->ctx->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const bool prof = B->ctx->prof;
+    B->ctx->prof = false;
+    int i = 0;
+    for (auto* k : B->buckets) {
+        float best = 1e30f;
+        int bbm = 64, bnb = 256;
+        for (int cbm : {64, 128})
+            for (int cnb : {128, 256}) {
+                if (cnb > k->N && cnb != 128) continue;
+                k->bm = cbm;
+                k->nbo = cnb;
+                bucket_pipeline(B, *k, s);  // warm
+                hipEventRecord(e0, s);
+                bucket_pipeline(B, *k, s);
+                hipEventRecord(e1, s);
+                hipEventSynchronize(e1);
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) best = ms, bbm = cbm, bnb = cnb;
+            }
+        k->bm = bbm;
+        k->nbo = bnb;
+        if (i < cap) {
+            if (bucket_n) bucket_n[i] = (int32_t)k->N;
+            if (bm) bm[i] = bbm;
+            if (nbo) nbo[i] = bnb;
+            if (best_ms) best_ms[i] = best;
+        }
+        ++i;
+    }
+    B->ctx->prof = prof;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    drop_graph(B);
+    if (nbuckets) *nbuckets = (int32_t)B->buckets.size();
+    return GPL_OK;
+}
+
+extern "C" int gpl_batch_set_tuning(gpl_batch* B, int32_t count, const int32_t* bucket_n, const int32_t* bm,
+                                    const int32_t* nbo) {
+    if (!B || (count > 0 && (!bucket_n || !bm || !nbo))) return fail(GPL_ERR_ARG, "NULL argument");
+    for (int j = 0; j < count; ++j)
+        for (auto* k : B->buckets)
+            if (k->N == bucket_n[j]) {
+                if ((bm[j] != 64 && bm[j] != 128) || (nbo[j] != 128 && nbo[j] != 256))
+                    return fail(GPL_ERR_ARG, "tile edge must be 64/128 and panel width 128/256");
+                k->bm = bm[j];
+                k->nbo = nbo[j];
+            }
+    drop_graph(B);
+    return GPL_OK;
+}
+
+extern "C" void gpl_batch_free(gpl_batch* B) {
+    if (!B) return;
+    hipSetDevice(B->ctx->device);
+    hipStreamSynchronize(B->ctx->stream);
+    delete B;
+}

@@ -19,7 +19,7 @@
 #include <string>
 #include <vector>
 
-#include "gpl_internal.h"
+#include "host_common.h"
 
 using namespace gpl;
 
@@ -29,121 +29,16 @@ using namespace gpl;
 static thread_local std::string g_err;
 static thread_local int64_t g_minor = 0;
 
-static int fail(int code, const std::string& msg) {
+namespace gpl {
+int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail(GPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-    } while (0)
+void set_minor(int64_t minor) { g_minor = minor; }
+}  // namespace gpl
 
 extern "C" const char* gpl_last_error(void) { return g_err.c_str(); }
 extern "C" int64_t gpl_last_error_minor(void) { return g_minor; }
-
-static inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
-
-// ---------------------------------------------------------------------------
-// context, profiling
-// ---------------------------------------------------------------------------
-struct ProfRec {
-    int cls;
-    hipEvent_t a, b;
-    double flops, bytes;
-};
-
-struct ProfClass {
-    std::string name;
-    double ms = 0, flops = 0, bytes = 0;
-    int64_t launches = 0;
-};
-
-struct gpl_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool prof = false;
-    std::vector<ProfRec> pending;
-    std::vector<hipEvent_t> pool;
-    std::vector<ProfClass> classes;
-    std::map<std::string, int> cls_index;
-    std::vector<void*> scratch;
-    std::vector<size_t> scratch_sz;
-    std::mutex mu;
-
-    hipEvent_t ev() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        hipEventCreate(&e);
-        return e;
-    }
-    void resolve() {
-        if (pending.empty()) return;
-        hipEventSynchronize(pending.back().b);
-        for (auto& r : pending) {
-            float ms = 0;
-            hipEventElapsedTime(&ms, r.a, r.b);
-            auto& c = classes[r.cls];
-            c.ms += ms;
-            c.launches += 1;
-            c.flops += r.flops;
-            c.bytes += r.bytes;
-            pool.push_back(r.a);
-            pool.push_back(r.b);
-        }
-        pending.clear();
-    }
-    int cls(const char* name) {
-        auto it = cls_index.find(name);
-        if (it != cls_index.end()) return it->second;
-        int id = (int)classes.size();
-        classes.push_back(ProfClass{name});
-        cls_index[name] = id;
-        return id;
-    }
-    void* buf(int slot, size_t bytes) {
-        if ((int)scratch.size() <= slot) {
-            scratch.resize(slot + 1, nullptr);
-            scratch_sz.resize(slot + 1, 0);
-        }
-        if (scratch_sz[slot] < bytes) {
-            if (scratch[slot]) hipFree(scratch[slot]);
-            scratch[slot] = nullptr;
-            scratch_sz[slot] = 0;
-            if (hipMalloc(&scratch[slot], bytes) != hipSuccess) return nullptr;
-            scratch_sz[slot] = bytes;
-        }
-        return scratch[slot];
-    }
-};
-
-// Times one launch group on the library stream when profiling is on.
-struct Prof {
-    gpl_ctx* c;
-    ProfRec r;
-    bool on;
-    Prof(gpl_ctx* ctx, const char* name, double flops, double bytes) : c(ctx), on(ctx->prof) {
-        if (!on) return;
-        r.cls = c->cls(name);
-        r.a = c->ev();
-        r.b = c->ev();
-        r.flops = flops;
-        r.bytes = bytes;
-        hipEventRecord(r.a, c->stream);
-    }
-    ~Prof() {
-        if (!on) return;
-        hipEventRecord(r.b, c->stream);
-        c->pending.push_back(r);
-        if (c->pending.size() > 8192) c->resolve();
-    }
-};
 
 extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (!out) return fail(GPL_ERR_ARG, "out is NULL");
@@ -154,6 +49,9 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     gpl_ctx* c = new gpl_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete c;
         return fail(GPL_ERR_HIP, hipGetErrorString(e));
@@ -170,6 +68,10 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     for (auto e : c->pool) hipEventDestroy(e);
     for (auto p : c->scratch)
         if (p) hipFree(p);
+    hipStreamSynchronize(c->side);
+    hipEventDestroy(c->ev_a);
+    hipEventDestroy(c->ev_p);
+    hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -250,7 +152,7 @@ extern "C" int32_t gpl_grad_len(const gpl_cf_term* terms, int32_t nterms, int32_
     return g;
 }
 
-static int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P) {
+int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P) {
     memset(&P, 0, sizeof(P));
     if (!terms || nterms < 1 || nterms > GPL_MAX_TERMS) return fail(GPL_ERR_ARG, "nterms must be in [1, 8]");
     int f = 0, g = 0;
@@ -318,32 +220,6 @@ static int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols,
     return GPL_OK;
 }
 
-// ---------------------------------------------------------------------------
-// model
-// ---------------------------------------------------------------------------
-struct DevBuf {
-    void* p = nullptr;
-    size_t sz = 0;
-    void free_() {
-        if (p) hipFree(p);
-        p = nullptr;
-        sz = 0;
-    }
-    bool ensure(size_t bytes) {
-        if (sz >= bytes && p) return true;
-        free_();
-        if (bytes == 0) bytes = 8;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        sz = bytes;
-        return true;
-    }
-    template <typename T>
-    T* as() const { return reinterpret_cast<T*>(p); }
-};
-
 enum {
     B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
     B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
@@ -361,6 +237,8 @@ struct gpl_model {
     bool have_xinv = false, have_binv = false;
     double logZ = 0;
     int32_t iters = 0;
+    int32_t refine_iters_done = 0;
+    double refine_rel = 0;
     DevBuf b[B_COUNT];
     ~gpl_model() {
         for (auto& x : b) x.free_();
@@ -373,47 +251,209 @@ struct gpl_model {
         if (!(m)->b[idx].ensure(bytes)) return fail(GPL_ERR_OOM, "device allocation failed"); \
     } while (0)
 
-// Blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
-static void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
-    const int64_t T = N / 64;
-    hipStream_t s = c->stream;
-    for (int64_t k = 0; k < T; ++k) {
+// Two-level blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
+// Outer panels of NB columns are factored with 64-column inner steps (diagonal
+// block in LDS/registers, panel solve against its inverse on MFMA, update of the
+// rest of the outer panel); the trailing matrix then receives one rank-NB update.
+static const int kOuterNB = 256;
+
+// Factor the outer panel of columns [K0, K0+W) (all rows >= K0) on stream st.
+static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
+                        double* ldiag, int* info) {
+    for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
+        const int64_t k = j0 / 64;
         {
-            Prof p(c, "potrf_diag", 64.0 * 64 * 64 / 3 + 64.0 * 64 * 64 / 3, 2 * 64.0 * 64 * 8);
-            potrf_diag_f64(s, L, N, k, Dinv, ldiag, info);
+            Prof p(c, "potrf_diag", 2.0 * 64 * 64 * 64 / 3, 2 * 64.0 * 64 * 8, st);
+            potrf_diag3_f64(st, L, N, k, Dinv, ldiag, info, 1);
         }
-        if (k + 1 < T) {
-            const int64_t M = (T - 1 - k) * 64;
-            double* L21 = L + (k + 1) * 64 + k * 64 * N;
-            {
-                Prof p(c, "trsm_panel", 2.0 * M * 64 * 64, 2.0 * M * 64 * 8);
-                gemm_f64(s, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0);
-            }
-            {
-                Prof p(c, "syrk_update", (double)M * M * 64, 8.0 * M * M + 8.0 * M * 64);
-                gemm_f64(s, 0, 1, M, M, 64, -1.0, L21, N, L21, N, 1.0, L + (k + 1) * 64 * (N + 1), N, 1);
-            }
+        const int64_t M = N - (j0 + 64);
+        if (M <= 0) continue;
+        double* L21 = L + (j0 + 64) + j0 * N;
+        {
+            Prof p(c, "trsm_panel", 2.0 * M * 64 * 64, 2.0 * M * 64 * 8, st);
+            gemm2_f64(st, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0, 0);
+        }
+        const int64_t Wr = K0 + W - (j0 + 64);
+        if (Wr > 0) {
+            Prof p(c, "panel_update", 2.0 * M * Wr * 64, 8.0 * (2.0 * M * Wr + M * 64), st);
+            gemm2_f64(st, 0, 1, M, Wr, 64, -1.0, L21, N, L21, N, 1.0, L + (j0 + 64) * (N + 1), N, 0, 0);
         }
     }
 }
 
-// X = L^-1 by blocked forward substitution against the identity.
-static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
-    const int64_t T = N / 64;
-    hipStream_t s = c->stream;
-    set_identity_lower_f64(s, X, N, N);
-    for (int64_t i = 0; i < T; ++i) {
-        const int64_t W = (i + 1) * 64;
+// Two-level blocked right-looking Cholesky with one-panel look-ahead: while the
+// main stream applies the rank-NB update to the trailing matrix, the side
+// stream factors the next panel (whose columns are updated first).
+static void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+    hipStream_t s = c->stream, s2 = c->side;
+    potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
+    for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
+        const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
+        const int64_t K1 = K0 + W, R = N - K1;
+        if (R <= 0) break;
+        const int64_t W1 = std::min<int64_t>(kOuterNB, R);
+        const double* P = L + K1 + K0 * N;  // rows >= K1 of the factored panel
         {
-            Prof p(c, "trtri_diag", 2.0 * 64 * 64 * W, 16.0 * 64 * W);
-            gemm_f64(s, 0, 0, 64, W, 64, 1.0, Dinv + i * 64 * 64, 64, X + i * 64, N, 0.0, X + i * 64, N, 0);
+            Prof p(c, "syrk_update", 2.0 * R * W1 * W, 8.0 * (2.0 * R * W1 + R * W));
+            gemm2_f64(s, 0, 1, R, W1, W, -1.0, P, N, P, N, 1.0, L + K1 * (N + 1), N, 0, 0);
         }
-        if (i + 1 < T) {
-            const int64_t M = (T - 1 - i) * 64;
-            Prof p(c, "trtri_update", 2.0 * M * W * 64, 16.0 * M * W);
-            gemm_f64(s, 0, 0, M, W, 64, -1.0, L + (i + 1) * 64 + i * 64 * N, N, X + i * 64, N, 1.0, X + (i + 1) * 64,
-                     N, 0);
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        potrf_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - W1;
+        if (R2 > 0) {
+            const double* P2 = L + (K1 + W1) + K0 * N;
+            Prof p(c, "syrk_update", (double)R2 * R2 * W, 8.0 * ((double)R2 * R2 + 2.0 * R2 * W));
+            gemm2_f64(s, 0, 1, R2, R2, W, -1.0, P2, N, P2, N, 1.0, L + (K1 + W1) * (N + 1), N, 1, 0);
         }
+        hipStreamWaitEvent(s, c->ev_p, 0);
+    }
+}
+
+// Rows [I0, I0+H) of X = L^-1: 64-row steps inside the panel, on stream st.
+static void trtri_panel(gpl_ctx* c, hipStream_t st, const double* L, const double* Dinv, int64_t N, int64_t I0,
+                        int64_t H, double* X) {
+    for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
+        const int64_t i = r0 / 64, Wc = r0 + 64;  // columns 0 .. r0+63 are non-zero
+        {
+            Prof p(c, "trtri_diag", 2.0 * 64 * 64 * Wc, 16.0 * 64 * Wc, st);
+            gemm2_f64(st, 0, 0, 64, Wc, 64, 1.0, Dinv + i * 64 * 64, 64, X + r0, N, 0.0, X + r0, N, 0, 0);
+        }
+        const int64_t Mr = I0 + H - (r0 + 64);
+        if (Mr > 0) {
+            Prof p(c, "trtri_panel", 2.0 * Mr * Wc * 64, 16.0 * Mr * Wc, st);
+            gemm2_f64(st, 0, 0, Mr, Wc, 64, -1.0, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.0, X + r0 + 64, N, 0, 0);
+        }
+    }
+}
+
+// X = L^-1 by blocked forward substitution against the identity, two levels with
+// look-ahead: the next row panel is updated first and solved on the side stream
+// while the main stream updates the remaining rows.
+static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
+    hipStream_t s = c->stream, s2 = c->side;
+    set_identity_lower_f64(s, X, N, N);
+    trtri_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
+    for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
+        const int64_t H = std::min<int64_t>(kOuterNB, N - I0);
+        const int64_t I1 = I0 + H, R = N - I1;
+        if (R <= 0) break;
+        const int64_t H1 = std::min<int64_t>(kOuterNB, R);
+        const int64_t Wc = I0 + H;
+        {
+            Prof p(c, "trtri_update", 2.0 * H1 * Wc * H, 8.0 * (2.0 * H1 * Wc + Wc * H));
+            gemm2_f64(s, 0, 0, H1, Wc, H, -1.0, L + I1 + I0 * N, N, X + I0, N, 1.0, X + I1, N, 0, 0);
+        }
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        trtri_panel(c, s2, L, Dinv, N, I1, H1, X);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - H1;
+        if (R2 > 0) {
+            Prof p(c, "trtri_update", 2.0 * R2 * Wc * H, 8.0 * (2.0 * R2 * Wc + Wc * H));
+            gemm2_f64(s, 0, 0, R2, Wc, H, -1.0, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.0, X + I1 + H1, N, 0, 0);
+        }
+        hipStreamWaitEvent(s, c->ev_p, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision (GPL_PREC_MIXED): FP32 storage and FP32 MFMA updates; every
+// 64x64 diagonal block is factored and inverted in FP64 and the panel solve
+// against it runs in FP64 arithmetic; the log-determinant accumulates the FP64
+// diagonal-block pivots.
+// ---------------------------------------------------------------------------
+static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
+                          double* ldiag, int* info) {
+    for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
+        const int64_t k = j0 / 64;
+        {
+            Prof p(c, "potrf_diag_mixed", 2.0 * 64 * 64 * 64 / 3, 64.0 * 64 * 12, st);
+            potrf_diag3_f32(st, L, N, k, Dinv, ldiag, info, 1);
+        }
+        const int64_t M = N - (j0 + 64);
+        if (M <= 0) continue;
+        float* L21 = L + (j0 + 64) + j0 * N;
+        {
+            Prof p(c, "trsm_panel_f64", 2.0 * M * 64 * 64, 2.0 * M * 64 * 4, st);
+            gemm_mixed_f32f64(st, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0);
+        }
+        const int64_t Wr = K0 + W - (j0 + 64);
+        if (Wr > 0) {
+            Prof p(c, "panel_update_f32", 2.0 * M * Wr * 64, 4.0 * (2.0 * M * Wr + M * 64), st);
+            gemm_f32(st, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0);
+        }
+    }
+}
+
+static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+    hipStream_t s = c->stream, s2 = c->side;
+    potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
+    for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
+        const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
+        const int64_t K1 = K0 + W, R = N - K1;
+        if (R <= 0) break;
+        const int64_t W1 = std::min<int64_t>(kOuterNB, R);
+        const float* P = L + K1 + K0 * N;
+        {
+            Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W));
+            gemm_f32(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, 0);
+        }
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        potrf32_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - W1;
+        if (R2 > 0) {
+            const float* P2 = L + (K1 + W1) + K0 * N;
+            Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W));
+            gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
+        }
+        hipStreamWaitEvent(s, c->ev_p, 0);
+    }
+}
+
+static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const double* Dinv, int64_t N, int64_t I0,
+                          int64_t H, float* X) {
+    for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
+        const int64_t i = r0 / 64, Wc = r0 + 64;
+        {
+            Prof p(c, "trtri_diag_f64", 2.0 * 64 * 64 * Wc, 8.0 * 64 * Wc, st);
+            gemm_mixed_f64f32(st, 0, 0, 64, Wc, 64, 1.0, Dinv + i * 64 * 64, 64, X + r0, N, 0.0, X + r0, N, 0);
+        }
+        const int64_t Mr = I0 + H - (r0 + 64);
+        if (Mr > 0) {
+            Prof p(c, "trtri_panel_f32", 2.0 * Mr * Wc * 64, 8.0 * Mr * Wc, st);
+            gemm_f32(st, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0, 0);
+        }
+    }
+}
+
+static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X) {
+    hipStream_t s = c->stream, s2 = c->side;
+    set_identity_lower_f32(s, X, N, N);
+    trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
+    for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
+        const int64_t H = std::min<int64_t>(kOuterNB, N - I0);
+        const int64_t I1 = I0 + H, R = N - I1;
+        if (R <= 0) break;
+        const int64_t H1 = std::min<int64_t>(kOuterNB, R);
+        const int64_t Wc = I0 + H;
+        {
+            Prof p(c, "trtri_update_f32", 2.0 * H1 * Wc * H, 4.0 * (2.0 * H1 * Wc + Wc * H));
+            gemm_f32(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, 0);
+        }
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        trtri32_panel(c, s2, L, Dinv, N, I1, H1, X);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - H1;
+        if (R2 > 0) {
+            Prof p(c, "trtri_update_f32", 2.0 * R2 * Wc * H, 4.0 * (2.0 * R2 * Wc + Wc * H));
+            gemm_f32(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0, 0);
+        }
+        hipStreamWaitEvent(s, c->ev_p, 0);
     }
 }
 
@@ -498,7 +538,21 @@ static int fit_gaussian(gpl_model* m) {
     lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
     potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
-    solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+    if (m->opts.flags & GPL_FLAG_INVERSE) {
+        // alpha = S X^T X S z with X = L^-1 (S = 1/sigma)
+        ENSURE(m, B_XINV, (size_t)N * N * 8);
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        m->have_xinv = true;
+        Prof p(c, "trmv", 4.0 * N * N / 2, 8.0 * N * N);
+        fill_f64(s, m->d(B_W1), N, 0.0);
+        vec_mul(s, n, m->d(B_SV), m->d(B_Z), m->d(B_W1));
+        trmv_lower_f64(s, m->d(B_XINV), N, N, m->d(B_W1), m->d(B_W2), 0, nullptr);
+        trmv_lower_f64(s, m->d(B_XINV), N, N, m->d(B_W2), m->d(B_W1), 1, nullptr);
+        fill_f64(s, m->d(B_ALPHA), N, 0.0);
+        vec_mul(s, n, m->d(B_SV), m->d(B_W1), m->d(B_ALPHA));
+    } else {
+        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+    }
     {
         Prof p(c, "kmatvec", 0, 0);
         launch_kmatvec(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV),
@@ -516,6 +570,86 @@ static int fit_gaussian(gpl_model* m) {
     r = read_scalars(m, h, 3);
     if (r) return r;
     // log_lik - 1/2 f^T alpha - 1/2 (logdet C - logdet V), logdet C - logdet V = 2 sum log L_ii
+    m->logZ = h[0] - 0.5 * h[1] - h[2];
+    m->iters = 1;
+    return GPL_OK;
+}
+
+// Gaussian fit in the mixed-precision form.  alpha = S B^-1 S y is obtained by
+// iterative refinement: corrections come from the FP32 inverse factor
+// (d = X^T X r), residuals r = S y - B u are formed in FP64 with a matrix-free
+// K*v that re-evaluates the covariance from the inputs, so the converged alpha
+// and f = K alpha carry FP64 accuracy.
+static int fit_gaussian_mixed(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const double sigma = m->lik.sigma;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    float* L = m->b[B_L].as<float>();
+    ENSURE(m, B_XINV, (size_t)N * N * 8);
+    float* X = m->b[B_XINV].as<float>();
+    {
+        Prof p(c, "build_B_f32", 0, 4.0 * N * (N + 64) / 2);
+        launch_build_sym_f32(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1, 1.0 / sigma, nullptr, L, N);
+    }
+    lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
+                      m->d(B_SV), m->b[B_INFO].as<int>() + 1);
+    potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
+    trtri32_blocked(c, L, m->d(B_DINV), N, X);
+    m->have_xinv = true;
+    double* rhs = m->d(B_TERM);
+    double* u = m->d(B_W1);
+    double* r = m->d(B_W2);
+    double* d = m->d(B_W3);
+    double* t1 = m->d(B_CAVM);
+    double* Ku = m->d(B_FMEAN);
+    fill_f64(s, rhs, N, 0.0);
+    vec_mul(s, n, m->d(B_SV), m->d(B_Z), rhs);  // S y
+    fill_f64(s, u, N, 0.0);
+    HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
+    const int maxit = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
+    int it = 0;
+    double rel = INFINITY;
+    for (it = 1; it <= maxit; ++it) {
+        {
+            Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
+            trmv_lower_f32(s, X, N, N, r, t1, 0);
+            trmv_lower_f32(s, X, N, N, t1, d, 1);
+        }
+        vec_axpby(s, n, 1.0, d, 1.0, u, u);
+        // r = S y - u - S K (S u), K including the jitter; Ku = K (S u) = f for alpha = S u
+        vec_mul(s, n, m->d(B_SV), u, t1);
+        {
+            Prof p(c, "kmatvec", 0, 0);
+            launch_kmatvec(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, t1, Ku, m->d(B_KMV), kmatvec_chunks(N));
+        }
+        vec_mul(s, n, m->d(B_SV), Ku, r);
+        vec_axpby(s, n, -1.0, r, -1.0, u, r);
+        vec_axpby(s, n, 1.0, r, 1.0, rhs, r);
+        rel_maxabs(s, d, u, n, m->d(B_SCAL) + 8);
+        double h;
+        HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        rel = h;
+        if (rel < 1e-14) break;
+    }
+    m->refine_iters_done = std::min(it, maxit);
+    m->refine_rel = rel;
+    fill_f64(s, m->d(B_ALPHA), N, 0.0);
+    vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
+    // f = K alpha is the last K (S u) (u updated before it)
+    lik_loglik(s, GPL_LIK_GAUSSIAN, 0, sigma, n, m->d(B_FMEAN), m->d(B_Y), nullptr,
+               m->has_offset ? m->d(B_OFF) : nullptr, m->d(B_W3));
+    double* sc = m->d(B_SCAL);
+    reduce_sum(s, m->d(B_W3), n, sc + 0);
+    reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
+    reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+    int rr = check_info(m);
+    if (rr) return rr;
+    double h[3];
+    rr = read_scalars(m, h, 3);
+    if (rr) return rr;
     m->logZ = h[0] - 0.5 * h[1] - h[2];
     m->iters = 1;
     return GPL_OK;
@@ -792,8 +926,12 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     upload_vec(m, B_TR, trials, n);
     upload_vec(m, B_OFF, offset, n);
     hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
+    if (o.precision == GPL_PREC_MIXED && !(lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_LAPLACE)) {
+        if (fresh) delete m;
+        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Gaussian likelihood");
+    }
     if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
-    else if (lik->kind == GPL_LIK_GAUSSIAN) r = fit_gaussian(m);
+    else if (lik->kind == GPL_LIK_GAUSSIAN) r = (o.precision == GPL_PREC_MIXED) ? fit_gaussian_mixed(m) : fit_gaussian(m);
     else r = fit_laplace(m);
     *inout = m;
     if (log_evidence) *log_evidence = m->logZ;
@@ -815,26 +953,38 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
     const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
     const int ng = m->prog.ngrad;
     ENSURE(m, B_PART, (size_t)ntiles * (ng > 0 ? ng : 1) * 8);
-    if (!m->have_xinv) {
-        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
-        m->have_xinv = true;
-    }
-    {
-        Prof p(c, "lauum", (double)N * N * N / 3, 8.0 * N * N);
-        gemm_f64(s, 1, 0, N, N, N, 1.0, m->d(B_XINV), N, m->d(B_XINV), N, 0.0, m->d(B_BINV), N, 2);
-    }
-    m->have_binv = true;
+    const bool mixed = m->opts.precision == GPL_PREC_MIXED;
     const double sigma = m->lik.sigma;
     const double s2 = sigma * sigma;
-    {
+    if (mixed) {
+        float* X = m->b[B_XINV].as<float>();
+        float* Bi = m->b[B_BINV].as<float>();
+        {
+            Prof p(c, "lauum_f32", (double)N * N * N / 3, 4.0 * N * N);
+            gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0);
+        }
+        Prof p(c, "grad", 0, 4.0 * N * (N + 64) / 2);
+        launch_grad_tiles_f32(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, Bi, N, 1.0 / s2,
+                              m->d(B_ALPHA), m->d(B_PART));
+    } else {
+        if (!m->have_xinv) {
+            trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+            m->have_xinv = true;
+        }
+        {
+            Prof p(c, "lauum", (double)N * N * N / 3, 8.0 * N * N);
+            gemm2_f64(s, 1, 0, N, N, N, 1.0, m->d(B_XINV), N, m->d(B_XINV), N, 0.0, m->d(B_BINV), N, 2, 0);
+        }
         Prof p(c, "grad", 0, 8.0 * N * (N + 64) / 2);
         launch_grad_tiles(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, m->d(B_BINV), N, 1.0 / s2,
                           m->d(B_ALPHA), m->d(B_PART));
     }
+    m->have_binv = true;
     double* sc = m->d(B_SCAL);
     reduce_cols(s, m->d(B_PART), ntiles, ng, ng, sc);
     reduce_dot(s, m->d(B_ALPHA), m->d(B_ALPHA), n, sc + ng);
-    colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);
+    if (mixed) colsumsq_f32(s, m->b[B_XINV].as<float>(), N, N, n, m->d(B_W3), 1);
+    else colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);
     reduce_sum(s, m->d(B_W3), n, sc + ng + 1);
     std::vector<double> h(ng + 2);
     HIP_TRY(hipMemcpyAsync(h.data(), sc, (ng + 2) * 8, hipMemcpyDeviceToHost, s));
@@ -874,20 +1024,23 @@ extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want
             double* A2 = (double*)c->buf(4, (size_t)N * M * 8);
             if (!A2) return fail(GPL_ERR_OOM, "device allocation failed");
             Prof p(c, "pred_trmm", (double)N * N * M, 8.0 * N * M * 2);
-            gemm_f64(s, 0, 0, N, M, N, 1.0, m->d(B_XINV), N, KtT, N, 0.0, A2, N, 3);
+            if (m->opts.precision == GPL_PREC_MIXED)
+                gemm_mixed_f32f64f64(s, 0, 0, N, M, N, 1.0, m->b[B_XINV].as<float>(), N, KtT, N, 0.0, A2, N, 3);
+            else
+                gemm2_f64(s, 0, 0, N, M, N, 1.0, m->d(B_XINV), N, KtT, N, 0.0, A2, N, 3, 0);
             KtT = A2;
         } else {
             for (int64_t i = 0; i < T; ++i) {
                 {
                     Prof p(c, "pred_trsm_diag", 2.0 * 64 * 64 * M, 16.0 * 64 * M);
-                    gemm_f64(s, 0, 0, 64, M, 64, 1.0, m->d(B_DINV) + i * 64 * 64, 64, KtT + i * 64, N, 0.0,
-                             KtT + i * 64, N, 0);
+                    gemm2_f64(s, 0, 0, 64, M, 64, 1.0, m->d(B_DINV) + i * 64 * 64, 64, KtT + i * 64, N, 0.0,
+                              KtT + i * 64, N, 0, 0);
                 }
                 if (i + 1 < T) {
                     const int64_t R = (T - 1 - i) * 64;
                     Prof p(c, "pred_trsm_update", 2.0 * R * M * 64, 16.0 * R * M);
-                    gemm_f64(s, 0, 0, R, M, 64, -1.0, m->d(B_L) + (i + 1) * 64 + i * 64 * N, N, KtT + i * 64, N, 1.0,
-                             KtT + (i + 1) * 64, N, 0);
+                    gemm2_f64(s, 0, 0, R, M, 64, -1.0, m->d(B_L) + (i + 1) * 64 + i * 64 * N, N, KtT + i * 64, N,
+                              1.0, KtT + (i + 1) * 64, N, 0, 0);
                 }
             }
         }

@@ -177,7 +177,16 @@ __device__ double prog_dparam(const DevProg* __restrict__ P, int p, const double
 // Features: raw columns -> feature columns (periodic: sin/cos and d/dlog p)
 // ---------------------------------------------------------------------------
 __global__ void k_features(const double* __restrict__ x, int64_t n, int64_t ldx, const DevProg* __restrict__ P,
-                           double* __restrict__ F, int64_t ldf, int64_t N) {
+                           double* __restrict__ F, int64_t ldf, int64_t N, const int64_t* __restrict__ xoff,
+                           const int64_t* __restrict__ nvec, int64_t sF) {
+    const int64_t b = blockIdx.y;  // batch: problem b has n = nvec[b] rows at x + xoff[b]
+    if (nvec) {
+        n = nvec[b];
+        ldx = n;
+        x += xoff[b];
+        P += b;
+        F += b * sF;
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int nt = P->nterms;
@@ -207,7 +216,13 @@ void launch_features(hipStream_t s, const double* x, int64_t n, int64_t ldx, int
     (void)prog;
     const int64_t N = ldf;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, x, n, ldx, prog_dev, F, ldf,
-                       N);
+                       N, (const int64_t*)nullptr, (const int64_t*)nullptr, (int64_t)0);
+}
+
+void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,
+                             const DevProg* progs, double* F, int64_t N, int64_t sF, int batch) {
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, x, (int64_t)0,
+                       (int64_t)0, progs, F, N, N, xoff, nvec, sF);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,10 +232,20 @@ template <typename OutT>
 __global__ __launch_bounds__(256) void k_build_sym(const DevProg* __restrict__ P, const double* __restrict__ F,
                                                    int64_t ldf, int nfeat, int64_t n, double jitter, int mode,
                                                    double sscal, const double* __restrict__ svec,
-                                                   OutT* __restrict__ out, int64_t ld) {
+                                                   OutT* __restrict__ out, int64_t ld, int64_t sF, int64_t sO,
+                                                   const int64_t* __restrict__ nvec,
+                                                   const double* __restrict__ scal) {
     extern __shared__ double sm[];
     double* Fr = sm;               // [nfeat][64]
     double* Fc = sm + nfeat * 64;  // [nfeat][64]
+    {
+        const int64_t b = blockIdx.y;  // batch
+        P += (nvec ? b : 0);
+        F += b * sF;
+        out += b * sO;
+        if (nvec) n = nvec[b];
+        if (scal) sscal = scal[b];
+    }
     int64_t ti, tj;
     lower_tile_c(xcd_remap_c(blockIdx.x, gridDim.x), ti, tj);
     const int t = threadIdx.x, i = t & 63, cg = t >> 6;
@@ -260,10 +285,22 @@ static void build_sym_launch(hipStream_t s, const DevProg* prog, const DevProg& 
     dim3 grid((unsigned)(T * (T + 1) / 2));
     if (out)
         hipLaunchKernelGGL(k_build_sym<double>, grid, dim3(256), shm, s, prog, F, N, nfeat, n, jitter, mode, sscal,
-                           svec, out, ld);
+                           svec, out, ld, (int64_t)0, (int64_t)0, (const int64_t*)nullptr, (const double*)nullptr);
     else
         hipLaunchKernelGGL(k_build_sym<float>, grid, dim3(256), shm, s, prog, F, N, nfeat, n, jitter, mode, sscal,
-                           svec, out32, ld);
+                           svec, out32, ld, (int64_t)0, (int64_t)0, (const int64_t*)nullptr, (const double*)nullptr);
+}
+
+// batch of problems of one padded size N: B_b = I + s_b (K_b + jitter I) s_b in FP32
+void launch_build_B_f32_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
+                                const int64_t* nvec, int64_t N, double jitter, const double* scal, float* out,
+                                int64_t sO, int batch) {
+    const int64_t T = N / 64;
+    nfeat = nfeat > 0 ? nfeat : 1;
+    const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
+    hipLaunchKernelGGL(k_build_sym<float>, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), shm, s,
+                       progs, F, N, nfeat, (int64_t)0, jitter, 1, 1.0, (const double*)nullptr, out, N, sF, sO, nvec,
+                       scal);
 }
 
 void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
@@ -347,7 +384,16 @@ int kmatvec_chunks(int64_t N) {
 
 __global__ __launch_bounds__(256) void k_kmatvec(const DevProg* __restrict__ P, const double* __restrict__ F,
                                                  int64_t ldf, int nfeat, int64_t n, int64_t T, int nchunk,
-                                                 const double* __restrict__ v, double* __restrict__ part) {
+                                                 const double* __restrict__ v, double* __restrict__ part,
+                                                 int64_t sF, int64_t sV, const int64_t* __restrict__ nvec) {
+    {
+        const int64_t b = blockIdx.z;  // batch
+        P += (nvec ? b : 0);
+        F += b * sF;
+        v += b * sV;
+        part += b * (int64_t)nchunk * T * 64;
+        if (nvec) n = nvec[b];
+    }
     extern __shared__ double sm[];
     double* Fr = sm;
     double* Fc = sm + nfeat * 64;
@@ -386,7 +432,15 @@ __global__ __launch_bounds__(256) void k_kmatvec(const DevProg* __restrict__ P, 
 }
 
 __global__ void k_kmatvec_fin(const double* __restrict__ part, int64_t N, int nchunk, int64_t n, double jitter,
-                              const double* __restrict__ v, double* __restrict__ y) {
+                              const double* __restrict__ v, double* __restrict__ y, int64_t sV,
+                              const int64_t* __restrict__ nvec) {
+    {
+        const int64_t b = blockIdx.y;
+        part += b * (int64_t)nchunk * N;
+        v += b * sV;
+        y += b * sV;
+        if (nvec) n = nvec[b];
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double s = 0.0;
@@ -400,9 +454,22 @@ void launch_kmatvec(hipStream_t s, const DevProg* prog, const DevProg& hp, const
     const int nfeat = hp.nfeat > 0 ? hp.nfeat : 1;
     const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
     hipLaunchKernelGGL(k_kmatvec, dim3((unsigned)T, (unsigned)nchunk), dim3(256), shm, s, prog, F, N, nfeat, n, T,
-                       nchunk, v, part);
+                       nchunk, v, part, (int64_t)0, (int64_t)0, (const int64_t*)nullptr);
     hipLaunchKernelGGL(k_kmatvec_fin, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, part, N, nchunk, n, jitter,
-                       v, y);
+                       v, y, (int64_t)0, (const int64_t*)nullptr);
+}
+
+// batch: part must hold batch * nchunk * N doubles
+void launch_kmatvec_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
+                            const int64_t* nvec, int64_t N, double jitter, const double* v, double* y, int64_t sV,
+                            double* part, int nchunk, int batch) {
+    const int64_t T = N / 64;
+    nfeat = nfeat > 0 ? nfeat : 1;
+    const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
+    hipLaunchKernelGGL(k_kmatvec, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), shm, s, progs, F,
+                       N, nfeat, (int64_t)0, T, nchunk, v, part, sF, sV, nvec);
+    hipLaunchKernelGGL(k_kmatvec_fin, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, N,
+                       nchunk, (int64_t)0, jitter, v, y, sV, nvec);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,7 +521,19 @@ template <typename WT, int KIND, int NF>
 __global__ __launch_bounds__(256) void k_grad_fast(const DevProg* __restrict__ P, const double* __restrict__ F,
                                                    int64_t ldf, int nfeat, int64_t n, const WT* __restrict__ W,
                                                    int64_t ldw, double wscale, const double* __restrict__ alpha,
-                                                   double* __restrict__ part, int ngp) {
+                                                   double* __restrict__ part, int ngp, int64_t sF, int64_t sW,
+                                                   int64_t sV, const int64_t* __restrict__ nvec,
+                                                   const double* __restrict__ wsc) {
+    {
+        const int64_t b = blockIdx.y;  // batch
+        P += (nvec ? b : 0);
+        F += b * sF;
+        W += b * sW;
+        alpha += b * sV;
+        part += b * (int64_t)gridDim.x * ngp;
+        if (nvec) n = nvec[b];
+        if (wsc) wscale = wsc[b];
+    }
     extern __shared__ double sm[];
     double* Fr = sm;
     double* Fc = sm + nfeat * 64;
@@ -511,7 +590,19 @@ template <typename WT>
 __global__ __launch_bounds__(256) void k_grad_generic(const DevProg* __restrict__ P, const double* __restrict__ F,
                                                       int64_t ldf, int nfeat, int64_t n, const WT* __restrict__ W,
                                                       int64_t ldw, double wscale, const double* __restrict__ alpha,
-                                                      double* __restrict__ part, int ngp) {
+                                                      double* __restrict__ part, int ngp, int64_t sF, int64_t sW,
+                                                      int64_t sV, const int64_t* __restrict__ nvec,
+                                                      const double* __restrict__ wsc) {
+    {
+        const int64_t b = blockIdx.y;  // batch
+        P += (nvec ? b : 0);
+        F += b * sF;
+        W += b * sW;
+        alpha += b * sV;
+        part += b * (int64_t)gridDim.x * ngp;
+        if (nvec) n = nvec[b];
+        if (wsc) wscale = wsc[b];
+    }
     extern __shared__ double sm[];
     double* Fr = sm;
     double* Fc = sm + nfeat * 64;
@@ -561,11 +652,13 @@ __global__ __launch_bounds__(256) void k_grad_generic(const DevProg* __restrict_
 
 template <typename WT>
 static void grad_launch(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
-                        int64_t N, const WT* W, int64_t ldw, double wscale, const double* alpha, double* part) {
+                        int64_t N, const WT* W, int64_t ldw, double wscale, const double* alpha, double* part,
+                        int batch = 1, int64_t sF = 0, int64_t sW = 0, int64_t sV = 0, const int64_t* nvec = nullptr,
+                        const double* wsc = nullptr) {
     const int64_t T = N / 64;
     const int nfeat = hp.nfeat > 0 ? hp.nfeat : 1;
     const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
-    dim3 grid((unsigned)(T * (T + 1) / 2)), block(256);
+    dim3 grid((unsigned)(T * (T + 1) / 2), (unsigned)batch), block(256);
     const int ngp = hp.ngrad;
     const DevTerm& t0 = hp.t[0];
     const bool fast = hp.nterms == 1 &&
@@ -575,7 +668,7 @@ static void grad_launch(hipStream_t s, const DevProg* prog, const DevProg& hp, c
         const int nfp = t0.nf <= 1 ? 1 : t0.nf <= 2 ? 2 : t0.nf <= 4 ? 4 : t0.nf <= 8 ? 8 : 16;
 #define GPL_GRAD_FAST(KD, NFv)                                                                                   \
     hipLaunchKernelGGL((k_grad_fast<WT, KD, NFv>), grid, block, shm, s, prog, F, N, nfeat, n, W, ldw, wscale, alpha, \
-                       part, ngp)
+                       part, ngp, sF, sW, sV, nvec, wsc)
 #define GPL_GRAD_NF(KD)                            \
     switch (nfp) {                                 \
         case 1: GPL_GRAD_FAST(KD, 1); break;       \
@@ -591,7 +684,7 @@ static void grad_launch(hipStream_t s, const DevProg* prog, const DevProg& hp, c
 #undef GPL_GRAD_FAST
     } else {
         hipLaunchKernelGGL(k_grad_generic<WT>, grid, block, shm, s, prog, F, N, nfeat, n, W, ldw, wscale, alpha,
-                           part, ngp);
+                           part, ngp, sF, sW, sV, nvec, wsc);
     }
 }
 
@@ -604,6 +697,13 @@ void launch_grad_tiles_f32(hipStream_t s, const DevProg* prog, const DevProg& hp
                            int64_t N, const float* W, int64_t ldw, double wscale, const double* alpha,
                            double* part) {
     grad_launch<float>(s, prog, hp, F, n, N, W, ldw, wscale, alpha, part);
+}
+
+// batch (all problems share the term structure of hp): part holds batch * ntiles * ngrad
+void launch_grad_tiles_f32_batched(hipStream_t s, const DevProg* progs, const DevProg& hp, const double* F,
+                                   int64_t sF, const int64_t* nvec, int64_t N, const float* W, int64_t sW,
+                                   const double* wsc, const double* alpha, int64_t sV, double* part, int batch) {
+    grad_launch<float>(s, progs, hp, F, 0, N, W, N, 1.0, alpha, part, batch, sF, sW, sV, nvec, wsc);
 }
 
 }  // namespace gpl

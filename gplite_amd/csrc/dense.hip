@@ -30,62 +30,90 @@ __device__ __forceinline__ void lower_tile(int64_t t, int64_t& ti, int64_t& tj) 
 }
 
 // ---------------------------------------------------------------------------
-// GEMM: C = beta*C + alpha * op(A) op(B).  64x64 output tile per 256-thread
-// workgroup; each wave owns a 32x32 quadrant as 2x2 v_mfma_f64_16x16x4 blocks.
-// Operands are read straight from global memory in MFMA operand layout.  Within
-// a 16-wide k chunk, lane group g = lane/16 takes k = 4g..4g+3 over the four
-// MFMA steps, so a transposed operand is a contiguous 32-byte read per lane and
-// a non-transposed one a 128-byte coalesced read per 16 lanes.
+// LDS-staged GEMM: BM x BM output tile per 256-thread workgroup (BM = 128 or 64),
+// 16-wide k chunks double-buffered in LDS as [k][m] (global -> registers for
+// chunk c+1 is issued before the MFMAs of chunk c and written to LDS after
+// them).  Each wave owns a (BM/2)^2 quadrant of v_mfma_f64_16x16x4 blocks.
+// M and N are multiples of 64; a 128 tile whose second half falls outside is
+// masked (zero loads, skipped stores).
 // ---------------------------------------------------------------------------
-template <int TA, int TB>
-__device__ __forceinline__ void gemm_load(const double* A, int64_t lda,
-                                          const double* B, int64_t ldb, int64_t row0,
-                                          int64_t col0, int64_t k0, int li, int g, double (&a)[2][4],
-                                          double (&b)[2][4]) {
-    const int64_t kk = k0 + 4 * g;
+template <typename T> struct vec2_of;
+template <> struct vec2_of<double> { typedef double2 type; };
+template <> struct vec2_of<float> { typedef float2 type; };
+
+// LDS tiles are [k][BM + 1]: the odd row stride keeps the eight lanes that
+// write one column's k values on distinct banks.
+template <int BM, int MCONT, typename T>
+__device__ __forceinline__ void lds_gemm_load(const T* P, int64_t ld, int64_t base, int64_t lim, int64_t k0,
+                                              double2 (&r)[BM / 64 * 2]) {
+    typedef typename vec2_of<T>::type V2;
+    constexpr int AL = BM / 64 * 2;
+    const int t = threadIdx.x;
+    if (MCONT) {  // element (m, k) at P[m + k*ld]: pairs along m, 128-byte rows per 8 lanes
+        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = 256 / (BM / 2);
+        const bool ok = base + mp < lim;
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        const int64_t r = row0 + bi * 16 + li;
-        if (TA) {
-            const double2* p = reinterpret_cast<const double2*>(A + kk + r * lda);
-            double2 u = p[0], v = p[1];
-            a[bi][0] = u.x; a[bi][1] = u.y; a[bi][2] = v.x; a[bi][3] = v.y;
-        } else {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) a[bi][s] = A[r + (kk + s) * lda];
+        for (int j = 0; j < AL; ++j) {
+            const int64_t k = k0 + kq + ks * j;
+            if (ok) {
+                const V2 v = *reinterpret_cast<const V2*>(P + base + mp + k * ld);
+                r[j] = double2{(double)v.x, (double)v.y};
+            } else {
+                r[j] = double2{0.0, 0.0};
+            }
         }
-    }
+    } else {  // element (m, k) at P[k + m*ld]: 8 lanes read one column's 16 k contiguously
+        const int kq = t & 7, m = t >> 3;
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-        const int64_t c = col0 + bj * 16 + li;
-        if (!TB) {
-            const double2* p = reinterpret_cast<const double2*>(B + kk + c * ldb);
-            double2 u = p[0], v = p[1];
-            b[bj][0] = u.x; b[bj][1] = u.y; b[bj][2] = v.x; b[bj][3] = v.y;
-        } else {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) b[bj][s] = B[c + (kk + s) * ldb];
+        for (int j = 0; j < AL; ++j) {
+            const int64_t mm = base + m + 32 * j;
+            if (mm < lim) {
+                const V2 v = *reinterpret_cast<const V2*>(P + (k0 + 2 * kq) + mm * ld);
+                r[j] = double2{(double)v.x, (double)v.y};
+            } else {
+                r[j] = double2{0.0, 0.0};
+            }
         }
     }
 }
 
-__device__ __forceinline__ void gemm_mma(const double (&a)[2][4], const double (&b)[2][4], d4 (&acc)[2][2]) {
+template <int BM, int MCONT>
+__device__ __forceinline__ void lds_gemm_store(double (*S)[BM + 1], const double2 (&r)[BM / 64 * 2]) {
+    constexpr int AL = BM / 64 * 2;
+    const int t = threadIdx.x;
+    if (MCONT) {
+        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = 256 / (BM / 2);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+        for (int j = 0; j < AL; ++j) {
+            S[kq + ks * j][mp] = r[j].x;
+            S[kq + ks * j][mp + 1] = r[j].y;
+        }
+    } else {
+        const int kq = t & 7, m = t >> 3;
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-                acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][s], b[bj][s], acc[bi][bj], 0, 0, 0);
+        for (int j = 0; j < AL; ++j) {
+            S[2 * kq][m + 32 * j] = r[j].x;
+            S[2 * kq + 1][m + 32 * j] = r[j].y;
+        }
+    }
 }
 
-template <int TA, int TB>
-__global__ __launch_bounds__(256) void k_gemm_f64(int64_t tilesM, int64_t K, double alpha,
-                                                  const double* A, int64_t lda,
-                                                  const double* B, int64_t ldb, double beta,
-                                                  double* C, int64_t ldc, int tiles) {
-    const int64_t nb = gridDim.x;
-    const int64_t bid = xcd_remap(blockIdx.x, nb);
+// C = beta*C + alpha*op(A) op(B), FP64 MFMA; A, B and C may be stored as float or double.
+template <int TA, int TB, int BM, typename TAs, typename TBs, typename TCs>
+__global__ __launch_bounds__(256) void k_gemm_lds(int64_t M, int64_t N, int64_t K, int64_t tilesM, double alpha,
+                                                  const TAs* A, int64_t lda, const TBs* B, int64_t ldb,
+                                                  double beta, TCs* C, int64_t ldc, int tiles, int64_t sA,
+                                                  int64_t sB, int64_t sC) {
+    constexpr int WT = BM / 2, NBK = WT / 16, AL = BM / 64 * 2;
+    A += blockIdx.y * sA;  // batch: one problem per grid row
+    B += blockIdx.y * sB;
+    C += blockIdx.y * sC;
+    __shared__ double As[2][16][BM + 1];
+    __shared__ double Bs[2][16][BM + 1];
+    // Contiguous tile ranges per XCD share operand panels in L2; with a
+    // row-dependent k range (tiles == 2) work per tile varies by 100x, so the
+    // hardware's round-robin placement is kept there to balance the XCDs.
+    const int64_t bid = (tiles == 2) ? (int64_t)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
     int64_t ti, tj;
     if (tiles == 0 || tiles == 3) {
         ti = bid % tilesM;
@@ -93,143 +121,273 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int64_t tilesM, int64_t K, dou
     } else {
         lower_tile(bid, ti, tj);
     }
+    const int64_t row0 = ti * BM, col0 = tj * BM;
+    const int64_t kbeg = (tiles == 2) ? ti * BM : 0;
+    const int64_t kend = (tiles == 3 && (ti + 1) * BM < K) ? (ti + 1) * BM : K;
+    const int64_t nch = kend > kbeg ? (kend - kbeg) / 16 : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g = lane >> 4;
-    const int64_t row0 = ti * 64 + (wave >> 1) * 32;
-    const int64_t col0 = tj * 64 + (wave & 1) * 32;
-    const int64_t kbeg = (tiles == 2) ? ti * 64 : 0;
-    if (tiles == 3 && (ti + 1) * 64 < K) K = (ti + 1) * 64;
+    const int wr = wave >> 1, wc = wave & 1;
 
-    d4 acc[2][2];
+    d4 acc[NBK][NBK];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NBK; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < NBK; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    double a0[2][4], b0[2][4], a1[2][4], b1[2][4];
-    int64_t k0 = kbeg;
-    if (k0 < K) gemm_load<TA, TB>(A, lda, B, ldb, row0, col0, k0, li, g, a0, b0);
-    while (k0 < K) {
-        if (k0 + 16 < K) gemm_load<TA, TB>(A, lda, B, ldb, row0, col0, k0 + 16, li, g, a1, b1);
-        gemm_mma(a0, b0, acc);
-        k0 += 16;
-        if (k0 >= K) break;
-        if (k0 + 16 < K) gemm_load<TA, TB>(A, lda, B, ldb, row0, col0, k0 + 16, li, g, a0, b0);
-        gemm_mma(a1, b1, acc);
-        k0 += 16;
+    double2 ra[AL], rb[AL];
+    if (nch > 0) {
+        lds_gemm_load<BM, !TA>(A, lda, row0, M, kbeg, ra);
+        lds_gemm_load<BM, TB>(B, ldb, col0, N, kbeg, rb);
+        lds_gemm_store<BM, !TA>(As[0], ra);
+        lds_gemm_store<BM, TB>(Bs[0], rb);
     }
-
-    // C may alias an operand (in-place panel solve): every wave finishes reading first.
     __syncthreads();
+    for (int64_t c = 0; c < nch; ++c) {
+        const int buf = (int)(c & 1);
+        const bool more = c + 1 < nch;
+        if (more) {
+            lds_gemm_load<BM, !TA>(A, lda, row0, M, kbeg + (c + 1) * 16, ra);
+            lds_gemm_load<BM, TB>(B, ldb, col0, N, kbeg + (c + 1) * 16, rb);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kk = 4 * g + s;
+            double a[NBK], b[NBK];
+#pragma unroll
+            for (int i = 0; i < NBK; ++i) a[i] = As[buf][kk][wr * WT + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < NBK; ++j) b[j] = Bs[buf][kk][wc * WT + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < NBK; ++i)
+#pragma unroll
+                for (int j = 0; j < NBK; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            lds_gemm_store<BM, !TA>(As[buf ^ 1], ra);
+            lds_gemm_store<BM, TB>(Bs[buf ^ 1], rb);
+        }
+        __syncthreads();
+    }
     // v_mfma_f64_16x16x4 D layout: lane holds rows g, g+4, g+8, g+12 of column li
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int i = 0; i < NBK; ++i)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int64_t c = col0 + bj * 16 + li;
-            const int64_t r = row0 + bi * 16 + g;
-            double* p = C + r + c * ldc;
+        for (int j = 0; j < NBK; ++j) {
+            const int64_t rb0 = row0 + wr * WT + i * 16, cb0 = col0 + wc * WT + j * 16;
+            if (rb0 >= M || cb0 >= N) continue;
+            TCs* p = C + (rb0 + g) + (cb0 + li) * ldc;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const double o = alpha * acc[bi][bj][v];
-                p[4 * v] = (beta == 0.0) ? o : beta * p[4 * v] + o;
+                const double o = alpha * acc[i][j][v];
+                p[4 * v] = (TCs)((beta == 0.0) ? o : beta * (double)p[4 * v] + o);
             }
         }
 }
 
-void gemm_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
-              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
-              int64_t ldc, int tiles) {
-    const int64_t tm = M / 64, tn = N / 64;
-    if (tm <= 0 || tn <= 0) return;
+template <int BM, typename TAs, typename TBs, typename TCs>
+static void gemm_lds_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                            const TAs* A, int64_t lda, const TBs* B, int64_t ldb, double beta, TCs* C,
+                            int64_t ldc, int tiles, int batch = 1, int64_t sA = 0, int64_t sB = 0,
+                            int64_t sC = 0) {
+    const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
     const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
-    dim3 grid((unsigned)nblk), block(256);
-    if (K <= 0) {  // pure scaling (alpha * 0 + beta*C)
-        if (beta == 1.0) return;
-    }
-    if (!ta && !tb)
-        hipLaunchKernelGGL((k_gemm_f64<0, 0>), grid, block, 0, s, tm, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
-    else if (!ta && tb)
-        hipLaunchKernelGGL((k_gemm_f64<0, 1>), grid, block, 0, s, tm, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
-    else if (ta && !tb)
-        hipLaunchKernelGGL((k_gemm_f64<1, 0>), grid, block, 0, s, tm, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
-    else
-        hipLaunchKernelGGL((k_gemm_f64<1, 1>), grid, block, 0, s, tm, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    dim3 grid((unsigned)nblk, (unsigned)batch), block(256);
+#define GPL_GL(TAv, TBv)                                                                                        \
+    hipLaunchKernelGGL((k_gemm_lds<TAv, TBv, BM, TAs, TBs, TCs>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, \
+                       B, ldb, beta, C, ldc, tiles, sA, sB, sC)
+    if (!ta && !tb) GPL_GL(0, 0);
+    else if (!ta && tb) GPL_GL(0, 1);
+    else if (ta && !tb) GPL_GL(1, 0);
+    else GPL_GL(1, 1);
+#undef GPL_GL
+}
+
+// 64x64 tiles measured faster than 128x128 for every shape on the path (the
+// 128 variant holds one wave per SIMD); the 128 variant stays selectable.
+static int pick_bm(int64_t M, int64_t N, int tiles) {
+    (void)M;
+    (void)N;
+    (void)tiles;
+    return 64;
+}
+
+void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+               int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles, int bm) {
+    if (M <= 0 || N <= 0) return;
+    if (bm == 0) bm = pick_bm(M, N, tiles);
+    if (bm == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+}
+
+// FP64-compute GEMM on FP32 storage for A/C with an FP64 B (panel solves of the mixed path)
+void gemm_mixed_f32f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
+                       int64_t lda, const double* B, int64_t ldb, double beta, float* C, int64_t ldc, int tiles) {
+    if (M <= 0 || N <= 0) return;
+    if (pick_bm(M, N, tiles) == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+}
+
+// FP64 B on the left: C(float) = alpha * A(double) op B(float)
+void gemm_mixed_f64f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                       const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc,
+                       int tiles) {
+    if (M <= 0 || N <= 0) return;
+    if (pick_bm(M, N, tiles) == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+}
+
+void gemm_mixed_f32f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                               const float* A, int64_t lda, const double* B, int64_t ldb, double beta, float* C,
+                               int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+}
+
+void gemm_mixed_f64f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                               const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C,
+                               int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+}
+
+void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                          const float* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                          int64_t ldc, int tiles) {
+    if (M <= 0 || N <= 0) return;
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block: right-looking Cholesky of a 64x64 block held in LDS (one
-// barrier per column), then its inverse by column-parallel substitution with
-// four lanes per column exchanging pivots through cross-lane shuffles.
+// Diagonal block (64x64), FP64 arithmetic on FP64 or FP32 storage.
+// Thread (row i = t%64, wave w = t/64) keeps elements (i, w+4q), q < 16, in
+// registers for the whole factorisation.  Step j: the wave owning column j
+// broadcasts the pivot with readlane, scales its column and publishes the
+// multipliers through a double-buffered LDS vector; after one barrier every
+// thread applies the rank-1 update to its own 16 elements.  The inverse
+// (written to Dinv, FP64) is then formed column-parallel with four lanes per
+// column exchanging pivots by DPP quad broadcasts.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int64_t ld, int64_t k,
-                                                    double* __restrict__ Dinv, double* __restrict__ ldiag,
-                                                    int* __restrict__ info) {
-    __shared__ double a[64][65];  // a[col][row]
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int KQ>
+__device__ __forceinline__ double quad_bcast_f64(double v) {
+    constexpr int ctrl = KQ | (KQ << 2) | (KQ << 4) | (KQ << 6);  // quad_perm [KQ,KQ,KQ,KQ]
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t ld, int64_t k,
+                                                     double* __restrict__ Dinv, double* __restrict__ ldiag,
+                                                     int* __restrict__ info, int want_inv, int64_t sA, int64_t sD,
+                                                     int64_t sL) {
+    // batch: one problem per workgroup
+    A += blockIdx.x * sA;
+    Dinv += blockIdx.x * sD;
+    ldiag += blockIdx.x * sL;
+    info += blockIdx.x * ((sA != 0) ? 1 : 0);
+    __shared__ double colbuf[2][64];
+    __shared__ double aL[64][65];  // factor, a[col][row], for the inverse
     __shared__ double rinv[64];
-    double* blk = A + k * 64 * (ld + 1);
-    const int t = threadIdx.x;
-    const int i = t & 63, grp = t >> 6;
+    __shared__ double piv[64];
+    T* blk = A + k * 64 * (ld + 1);
+    const int t = threadIdx.x, i = t & 63, w = t >> 6;
+    double a[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = grp + 4 * q;
-        a[c][i] = (i >= c) ? blk[i + (int64_t)c * ld] : 0.0;
+        const int c = w + 4 * q;
+        a[q] = (c <= i) ? (double)blk[i + (int64_t)c * ld] : 0.0;
     }
-    __syncthreads();
-    for (int j = 0; j < 64; ++j) {
-        const double d = a[j][j];
-        const double r = sqrt(d);
-        const double inv = 1.0 / r;
-        if (t == 0 && !(d > 0.0)) atomicCAS(info, 0, (int)(k * 64 + j + 1));
-        const double lij = a[j][i] * inv;
-        if (i > j) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int c = grp + 4 * q;
-                if (c > j && c <= i) a[c][i] -= lij * (a[j][c] * inv);
+    for (int j = 0; j < 64; ++j) {
+        const int wj = j & 3, qj = j >> 2;
+        if (w == wj) {
+            const double d = readlane_f64(a[qj], j);
+            const double rt = sqrt(d);
+            const double inv = 1.0 / rt;
+            const double l = a[qj] * inv;
+            a[qj] = (i > j) ? l : ((i == j) ? rt : 0.0);
+            colbuf[j & 1][i] = (i > j) ? l : 0.0;
+            if (i == j) {
+                rinv[j] = inv;
+                piv[j] = d;
             }
         }
         __syncthreads();
-        if (grp == 0) {
-            if (i > j) a[j][i] = lij;
-            else if (i == j) { a[j][j] = r; rinv[j] = inv; }
+        const double lij = colbuf[j & 1][i];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (w + 4 * q > j) a[q] -= lij * colbuf[j & 1][w + 4 * q];
         }
     }
-    __syncthreads();
-    // store L (upper part of the tile zeroed) and log of the diagonal
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = grp + 4 * q;
-        blk[i + (int64_t)c * ld] = (i >= c) ? a[c][i] : 0.0;
+        const int c = w + 4 * q;
+        const double v = (c <= i) ? a[q] : 0.0;
+        blk[i + (int64_t)c * ld] = (T)v;
+        aL[c][i] = (c < i) ? v : 0.0;  // strictly lower part drives the substitution
     }
-    if (t < 64) ldiag[k * 64 + t] = log(a[t][t]);
     __syncthreads();
-    if (t < 64) a[t][t] = 0.0;  // strictly lower part drives the substitution below
-    __syncthreads();
-
-    // inverse: column c = 16*wave + lane/4; this lane owns rows 4r+q
+    if (t < 64) {
+        ldiag[k * 64 + t] = 0.5 * log(piv[t]);
+        const unsigned long long bad = __ballot(!(piv[t] > 0.0));
+        if (t == 0 && bad) atomicCAS(info, 0, (int)(k * 64 + __ffsll((long long)bad)));
+    }
+    if (!want_inv) return;
+    // inverse: column c = 16*w + lane/4, this lane owns rows 4r + q
     const int lane = t & 63;
-    const int c = grp * 16 + (lane >> 2);
+    const int c = w * 16 + (lane >> 2);
     const int q = lane & 3;
     double b[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) b[r] = (4 * r + q == c) ? 1.0 : 0.0;
 #pragma unroll
     for (int kk = 0; kk < 64; ++kk) {
-        const int kr = kk >> 2, kq = kk & 3;
-        double xk = (q == kq) ? b[kr] * rinv[kk] : 0.0;
-        xk = __shfl(xk, (lane & ~3) | kq, 64);
-        if (q == kq) b[kr] = xk;
+        const int kr = kk >> 2;
+        constexpr int dummy = 0;
+        (void)dummy;
+        double xk = b[kr] * rinv[kk];
+        switch (kk & 3) {
+            case 0: xk = quad_bcast_f64<0>(xk); break;
+            case 1: xk = quad_bcast_f64<1>(xk); break;
+            case 2: xk = quad_bcast_f64<2>(xk); break;
+            default: xk = quad_bcast_f64<3>(xk); break;
+        }
+        if (q == (kk & 3)) b[kr] = xk;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) b[r] -= a[kk][4 * r + q] * xk;
+        for (int r = 0; r < 16; ++r) b[r] -= aL[kk][4 * r + q] * xk;
     }
     double* D = Dinv + k * 64 * 64;
 #pragma unroll
     for (int r = 0; r < 16; ++r) D[(4 * r + q) + c * 64] = b[r];
 }
 
-void potrf_diag_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info) {
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info);
+void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                     int want_inv) {
+    hipLaunchKernelGGL(k_potrf_diag3<double>, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, want_inv,
+                       (int64_t)0, (int64_t)0, (int64_t)0);
+}
+
+void potrf_diag3_f32(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                     int want_inv) {
+    hipLaunchKernelGGL(k_potrf_diag3<float>, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, want_inv,
+                       (int64_t)0, (int64_t)0, (int64_t)0);
+}
+
+// batch of problems, each with its own info slot (info[b])
+void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                             int batch, int64_t sA, int64_t sD, int64_t sL) {
+    hipLaunchKernelGGL(k_potrf_diag3<float>, dim3((unsigned)batch), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, 1,
+                       sA, sD, sL);
 }
 
 // ---------------------------------------------------------------------------

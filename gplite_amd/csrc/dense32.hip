@@ -1,0 +1,323 @@
+// FP32 dense kernels for the mixed-precision form on gfx950: an LDS-staged
+// GEMM on v_mfma_f32_32x32x2f32 with FP32 accumulation (trailing updates,
+// triangular-inverse updates, B^-1 = X^T X), and FP32-matrix / FP64-vector
+// helpers used by iterative refinement.
+#include "gpl_internal.h"
+
+#include <cmath>
+
+namespace gpl {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int64_t xcd_remap32(int64_t b, int64_t nb) {
+    const int64_t per = nb / 8, rem = nb % 8;
+    const int64_t x = b % 8, q = b / 8;
+    return (x < rem) ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
+}
+
+__device__ __forceinline__ void lower_tile32(int64_t t, int64_t& ti, int64_t& tj) {
+    int64_t r = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    ti = r;
+    tj = t - r * (r + 1) / 2;
+}
+
+// chunk of 16 k x BM m, FP32, into registers
+template <int BM, int MCONT>
+__device__ __forceinline__ void g32_load(const float* P, int64_t ld, int64_t base, int64_t lim, int64_t k0,
+                                         float4 (&r)[BM / 64]) {
+    constexpr int AL = BM / 64;
+    const int t = threadIdx.x;
+    if (MCONT) {  // (m, k) at P[m + k*ld]: float4 along m
+        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = 256 / (BM / 4);
+        const bool ok = base + m4 < lim;
+#pragma unroll
+        for (int j = 0; j < AL; ++j) {
+            const int64_t k = k0 + kq + ks * j;
+            r[j] = ok ? *reinterpret_cast<const float4*>(P + base + m4 + k * ld) : float4{0.f, 0.f, 0.f, 0.f};
+        }
+    } else {  // (m, k) at P[k + m*ld]: four lanes read one column's 16 k
+        const int kq = t & 3, m = t >> 2;
+#pragma unroll
+        for (int j = 0; j < AL; ++j) {
+            const int64_t mm = base + m + 64 * j;
+            r[j] = (mm < lim) ? *reinterpret_cast<const float4*>(P + (k0 + 4 * kq) + mm * ld)
+                              : float4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+}
+
+template <int BM, int MCONT>
+__device__ __forceinline__ void g32_store(float (*S)[BM + 4], const float4 (&r)[BM / 64]) {
+    constexpr int AL = BM / 64;
+    const int t = threadIdx.x;
+    if (MCONT) {
+        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = 256 / (BM / 4);
+#pragma unroll
+        for (int j = 0; j < AL; ++j) *reinterpret_cast<float4*>(&S[kq + ks * j][m4]) = r[j];
+    } else {
+        const int kq = t & 3, m = t >> 2;
+#pragma unroll
+        for (int j = 0; j < AL; ++j) {
+            S[4 * kq + 0][m + 64 * j] = r[j].x;
+            S[4 * kq + 1][m + 64 * j] = r[j].y;
+            S[4 * kq + 2][m + 64 * j] = r[j].z;
+            S[4 * kq + 3][m + 64 * j] = r[j].w;
+        }
+    }
+}
+
+// C = beta*C + alpha*op(A) op(B) in FP32.  Tile modes as gemm2_f64.
+// Each wave owns a (BM/2)^2 quadrant of 32x32 blocks; lane half h = lane/32
+// takes k = 8h..8h+7 of a 16-wide chunk over eight MFMA steps.
+template <int TA, int TB, int BM>
+__global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
+                                                const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                                                float* C, int64_t ldc, int tiles, int64_t sA, int64_t sB,
+                                                int64_t sC) {
+    constexpr int WT = BM / 2, NBK = WT / 32, AL = BM / 64;
+    A += blockIdx.y * sA;  // batch: one problem per grid row
+    B += blockIdx.y * sB;
+    C += blockIdx.y * sC;
+    __shared__ float As[2][16][BM + 4];
+    __shared__ float Bs[2][16][BM + 4];
+    const int64_t bid = (tiles == 2) ? (int64_t)blockIdx.x : xcd_remap32(blockIdx.x, gridDim.x);
+    int64_t ti, tj;
+    if (tiles == 0 || tiles == 3) {
+        ti = bid % tilesM;
+        tj = bid / tilesM;
+    } else {
+        lower_tile32(bid, ti, tj);
+    }
+    const int64_t row0 = ti * BM, col0 = tj * BM;
+    const int64_t kbeg = (tiles == 2) ? ti * BM : 0;
+    const int64_t kend = (tiles == 3 && (ti + 1) * BM < K) ? (ti + 1) * BM : K;
+    const int64_t nch = kend > kbeg ? (kend - kbeg) / 16 : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    f16v acc[NBK][NBK];
+#pragma unroll
+    for (int i = 0; i < NBK; ++i)
+#pragma unroll
+        for (int j = 0; j < NBK; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+    float4 ra[AL], rb[AL];
+    if (nch > 0) {
+        g32_load<BM, !TA>(A, lda, row0, M, kbeg, ra);
+        g32_load<BM, TB>(B, ldb, col0, N, kbeg, rb);
+        g32_store<BM, !TA>(As[0], ra);
+        g32_store<BM, TB>(Bs[0], rb);
+    }
+    __syncthreads();
+    for (int64_t c = 0; c < nch; ++c) {
+        const int buf = (int)(c & 1);
+        const bool more = c + 1 < nch;
+        if (more) {
+            g32_load<BM, !TA>(A, lda, row0, M, kbeg + (c + 1) * 16, ra);
+            g32_load<BM, TB>(B, ldb, col0, N, kbeg + (c + 1) * 16, rb);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int kk = 8 * h + s;
+            float a[NBK], b[NBK];
+#pragma unroll
+            for (int i = 0; i < NBK; ++i) a[i] = As[buf][kk][wr * WT + i * 32 + l32];
+#pragma unroll
+            for (int j = 0; j < NBK; ++j) b[j] = Bs[buf][kk][wc * WT + j * 32 + l32];
+#pragma unroll
+            for (int i = 0; i < NBK; ++i)
+#pragma unroll
+                for (int j = 0; j < NBK; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            g32_store<BM, !TA>(As[buf ^ 1], ra);
+            g32_store<BM, TB>(Bs[buf ^ 1], rb);
+        }
+        __syncthreads();
+    }
+    // v_mfma_f32_32x32x2f32 D layout: item v -> row 8*(v/4) + 4*(lane/32) + v%4, column lane%32
+#pragma unroll
+    for (int i = 0; i < NBK; ++i)
+#pragma unroll
+        for (int j = 0; j < NBK; ++j) {
+            const int64_t rb0 = row0 + wr * WT + i * 32, cb0 = col0 + wc * WT + j * 32;
+            if (rb0 >= M || cb0 >= N) continue;
+            float* p = C + rb0 + (cb0 + l32) * ldc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int r = 8 * (v >> 2) + 4 * h + (v & 3);
+                const float o = alpha * acc[i][j][v];
+                p[r] = (beta == 0.f) ? o : beta * p[r] + o;
+            }
+        }
+}
+
+template <int BM>
+static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha,
+                          const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+                          int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
+    const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
+    const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
+    dim3 grid((unsigned)nblk, (unsigned)batch), block(256);
+#define GPL_G32(TAv, TBv)                                                                                           \
+    hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, beta, C, ldc, \
+                       tiles, sA, sB, sC)
+    if (!ta && !tb) GPL_G32(0, 0);
+    else if (!ta && tb) GPL_G32(0, 1);
+    else if (ta && !tb) GPL_G32(1, 0);
+    else GPL_G32(1, 1);
+#undef GPL_G32
+}
+
+void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+                      int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
+                      int batch, int64_t sA, int64_t sB, int64_t sC) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    if (bm == 0) {
+        const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
+        const int64_t n128 = ((tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2) * batch;
+        bm = (n128 >= 256) ? 128 : 64;
+    }
+    if (bm == 128) gemm32_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+    else gemm32_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+}
+
+void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm) {
+    gemm_f32_batched(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, bm, 1, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-matrix helpers with FP64 vectors
+// ---------------------------------------------------------------------------
+__global__ void k_set_identity_lower32(float* __restrict__ A, int64_t ld, int64_t sA) {
+    A += blockIdx.y * sA;
+    int64_t ti, tj;
+    lower_tile32(blockIdx.x, ti, tj);
+    const int t = threadIdx.x, i = t & 63;
+    for (int c = t >> 6; c < 64; c += 4) {
+        const int64_t gi = ti * 64 + i, gj = tj * 64 + c;
+        A[gi + gj * ld] = (gi == gj) ? 1.f : 0.f;
+    }
+}
+
+void set_identity_lower_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_set_identity_lower32, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), 0, s, A,
+                       ld, sA);
+}
+
+void set_identity_lower_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
+    set_identity_lower_f32_batched(s, A, ld, N, 1, 0);
+}
+
+// y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32, vectors FP64
+__global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int64_t ld, int64_t T,
+                                                const double* __restrict__ b, double* __restrict__ y, int trans,
+                                                int64_t sX, int64_t sv) {
+    __shared__ double red[256];
+    X += blockIdx.y * sX;
+    b += blockIdx.y * sv;
+    y += blockIdx.y * sv;
+    const int64_t ti = blockIdx.x;
+    const int t = threadIdx.x, e = t & 63, p = t >> 6;
+    double s = 0.0;
+    if (!trans) {
+        for (int64_t tj = 0; tj <= ti; ++tj) {
+            const float* Xt = X + ti * 64 + tj * 64 * ld;
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int c = p * 16 + u;
+                s += (double)Xt[e + (int64_t)c * ld] * b[tj * 64 + c];
+            }
+        }
+    } else {
+        for (int64_t tr = ti; tr < T; ++tr) {
+            const float* Xt = X + tr * 64 + ti * 64 * ld;
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int r = p * 16 + u;
+                s += (double)Xt[r + (int64_t)e * ld] * b[tr * 64 + r];
+            }
+        }
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < 64) y[ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+}
+
+void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
+                            int trans, int batch, int64_t sX, int64_t sv) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_trmv32, dim3((unsigned)T, (unsigned)batch), dim3(256), 0, s, X, ld, T, b, y, trans, sX, sv);
+}
+
+void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans) {
+    trmv_lower_f32_batched(s, X, ld, N, b, y, trans, 1, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_colsumsq32(const float* __restrict__ A, int64_t ld, int64_t nrow,
+                                                    int64_t ncol, double* __restrict__ out, int lower, int64_t sA,
+                                                    int64_t sO) {
+    A += blockIdx.y * sA;
+    out += blockIdx.y * sO;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= ncol) return;
+    const float* col = A + j * ld;
+    double s = 0.0;
+    for (int64_t i = (lower ? (j / 64) * 64 : 0) + lane; i < nrow; i += 64) {
+        const double v = col[i];
+        s += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[j] = s;
+}
+
+void colsumsq_f32_batched(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
+                          int lower, int batch, int64_t sA, int64_t sO) {
+    if (ncol <= 0) return;
+    hipLaunchKernelGGL(k_colsumsq32, dim3((unsigned)((ncol + 3) / 4), (unsigned)batch), dim3(256), 0, s, A, ld, nrow,
+                       ncol, out, lower, sA, sO);
+}
+
+void colsumsq_f32(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out, int lower) {
+    colsumsq_f32_batched(s, A, ld, nrow, ncol, out, lower, 1, 0, 0);
+}
+
+// max |d| / max |u| (one workgroup): the refinement stopping test
+__global__ __launch_bounds__(1024) void k_relmax(const double* __restrict__ d, const double* __restrict__ u,
+                                                 int64_t n, double* __restrict__ out) {
+    __shared__ double rd[1024], ru[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int64_t i = t; i < n; i += 1024) {
+        a = fmax(a, fabs(d[i]));
+        b = fmax(b, fabs(u[i]));
+    }
+    rd[t] = a;
+    ru[t] = b;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) {
+            rd[t] = fmax(rd[t], rd[t + o]);
+            ru[t] = fmax(ru[t], ru[t + o]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[0] = (ru[0] > 0.0) ? rd[0] / ru[0] : rd[0];
+}
+
+void rel_maxabs(hipStream_t s, const double* d, const double* u, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_relmax, dim3(1), dim3(1024), 0, s, d, u, n, out);
+}
+
+}  // namespace gpl

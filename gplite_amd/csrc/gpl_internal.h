@@ -78,13 +78,31 @@ void launch_grad_tiles_f32(hipStream_t s, const DevProg* prog, const DevProg& hp
 // ta/tb: operand stored transposed.  tiles: 0 = full grid, 1 = lower tiles of a square C,
 // 2 = lower tiles and k starts at the row tile (C = X^T X with X lower triangular),
 // 3 = full grid and k stops after the row tile (C = X B with X lower triangular).
-void gemm_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
-              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
-              int64_t ldc, int tiles);
-// Factor the 64x64 diagonal block at A + k*64*(ld+1) in place; write its inverse to
-// Dinv + k*64*64 and log(L_ii) to ldiag + k*64; info = first failing row + 1.
-void potrf_diag_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag,
-                    int* info);
+// bm: output tile edge (64 or 128; 0 = choose by grid size).
+void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+               int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles, int bm);
+// FP64 MFMA on FP32 storage: C(f32) = beta*C + alpha op(A(f32)) op(B(f64)) and the mirror case.
+void gemm_mixed_f32f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
+                       int64_t lda, const double* B, int64_t ldb, double beta, float* C, int64_t ldc, int tiles);
+void gemm_mixed_f64f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                       const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc,
+                       int tiles);
+void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                          const float* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                          int64_t ldc, int tiles);
+// FP32 storage and FP32 MFMA with FP32 accumulation (dense32.hip)
+void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm);
+void set_identity_lower_f32(hipStream_t s, float* A, int64_t ld, int64_t N);
+void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans);
+void colsumsq_f32(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out, int lower);
+void rel_maxabs(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
+// Factor the 64x64 diagonal block at A + k*64*(ld+1) in place (FP64 arithmetic); write its
+// inverse to Dinv + k*64*64 (FP64) and log(L_ii) to ldiag + k*64; info = first failing row + 1.
+void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                     int want_inv);
+void potrf_diag3_f32(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                     int want_inv);
 // triangular solve with the blocked factor (L lower, Dinv diagonal inverses), in place on b.
 void trsv_lower_f64(hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N,
                     double* b, double* x, int trans);
@@ -132,5 +150,43 @@ void vec_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, co
                double* out);
 void vec_mul(hipStream_t s, int64_t n, const double* a, const double* b, double* out);
 void vec_recip(hipStream_t s, int64_t n, const double* a, double* out);
+
+// ---- batched forms (one problem per grid row / workgroup, stride per problem) ----
+void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,
+                             const DevProg* progs, double* F, int64_t N, int64_t sF, int batch);
+void launch_build_B_f32_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
+                                const int64_t* nvec, int64_t N, double jitter, const double* scal, float* out,
+                                int64_t sO, int batch);
+void launch_kmatvec_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
+                            const int64_t* nvec, int64_t N, double jitter, const double* v, double* y, int64_t sV,
+                            double* part, int nchunk, int batch);
+void launch_grad_tiles_f32_batched(hipStream_t s, const DevProg* progs, const DevProg& hp, const double* F,
+                                   int64_t sF, const int64_t* nvec, int64_t N, const float* W, int64_t sW,
+                                   const double* wsc, const double* alpha, int64_t sV, double* part, int batch);
+void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+                      int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
+                      int batch, int64_t sA, int64_t sB, int64_t sC);
+void gemm_mixed_f32f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                               const float* A, int64_t lda, const double* B, int64_t ldb, double beta, float* C,
+                               int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC);
+void gemm_mixed_f64f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                               const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C,
+                               int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC);
+void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
+                             int batch, int64_t sA, int64_t sD, int64_t sL);
+void set_identity_lower_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA);
+void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
+                            int trans, int batch, int64_t sX, int64_t sv);
+void colsumsq_f32_batched(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
+                          int lower, int batch, int64_t sA, int64_t sO);
+void seg_scale(hipStream_t s, const double* in, const double* scal, const int64_t* nvec, int64_t N, double* out,
+               int batch);
+void seg_dot(hipStream_t s, const double* a, const double* b, const int64_t* nvec, int64_t N, double* out,
+             int ostride, int slot, int batch);
+void seg_cols(hipStream_t s, const double* part, int64_t ntiles, int ng, double* out, int ostride, int slot0,
+              int batch);
+void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
+void batch_finish(hipStream_t s, const double* red, const int64_t* nvec, const double* sigma, const int32_t* idx, int ng,
+                  int nb, double* logz, double* grad, int gstride);
 
 }  // namespace gpl

@@ -1,0 +1,162 @@
+// Host-side pieces shared by the C-ABI translation units: device context,
+// per-kernel-class event profiling, device buffers and error reporting.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpl_internal.h"
+
+namespace gpl {
+
+int fail(int code, const std::string& msg);
+void set_minor(int64_t minor);
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::gpl::fail(GPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P);
+
+}  // namespace gpl
+
+using gpl::fail;
+
+inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
+
+// ---------------------------------------------------------------------------
+// context, profiling
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+
+struct ProfClass {
+    std::string name;
+    double ms = 0, flops = 0, bytes = 0;
+    int64_t launches = 0;
+};
+
+struct gpl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
+    hipEvent_t ev_a = nullptr, ev_p = nullptr;
+    bool prof = false;
+    std::vector<ProfRec> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<ProfClass> classes;
+    std::map<std::string, int> cls_index;
+    std::vector<void*> scratch;
+    std::vector<size_t> scratch_sz;
+    std::mutex mu;
+
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    void resolve() {
+        if (pending.empty()) return;
+        hipEventSynchronize(pending.back().b);
+        for (auto& r : pending) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, r.a, r.b);
+            auto& c = classes[r.cls];
+            c.ms += ms;
+            c.launches += 1;
+            c.flops += r.flops;
+            c.bytes += r.bytes;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    int cls(const char* name) {
+        auto it = cls_index.find(name);
+        if (it != cls_index.end()) return it->second;
+        int id = (int)classes.size();
+        classes.push_back(ProfClass{name});
+        cls_index[name] = id;
+        return id;
+    }
+    void* buf(int slot, size_t bytes) {
+        if ((int)scratch.size() <= slot) {
+            scratch.resize(slot + 1, nullptr);
+            scratch_sz.resize(slot + 1, 0);
+        }
+        if (scratch_sz[slot] < bytes) {
+            if (scratch[slot]) hipFree(scratch[slot]);
+            scratch[slot] = nullptr;
+            scratch_sz[slot] = 0;
+            if (hipMalloc(&scratch[slot], bytes) != hipSuccess) return nullptr;
+            scratch_sz[slot] = bytes;
+        }
+        return scratch[slot];
+    }
+};
+
+// Times one launch group on the library stream when profiling is on.
+struct Prof {
+    gpl_ctx* c;
+    ProfRec r;
+    bool on;
+    hipStream_t st;
+    Prof(gpl_ctx* ctx, const char* name, double flops, double bytes, hipStream_t stream = nullptr)
+        : c(ctx), on(ctx->prof), st(stream ? stream : ctx->stream) {
+        if (!on) return;
+        r.cls = c->cls(name);
+        r.a = c->ev();
+        r.b = c->ev();
+        r.flops = flops;
+        r.bytes = bytes;
+        hipEventRecord(r.a, st);
+    }
+    ~Prof() {
+        if (!on) return;
+        hipEventRecord(r.b, st);
+        c->pending.push_back(r);
+        if (c->pending.size() > 8192) c->resolve();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// model
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t sz = 0;
+    void free_() {
+        if (p) hipFree(p);
+        p = nullptr;
+        sz = 0;
+    }
+    bool ensure(size_t bytes) {
+        if (sz >= bytes && p) return true;
+        free_();
+        if (bytes == 0) bytes = 8;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        sz = bytes;
+        return true;
+    }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+

@@ -275,3 +275,127 @@ void reduce_cols(hipStream_t s, const double* part, int64_t rows, int64_t ld, in
 }
 
 }  // namespace gpl
+
+namespace gpl {
+
+// ---------------------------------------------------------------------------
+// Segmented (one segment per batched problem, stride N) helpers
+// ---------------------------------------------------------------------------
+// out[b*N + i] = (i < n_b) ? scal[b] * (in ? in[b*N + i] : 1) : 0
+__global__ void k_seg_scale(const double* __restrict__ in, const double* __restrict__ scal,
+                            const int64_t* __restrict__ nvec, int64_t N, double* __restrict__ out) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t o = b * N + i;
+    out[o] = (i < nvec[b]) ? scal[b] * (in ? in[o] : 1.0) : 0.0;
+}
+
+void seg_scale(hipStream_t s, const double* in, const double* scal, const int64_t* nvec, int64_t N, double* out,
+               int batch) {
+    hipLaunchKernelGGL(k_seg_scale, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, in, scal,
+                       nvec, N, out);
+}
+
+// per segment: out[b*ostride + slot] = sum_i a_i (b == null) or sum_i a_i b_i over i < n_b
+__global__ __launch_bounds__(256) void k_seg_dot(const double* __restrict__ a, const double* __restrict__ bv,
+                                                 const int64_t* __restrict__ nvec, int64_t N,
+                                                 double* __restrict__ out, int ostride, int slot) {
+    __shared__ double red[256];
+    const int64_t b = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t n = nvec[b];
+    const double* pa = a + b * N;
+    const double* pb = bv ? bv + b * N : nullptr;
+    double s = 0.0;
+    for (int64_t i = t; i < n; i += 256) s += pb ? pa[i] * pb[i] : pa[i];
+    red[t] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) out[b * ostride + slot] = red[0];
+}
+
+void seg_dot(hipStream_t s, const double* a, const double* b, const int64_t* nvec, int64_t N, double* out,
+             int ostride, int slot, int batch) {
+    hipLaunchKernelGGL(k_seg_dot, dim3((unsigned)batch), dim3(256), 0, s, a, b, nvec, N, out, ostride, slot);
+}
+
+// out[b*ostride + slot0 + p] = sum_t part[(b*ntiles + t)*ng + p]
+__global__ __launch_bounds__(256) void k_seg_cols(const double* __restrict__ part, int64_t ntiles, int ng,
+                                                  double* __restrict__ out, int ostride, int slot0) {
+    __shared__ double red[256];
+    const int64_t b = blockIdx.y;
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    const double* pp = part + b * ntiles * ng;
+    double s = 0.0;
+    for (int64_t r = t; r < ntiles; r += 256) s += pp[r * ng + p];
+    red[t] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) out[b * ostride + slot0 + p] = red[0];
+}
+
+void seg_cols(hipStream_t s, const double* part, int64_t ntiles, int ng, double* out, int ostride, int slot0,
+              int batch) {
+    hipLaunchKernelGGL(k_seg_cols, dim3((unsigned)ng, (unsigned)batch), dim3(256), 0, s, part, ntiles, ng, out, ostride,
+                       slot0);
+}
+
+// max over all entries of |d| / max |u| (whole batch)
+__global__ __launch_bounds__(1024) void k_relmax_all(const double* __restrict__ d, const double* __restrict__ u,
+                                                     int64_t n, double* __restrict__ out) {
+    __shared__ double rd[1024], ru[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, c = 0.0;
+    for (int64_t i = t; i < n; i += 1024) {
+        a = fmax(a, fabs(d[i]));
+        c = fmax(c, fabs(u[i]));
+    }
+    rd[t] = a;
+    ru[t] = c;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) {
+            rd[t] = fmax(rd[t], rd[t + o]);
+            ru[t] = fmax(ru[t], ru[t + o]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[0] = fmax(out[0], (ru[0] > 0.0) ? rd[0] / ru[0] : rd[0]);
+}
+
+void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_relmax_all, dim3(1), dim3(1024), 0, s, d, u, n, out);
+}
+
+// Per problem: log evidence and gradient from the segment sums
+// red[b*8 + 0] = y.alpha, 1 = sum log L_ii, 2 = alpha.alpha, 3 = tr(B^-1), 4.. = gradient partial sums
+__global__ void k_batch_finish(const double* __restrict__ red, const int64_t* __restrict__ nvec,
+                               const double* __restrict__ sigma, const int32_t* __restrict__ idx, int ng, int nb,
+                               double* __restrict__ logz, double* __restrict__ grad, int gstride) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const double* r = red + (int64_t)b * 8;
+    const double n = (double)nvec[b];
+    const double sg = sigma[b], s2 = sg * sg;
+    const int p = idx[b];
+    // log N(y | 0, K + sigma^2 I) with K + sigma^2 I = sigma^2 B:  -1/2 y^T alpha - sum log L_ii - n log sigma - n/2 log 2 pi
+    logz[p] = -0.5 * r[0] - r[1] - n * log(sg) - 0.5 * n * kLog2Pi;
+    for (int q = 0; q < ng; ++q) grad[(int64_t)p * gstride + q] = -0.5 * r[4 + q];
+    grad[(int64_t)p * gstride + ng] = -s2 * r[2] + r[3];
+}
+
+void batch_finish(hipStream_t s, const double* red, const int64_t* nvec, const double* sigma, const int32_t* idx, int ng,
+                  int nb, double* logz, double* grad, int gstride) {
+    hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, red, nvec, sigma, idx, ng, nb,
+                       logz, grad, gstride);
+}
+
+}  // namespace gpl

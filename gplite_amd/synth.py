@@ -67,3 +67,17 @@ def c3(n=4096, d=4, seed=43004):
     del K
     y = (r.uniform(size=n) < 1 / (1 + np.exp(-f))).astype(float)
     return dict(x=x, y=y, lscale=0.8, magn=2.0)
+
+
+def c4(nprob=2048, nmin=64, nmax=2048, seed=44096):
+    """nprob independent regressions: n ~ round(exp(U[ln nmin, ln nmax])), d ~ U{1..8},
+    x ~ U[-2,2]^d, lscale ~ U[0.3, 1.5], magn = 1, y = sin(sum x) + 0.1 N(0,1), sigma = 0.1."""
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(nprob):
+        n = int(round(np.exp(r.uniform(np.log(nmin), np.log(nmax)))))
+        d = int(r.integers(1, 9))
+        x = r.uniform(-2, 2, (n, d))
+        y = np.sin(x.sum(1)) + 0.1 * r.standard_normal(n)
+        out.append(dict(x=x, y=y, lscale=float(r.uniform(0.3, 1.5)), magn=1.0, sigma=0.1))
+    return out

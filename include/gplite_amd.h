@@ -93,8 +93,13 @@ typedef struct gpl_opts {
     int32_t precision;    /* GPL_PREC_FP64 (plain) or GPL_PREC_MIXED (FP32 MFMA + FP64 refinement) */
     double jitter;        /* 1e-6 (R/gp.R:276-281) */
     int32_t refine_iters; /* mixed precision: max iterative-refinement sweeps (default 8) */
-    int32_t flags;        /* reserved, 0 */
+    int32_t flags;        /* GPL_FLAG_* bits */
 } gpl_opts;
+
+/* gpl_opts.flags: form L^-1 during a Gaussian fit (alpha then comes from two
+ * triangular matrix-vector products, and a following gpl_energy_grad or
+ * gpl_pred reuses it).  Set it when the gradient will be requested. */
+#define GPL_FLAG_INVERSE 1
 
 void gpl_opts_default(gpl_opts* o);
 
@@ -172,6 +177,36 @@ int gpl_model_vector(gpl_model* model, int32_t which, double* out /* n */);
 int gpl_model_chol(gpl_model* model, double* out);
 int gpl_model_info(gpl_model* model, int64_t* n, int32_t* iters, double* log_evidence);
 void gpl_model_free(gpl_model* model);
+
+/* ---- batched form: many independent Gaussian-likelihood GPs ------------- */
+/* Energy and gradient of `nprob` independent GPs of uneven n in one device
+ * pass (new: the reference fits one GP per R call).  All problems share the
+ * covariance structure (term kinds, ops, n_lscale); inputs, n, d and
+ * hyperparameters differ.  terms is nprob x nterms (row = problem).  Problems
+ * are bucketed by padded size and each bucket runs the mixed-precision
+ * pipeline (opts->refine_iters fixed refinement sweeps, default 4); the whole
+ * pass is captured in one hipGraph at the first call and replayed. */
+typedef struct gpl_batch gpl_batch;
+int gpl_batch_create(gpl_ctx* ctx, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
+                     const double* const* x, const int64_t* n, const int64_t* ncols,
+                     const double* const* y, const double* sigma, const gpl_opts* opts,
+                     gpl_batch** out);
+/* New hyperparameters (same structure) for every problem. */
+int gpl_batch_set_params(gpl_batch* batch, const gpl_cf_term* terms, const double* sigma);
+/* log_evidence[nprob]; grad[nprob * gpl_batch_grad_len()] in the layout of gpl_grad_len. */
+int gpl_batch_energy_grad(gpl_batch* batch, double* log_evidence, double* grad);
+int32_t gpl_batch_grad_len(gpl_batch* batch);
+/* Number of buckets and the largest relative last refinement correction. */
+int gpl_batch_info(gpl_batch* batch, int32_t* nbuckets, double* refine_rel);
+/* Autotune: times GEMM tile edge {64,128} x outer panel {128,256} per bucket,
+ * keeps the fastest; writes up to cap rows (bucket n, tile, panel, ms) and the
+ * number of buckets. */
+int gpl_batch_tune(gpl_batch* batch, int32_t cap, int32_t* bucket_n, int32_t* bm, int32_t* nbo,
+                   double* best_ms, int32_t* nbuckets);
+/* Apply a persisted tuning table. */
+int gpl_batch_set_tuning(gpl_batch* batch, int32_t count, const int32_t* bucket_n, const int32_t* bm,
+                         const int32_t* nbo);
+void gpl_batch_free(gpl_batch* batch);
 
 #ifdef __cplusplus
 }
