@@ -95,6 +95,16 @@ def _sig(lib):
                                      C.POINTER(i32)]),
         "gpl_batch_set_tuning": (C.c_int, [_vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "gpl_batch_free": (None, [_vp]),
+        "gpl_server_create": (C.c_int, [i32, i32, i64, C.POINTER(_vp)]),
+        "gpl_server_submit_fit": (C.c_int, [_vp, C.POINTER(CfTerm), i32, _dp, i64, i64, _dp, _dp, _dp,
+                                            C.POINTER(Lik), C.POINTER(Opts), C.POINTER(i64), C.POINTER(i64)]),
+        "gpl_server_submit_pred": (C.c_int, [_vp, i64, _dp, i64, i32, _dp, _dp, C.POINTER(i64)]),
+        "gpl_server_wait": (C.c_int, [_vp, i64, _dp, _dp, _dp, C.POINTER(i32)]),
+        "gpl_server_release_model": (C.c_int, [_vp, i64]),
+        "gpl_server_stats": (C.c_int, [_vp] + [C.POINTER(i64)] * 6),
+        "gpl_server_destroy": (None, [_vp]),
+        "gpl_pred_ctx": (C.c_int, [_vp, _vp, _dp, i64, i32, _dp, _dp]),
+        "gpl_model_bytes": (i64, [_vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)

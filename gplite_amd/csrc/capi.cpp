@@ -995,10 +995,12 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
     return GPL_OK;
 }
 
-extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var) {
-    if (!m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
+// Prediction on the streams of context c (which may differ from the context
+// that fitted the model: the model's device buffers are read-only here).
+int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean,
+                 double* var) {
+    if (!c || !m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
     if (mt <= 0) return GPL_OK;
-    gpl_ctx* c = m->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -1052,6 +1054,23 @@ extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want
     HIP_TRY(hipMemcpyAsync(mean, vbuf, mt * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return GPL_OK;
+}
+
+extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var) {
+    if (!m) return fail(GPL_ERR_ARG, "NULL model");
+    return pred_on(m->ctx, m, xt, mt, want_var, mean, var);
+}
+
+extern "C" int gpl_pred_ctx(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean,
+                            double* var) {
+    return pred_on(c, m, xt, mt, want_var, mean, var);
+}
+
+extern "C" int64_t gpl_model_bytes(gpl_model* m) {
+    if (!m) return 0;
+    int64_t b = 0;
+    for (auto& x : m->b) b += (int64_t)x.sz;
+    return b;
 }
 
 extern "C" int gpl_model_vector(gpl_model* m, int32_t which, double* out) {

@@ -208,6 +208,40 @@ int gpl_batch_set_tuning(gpl_batch* batch, int32_t count, const int32_t* bucket_
                          const int32_t* nbo);
 void gpl_batch_free(gpl_batch* batch);
 
+/* ---- serving form: continuous batching of fit / predict requests -------- */
+/* New (the reference answers one R call at a time).  Worker threads, each with
+ * its own streams, take the oldest request that can run: predictions wait for
+ * their model's fit, and every request waits until its estimated HBM footprint
+ * fits the budget next to the resident models and the work in flight (idle
+ * resident models are evicted least recently used first).  Queued predictions
+ * against one model are served by one launch; every output is computed
+ * independently, so results are the same alone and in a mix.  Inputs are
+ * copied at submission; prediction outputs are written into the caller's
+ * buffers, which must stay valid until gpl_server_wait returns. */
+typedef struct gpl_server gpl_server;
+int gpl_server_create(int32_t device, int32_t n_workers, int64_t mem_budget_bytes, gpl_server** out);
+int gpl_server_submit_fit(gpl_server* srv, const gpl_cf_term* terms, int32_t nterms, const double* x,
+                          int64_t n, int64_t ncols, const double* y, const double* trials,
+                          const double* offset, const gpl_lik* lik, const gpl_opts* opts,
+                          int64_t* model_id, int64_t* ticket);
+int gpl_server_submit_pred(gpl_server* srv, int64_t model_id, const double* xt, int64_t m,
+                           int32_t want_var, double* mean, double* var, int64_t* ticket);
+/* Blocks until done; latency and queueing delay in ms, the fit's log evidence,
+ * and how many requests the serving launch covered. */
+int gpl_server_wait(gpl_server* srv, int64_t ticket, double* latency_ms, double* queue_ms,
+                    double* log_evidence, int32_t* coalesced);
+int gpl_server_release_model(gpl_server* srv, int64_t model_id);
+int gpl_server_stats(gpl_server* srv, int64_t* fits, int64_t* preds, int64_t* launches,
+                     int64_t* evictions, int64_t* bytes_in_use, int64_t* resident_models);
+void gpl_server_destroy(gpl_server* srv);
+
+/* Prediction on the streams of ctx (a model fitted on another context of the
+ * same device can be used concurrently by several contexts). */
+int gpl_pred_ctx(gpl_ctx* ctx, gpl_model* model, const double* xt, int64_t m, int32_t want_var,
+                 double* mean, double* var);
+/* Device bytes held by a fitted model. */
+int64_t gpl_model_bytes(gpl_model* model);
+
 #ifdef __cplusplus
 }
 #endif
