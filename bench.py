@@ -41,7 +41,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,mixed,mixed_batched")
+    ap.add_argument("--restarts", type=int, default=8, help="problems per pass in the batched form")
+    ap.add_argument("--refine", type=int, default=4, help="refinement sweeps in the batched form")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -111,11 +113,52 @@ def main():
         lib.gpl_model_free(handle)
         return res
 
+    def run_batched(nrest):
+        """nrest hyperparameter restarts of the same regression (ARD length-scales
+        perturbed by exp(0.2 N(0,1)); restart 0 keeps the base values) in one
+        batched mixed-precision pass per step."""
+        from gplite_amd.batch import GPBatch
+        rr = np.random.default_rng(7)
+        gps = []
+        for i in range(nrest):
+            ls = w["lscale"] * (np.exp(0.2 * rr.standard_normal(d)) if i else 1.0)
+            gps.append(G.gp_init(G.cf_matern52(lscale=ls, magn=w["magn"]), G.lik_gaussian(sigma=w["sigma"])))
+        Bt = GPBatch(gps, [w["x"]] * nrest, [y] * nrest, refine_iters=args.refine)
+        table = Bt.tune()
+        log(f"[batched] tuning: {table}")
+        ngb = lib.gpl_batch_grad_len(Bt.handle)
+        lz = np.empty(nrest)
+        gb = np.empty(nrest * ngb)
+
+        def step():
+            L.check(lib.gpl_batch_energy_grad(Bt.handle, L.dptr(lz), L.dptr(gb)))
+
+        for i in range(args.warmup):
+            step()
+        lat = []
+        t_all = time.perf_counter()
+        for i in range(args.steps):
+            ts = time.perf_counter()
+            step()
+            lat.append(time.perf_counter() - ts)
+        elapsed = time.perf_counter() - t_all
+        log(f"[mixed_batched] {args.steps} passes x {nrest} problems in {elapsed:.3f}s -> "
+            f"{args.steps * nrest / elapsed:.2f} evals/s; refine rel {Bt.info()['last_refine_rel']:.2e}")
+        L.profile(enable=True, reset=True)
+        step()
+        prof = L.profile(enable=False)
+        lat_ms = np.array(lat) * 1e3
+        res = {"evals_per_s": args.steps * nrest / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
+               "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+               "logz": float(lz[0]), "grad": gb[:ngb].copy(), "prof": prof, "problems": nrest,
+               "tuning": table, "refine_rel": Bt.info()["last_refine_rel"]}
+        del Bt
+        return res
+
     forms = args.forms.split(",")
-    R = {f: run_form(f) for f in forms}
-    head = "mixed" if "mixed" in R else forms[0]
+    R = {f: (run_batched(args.restarts) if f == "mixed_batched" else run_form(f)) for f in forms}
+    head = "mixed_batched" if "mixed_batched" in R else ("mixed" if "mixed" in R else forms[0])
     H = R[head]
-    peak = PEAK["f32"] if head == "mixed" else PEAK["f64"]
     prof = H["prof"]
     mm = {k: v for k, v in prof.items() if v["flops"] > 0}
     dom = max(mm.items(), key=lambda kv: kv[1]["ms"])
@@ -138,6 +181,10 @@ def main():
                                                        / np.max(np.abs(b["pred_mean"]))),
                   "rel_pred_var_mixed_vs_fp64": float(np.max(np.abs(a["pred_var"] - b["pred_var"]))
                                                       / np.max(np.abs(b["pred_var"])))}
+        if "mixed_batched" in R:
+            c = R["mixed_batched"]
+            parity["rel_logz_batched_vs_fp64"] = abs(c["logz"] / b["logz"] - 1)
+            parity["rel_grad_batched_vs_fp64"] = float(np.max(np.abs(c["grad"] - b["grad"])) / np.max(np.abs(b["grad"])))
         log("parity:", parity)
 
     cpu = None
@@ -158,7 +205,7 @@ def main():
         log("cpu baseline:", cpu)
 
     out = {
-        "metric": "GP energy+grad evals/s (n=8192, d=8, Matern-5/2 ARD; mixed-precision form, rel logZ err vs FP64)",
+        "metric": "GP energy+grad evals/s (n=8192, d=8, Matern-5/2 ARD) & p50/p99 latency vs plain FP64, rel logZ err",
         "value": round(H["evals_per_s"], 4),
         "unit": "evals/s",
         "n_gpus": 1,
@@ -168,16 +215,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": "f32+f64" if head.startswith("mixed") else "f64",
         "data": "synthetic (seeded Matern-5/2 GP sample, numpy PCG64)",
         "config": {"workload": "energy+gradient per step, n=8192, d=8, cf_matern52 ARD, lik_gaussian(sigma=0.05); "
-                               "pred at 4096 points", "n": n, "d": d, "form": head},
+                               "pred at 4096 points; headline form: " + head
+                               + (f" ({args.restarts} hyperparameter restarts per pass)" if head == "mixed_batched" else ""),
+                    "n": n, "d": d, "form": head},
         "latency_ms": {"p50": round(H["p50_ms"], 3), "p99": round(H["p99_ms"], 3)},
         "forms": {f: {"evals_per_s": round(r["evals_per_s"], 3), "ms_per_step": round(r["ms_per_step"], 3),
                       "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
-                      "pred_4096_ms": round(r["pred_ms"], 3)} for f, r in R.items()},
-        "speedup_mixed_vs_fp64": (round(R["mixed"]["evals_per_s"] / R["fp64"]["evals_per_s"], 3)
-                                  if "mixed" in R and "fp64" in R else None),
+                      **({"pred_4096_ms": round(r["pred_ms"], 3)} if "pred_ms" in r else {}),
+                      **({"problems_per_pass": r["problems"]} if "problems" in r else {})} for f, r in R.items()},
+        "speedup_vs_fp64": ({f: round(r["evals_per_s"] / R["fp64"]["evals_per_s"], 3) for f, r in R.items()
+                             if f != "fp64"} if "fp64" in R else None),
         "parity": parity,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -23,7 +23,7 @@ namespace {
 
 enum {
     BB_X, BB_XOFF, BB_N, BB_PROG, BB_F, BB_Y, BB_SV, BB_SCAL, BB_SIG, BB_WSC, BB_IDX, BB_RHS, BB_U, BB_R, BB_D, BB_T1,
-    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_COUNT
+    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_CORR, BB_DACC, BB_COUNT
 };
 
 struct Bucket {
@@ -64,6 +64,9 @@ struct gpl_batch {
     }
 };
 
+// scalars per problem in BB_RED: five fixed slots and the gradient partials
+static int red_stride(const gpl_batch* B) { return 6 + std::max(1, B->hp.ngrad); }
+
 // --------------------------------------------------------------------------
 // the per-bucket pipeline (enqueued on one stream; captured into the graph)
 // --------------------------------------------------------------------------
@@ -78,13 +81,15 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     float* X = k.f(BB_X32);
     float* Bi = k.f(BB_BI);
     double* Dinv = k.d(BB_DINV);
+    double* Dacc = k.d(BB_DACC);  // FP64 diagonal tiles, same layout as Dinv
     double* ldiag = k.d(BB_LDIAG);
     int* info = k.b[BB_INFO].as<int>();
     const int bm = k.bm;
     const int64_t NBo = k.nbo;
     {
         Prof p(c, "batch_build_B_f32", 0, 4.0 * nb * N * (N + 64) / 2, s);
-        launch_build_B_f32_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), L, N2, nb);
+        launch_build_B_f32_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), L, N2, nb,
+                                   Dacc, N * 64);
     }
     // blocked Cholesky (FP32 storage, FP64 diagonal blocks and panel solves)
     for (int64_t K0 = 0; K0 < N; K0 += NBo) {
@@ -93,7 +98,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             const int64_t kk = j0 / 64;
             {
                 Prof p(c, "batch_potrf_diag", 0, 0, s);
-                potrf_diag3_f32_batched(s, L, N, kk, Dinv, ldiag, info, nb, N2, N * 64, N);
+                potrf_diag3_f32_batched(s, L, N, kk, Dinv, ldiag, info, nb, N2, N * 64, N, Dacc);
             }
             const int64_t M = N - (j0 + 64);
             if (M <= 0) continue;
@@ -108,6 +113,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
                 Prof p(c, "batch_panel_f32", 2.0 * nb * M * Wr * 64, 0, s);
                 gemm_f32_batched(s, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, bm, nb, N2,
                                  N2, N2);
+                dacc_update(s, Dacc + (kk + 1) * 64 * 64, L21, N, Wr / 64, 64, nb, N * 64, N2);
             }
         }
         const int64_t R = N - (K0 + W);
@@ -115,10 +121,11 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             const float* P = L + (K0 + W) + K0 * N;
             Prof p(c, "batch_syrk_f32", (double)nb * R * R * W, 0, s);
             gemm_f32_batched(s, 0, 1, R, R, W, -1.f, P, N, P, N, 1.f, L + (K0 + W) * (N + 1), N, 1, bm, nb, N2, N2, N2);
+            dacc_update(s, Dacc + ((K0 + W) / 64) * 64 * 64, P, N, R / 64, W, nb, N * 64, N2);
         }
     }
     // X = L^-1 (FP32, FP64 diagonal-block applications)
-    set_identity_lower_f32_batched(s, X, N, N, nb, N2);
+    set_identity_f32_batched(s, X, N, N, nb, N2);
     for (int64_t I0 = 0; I0 < N; I0 += NBo) {
         const int64_t H = std::min<int64_t>(NBo, N - I0);
         for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
@@ -188,14 +195,21 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         launch_grad_tiles_f32_batched(s, progs, B->hp, k.d(BB_F), sF, nvec, N, Bi, N2, k.d(BB_WSC), alpha, N,
                                       k.d(BB_PART), nb);
     }
+    {
+        Prof p(c, "batch_round_corr", 0, 0, s);
+        launch_round_corr_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), Bi, N2,
+                                  k.d(BB_CORR), nb);
+    }
     double* red = k.d(BB_RED);
-    seg_dot(s, k.d(BB_Y), alpha, nvec, N, red, 8, 0, nb);
-    seg_dot(s, ldiag, nullptr, nvec, N, red, 8, 1, nb);
-    seg_dot(s, alpha, alpha, nvec, N, red, 8, 2, nb);
+    const int rs = red_stride(B);
+    seg_dot(s, k.d(BB_Y), alpha, nvec, N, red, rs, 0, nb);
+    seg_dot(s, ldiag, nullptr, nvec, N, red, rs, 1, nb);
+    seg_dot(s, alpha, alpha, nvec, N, red, rs, 2, nb);
     colsumsq_f32_batched(s, X, N, N, N, t1, 1, nb, N2, N);
-    seg_dot(s, t1, nullptr, nvec, N, red, 8, 3, nb);
-    seg_cols(s, k.d(BB_PART), ntiles, B->hp.ngrad, red, 8, 4, nb);
-    batch_finish(s, red, nvec, k.d(BB_SIG), k.b[BB_IDX].as<int32_t>(), B->hp.ngrad, nb, B->res_logz.as<double>(),
+    seg_dot(s, t1, nullptr, nvec, N, red, rs, 3, nb);
+    seg_cols(s, k.d(BB_CORR), ntiles, 1, red, rs, 4, nb);
+    seg_cols(s, k.d(BB_PART), ntiles, B->hp.ngrad, red, rs, 5, nb);
+    batch_finish(s, red, rs, nvec, k.d(BB_SIG), k.b[BB_IDX].as<int32_t>(), B->hp.ngrad, nb, B->res_logz.as<double>(),
                  B->res_grad.as<double>(), B->ng);
 }
 
@@ -319,9 +333,12 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
                   k->b[BB_F].ensure((size_t)nb * N * MAX_FEAT * 8) && k->b[BB_SCAL].ensure(nb * 8) &&
                   k->b[BB_SIG].ensure(nb * 8) && k->b[BB_WSC].ensure(nb * 8) && k->b[BB_IDX].ensure(nb * 4) &&
                   k->b[BB_L].ensure(mat32) && k->b[BB_X32].ensure(mat32) && k->b[BB_BI].ensure(mat32) &&
-                  k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_LDIAG].ensure(vec) &&
+                  k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_DACC].ensure((size_t)nb * N * 64 * 8) &&
+                  k->b[BB_LDIAG].ensure(vec) &&
                   k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&
-                  k->b[BB_KMV].ensure((size_t)nb * k->nchunk * N * 8) && k->b[BB_RED].ensure((size_t)nb * 8 * 8);
+                  k->b[BB_KMV].ensure((size_t)nb * k->nchunk * N * 8) &&
+                  k->b[BB_RED].ensure((size_t)nb * red_stride(B) * 8) &&
+                  k->b[BB_CORR].ensure((size_t)nb * T * (T + 1) / 2 * 8);
         for (int i : {BB_Y, BB_SV, BB_RHS, BB_U, BB_R, BB_D, BB_T1, BB_KU, BB_ALPHA}) ok = ok && k->b[i].ensure(vec);
         if (!ok) {
             delete B;

@@ -223,7 +223,7 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
 enum {
     B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
     B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
-    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO,
+    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC,
     B_COUNT
 };
 
@@ -235,6 +235,7 @@ struct gpl_model {
     gpl_opts opts{};
     bool has_trials = false, has_offset = false;
     bool have_xinv = false, have_binv = false;
+    double round_corr = 0.0;  // mixed form: first-order log|B| correction applied to logZ
     double logZ = 0;
     int32_t iters = 0;
     int32_t refine_iters_done = 0;
@@ -333,7 +334,7 @@ static void trtri_panel(gpl_ctx* c, hipStream_t st, const double* L, const doubl
 // while the main stream updates the remaining rows.
 static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
     hipStream_t s = c->stream, s2 = c->side;
-    set_identity_lower_f64(s, X, N, N);
+    set_identity_f64(s, X, N, N);
     trtri_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
         const int64_t H = std::min<int64_t>(kOuterNB, N - I0);
@@ -365,12 +366,12 @@ static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64
 // diagonal-block pivots.
 // ---------------------------------------------------------------------------
 static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
-                          double* ldiag, int* info) {
+                          double* ldiag, int* info, double* Dacc) {
     for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
         const int64_t k = j0 / 64;
         {
             Prof p(c, "potrf_diag_mixed", 2.0 * 64 * 64 * 64 / 3, 64.0 * 64 * 12, st);
-            potrf_diag3_f32(st, L, N, k, Dinv, ldiag, info, 1);
+            potrf_diag3_f32(st, L, N, k, Dinv, ldiag, info, 1, Dacc);
         }
         const int64_t M = N - (j0 + 64);
         if (M <= 0) continue;
@@ -383,13 +384,15 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
         if (Wr > 0) {
             Prof p(c, "panel_update_f32", 2.0 * M * Wr * 64, 4.0 * (2.0 * M * Wr + M * 64), st);
             gemm_f32(st, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0);
+            dacc_update(st, Dacc + (k + 1) * 64 * 64, L21, N, Wr / 64, 64, 1, 0, 0);
         }
     }
 }
 
-static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+// Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build
+static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc) {
     hipStream_t s = c->stream, s2 = c->side;
-    potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
+    potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info, Dacc);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
         const int64_t K1 = K0 + W, R = N - K1;
@@ -399,16 +402,18 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         {
             Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W));
             gemm_f32(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, 0);
+            dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, 1, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
         hipStreamWaitEvent(s2, c->ev_a, 0);
-        potrf32_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info);
+        potrf32_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info, Dacc);
         hipEventRecord(c->ev_p, s2);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             const float* P2 = L + (K1 + W1) + K0 * N;
             Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W));
             gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
+            dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, 1, 0, 0);
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
@@ -432,7 +437,7 @@ static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const doub
 
 static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X) {
     hipStream_t s = c->stream, s2 = c->side;
-    set_identity_lower_f32(s, X, N, N);
+    set_identity_f32(s, X, N, N);
     trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
         const int64_t H = std::min<int64_t>(kOuterNB, N - I0);
@@ -483,13 +488,14 @@ static int check_info(gpl_model* m) {
     return GPL_OK;
 }
 
-static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k) {
+static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool mixed) {
     const int64_t N = pad64(n);
     m->n = n;
     m->N = N;
     m->ncols = ncols;
     const size_t vec = (size_t)N * 8;
-    const size_t mat = (size_t)N * N * 8;
+    // the mixed form keeps its factor in FP32
+    const size_t mat = (size_t)N * N * (mixed ? 4 : 8);
     ENSURE(m, B_X, (size_t)n * ncols * 8);
     ENSURE(m, B_F, (size_t)N * MAX_FEAT * 8);
     ENSURE(m, B_Y, vec);
@@ -587,17 +593,36 @@ static int fit_gaussian_mixed(gpl_model* m) {
     const double sigma = m->lik.sigma;
     const DevProg* pd = m->b[B_PROG].as<DevProg>();
     float* L = m->b[B_L].as<float>();
-    ENSURE(m, B_XINV, (size_t)N * N * 8);
+    ENSURE(m, B_XINV, (size_t)N * N * 4);
+    ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     float* X = m->b[B_XINV].as<float>();
     {
         Prof p(c, "build_B_f32", 0, 4.0 * N * (N + 64) / 2);
-        launch_build_sym_f32(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1, 1.0 / sigma, nullptr, L, N);
+        launch_build_sym_f32(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1, 1.0 / sigma, nullptr, L, N,
+                             m->d(B_DACC));
     }
     lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
-    potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
+    potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), m->d(B_DACC));
     trtri32_blocked(c, L, m->d(B_DINV), N, X);
     m->have_xinv = true;
+    // B^-1 = X^T X (kept for the gradient) and the first-order log|B| correction
+    // for having factored fl32(B) rather than B
+    ENSURE(m, B_BINV, (size_t)N * N * 4);
+    const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
+    ENSURE(m, B_PART, (size_t)ntiles * std::max(1, m->prog.ngrad) * 8);
+    float* Bi = m->b[B_BINV].as<float>();
+    {
+        Prof p(c, "lauum_f32", (double)N * N * N / 3, 4.0 * N * N);
+        gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0);
+    }
+    m->have_binv = true;
+    {
+        Prof p(c, "round_corr", 0, 4.0 * N * (N + 64) / 2);
+        launch_round_corr(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1.0 / sigma, nullptr, Bi, N,
+                          m->d(B_PART));
+    }
+    reduce_sum(s, m->d(B_PART), ntiles, m->d(B_SCAL) + 3);
     double* rhs = m->d(B_TERM);
     double* u = m->d(B_W1);
     double* r = m->d(B_W2);
@@ -647,10 +672,11 @@ static int fit_gaussian_mixed(gpl_model* m) {
     reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
     int rr = check_info(m);
     if (rr) return rr;
-    double h[3];
-    rr = read_scalars(m, h, 3);
+    double h[4];
+    rr = read_scalars(m, h, 4);
     if (rr) return rr;
-    m->logZ = h[0] - 0.5 * h[1] - h[2];
+    m->logZ = h[0] - 0.5 * h[1] - (h[2] + 0.5 * h[3]);
+    m->round_corr = h[3];
     m->iters = 1;
     return GPL_OK;
 }
@@ -896,6 +922,9 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
                                  "exact results");  // R/gp.R:54-56
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
     if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
+    const bool mixed = o.precision == GPL_PREC_MIXED;
+    if (mixed && !(lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_LAPLACE))
+        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Gaussian likelihood");
     DevProg P;
     int r = prepare_prog(terms, nterms, ncols, P);
     if (r) return r;
@@ -907,7 +936,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
         fresh = true;
     }
     const bool need_k = !(lik->kind == GPL_LIK_GAUSSIAN);
-    r = alloc_model(m, n, ncols, need_k);
+    r = alloc_model(m, n, ncols, need_k, mixed);
     if (r) {
         if (fresh) delete m;
         return r;
@@ -926,12 +955,8 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     upload_vec(m, B_TR, trials, n);
     upload_vec(m, B_OFF, offset, n);
     hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
-    if (o.precision == GPL_PREC_MIXED && !(lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_LAPLACE)) {
-        if (fresh) delete m;
-        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Gaussian likelihood");
-    }
     if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
-    else if (lik->kind == GPL_LIK_GAUSSIAN) r = (o.precision == GPL_PREC_MIXED) ? fit_gaussian_mixed(m) : fit_gaussian(m);
+    else if (lik->kind == GPL_LIK_GAUSSIAN) r = mixed ? fit_gaussian_mixed(m) : fit_gaussian(m);
     else r = fit_laplace(m);
     *inout = m;
     if (log_evidence) *log_evidence = m->logZ;
@@ -948,18 +973,19 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = m->n, N = m->N;
-    ENSURE(m, B_XINV, (size_t)N * N * 8);
-    ENSURE(m, B_BINV, (size_t)N * N * 8);
+    const bool mixed = m->opts.precision == GPL_PREC_MIXED;
+    const size_t esz = mixed ? 4 : 8;
+    ENSURE(m, B_XINV, (size_t)N * N * esz);
+    ENSURE(m, B_BINV, (size_t)N * N * esz);
     const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
     const int ng = m->prog.ngrad;
     ENSURE(m, B_PART, (size_t)ntiles * (ng > 0 ? ng : 1) * 8);
-    const bool mixed = m->opts.precision == GPL_PREC_MIXED;
     const double sigma = m->lik.sigma;
     const double s2 = sigma * sigma;
     if (mixed) {
         float* X = m->b[B_XINV].as<float>();
         float* Bi = m->b[B_BINV].as<float>();
-        {
+        if (!m->have_binv) {
             Prof p(c, "lauum_f32", (double)N * N * N / 3, 4.0 * N * N);
             gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0);
         }

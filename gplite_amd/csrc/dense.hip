@@ -289,12 +289,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t ld, int64_t k,
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, int want_inv, int64_t sA, int64_t sD,
-                                                     int64_t sL) {
+                                                     int64_t sL, const double* __restrict__ Dsrc) {
     // batch: one problem per workgroup
     A += blockIdx.x * sA;
     Dinv += blockIdx.x * sD;
     ldiag += blockIdx.x * sL;
     info += blockIdx.x * ((sA != 0) ? 1 : 0);
+    // Dsrc (optional): the FP64 diagonal tiles (64x64 column-major, one per block
+    // row, same batch stride as Dinv) to factor in place of the stored tile of A
+    if (Dsrc) Dsrc += blockIdx.x * sD + k * 64 * 64;
     __shared__ double colbuf[2][64];
     __shared__ double aL[64][65];  // factor, a[col][row], for the inverse
     __shared__ double rinv[64];
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int c = w + 4 * q;
-        a[q] = (c <= i) ? (double)blk[i + (int64_t)c * ld] : 0.0;
+        a[q] = (c <= i) ? (Dsrc ? Dsrc[i + c * 64] : (double)blk[i + (int64_t)c * ld]) : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
@@ -374,20 +377,63 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
                      int want_inv) {
     hipLaunchKernelGGL(k_potrf_diag3<double>, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, want_inv,
-                       (int64_t)0, (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, (int64_t)0, (const double*)nullptr);
 }
 
 void potrf_diag3_f32(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
-                     int want_inv) {
+                     int want_inv, const double* Dacc) {
     hipLaunchKernelGGL(k_potrf_diag3<float>, dim3(1), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, want_inv,
-                       (int64_t)0, (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, (int64_t)0, Dacc);
 }
 
 // batch of problems, each with its own info slot (info[b])
 void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
-                             int batch, int64_t sA, int64_t sD, int64_t sL) {
+                             int batch, int64_t sA, int64_t sD, int64_t sL, const double* Dacc) {
     hipLaunchKernelGGL(k_potrf_diag3<float>, dim3((unsigned)batch), dim3(256), 0, s, A, ld, k, Dinv, ldiag, info, 1,
-                       sA, sD, sL);
+                       sA, sD, sL, Dacc);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagonal tiles of the mixed-precision factorization.  Every FP32 update
+// A -= P P^T that reaches diagonal tiles is also applied to their FP64 copies
+// D_j -= P_j P_j^T, where P_j is the FP32 panel's rows of block j: the products
+// of FP32 values are exact in FP64 and are accumulated in FP64, so the diagonal
+// blocks that are factored carry no FP32 accumulation error.
+// D: first tile to update (64x64 column-major tiles, consecutive), P: the panel
+// rows of that tile (leading dimension ldp), kdepth: panel width (multiple of 64).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dacc_update(double* __restrict__ D, const float* __restrict__ P, int64_t ldp,
+                                                     int64_t kdepth, int64_t sD, int64_t sP) {
+    D += blockIdx.y * sD + (int64_t)blockIdx.x * 64 * 64;
+    P += blockIdx.y * sP + (int64_t)blockIdx.x * 64;
+    __shared__ double Ps[64][65];
+    const int t = threadIdx.x, i = t & 63, cg = t >> 6;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int64_t k0 = 0; k0 < kdepth; k0 += 64) {
+        for (int e = t; e < 64 * 64; e += 256) {
+            const int r = e & 63, kk = e >> 6;
+            Ps[kk][r] = (double)P[r + (k0 + kk) * ldp];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 64; ++kk) {
+            const double pi = Ps[kk][i];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] += pi * Ps[kk][cg + 4 * q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) D[i + (cg + 4 * q) * 64] -= acc[q];
+}
+
+void dacc_update(hipStream_t s, double* D, const float* P, int64_t ldp, int64_t ntiles, int64_t kdepth, int batch,
+                 int64_t sD, int64_t sP) {
+    if (ntiles <= 0 || batch <= 0) return;
+    hipLaunchKernelGGL(k_dacc_update, dim3((unsigned)ntiles, (unsigned)batch), dim3(256), 0, s, D, P, ldp, kdepth, sD,
+                       sP);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,9 +629,10 @@ void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int6
 // ---------------------------------------------------------------------------
 // Elementwise helpers over lower tiles / vectors
 // ---------------------------------------------------------------------------
-__global__ void k_set_identity_lower(double* __restrict__ A, int64_t ld, int64_t T) {
-    int64_t ti, tj;
-    lower_tile(blockIdx.x, ti, tj);
+// A = I over all T x T tiles: the triangular GEMM modes read whole tiles, so the
+// upper tiles of a triangular factor must hold zeros.
+__global__ void k_set_identity(double* __restrict__ A, int64_t ld, int64_t T) {
+    const int64_t ti = blockIdx.x % T, tj = blockIdx.x / T;
     const int t = threadIdx.x, i = t & 63;
     for (int c = t >> 6; c < 64; c += 4) {
         const int64_t gi = ti * 64 + i, gj = tj * 64 + c;
@@ -593,9 +640,9 @@ __global__ void k_set_identity_lower(double* __restrict__ A, int64_t ld, int64_t
     }
 }
 
-void set_identity_lower_f64(hipStream_t s, double* A, int64_t ld, int64_t N) {
+void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N) {
     const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_set_identity_lower, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, A, ld, T);
+    hipLaunchKernelGGL(k_set_identity, dim3((unsigned)(T * T)), dim3(256), 0, s, A, ld, T);
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {

@@ -69,9 +69,14 @@ __device__ __forceinline__ void g32_store(float (*S)[BM + 4], const float4 (&r)[
     }
 }
 
-// C = beta*C + alpha*op(A) op(B) in FP32.  Tile modes as gemm2_f64.
-// Each wave owns a (BM/2)^2 quadrant of 32x32 blocks; lane half h = lane/32
-// takes k = 8h..8h+7 of a 16-wide chunk over eight MFMA steps.
+// C = beta*C + alpha*op(A) op(B), FP32 storage and FP32 MFMA products.  Tile
+// modes as gemm2_f64.  Each wave owns a (BM/2)^2 quadrant of 32x32 blocks;
+// lane half h = lane/32 takes k = 8h..8h+7 of a 16-wide chunk over eight MFMA
+// steps.  The FP32 accumulators cover at most 64 consecutive k: every 64 k
+// they are added into FP64 accumulators and cleared, and the epilogue forms
+// beta*C + alpha*acc in FP64 with one rounding to FP32.  A long FP32 running
+// sum loses about sqrt(k) ulps of the partial sums; in trailing updates that
+// cancel (A - P P^T, with |P P^T| >> |result|) that error dominates log|B|.
 template <int TA, int TB, int BM>
 __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
                                                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
@@ -100,12 +105,16 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
     const int wr = wave >> 1, wc = wave & 1;
 
     f16v acc[NBK][NBK];
+    double acc64[NBK][NBK][16];
 #pragma unroll
     for (int i = 0; i < NBK; ++i)
 #pragma unroll
         for (int j = 0; j < NBK; ++j)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+            for (int v = 0; v < 16; ++v) {
+                acc[i][j][v] = 0.f;
+                acc64[i][j][v] = 0.0;
+            }
 
     float4 ra[AL], rb[AL];
     if (nch > 0) {
@@ -136,6 +145,17 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
                 for (int j = 0; j < NBK; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
+        if ((c & 3) == 3 || !more) {  // fold 64 k of FP32 sums into FP64 (block-uniform branch)
+#pragma unroll
+            for (int i = 0; i < NBK; ++i)
+#pragma unroll
+                for (int j = 0; j < NBK; ++j)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        acc64[i][j][v] += (double)acc[i][j][v];
+                        acc[i][j][v] = 0.f;
+                    }
+        }
         if (more) {
             g32_store<BM, !TA>(As[buf ^ 1], ra);
             g32_store<BM, TB>(Bs[buf ^ 1], rb);
@@ -153,8 +173,8 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int r = 8 * (v >> 2) + 4 * h + (v & 3);
-                const float o = alpha * acc[i][j][v];
-                p[r] = (beta == 0.f) ? o : beta * p[r] + o;
+                const double o = (double)alpha * acc64[i][j][v];
+                p[r] = (float)((beta == 0.f) ? o : (double)beta * (double)p[r] + o);
             }
         }
 }
@@ -197,10 +217,11 @@ void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, fl
 // ---------------------------------------------------------------------------
 // FP32-matrix helpers with FP64 vectors
 // ---------------------------------------------------------------------------
-__global__ void k_set_identity_lower32(float* __restrict__ A, int64_t ld, int64_t sA) {
+// A = I over all T x T tiles (the 128-wide triangular GEMM modes read the tile
+// just above each diagonal tile, so upper tiles must hold zeros)
+__global__ void k_set_identity32(float* __restrict__ A, int64_t ld, int64_t T, int64_t sA) {
     A += blockIdx.y * sA;
-    int64_t ti, tj;
-    lower_tile32(blockIdx.x, ti, tj);
+    const int64_t ti = blockIdx.x % T, tj = blockIdx.x / T;
     const int t = threadIdx.x, i = t & 63;
     for (int c = t >> 6; c < 64; c += 4) {
         const int64_t gi = ti * 64 + i, gj = tj * 64 + c;
@@ -208,14 +229,13 @@ __global__ void k_set_identity_lower32(float* __restrict__ A, int64_t ld, int64_
     }
 }
 
-void set_identity_lower_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA) {
+void set_identity_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA) {
     const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_set_identity_lower32, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), 0, s, A,
-                       ld, sA);
+    hipLaunchKernelGGL(k_set_identity32, dim3((unsigned)(T * T), (unsigned)batch), dim3(256), 0, s, A, ld, T, sA);
 }
 
-void set_identity_lower_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
-    set_identity_lower_f32_batched(s, A, ld, N, 1, 0);
+void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
+    set_identity_f32_batched(s, A, ld, N, 1, 0);
 }
 
 // y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32, vectors FP64

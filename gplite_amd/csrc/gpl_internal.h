@@ -50,9 +50,9 @@ void launch_features(hipStream_t s, const double* x, int64_t n, int64_t ldx, int
 void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
                           int64_t n, int64_t N, double jitter, int mode, double sscal,
                           const double* svec, double* out, int64_t ld);
-void launch_build_sym_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F,
-                          int64_t n, int64_t N, double jitter, int mode, double sscal,
-                          const double* svec, float* out, int64_t ld);
+void launch_build_sym_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
+                          int64_t N, double jitter, int mode, double sscal, const double* svec, float* out,
+                          int64_t ld, double* dacc);
 // out (Na x Nb col-major, ld) = K(Fa, Fb); zero outside [na) x [nb).
 void launch_build_cross(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* Fa,
                         int64_t lda_f, int64_t na, int64_t Na, const double* Fb, int64_t ldb_f,
@@ -93,7 +93,7 @@ void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 // FP32 storage and FP32 MFMA with FP32 accumulation (dense32.hip)
 void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
               int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm);
-void set_identity_lower_f32(hipStream_t s, float* A, int64_t ld, int64_t N);
+void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N);
 void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans);
 void colsumsq_f32(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out, int lower);
 void rel_maxabs(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
@@ -102,7 +102,10 @@ void rel_maxabs(hipStream_t s, const double* d, const double* u, int64_t n, doub
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
                      int want_inv);
 void potrf_diag3_f32(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
-                     int want_inv);
+                     int want_inv, const double* Dacc);
+// D_j -= P_j P_j^T in FP64 for ntiles consecutive diagonal tiles (FP32 panel P)
+void dacc_update(hipStream_t s, double* D, const float* P, int64_t ldp, int64_t ntiles, int64_t kdepth, int batch,
+                 int64_t sD, int64_t sP);
 // triangular solve with the blocked factor (L lower, Dinv diagonal inverses), in place on b.
 void trsv_lower_f64(hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N,
                     double* b, double* x, int trans);
@@ -118,7 +121,7 @@ void gemv_t_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_
 // out[j] = sum_i A[i + j*ld]^2 for j < ncol (deterministic column sums of squares)
 void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
                   int lower);
-void set_identity_lower_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
+void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
 // B (lower tiles) = I + s_i K_ij s_j for i,j < n; identity on padding
@@ -154,9 +157,15 @@ void vec_recip(hipStream_t s, int64_t n, const double* a, double* out);
 // ---- batched forms (one problem per grid row / workgroup, stride per problem) ----
 void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,
                              const DevProg* progs, double* F, int64_t N, int64_t sF, int batch);
+// first-order log|B| correction for factors of fl32(B) (per lower tile partial sums)
+void launch_round_corr(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n, int64_t N,
+                       double jitter, double sscal, const double* svec, const float* W, int64_t ldw, double* part);
+void launch_round_corr_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
+                               const int64_t* nvec, int64_t N, double jitter, const double* scal, const float* W,
+                               int64_t sW, double* part, int batch);
 void launch_build_B_f32_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
                                 const int64_t* nvec, int64_t N, double jitter, const double* scal, float* out,
-                                int64_t sO, int batch);
+                                int64_t sO, int batch, double* dacc, int64_t sDa);
 void launch_kmatvec_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
                             const int64_t* nvec, int64_t N, double jitter, const double* v, double* y, int64_t sV,
                             double* part, int nchunk, int batch);
@@ -173,8 +182,8 @@ void gemm_mixed_f64f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t
                                const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C,
                                int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC);
 void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
-                             int batch, int64_t sA, int64_t sD, int64_t sL);
-void set_identity_lower_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA);
+                             int batch, int64_t sA, int64_t sD, int64_t sL, const double* Dacc);
+void set_identity_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA);
 void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
                             int trans, int batch, int64_t sX, int64_t sv);
 void colsumsq_f32_batched(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
@@ -186,7 +195,7 @@ void seg_dot(hipStream_t s, const double* a, const double* b, const int64_t* nve
 void seg_cols(hipStream_t s, const double* part, int64_t ntiles, int ng, double* out, int ostride, int slot0,
               int batch);
 void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
-void batch_finish(hipStream_t s, const double* red, const int64_t* nvec, const double* sigma, const int32_t* idx, int ng,
-                  int nb, double* logz, double* grad, int gstride);
+void batch_finish(hipStream_t s, const double* red, int rs, const int64_t* nvec, const double* sigma,
+                  const int32_t* idx, int ng, int nb, double* logz, double* grad, int gstride);
 
 }  // namespace gpl

@@ -375,27 +375,28 @@ void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t 
     hipLaunchKernelGGL(k_relmax_all, dim3(1), dim3(1024), 0, s, d, u, n, out);
 }
 
-// Per problem: log evidence and gradient from the segment sums
-// red[b*8 + 0] = y.alpha, 1 = sum log L_ii, 2 = alpha.alpha, 3 = tr(B^-1), 4.. = gradient partial sums
-__global__ void k_batch_finish(const double* __restrict__ red, const int64_t* __restrict__ nvec,
+// Per problem: log evidence and gradient from the segment sums.
+// red per problem (stride rs): [0] y^T alpha, [1] sum log L_ii, [2] alpha^T alpha,
+// [3] tr(B^-1), [4] first-order log|B| correction of the FP32 factor, [5..] gradient partials
+__global__ void k_batch_finish(const double* __restrict__ red, int rs, const int64_t* __restrict__ nvec,
                                const double* __restrict__ sigma, const int32_t* __restrict__ idx, int ng, int nb,
                                double* __restrict__ logz, double* __restrict__ grad, int gstride) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const double* r = red + (int64_t)b * 8;
+    const double* r = red + (int64_t)b * rs;
     const double n = (double)nvec[b];
     const double sg = sigma[b], s2 = sg * sg;
     const int p = idx[b];
-    // log N(y | 0, K + sigma^2 I) with K + sigma^2 I = sigma^2 B:  -1/2 y^T alpha - sum log L_ii - n log sigma - n/2 log 2 pi
-    logz[p] = -0.5 * r[0] - r[1] - n * log(sg) - 0.5 * n * kLog2Pi;
-    for (int q = 0; q < ng; ++q) grad[(int64_t)p * gstride + q] = -0.5 * r[4 + q];
+    // log N(y | 0, K + sigma^2 I) with K + sigma^2 I = sigma^2 B:  -1/2 y^T alpha - 1/2 log|B| - n log sigma - n/2 log 2 pi
+    logz[p] = -0.5 * r[0] - (r[1] + 0.5 * r[4]) - n * log(sg) - 0.5 * n * kLog2Pi;
+    for (int q = 0; q < ng; ++q) grad[(int64_t)p * gstride + q] = -0.5 * r[5 + q];
     grad[(int64_t)p * gstride + ng] = -s2 * r[2] + r[3];
 }
 
-void batch_finish(hipStream_t s, const double* red, const int64_t* nvec, const double* sigma, const int32_t* idx, int ng,
-                  int nb, double* logz, double* grad, int gstride) {
-    hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, red, nvec, sigma, idx, ng, nb,
-                       logz, grad, gstride);
+void batch_finish(hipStream_t s, const double* red, int rs, const int64_t* nvec, const double* sigma,
+                  const int32_t* idx, int ng, int nb, double* logz, double* grad, int gstride) {
+    hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, red, rs, nvec, sigma, idx, ng,
+                       nb, logz, grad, gstride);
 }
 
 }  // namespace gpl

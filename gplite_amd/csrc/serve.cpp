@@ -78,12 +78,15 @@ struct Resident {
 
 size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
     const size_t N = (size_t)pad64(n);
-    size_t mats = 1;                                  // factor of B
-    if (lik.kind != GPL_LIK_GAUSSIAN) mats += 1;      // stored K
-    if (o.approx == GPL_APPROX_EP) mats += 1;         // L^-1
-    if (o.flags & GPL_FLAG_INVERSE) mats += 1;
-    const size_t esz = (o.precision == GPL_PREC_MIXED) ? 4 : 8;
-    return mats * N * N * esz + N * 64 * 8 + N * (MAX_FEAT + 40) * 8;
+    const bool mixed = o.precision == GPL_PREC_MIXED;
+    size_t mats = 1;                                       // factor of B
+    if (lik.kind != GPL_LIK_GAUSSIAN) mats += 1;           // stored K
+    if (o.approx == GPL_APPROX_EP) mats += 1;              // L^-1
+    if (mixed) mats += 2;                                  // L^-1 and B^-1
+    else if (o.flags & GPL_FLAG_INVERSE) mats += 1;        // L^-1
+    const size_t esz = mixed ? 4 : 8;
+    const size_t T = N / 64;
+    return mats * N * N * esz + N * 64 * 8 + N * (MAX_FEAT + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + 4096;
 }
 
 size_t pred_footprint(int64_t n, int64_t m) {
@@ -95,7 +98,8 @@ size_t pred_footprint(int64_t n, int64_t m) {
 
 struct gpl_server {
     int device = 0;
-    size_t budget = 0, in_use = 0;
+    size_t budget = 0, in_use = 0;  // in_use: resident models + work in flight
+    size_t inflight = 0;             // footprints of the jobs running now
     int64_t max_coalesce = 4096;
     std::vector<gpl_ctx*> ctxs;
     std::vector<std::thread> workers;
@@ -111,6 +115,29 @@ struct gpl_server {
 
 // Must hold s->mu.  Pick the oldest runnable request (and, for a prediction,
 // the queued predictions on the same model that coalesce with it).
+// Evict idle resident models (no queued or running predictions), least recently
+// used first, until `need` more bytes fit in the budget; `keep` is never evicted.
+// Caller holds s->mu.
+static void evict_until(gpl_server* s, size_t need, int64_t keep) {
+    while (s->in_use + need > s->budget) {
+        int64_t victim = -1;
+        double oldest = 1e300;
+        for (auto& kv : s->models) {
+            const Resident& md = kv.second;
+            if (md.ready && md.refs == 0 && md.model && md.last_use < oldest && kv.first != keep) {
+                oldest = md.last_use;
+                victim = kv.first;
+            }
+        }
+        if (victim < 0) return;
+        Resident& md = s->models[victim];
+        gpl_model_free(md.model);
+        s->in_use -= std::min(s->in_use, md.bytes);
+        s->models.erase(victim);
+        s->evictions++;
+    }
+}
+
 static std::vector<Req*> take_runnable(gpl_server* s) {
     std::vector<Req*> out;
     for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
@@ -129,25 +156,10 @@ static std::vector<Req*> take_runnable(gpl_server* s) {
             if (!mit->second.ready) continue;
         }
         // admission by footprint, evicting idle resident models (LRU) if needed
-        while (s->in_use + r->footprint > s->budget) {
-            int64_t victim = -1;
-            double oldest = 1e300;
-            for (auto& kv : s->models) {
-                const Resident& md = kv.second;
-                if (md.ready && md.refs == 0 && md.model && md.last_use < oldest &&
-                    !(r->type == REQ_PRED && kv.first == r->model_id)) {
-                    oldest = md.last_use;
-                    victim = kv.first;
-                }
-            }
-            if (victim < 0) break;
-            Resident& md = s->models[victim];
-            gpl_model_free(md.model);
-            s->in_use -= std::min(s->in_use, md.bytes);
-            s->models.erase(victim);
-            s->evictions++;
-        }
-        if (s->in_use + r->footprint > s->budget && s->in_use > 0) continue;  // wait for memory
+        evict_until(s, r->footprint, r->type == REQ_PRED ? r->model_id : -1);
+        // wait for memory while running work will free some; with nothing running and
+        // nothing left to evict the request is admitted anyway, so the queue cannot stall
+        if (s->in_use + r->footprint > s->budget && s->inflight > 0) continue;
         out.push_back(r);
         s->queue.erase(it);
         if (r->type == REQ_PRED) {
@@ -190,6 +202,7 @@ static void worker_main(gpl_server* s, int w) {
                 fp += r->footprint;
             }
             s->in_use += fp;
+            s->inflight += fp;
             s->launches++;
         }
         Req* r0 = job[0];
@@ -204,12 +217,16 @@ static void worker_main(gpl_server* s, int w) {
             r0->log_evidence = lz;
             Resident& md = s->models[r0->model_id];
             s->in_use -= std::min(s->in_use, fp);
+            s->inflight -= std::min(s->inflight, fp);
             if (m && (st == GPL_OK || st == GPL_ERR_NOCONV)) {
                 gpl_model_info(m, nullptr, &r0->iters, nullptr);
                 md.model = m;
                 md.ready = true;
                 md.bytes = (size_t)gpl_model_bytes(m);
                 s->in_use += md.bytes;
+                md.last_use = now_ms();
+                // the resident model may be larger than the admission estimate
+                evict_until(s, 0, r0->model_id);
             } else {
                 if (m) gpl_model_free(m);
                 md.failed = true;
@@ -246,6 +263,7 @@ static void worker_main(gpl_server* s, int w) {
             }
             std::unique_lock<std::mutex> lk(s->mu);
             s->in_use -= std::min(s->in_use, fp);
+            s->inflight -= std::min(s->inflight, fp);
             Resident& md = s->models[r0->model_id];
             md.refs -= (int)job.size();
             md.last_use = now_ms();
