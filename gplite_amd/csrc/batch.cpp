@@ -23,7 +23,7 @@ namespace {
 
 enum {
     BB_X, BB_XOFF, BB_N, BB_PROG, BB_F, BB_Y, BB_SV, BB_SCAL, BB_SIG, BB_WSC, BB_IDX, BB_RHS, BB_U, BB_R, BB_D, BB_T1,
-    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_CORR, BB_DACC, BB_COUNT
+    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_CORR, BB_DACC, BB_B64, BB_COUNT
 };
 
 struct Bucket {
@@ -89,7 +89,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     {
         Prof p(c, "batch_build_B_f32", 0, 4.0 * nb * N * (N + 64) / 2, s);
         launch_build_B_f32_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), L, N2, nb,
-                                   Dacc, N * 64);
+                                   Dacc, N * 64, k.d(BB_B64));
     }
     // blocked Cholesky (FP32 storage, FP64 diagonal blocks and panel solves)
     for (int64_t K0 = 0; K0 < N; K0 += NBo) {
@@ -111,7 +111,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             const int64_t Wr = K0 + W - (j0 + 64);
             if (Wr > 0) {
                 Prof p(c, "batch_panel_f32", 2.0 * nb * M * Wr * 64, 0, s);
-                gemm_f32_batched(s, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, bm, nb, N2,
+                gemm_f32_batched(s, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0, nb, N2,
                                  N2, N2);
                 dacc_update(s, Dacc + (kk + 1) * 64 * 64, L21, N, Wr / 64, 64, nb, N * 64, N2);
             }
@@ -120,7 +120,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         if (R > 0) {
             const float* P = L + (K0 + W) + K0 * N;
             Prof p(c, "batch_syrk_f32", (double)nb * R * R * W, 0, s);
-            gemm_f32_batched(s, 0, 1, R, R, W, -1.f, P, N, P, N, 1.f, L + (K0 + W) * (N + 1), N, 1, bm, nb, N2, N2, N2);
+            gemm_f32_batched(s, 0, 1, R, R, W, -1.f, P, N, P, N, 1.f, L + (K0 + W) * (N + 1), N, 1, 0, nb, N2, N2, N2);
             dacc_update(s, Dacc + ((K0 + W) / 64) * 64 * 64, P, N, R / 64, W, nb, N * 64, N2);
         }
     }
@@ -139,7 +139,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             if (Mr > 0) {
                 Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * Mr * Wc * 64, 0, s);
                 gemm_f32_batched(s, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0,
-                                 bm, nb, N2, N2, N2);
+                                 bm, nb, N2, N2, N2, kFoldInverse);
             }
         }
         const int64_t R = N - (I0 + H);
@@ -147,7 +147,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             const int64_t Wc = I0 + H;
             Prof p(c, "batch_trtri_f32", 2.0 * nb * R * Wc * H, 0, s);
             gemm_f32_batched(s, 0, 0, R, Wc, H, -1.f, L + (I0 + H) + I0 * N, N, X + I0, N, 1.f, X + I0 + H, N, 0, bm, nb,
-                             N2, N2, N2);
+                             N2, N2, N2, kFoldInverse);
         }
     }
     // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is static)
@@ -172,22 +172,18 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         }
         vec_axpby(s, NV, 1.0, dd, 1.0, u, u);
         if (it == sweeps - 1) relmax_accumulate(s, dd, u, NV, B->relmax.as<double>());
-        vec_mul(s, NV, sv, u, t1);
-        {
-            Prof p(c, "batch_kmatvec_f64", 0, 0, s);
-            launch_kmatvec_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, t1, Ku, N, k.d(BB_KMV),
-                                   k.nchunk, nb);
+        {  // r = S y - B u with the FP64 copy of B
+            Prof p(c, "batch_symv_b64", 2.0 * nb * N * N, 0, s);
+            symv_lt_f64(s, k.d(BB_B64), N, u, Ku, k.d(BB_KMV), k.nchunk, nb, N2, N);
         }
-        vec_mul(s, NV, sv, Ku, r);
-        vec_axpby(s, NV, -1.0, r, -1.0, u, r);
-        vec_axpby(s, NV, 1.0, r, 1.0, rhs, r);
+        vec_axpby(s, NV, 1.0, rhs, -1.0, Ku, r);
     }
     double* alpha = k.d(BB_ALPHA);
     vec_mul(s, NV, sv, u, alpha);
     // B^-1 = X^T X and the gradient
     {
         Prof p(c, "batch_lauum_f32", (double)nb * N * N * N / 3, 0, s);
-        gemm_f32_batched(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, bm, nb, N2, N2, N2);
+        gemm_f32_batched(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, bm, nb, N2, N2, N2, kFoldInverse);
     }
     const int64_t ntiles = T * (T + 1) / 2;
     {
@@ -197,8 +193,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     }
     {
         Prof p(c, "batch_round_corr", 0, 0, s);
-        launch_round_corr_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), Bi, N2,
-                                  k.d(BB_CORR), nb);
+        launch_round_corr(s, k.d(BB_B64), Bi, N, k.d(BB_CORR), nb, N2, N2);
     }
     double* red = k.d(BB_RED);
     const int rs = red_stride(B);
@@ -333,6 +328,7 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
                   k->b[BB_F].ensure((size_t)nb * N * MAX_FEAT * 8) && k->b[BB_SCAL].ensure(nb * 8) &&
                   k->b[BB_SIG].ensure(nb * 8) && k->b[BB_WSC].ensure(nb * 8) && k->b[BB_IDX].ensure(nb * 4) &&
                   k->b[BB_L].ensure(mat32) && k->b[BB_X32].ensure(mat32) && k->b[BB_BI].ensure(mat32) &&
+                  k->b[BB_B64].ensure(mat32 * 2) &&
                   k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_DACC].ensure((size_t)nb * N * 64 * 8) &&
                   k->b[BB_LDIAG].ensure(vec) &&
                   k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&

@@ -223,7 +223,7 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
 enum {
     B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
     B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
-    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC,
+    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC, B_B64,
     B_COUNT
 };
 
@@ -430,7 +430,8 @@ static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const doub
         const int64_t Mr = I0 + H - (r0 + 64);
         if (Mr > 0) {
             Prof p(c, "trtri_panel_f32", 2.0 * Mr * Wc * 64, 8.0 * Mr * Wc, st);
-            gemm_f32(st, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0, 0);
+            gemm_f32(st, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0, 0,
+                     kFoldInverse);
         }
     }
 }
@@ -447,7 +448,7 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         const int64_t Wc = I0 + H;
         {
             Prof p(c, "trtri_update_f32", 2.0 * H1 * Wc * H, 4.0 * (2.0 * H1 * Wc + Wc * H));
-            gemm_f32(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, 0);
+            gemm_f32(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, 0, kFoldInverse);
         }
         hipEventRecord(c->ev_a, s);
         hipStreamWaitEvent(s2, c->ev_a, 0);
@@ -456,7 +457,8 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         const int64_t R2 = R - H1;
         if (R2 > 0) {
             Prof p(c, "trtri_update_f32", 2.0 * R2 * Wc * H, 4.0 * (2.0 * R2 * Wc + Wc * H));
-            gemm_f32(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0, 0);
+            gemm_f32(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0, 0,
+                     kFoldInverse);
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
@@ -583,9 +585,9 @@ static int fit_gaussian(gpl_model* m) {
 
 // Gaussian fit in the mixed-precision form.  alpha = S B^-1 S y is obtained by
 // iterative refinement: corrections come from the FP32 inverse factor
-// (d = X^T X r), residuals r = S y - B u are formed in FP64 with a matrix-free
-// K*v that re-evaluates the covariance from the inputs, so the converged alpha
-// and f = K alpha carry FP64 accuracy.
+// (d = X^T X r), residuals r = S y - B u are formed in FP64 from the FP64 copy
+// of B written by the build, so the converged alpha and f = K alpha carry FP64
+// accuracy.
 static int fit_gaussian_mixed(gpl_model* m) {
     gpl_ctx* c = m->ctx;
     hipStream_t s = c->stream;
@@ -595,11 +597,15 @@ static int fit_gaussian_mixed(gpl_model* m) {
     float* L = m->b[B_L].as<float>();
     ENSURE(m, B_XINV, (size_t)N * N * 4);
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
+    ENSURE(m, B_B64, (size_t)N * N * 8);
+    const int nch = kmatvec_chunks(N);
+    ENSURE(m, B_KMV, (size_t)N * nch * 8);
     float* X = m->b[B_XINV].as<float>();
+    double* B64 = m->d(B_B64);
     {
-        Prof p(c, "build_B_f32", 0, 4.0 * N * (N + 64) / 2);
+        Prof p(c, "build_B_f32", 0, 12.0 * N * (N + 64) / 2);
         launch_build_sym_f32(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1, 1.0 / sigma, nullptr, L, N,
-                             m->d(B_DACC));
+                             m->d(B_DACC), B64);
     }
     lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
@@ -614,13 +620,12 @@ static int fit_gaussian_mixed(gpl_model* m) {
     float* Bi = m->b[B_BINV].as<float>();
     {
         Prof p(c, "lauum_f32", (double)N * N * N / 3, 4.0 * N * N);
-        gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0);
+        gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0, kFoldInverse);
     }
     m->have_binv = true;
     {
-        Prof p(c, "round_corr", 0, 4.0 * N * (N + 64) / 2);
-        launch_round_corr(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 1.0 / sigma, nullptr, Bi, N,
-                          m->d(B_PART));
+        Prof p(c, "round_corr", 0, 12.0 * N * (N + 64) / 2);
+        launch_round_corr(s, B64, Bi, N, m->d(B_PART), 1, 0, 0);
     }
     reduce_sum(s, m->d(B_PART), ntiles, m->d(B_SCAL) + 3);
     double* rhs = m->d(B_TERM);
@@ -643,15 +648,12 @@ static int fit_gaussian_mixed(gpl_model* m) {
             trmv_lower_f32(s, X, N, N, t1, d, 1);
         }
         vec_axpby(s, n, 1.0, d, 1.0, u, u);
-        // r = S y - u - S K (S u), K including the jitter; Ku = K (S u) = f for alpha = S u
-        vec_mul(s, n, m->d(B_SV), u, t1);
+        // r = S y - B u with the FP64 B (t1 = B u)
         {
-            Prof p(c, "kmatvec", 0, 0);
-            launch_kmatvec(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, t1, Ku, m->d(B_KMV), kmatvec_chunks(N));
+            Prof p(c, "symv_b64", 2.0 * N * N, 8.0 * N * (N + 64) / 2);
+            symv_lt_f64(s, B64, N, u, t1, m->d(B_KMV), nch, 1, 0, 0);
         }
-        vec_mul(s, n, m->d(B_SV), Ku, r);
-        vec_axpby(s, n, -1.0, r, -1.0, u, r);
-        vec_axpby(s, n, 1.0, r, 1.0, rhs, r);
+        vec_axpby(s, n, 1.0, rhs, -1.0, t1, r);
         rel_maxabs(s, d, u, n, m->d(B_SCAL) + 8);
         double h;
         HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
@@ -663,7 +665,8 @@ static int fit_gaussian_mixed(gpl_model* m) {
     m->refine_rel = rel;
     fill_f64(s, m->d(B_ALPHA), N, 0.0);
     vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
-    // f = K alpha is the last K (S u) (u updated before it)
+    // f = (K + jitter I) alpha = sigma (B u - u), B u from the last sweep (u updated before it)
+    vec_axpby(s, n, sigma, t1, -sigma, u, Ku);
     lik_loglik(s, GPL_LIK_GAUSSIAN, 0, sigma, n, m->d(B_FMEAN), m->d(B_Y), nullptr,
                m->has_offset ? m->d(B_OFF) : nullptr, m->d(B_W3));
     double* sc = m->d(B_SCAL);
@@ -987,7 +990,7 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
         float* Bi = m->b[B_BINV].as<float>();
         if (!m->have_binv) {
             Prof p(c, "lauum_f32", (double)N * N * N / 3, 4.0 * N * N);
-            gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0);
+            gemm_f32(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, 0, kFoldInverse);
         }
         Prof p(c, "grad", 0, 4.0 * N * (N + 64) / 2);
         launch_grad_tiles_f32(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, Bi, N, 1.0 / s2,

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_build_sym(const DevProg* __restrict__ P
                                                    OutT* __restrict__ out, int64_t ld, int64_t sF, int64_t sO,
                                                    const int64_t* __restrict__ nvec,
                                                    const double* __restrict__ scal, double* __restrict__ dacc,
-                                                   int64_t sDa) {
+                                                   int64_t sDa, double* __restrict__ out64, int64_t sO64) {
     extern __shared__ double sm[];
     double* Fr = sm;               // [nfeat][64]
     double* Fc = sm + nfeat * 64;  // [nfeat][64]
@@ -245,12 +245,14 @@ __global__ __launch_bounds__(256) void k_build_sym(const DevProg* __restrict__ P
         F += b * sF;
         out += b * sO;
         if (dacc) dacc += b * sDa;
+        if (out64) out64 += b * sO64;
         if (nvec) n = nvec[b];
         if (scal) sscal = scal[b];
     }
     int64_t ti, tj;
     lower_tile_c(xcd_remap_c(blockIdx.x, gridDim.x), ti, tj);
-    // dacc (optional): unrounded FP64 copies of the diagonal tiles, 64x64 column-major per block row
+    // dacc (optional): unrounded FP64 copies of the diagonal tiles, 64x64 column-major per block row;
+    // out64 (optional): an FP64 copy of the lower tiles (with the FP32 out)
     double* dt = (dacc && ti == tj) ? dacc + ti * 64 * 64 : nullptr;
     const int t = threadIdx.x, i = t & 63, cg = t >> 6;
     for (int e = t; e < nfeat * 64; e += 256) {
@@ -277,13 +279,14 @@ __global__ __launch_bounds__(256) void k_build_sym(const DevProg* __restrict__ P
             v = (gi == gj) ? 1.0 : 0.0;
         }
         out[gi + gj * ld] = (OutT)v;
+        if (out64) out64[gi + gj * ld] = v;
         if (dt) dt[i + c * 64] = v;
     }
 }
 
 static void build_sym_launch(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
                              int64_t N, double jitter, int mode, double sscal, const double* svec, double* out,
-                             float* out32, int64_t ld, double* dacc = nullptr) {
+                             float* out32, int64_t ld, double* dacc = nullptr, double* out64 = nullptr) {
     const int64_t T = N / 64;
     const int nfeat = hp.nfeat > 0 ? hp.nfeat : 1;
     const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
@@ -291,23 +294,23 @@ static void build_sym_launch(hipStream_t s, const DevProg* prog, const DevProg& 
     if (out)
         hipLaunchKernelGGL(k_build_sym<double>, grid, dim3(256), shm, s, prog, F, N, nfeat, n, jitter, mode, sscal,
                            svec, out, ld, (int64_t)0, (int64_t)0, (const int64_t*)nullptr, (const double*)nullptr,
-                           dacc, (int64_t)0);
+                           dacc, (int64_t)0, (double*)nullptr, (int64_t)0);
     else
         hipLaunchKernelGGL(k_build_sym<float>, grid, dim3(256), shm, s, prog, F, N, nfeat, n, jitter, mode, sscal,
                            svec, out32, ld, (int64_t)0, (int64_t)0, (const int64_t*)nullptr, (const double*)nullptr,
-                           dacc, (int64_t)0);
+                           dacc, (int64_t)0, out64, (int64_t)0);
 }
 
 // batch of problems of one padded size N: B_b = I + s_b (K_b + jitter I) s_b in FP32
 void launch_build_B_f32_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
                                 const int64_t* nvec, int64_t N, double jitter, const double* scal, float* out,
-                                int64_t sO, int batch, double* dacc, int64_t sDa) {
+                                int64_t sO, int batch, double* dacc, int64_t sDa, double* out64) {
     const int64_t T = N / 64;
     nfeat = nfeat > 0 ? nfeat : 1;
     const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
     hipLaunchKernelGGL(k_build_sym<float>, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), shm, s,
                        progs, F, N, nfeat, (int64_t)0, jitter, 1, 1.0, (const double*)nullptr, out, N, sF, sO, nvec,
-                       scal, dacc, sDa);
+                       scal, dacc, sDa, out64, sO);
 }
 
 void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
@@ -318,63 +321,39 @@ void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp,
 
 void launch_build_sym_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
                           int64_t N, double jitter, int mode, double sscal, const double* svec, float* out,
-                          int64_t ld, double* dacc) {
-    build_sym_launch(s, prog, hp, F, n, N, jitter, mode, sscal, svec, nullptr, out, ld, dacc);
+                          int64_t ld, double* dacc, double* out64) {
+    build_sym_launch(s, prog, hp, F, n, N, jitter, mode, sscal, svec, nullptr, out, ld, dacc, out64);
 }
 
 // ---------------------------------------------------------------------------
 // First-order log-determinant correction for a factor of the FP32-rounded B.
 // The FP32 path factors Bt, whose off-diagonal tiles are fl32(B) (its diagonal
 // tiles are kept in FP64), so log|B| = log|Bt| + tr(Bt^-1 (B - Bt))
-// + O(|Bt^-1 (B - Bt)|^2).  Per off-diagonal lower tile this kernel re-evaluates
-// B_ij in FP64 exactly as k_build_sym (mode 1) does, takes its FP32 rounding
-// error and accumulates 2 (B^-1)_ij (B_ij - fl32(B_ij)).
-// part[b * tiles + tile] receives the tile sum (deterministic order; 0 for
-// diagonal tiles).
+// + O(|Bt^-1 (B - Bt)|^2).  Per off-diagonal lower tile this kernel reads the
+// FP64 copy of B, takes its FP32 rounding error and accumulates
+// 2 (B^-1)_ij (B_ij - fl32(B_ij)).  part[b * tiles + tile] receives the tile sum
+// (deterministic order; 0 for diagonal tiles).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_round_corr(const DevProg* __restrict__ P, const double* __restrict__ F,
-                                                    int64_t ldf, int nfeat, int64_t n, double jitter, double sscal,
-                                                    const double* __restrict__ svec, const float* __restrict__ W,
-                                                    int64_t ldw, double* __restrict__ part, int64_t sF, int64_t sW,
-                                                    const int64_t* __restrict__ nvec, const double* __restrict__ scal) {
-    extern __shared__ double sm[];
-    double* Fr = sm;
-    double* Fc = sm + nfeat * 64;
+__global__ __launch_bounds__(256) void k_round_corr(const double* __restrict__ B64, const float* __restrict__ W,
+                                                    int64_t ld, double* __restrict__ part, int64_t sB, int64_t sW) {
     __shared__ double red[256];
-    {
-        const int64_t b = blockIdx.y;
-        P += (nvec ? b : 0);
-        F += b * sF;
-        W += b * sW;
-        part += b * (int64_t)gridDim.x;
-        if (nvec) n = nvec[b];
-        if (scal) sscal = scal[b];
-    }
+    B64 += blockIdx.y * sB;
+    W += blockIdx.y * sW;
+    part += blockIdx.y * (int64_t)gridDim.x;
     int64_t ti, tj;
     lower_tile_c(blockIdx.x, ti, tj);
     const int t = threadIdx.x, i = t & 63, cg = t >> 6;
-    for (int e = t; e < nfeat * 64; e += 256) {
-        const int f = e >> 6, p = e & 63;
-        Fr[e] = F[ti * 64 + p + (int64_t)f * ldf];
-        Fc[e] = F[tj * 64 + p + (int64_t)f * ldf];
-    }
-    __syncthreads();
-    const int64_t gi = ti * 64 + i;
     double acc = 0.0;
-    if (gi < n && ti != tj) {
-        const double si = svec ? svec[gi] : sscal;
+    if (ti != tj) {
+        const int64_t gi = ti * 64 + i;
+#pragma unroll 4
         for (int q = 0; q < 16; ++q) {
-            const int c = cg + 4 * q;
-            const int64_t gj = tj * 64 + c;
-            if (gj >= n) continue;
-            double v = prog_value(P, Fr + i, 64, Fc + c, 64);
-            const double sj = svec ? svec[gj] : sscal;
-            v = si * v * sj;
-            const double dv = v - (double)(float)v;
-            acc += 2.0 * (double)W[gi + gj * ldw] * dv;
+            const int64_t gj = tj * 64 + cg + 4 * q;
+            const double v = B64[gi + gj * ld];
+            acc += (double)W[gi + gj * ld] * (v - (double)(float)v);
         }
     }
-    red[t] = acc;
+    red[t] = 2.0 * acc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (t < o) red[t] += red[t + o];
@@ -383,24 +362,11 @@ __global__ __launch_bounds__(256) void k_round_corr(const DevProg* __restrict__ 
     if (t == 0) part[blockIdx.x] = red[0];
 }
 
-void launch_round_corr(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n, int64_t N,
-                       double jitter, double sscal, const double* svec, const float* W, int64_t ldw, double* part) {
+void launch_round_corr(hipStream_t s, const double* B64, const float* W, int64_t N, double* part, int batch,
+                       int64_t sB, int64_t sW) {
     const int64_t T = N / 64;
-    const int nfeat = hp.nfeat > 0 ? hp.nfeat : 1;
-    const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
-    hipLaunchKernelGGL(k_round_corr, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), shm, s, prog, F, N, nfeat, n,
-                       jitter, sscal, svec, W, ldw, part, (int64_t)0, (int64_t)0, (const int64_t*)nullptr,
-                       (const double*)nullptr);
-}
-
-void launch_round_corr_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
-                               const int64_t* nvec, int64_t N, double jitter, const double* scal, const float* W,
-                               int64_t sW, double* part, int batch) {
-    const int64_t T = N / 64;
-    nfeat = nfeat > 0 ? nfeat : 1;
-    const size_t shm = (size_t)nfeat * 64 * 2 * sizeof(double);
-    hipLaunchKernelGGL(k_round_corr, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), shm, s, progs, F,
-                       N, nfeat, (int64_t)0, jitter, 1.0, (const double*)nullptr, W, N, part, sF, sW, nvec, scal);
+    hipLaunchKernelGGL(k_round_corr, dim3((unsigned)(T * (T + 1) / 2), (unsigned)batch), dim3(256), 0, s, B64, W, N,
+                       part, sB, sW);
 }
 
 // ---------------------------------------------------------------------------

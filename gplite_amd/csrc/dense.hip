@@ -584,6 +584,81 @@ void symv_lower_f64(hipStream_t s, const double* K, int64_t ld, int64_t N, const
     hipLaunchKernelGGL(k_symv, dim3((unsigned)(N / 64)), dim3(256), 0, s, K, ld, N / 64, v, y);
 }
 
+// y = A v for a symmetric FP64 matrix held as its lower 64x64 tiles (column-
+// major, leading dimension ld; diagonal tiles complete).  Grid (T, nchunk,
+// batch): block (ti, ch) sums the tiles of block row ti whose column tile lies
+// in chunk ch; tiles above the diagonal are read as the transpose of the
+// stored tile (tj, ti), staged through LDS so every global read is coalesced.
+// A second pass adds the chunk partials in a fixed order.
+__global__ __launch_bounds__(256) void k_symv_lt(const double* __restrict__ A, int64_t ld, int64_t T, int nchunk,
+                                                 const double* __restrict__ v, double* __restrict__ part, int64_t sA,
+                                                 int64_t sV) {
+    const int64_t b = blockIdx.z;
+    A += b * sA;
+    v += b * sV;
+    part += b * (int64_t)nchunk * T * 64;
+    __shared__ double tile[64][65];
+    __shared__ double vc[64];
+    __shared__ double red[256];
+    const int64_t ti = blockIdx.x;
+    const int ch = blockIdx.y;
+    const int t = threadIdx.x, e = t & 63, p = t >> 6;
+    const int64_t per = (T + nchunk - 1) / nchunk, tb = ch * per, te = (tb + per < T) ? tb + per : T;
+    double s = 0.0;
+    for (int64_t tj = tb; tj < te; ++tj) {
+        __syncthreads();
+        if (t < 64) vc[t] = v[tj * 64 + t];
+        if (tj > ti) {
+            const double* At = A + tj * 64 + ti * 64 * ld;  // stored tile (tj, ti)
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int c = p * 16 + u;
+                tile[c][e] = At[e + (int64_t)c * ld];
+            }
+        }
+        __syncthreads();
+        if (tj <= ti) {
+            const double* At = A + ti * 64 + tj * 64 * ld;
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int c = p * 16 + u;
+                s += At[e + (int64_t)c * ld] * vc[c];
+            }
+        } else {
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int r = p * 16 + u;
+                s += tile[e][r] * vc[r];
+            }
+        }
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < 64) part[(int64_t)ch * T * 64 + ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+}
+
+__global__ void k_symv_lt_fin(const double* __restrict__ part, int64_t N, int nchunk, double* __restrict__ y,
+                              int64_t sV) {
+    const int64_t b = blockIdx.y;
+    part += b * (int64_t)nchunk * N;
+    y += b * sV;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * N + i];
+    y[i] = s;
+}
+
+// part: batch * nchunk * N doubles
+void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
+                 int batch, int64_t sA, int64_t sV) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_symv_lt, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, A, N, T,
+                       nchunk, v, part, sA, sV);
+    hipLaunchKernelGGL(k_symv_lt_fin, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, N,
+                       nchunk, y, sV);
+}
+
 // one wave per column: out[j] = sum_i A[i + j*ld]^2; lower != 0 starts at the
 // column's diagonal tile (the upper tiles of a lower-triangular buffer are never written)
 __global__ __launch_bounds__(256) void k_colsumsq(const double* __restrict__ A, int64_t ld, int64_t nrow,

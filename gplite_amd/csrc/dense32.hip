@@ -72,17 +72,19 @@ __device__ __forceinline__ void g32_store(float (*S)[BM + 4], const float4 (&r)[
 // C = beta*C + alpha*op(A) op(B), FP32 storage and FP32 MFMA products.  Tile
 // modes as gemm2_f64.  Each wave owns a (BM/2)^2 quadrant of 32x32 blocks;
 // lane half h = lane/32 takes k = 8h..8h+7 of a 16-wide chunk over eight MFMA
-// steps.  The FP32 accumulators cover at most 64 consecutive k: every 64 k
-// they are added into FP64 accumulators and cleared, and the epilogue forms
-// beta*C + alpha*acc in FP64 with one rounding to FP32.  A long FP32 running
-// sum loses about sqrt(k) ulps of the partial sums; in trailing updates that
-// cancel (A - P P^T, with |P P^T| >> |result|) that error dominates log|B|.
-template <int TA, int TB, int BM>
+// steps.  With FOLD > 0 the FP32 accumulators cover at most 16*FOLD
+// consecutive k: they are then added into FP64 accumulators and cleared, and
+// the epilogue forms beta*C + alpha*acc in FP64 with one rounding to FP32.  A
+// long FP32 running sum loses about sqrt(k) ulps of its partial sums; in
+// trailing updates that cancel (A - P P^T with |P P^T| >> |result|) that error
+// dominates log|B|.  FOLD == 0 keeps plain FP32 accumulation.
+template <int TA, int TB, int BM, int FOLD>
 __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
                                                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
                                                 float* C, int64_t ldc, int tiles, int64_t sA, int64_t sB,
                                                 int64_t sC) {
     constexpr int WT = BM / 2, NBK = WT / 32, AL = BM / 64;
+    constexpr int NF = FOLD > 0 ? NBK : 1;
     A += blockIdx.y * sA;  // batch: one problem per grid row
     B += blockIdx.y * sB;
     C += blockIdx.y * sC;
@@ -105,16 +107,21 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
     const int wr = wave >> 1, wc = wave & 1;
 
     f16v acc[NBK][NBK];
-    double acc64[NBK][NBK][16];
+    double acc64[NF][NF][FOLD > 0 ? 16 : 1];
 #pragma unroll
     for (int i = 0; i < NBK; ++i)
 #pragma unroll
         for (int j = 0; j < NBK; ++j)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                acc[i][j][v] = 0.f;
-                acc64[i][j][v] = 0.0;
-            }
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    if constexpr (FOLD > 0) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc64[i][j][v] = 0.0;
+    }
 
     float4 ra[AL], rb[AL];
     if (nch > 0) {
@@ -145,16 +152,18 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
                 for (int j = 0; j < NBK; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        if ((c & 3) == 3 || !more) {  // fold 64 k of FP32 sums into FP64 (block-uniform branch)
+        if constexpr (FOLD > 0) {
+            if ((c % FOLD) == FOLD - 1 || !more) {  // block-uniform branch
 #pragma unroll
-            for (int i = 0; i < NBK; ++i)
+                for (int i = 0; i < NBK; ++i)
 #pragma unroll
-                for (int j = 0; j < NBK; ++j)
+                    for (int j = 0; j < NBK; ++j)
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        acc64[i][j][v] += (double)acc[i][j][v];
-                        acc[i][j][v] = 0.f;
-                    }
+                        for (int v = 0; v < 16; ++v) {
+                            acc64[i][j][v] += (double)acc[i][j][v];
+                            acc[i][j][v] = 0.f;
+                        }
+            }
         }
         if (more) {
             g32_store<BM, !TA>(As[buf ^ 1], ra);
@@ -162,33 +171,44 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
         }
         __syncthreads();
     }
-    // v_mfma_f32_32x32x2f32 D layout: item v -> row 8*(v/4) + 4*(lane/32) + v%4, column lane%32
+    // v_mfma_f32_32x32x2f32 D layout: item v -> row 8*(v/4) + 4*(lane/32) + v%4, column lane%32,
+    // so items 4g..4g+3 of a lane are four consecutive rows: one float4 per group
 #pragma unroll
     for (int i = 0; i < NBK; ++i)
 #pragma unroll
         for (int j = 0; j < NBK; ++j) {
             const int64_t rb0 = row0 + wr * WT + i * 32, cb0 = col0 + wc * WT + j * 32;
             if (rb0 >= M || cb0 >= N) continue;
-            float* p = C + rb0 + (cb0 + l32) * ldc;
+            float* p = C + rb0 + 4 * h + (cb0 + l32) * ldc;
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int r = 8 * (v >> 2) + 4 * h + (v & 3);
-                const double o = (double)alpha * acc64[i][j][v];
-                p[r] = (float)((beta == 0.f) ? o : (double)beta * (double)p[r] + o);
+            for (int g = 0; g < 4; ++g) {
+                float4* q = reinterpret_cast<float4*>(p + 8 * g);
+                float4 cv = (beta == 0.f) ? float4{0.f, 0.f, 0.f, 0.f} : *q;
+                float o[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int v = 4 * g + e;
+                    const float cvi = (e == 0) ? cv.x : (e == 1) ? cv.y : (e == 2) ? cv.z : cv.w;
+                    if constexpr (FOLD > 0)
+                        o[e] = (float)((double)beta * (double)cvi + (double)alpha * acc64[i][j][v]);
+                    else
+                        o[e] = beta * cvi + alpha * acc[i][j][v];
+                }
+                *q = float4{o[0], o[1], o[2], o[3]};
             }
         }
 }
 
-template <int BM>
+template <int BM, int FOLD>
 static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha,
                           const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
                           int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
     const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
     dim3 grid((unsigned)nblk, (unsigned)batch), block(256);
-#define GPL_G32(TAv, TBv)                                                                                           \
-    hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, beta, C, ldc, \
-                       tiles, sA, sB, sC)
+#define GPL_G32(TAv, TBv)                                                                                             \
+    hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM, FOLD>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, beta, C, \
+                       ldc, tiles, sA, sB, sC)
     if (!ta && !tb) GPL_G32(0, 0);
     else if (!ta && tb) GPL_G32(0, 1);
     else if (ta && !tb) GPL_G32(1, 0);
@@ -196,22 +216,34 @@ static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 #undef GPL_G32
 }
 
+// fold != 0: FP32 sums folded into FP64 every 64 k (64-wide tiles unless bm
+// asks for 128: the FP64 accumulators of a 128-wide tile halve occupancy).
+// fold == 0: plain FP32 accumulation, 128-wide tiles when the grid is large.
 void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
-                      int batch, int64_t sA, int64_t sB, int64_t sC) {
+                      int batch, int64_t sA, int64_t sB, int64_t sC, int fold) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
     if (bm == 0) {
         const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
         const int64_t n128 = ((tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2) * batch;
-        bm = (n128 >= 256) ? 128 : 64;
+        bm = (!fold && n128 >= 256) ? 128 : 64;
     }
-    if (bm == 128) gemm32_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
-    else gemm32_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+    if (fold) {
+        if (bm == 128)
+            gemm32_launch<128, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+        else
+            gemm32_launch<64, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+    } else {
+        if (bm == 128)
+            gemm32_launch<128, 0>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+        else
+            gemm32_launch<64, 0>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+    }
 }
 
 void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
-              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm) {
-    gemm_f32_batched(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, bm, 1, 0, 0, 0);
+              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm, int fold) {
+    gemm_f32_batched(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, bm, 1, 0, 0, 0, fold);
 }
 
 // ---------------------------------------------------------------------------

@@ -52,7 +52,7 @@ void launch_build_sym_f64(hipStream_t s, const DevProg* prog, const DevProg& hp,
                           const double* svec, double* out, int64_t ld);
 void launch_build_sym_f32(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n,
                           int64_t N, double jitter, int mode, double sscal, const double* svec, float* out,
-                          int64_t ld, double* dacc);
+                          int64_t ld, double* dacc, double* out64);
 // out (Na x Nb col-major, ld) = K(Fa, Fb); zero outside [na) x [nb).
 void launch_build_cross(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* Fa,
                         int64_t lda_f, int64_t na, int64_t Na, const double* Fb, int64_t ldb_f,
@@ -92,7 +92,8 @@ void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
                           int64_t ldc, int tiles);
 // FP32 storage and FP32 MFMA with FP32 accumulation (dense32.hip)
 void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
-              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm);
+              int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
+              int fold = 1);
 void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N);
 void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans);
 void colsumsq_f32(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out, int lower);
@@ -121,6 +122,9 @@ void gemv_t_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_
 // out[j] = sum_i A[i + j*ld]^2 for j < ncol (deterministic column sums of squares)
 void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
                   int lower);
+// y = A v, A symmetric FP64 held as lower 64x64 tiles (part: batch * nchunk * N)
+void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
+                 int batch, int64_t sA, int64_t sV);
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
@@ -158,14 +162,11 @@ void vec_recip(hipStream_t s, int64_t n, const double* a, double* out);
 void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,
                              const DevProg* progs, double* F, int64_t N, int64_t sF, int batch);
 // first-order log|B| correction for factors of fl32(B) (per lower tile partial sums)
-void launch_round_corr(hipStream_t s, const DevProg* prog, const DevProg& hp, const double* F, int64_t n, int64_t N,
-                       double jitter, double sscal, const double* svec, const float* W, int64_t ldw, double* part);
-void launch_round_corr_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
-                               const int64_t* nvec, int64_t N, double jitter, const double* scal, const float* W,
-                               int64_t sW, double* part, int batch);
+void launch_round_corr(hipStream_t s, const double* B64, const float* W, int64_t N, double* part, int batch,
+                       int64_t sB, int64_t sW);
 void launch_build_B_f32_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
                                 const int64_t* nvec, int64_t N, double jitter, const double* scal, float* out,
-                                int64_t sO, int batch, double* dacc, int64_t sDa);
+                                int64_t sO, int batch, double* dacc, int64_t sDa, double* out64);
 void launch_kmatvec_batched(hipStream_t s, const DevProg* progs, int nfeat, const double* F, int64_t sF,
                             const int64_t* nvec, int64_t N, double jitter, const double* v, double* y, int64_t sV,
                             double* part, int nchunk, int batch);
@@ -174,7 +175,10 @@ void launch_grad_tiles_f32_batched(hipStream_t s, const DevProg* progs, const De
                                    const double* wsc, const double* alpha, int64_t sV, double* part, int batch);
 void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
-                      int batch, int64_t sA, int64_t sB, int64_t sC);
+                      int batch, int64_t sA, int64_t sB, int64_t sC, int fold = 1);
+// FP64 folding in the GEMMs of L^-1 and B^-1 = X^T X (they feed the gradient and
+// the predictive variance, not the log evidence)
+constexpr int kFoldInverse = 1;
 void gemm_mixed_f32f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                                const float* A, int64_t lda, const double* B, int64_t ldb, double beta, float* C,
                                int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC);

@@ -82,7 +82,7 @@ size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
     size_t mats = 1;                                       // factor of B
     if (lik.kind != GPL_LIK_GAUSSIAN) mats += 1;           // stored K
     if (o.approx == GPL_APPROX_EP) mats += 1;              // L^-1
-    if (mixed) mats += 2;                                  // L^-1 and B^-1
+    if (mixed) mats += 4;                                  // L^-1, B^-1 and the FP64 copy of B
     else if (o.flags & GPL_FLAG_INVERSE) mats += 1;        // L^-1
     const size_t esz = mixed ? 4 : 8;
     const size_t T = N / 64;
