@@ -42,8 +42,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--forms", default="fp64,mixed,mixed_batched")
-    ap.add_argument("--restarts", type=int, default=8, help="problems per pass in the batched form")
-    ap.add_argument("--refine", type=int, default=4, help="refinement sweeps in the batched form")
+    ap.add_argument("--restarts", type=int, default=16, help="problems per pass in the batched form")
+    ap.add_argument("--refine", type=int, default=3, help="refinement sweeps in the batched form")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

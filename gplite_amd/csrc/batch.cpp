@@ -68,6 +68,70 @@ struct gpl_batch {
 static int red_stride(const gpl_batch* B) { return 6 + std::max(1, B->hp.ngrad); }
 
 // --------------------------------------------------------------------------
+// Recursive panel steps (every problem of the bucket at once).  A panel of W
+// columns is split into halves: factor the left half, update the right half
+// with one GEMM of depth W/2 over all rows below, factor the right half.  The
+// 64-column leaves factor their FP64 diagonal tile and solve the rows below in
+// FP64; the update GEMMs get depths 256/128/64 instead of a chain of depth-64
+// updates.
+// --------------------------------------------------------------------------
+static void potrf_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t K0, int64_t W) {
+    const int nb = (int)k.idx.size();
+    const int64_t N = k.N, N2 = N * N;
+    float* L = k.f(BB_L);
+    double* Dinv = k.d(BB_DINV);
+    double* Dacc = k.d(BB_DACC);
+    if (W == 64) {
+        const int64_t kk = K0 / 64;
+        {
+            Prof p(c, "batch_potrf_diag", 0, 0, s);
+            potrf_diag3_f32_batched(s, L, N, kk, Dinv, k.d(BB_LDIAG), k.b[BB_INFO].as<int>(), nb, N2, N * 64, N, Dacc);
+        }
+        const int64_t M = N - (K0 + 64);
+        if (M > 0) {
+            float* L21 = L + (K0 + 64) + K0 * N;
+            Prof p(c, "batch_trsm_f64", 2.0 * nb * M * 64 * 64, 0, s);
+            gemm_mixed_f32f64_batched(s, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + kk * 64 * 64, 64, 0.0, L21, N, 0, nb, N2,
+                                      N * 64, N2);
+        }
+        return;
+    }
+    const int64_t h = (W / 128) * 64, W2 = W - h, M = N - (K0 + h);
+    potrf_panel_rec(c, s, k, K0, h);
+    const float* P = L + (K0 + h) + K0 * N;
+    {
+        Prof p(c, "batch_panel_f32", 2.0 * nb * M * W2 * h, 0, s);
+        gemm_f32_batched(s, 0, 1, M, W2, h, -1.f, P, N, P, N, 1.f, L + (K0 + h) * (N + 1), N, 0, k.bm, nb, N2, N2,
+                         N2);
+        dacc_update(s, Dacc + ((K0 + h) / 64) * 64 * 64, P, N, W2 / 64, h, nb, N * 64, N2);
+    }
+    potrf_panel_rec(c, s, k, K0 + h, W2);
+}
+
+// rows [I0, I0+H) of X = L^-1, given the updates from all rows above I0
+static void trtri_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t I0, int64_t H) {
+    const int nb = (int)k.idx.size();
+    const int64_t N = k.N, N2 = N * N;
+    const float* L = k.f(BB_L);
+    float* X = k.f(BB_X32);
+    if (H == 64) {
+        const int64_t i = I0 / 64, Wc = I0 + 64;
+        Prof p(c, "batch_trtri_diag_f64", 2.0 * nb * 64 * 64 * Wc, 0, s);
+        gemm_mixed_f64f32_batched(s, 0, 0, 64, Wc, 64, 1.0, k.d(BB_DINV) + i * 64 * 64, 64, X + I0, N, 0.0, X + I0, N,
+                                  0, nb, N * 64, N2, N2);
+        return;
+    }
+    const int64_t h = (H / 128) * 64, H2 = H - h, Wc = I0 + h;
+    trtri_panel_rec(c, s, k, I0, h);
+    {
+        Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * H2 * Wc * h, 0, s);
+        gemm_f32_batched(s, 0, 0, H2, Wc, h, -1.f, L + (I0 + h) + I0 * N, N, X + I0, N, 1.f, X + I0 + h, N, 0, k.bm, nb,
+                         N2, N2, N2, kFoldInverse);
+    }
+    trtri_panel_rec(c, s, k, I0 + h, H2);
+}
+
+// --------------------------------------------------------------------------
 // the per-bucket pipeline (enqueued on one stream; captured into the graph)
 // --------------------------------------------------------------------------
 static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
@@ -80,10 +144,8 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     float* L = k.f(BB_L);
     float* X = k.f(BB_X32);
     float* Bi = k.f(BB_BI);
-    double* Dinv = k.d(BB_DINV);
     double* Dacc = k.d(BB_DACC);  // FP64 diagonal tiles, same layout as Dinv
     double* ldiag = k.d(BB_LDIAG);
-    int* info = k.b[BB_INFO].as<int>();
     const int bm = k.bm;
     const int64_t NBo = k.nbo;
     {
@@ -91,66 +153,67 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         launch_build_B_f32_batched(s, progs, k.nfeat, k.d(BB_F), sF, nvec, N, B->opts.jitter, k.d(BB_SCAL), L, N2, nb,
                                    Dacc, N * 64, k.d(BB_B64));
     }
-    // blocked Cholesky (FP32 storage, FP64 diagonal blocks and panel solves)
+    // blocked Cholesky (FP32 storage, FP64 diagonal blocks and panel solves) with
+    // one panel of look-ahead: the next panel's columns are updated first and
+    // factored on the side stream while the main stream applies the rest of the
+    // trailing update.  The fork and join are events, so graph capture records
+    // them as dependencies between the two streams.
+    hipStream_t s2 = c->side;
+    potrf_panel_rec(c, s, k, 0, std::min<int64_t>(NBo, N));
     for (int64_t K0 = 0; K0 < N; K0 += NBo) {
         const int64_t W = std::min<int64_t>(NBo, N - K0);
-        for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
-            const int64_t kk = j0 / 64;
-            {
-                Prof p(c, "batch_potrf_diag", 0, 0, s);
-                potrf_diag3_f32_batched(s, L, N, kk, Dinv, ldiag, info, nb, N2, N * 64, N, Dacc);
-            }
-            const int64_t M = N - (j0 + 64);
-            if (M <= 0) continue;
-            float* L21 = L + (j0 + 64) + j0 * N;
-            {
-                Prof p(c, "batch_trsm_f64", 2.0 * nb * M * 64 * 64, 0, s);
-                gemm_mixed_f32f64_batched(s, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + kk * 64 * 64, 64, 0.0, L21, N, 0, nb,
-                                          N2, N * 64, N2);
-            }
-            const int64_t Wr = K0 + W - (j0 + 64);
-            if (Wr > 0) {
-                Prof p(c, "batch_panel_f32", 2.0 * nb * M * Wr * 64, 0, s);
-                gemm_f32_batched(s, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0, nb, N2,
-                                 N2, N2);
-                dacc_update(s, Dacc + (kk + 1) * 64 * 64, L21, N, Wr / 64, 64, nb, N * 64, N2);
-            }
+        const int64_t K1 = K0 + W, R = N - K1;
+        if (R <= 0) break;
+        const int64_t W1 = std::min<int64_t>(NBo, R);
+        const float* P = L + K1 + K0 * N;
+        {
+            Prof p(c, "batch_syrk_f32", 2.0 * nb * R * W1 * W, 0, s);
+            gemm_f32_batched(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, bm, nb, N2, N2, N2);
+            dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, nb, N * 64, N2);
         }
-        const int64_t R = N - (K0 + W);
-        if (R > 0) {
-            const float* P = L + (K0 + W) + K0 * N;
-            Prof p(c, "batch_syrk_f32", (double)nb * R * R * W, 0, s);
-            gemm_f32_batched(s, 0, 1, R, R, W, -1.f, P, N, P, N, 1.f, L + (K0 + W) * (N + 1), N, 1, 0, nb, N2, N2, N2);
-            dacc_update(s, Dacc + ((K0 + W) / 64) * 64 * 64, P, N, R / 64, W, nb, N * 64, N2);
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        potrf_panel_rec(c, s2, k, K1, W1);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - W1;
+        if (R2 > 0) {
+            const float* P2 = L + (K1 + W1) + K0 * N;
+            Prof p(c, "batch_syrk_f32", (double)nb * R2 * R2 * W, 0, s);
+            gemm_f32_batched(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, bm, nb, N2,
+                             N2, N2);
+            dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
         }
+        hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    // X = L^-1 (FP32, FP64 diagonal-block applications)
+    // X = L^-1 (FP32, FP64 diagonal-block applications), same look-ahead over row panels
     set_identity_f32_batched(s, X, N, N, nb, N2);
+    trtri_panel_rec(c, s, k, 0, std::min<int64_t>(NBo, N));
     for (int64_t I0 = 0; I0 < N; I0 += NBo) {
         const int64_t H = std::min<int64_t>(NBo, N - I0);
-        for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
-            const int64_t i = r0 / 64, Wc = r0 + 64;
-            {
-                Prof p(c, "batch_trtri_diag_f64", 2.0 * nb * 64 * 64 * Wc, 0, s);
-                gemm_mixed_f64f32_batched(s, 0, 0, 64, Wc, 64, 1.0, Dinv + i * 64 * 64, 64, X + r0, N, 0.0, X + r0, N, 0,
-                                          nb, N * 64, N2, N2);
-            }
-            const int64_t Mr = I0 + H - (r0 + 64);
-            if (Mr > 0) {
-                Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * Mr * Wc * 64, 0, s);
-                gemm_f32_batched(s, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0,
-                                 bm, nb, N2, N2, N2, kFoldInverse);
-            }
+        const int64_t I1 = I0 + H, R = N - I1;
+        if (R <= 0) break;
+        const int64_t H1 = std::min<int64_t>(NBo, R), Wc = I0 + H;
+        {
+            Prof p(c, "batch_trtri_f32", 2.0 * nb * H1 * Wc * H, 0, s);
+            gemm_f32_batched(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, bm, nb, N2,
+                             N2, N2, kFoldInverse);
         }
-        const int64_t R = N - (I0 + H);
-        if (R > 0) {
-            const int64_t Wc = I0 + H;
-            Prof p(c, "batch_trtri_f32", 2.0 * nb * R * Wc * H, 0, s);
-            gemm_f32_batched(s, 0, 0, R, Wc, H, -1.f, L + (I0 + H) + I0 * N, N, X + I0, N, 1.f, X + I0 + H, N, 0, bm, nb,
-                             N2, N2, N2, kFoldInverse);
+        hipEventRecord(c->ev_a, s);
+        hipStreamWaitEvent(s2, c->ev_a, 0);
+        trtri_panel_rec(c, s2, k, I1, H1);
+        hipEventRecord(c->ev_p, s2);
+        const int64_t R2 = R - H1;
+        if (R2 > 0) {
+            Prof p(c, "batch_trtri_f32", 2.0 * nb * R2 * Wc * H, 0, s);
+            gemm_f32_batched(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0,
+                             bm, nb, N2, N2, N2, kFoldInverse);
         }
+        hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is static)
+    // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is
+    // static) on the side stream, while the main stream forms B^-1 = X^T X
+    hipEventRecord(c->ev_a, s);
+    hipStreamWaitEvent(s2, c->ev_a, 0);
     const int64_t NV = (int64_t)nb * N;
     double* rhs = k.d(BB_RHS);
     double* u = k.d(BB_U);
@@ -159,32 +222,34 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     double* t1 = k.d(BB_T1);
     double* Ku = k.d(BB_KU);
     double* sv = k.d(BB_SV);
-    seg_scale(s, nullptr, k.d(BB_SCAL), nvec, N, sv, nb);       // S = 1/sigma on valid rows, 0 on padding
-    vec_mul(s, NV, sv, k.d(BB_Y), rhs);                         // S y
-    fill_f64(s, u, NV, 0.0);
-    vec_axpby(s, NV, 1.0, rhs, 0.0, nullptr, r);
+    seg_scale(s2, nullptr, k.d(BB_SCAL), nvec, N, sv, nb);      // S = 1/sigma on valid rows, 0 on padding
+    vec_mul(s2, NV, sv, k.d(BB_Y), rhs);                        // S y
+    fill_f64(s2, u, NV, 0.0);
+    vec_axpby(s2, NV, 1.0, rhs, 0.0, nullptr, r);
     const int sweeps = B->opts.refine_iters > 0 ? B->opts.refine_iters : 4;
     for (int it = 0; it < sweeps; ++it) {
         {
-            Prof p(c, "batch_refine_f32", 0, 0, s);
-            trmv_lower_f32_batched(s, X, N, N, r, t1, 0, nb, N2, N);
-            trmv_lower_f32_batched(s, X, N, N, t1, dd, 1, nb, N2, N);
+            Prof p(c, "batch_refine_f32", 0, 0, s2);
+            trmv_lower_f32_batched(s2, X, N, N, r, t1, 0, nb, N2, N, k.d(BB_KMV), k.nchunk);
+            trmv_lower_f32_batched(s2, X, N, N, t1, dd, 1, nb, N2, N, k.d(BB_KMV), k.nchunk);
         }
-        vec_axpby(s, NV, 1.0, dd, 1.0, u, u);
-        if (it == sweeps - 1) relmax_accumulate(s, dd, u, NV, B->relmax.as<double>());
+        vec_axpby(s2, NV, 1.0, dd, 1.0, u, u);
+        if (it == sweeps - 1) relmax_accumulate(s2, dd, u, NV, B->relmax.as<double>());
         {  // r = S y - B u with the FP64 copy of B
-            Prof p(c, "batch_symv_b64", 2.0 * nb * N * N, 0, s);
-            symv_lt_f64(s, k.d(BB_B64), N, u, Ku, k.d(BB_KMV), k.nchunk, nb, N2, N);
+            Prof p(c, "batch_symv_b64", 2.0 * nb * N * N, 0, s2);
+            symv_lt_f64(s2, k.d(BB_B64), N, u, Ku, k.d(BB_KMV), k.nchunk, nb, N2, N);
         }
-        vec_axpby(s, NV, 1.0, rhs, -1.0, Ku, r);
+        vec_axpby(s2, NV, 1.0, rhs, -1.0, Ku, r);
     }
     double* alpha = k.d(BB_ALPHA);
-    vec_mul(s, NV, sv, u, alpha);
+    vec_mul(s2, NV, sv, u, alpha);
+    hipEventRecord(c->ev_p, s2);
     // B^-1 = X^T X and the gradient
     {
         Prof p(c, "batch_lauum_f32", (double)nb * N * N * N / 3, 0, s);
         gemm_f32_batched(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, bm, nb, N2, N2, N2, kFoldInverse);
     }
+    hipStreamWaitEvent(s, c->ev_p, 0);
     const int64_t ntiles = T * (T + 1) / 2;
     {
         Prof p(c, "batch_grad", 0, 0, s);
@@ -465,7 +530,7 @@ extern "C" int gpl_batch_tune(gpl_batch* B, int32_t cap, int32_t* bucket_n, int3
         float best = 1e30f;
         int bbm = 64, bnb = 256;
         for (int cbm : {64, 128})
-            for (int cnb : {128, 256}) {
+            for (int cnb : {128, 256, 512, 1024}) {
                 if (cnb > k->N && cnb != 128) continue;
                 k->bm = cbm;
                 k->nbo = cnb;
@@ -502,8 +567,9 @@ extern "C" int gpl_batch_set_tuning(gpl_batch* B, int32_t count, const int32_t* 
     for (int j = 0; j < count; ++j)
         for (auto* k : B->buckets)
             if (k->N == bucket_n[j]) {
-                if ((bm[j] != 64 && bm[j] != 128) || (nbo[j] != 128 && nbo[j] != 256))
-                    return fail(GPL_ERR_ARG, "tile edge must be 64/128 and panel width 128/256");
+                if ((bm[j] != 64 && bm[j] != 128) ||
+                    (nbo[j] != 128 && nbo[j] != 256 && nbo[j] != 512 && nbo[j] != 1024))
+                    return fail(GPL_ERR_ARG, "tile edge must be 64/128 and panel width 128/256/512/1024");
                 k->bm = bm[j];
                 k->nbo = nbo[j];
             }

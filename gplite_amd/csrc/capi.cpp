@@ -644,8 +644,8 @@ static int fit_gaussian_mixed(gpl_model* m) {
     for (it = 1; it <= maxit; ++it) {
         {
             Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
-            trmv_lower_f32(s, X, N, N, r, t1, 0);
-            trmv_lower_f32(s, X, N, N, t1, d, 1);
+            trmv_lower_f32(s, X, N, N, r, t1, 0, m->d(B_KMV), nch);
+            trmv_lower_f32(s, X, N, N, t1, d, 1, m->d(B_KMV), nch);
         }
         vec_axpby(s, n, 1.0, d, 1.0, u, u);
         // r = S y - B u with the FP64 B (t1 = B u)

@@ -404,29 +404,53 @@ void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, dou
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_dacc_update(double* __restrict__ D, const float* __restrict__ P, int64_t ldp,
                                                      int64_t kdepth, int64_t sD, int64_t sP) {
+    // D_j -= P_j P_j^T on v_mfma_f64_16x16x4f64: each wave owns a 32x32 quadrant
+    // (2x2 blocks of 16x16); P_j is staged 16 k at a time in LDS as FP64.
+    // f64 16x16x4 layouts: A lane l -> A[l%16][k=l/16], B lane l -> B[k=l/16][l%16],
+    // D item v of lane l -> row l/16 + 4v, column l%16.
     D += blockIdx.y * sD + (int64_t)blockIdx.x * 64 * 64;
     P += blockIdx.y * sP + (int64_t)blockIdx.x * 64;
-    __shared__ double Ps[64][65];
-    const int t = threadIdx.x, i = t & 63, cg = t >> 6;
-    double acc[16];
+    __shared__ double Ps[16][64 + 2];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, lk = lane >> 4;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    d4 acc[2][2];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    for (int64_t k0 = 0; k0 < kdepth; k0 += 64) {
-        for (int e = t; e < 64 * 64; e += 256) {
-            const int r = e & 63, kk = e >> 6;
-            Ps[kk][r] = (double)P[r + (k0 + kk) * ldp];
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = 0; k0 < kdepth; k0 += 16) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = (t >> 6) + 4 * q;
+            Ps[kk][lane] = (double)P[lane + (k0 + kk) * ldp];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 64; ++kk) {
-            const double pi = Ps[kk][i];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) acc[q] += pi * Ps[kk][cg + 4 * q];
+        for (int ks = 0; ks < 16; ks += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) a[bi] = Ps[ks + lk][32 * wr + 16 * bi + l16];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) b[bj] = Ps[ks + lk][32 * wc + 16 * bj + l16];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
         }
-        __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < 16; ++q) D[i + (cg + 4 * q) * 64] -= acc[q];
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 32 * wr + 16 * bi + lk + 4 * v, col = 32 * wc + 16 * bj + l16;
+                D[row + col * 64] -= acc[bi][bj][v];
+            }
 }
 
 void dacc_update(hipStream_t s, double* D, const float* P, int64_t ldp, int64_t ntiles, int64_t kdepth, int batch,
@@ -637,8 +661,8 @@ __global__ __launch_bounds__(256) void k_symv_lt(const double* __restrict__ A, i
     if (t < 64) part[(int64_t)ch * T * 64 + ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
 }
 
-__global__ void k_symv_lt_fin(const double* __restrict__ part, int64_t N, int nchunk, double* __restrict__ y,
-                              int64_t sV) {
+__global__ void k_chunk_sum(const double* __restrict__ part, int64_t N, int nchunk, double* __restrict__ y,
+                            int64_t sV) {
     const int64_t b = blockIdx.y;
     part += b * (int64_t)nchunk * N;
     y += b * sV;
@@ -650,12 +674,18 @@ __global__ void k_symv_lt_fin(const double* __restrict__ part, int64_t N, int nc
 }
 
 // part: batch * nchunk * N doubles
+void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV);
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV) {
     const int64_t T = N / 64;
     hipLaunchKernelGGL(k_symv_lt, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, A, N, T,
                        nchunk, v, part, sA, sV);
-    hipLaunchKernelGGL(k_symv_lt_fin, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, N,
+    chunk_sum_f64(s, part, N, nchunk, y, batch, sV);
+}
+
+// y[b][i] = sum_c part[b][c][i] (fixed order)
+void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV) {
+    hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, N,
                        nchunk, y, sV);
 }
 

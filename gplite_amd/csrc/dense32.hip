@@ -24,72 +24,77 @@ __device__ __forceinline__ void lower_tile32(int64_t t, int64_t& ti, int64_t& tj
     tj = t - r * (r + 1) / 2;
 }
 
-// chunk of 16 k x BM m, FP32, into registers
-template <int BM, int MCONT>
+// chunk of KC k x BM m, FP32, into registers (NT threads, KC*BM/(4*NT) float4 each)
+template <int BM, int MCONT, int NT, int KC>
 __device__ __forceinline__ void g32_load(const float* P, int64_t ld, int64_t base, int64_t lim, int64_t k0,
-                                         float4 (&r)[BM / 64]) {
-    constexpr int AL = BM / 64;
+                                         float4 (&r)[KC * BM / (4 * NT)]) {
+    constexpr int AL = KC * BM / (4 * NT);
     const int t = threadIdx.x;
     if (MCONT) {  // (m, k) at P[m + k*ld]: float4 along m
-        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = 256 / (BM / 4);
+        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = NT / (BM / 4);
         const bool ok = base + m4 < lim;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
             const int64_t k = k0 + kq + ks * j;
             r[j] = ok ? *reinterpret_cast<const float4*>(P + base + m4 + k * ld) : float4{0.f, 0.f, 0.f, 0.f};
         }
-    } else {  // (m, k) at P[k + m*ld]: four lanes read one column's 16 k
-        const int kq = t & 3, m = t >> 2;
+    } else {  // (m, k) at P[k + m*ld]: KC/4 lanes read one column's KC k
+        constexpr int LPC = KC / 4, CPP = NT / LPC;  // lanes per column, columns per pass
+        const int kq = t % LPC, m = t / LPC;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
-            const int64_t mm = base + m + 64 * j;
+            const int64_t mm = base + m + CPP * j;
             r[j] = (mm < lim) ? *reinterpret_cast<const float4*>(P + (k0 + 4 * kq) + mm * ld)
                               : float4{0.f, 0.f, 0.f, 0.f};
         }
     }
 }
 
-template <int BM, int MCONT>
-__device__ __forceinline__ void g32_store(float (*S)[BM + 4], const float4 (&r)[BM / 64]) {
-    constexpr int AL = BM / 64;
+template <int BM, int MCONT, int NT, int KC>
+__device__ __forceinline__ void g32_store(float (*S)[BM + 4], const float4 (&r)[KC * BM / (4 * NT)]) {
+    constexpr int AL = KC * BM / (4 * NT);
     const int t = threadIdx.x;
     if (MCONT) {
-        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = 256 / (BM / 4);
+        const int m4 = (t % (BM / 4)) * 4, kq = t / (BM / 4), ks = NT / (BM / 4);
 #pragma unroll
         for (int j = 0; j < AL; ++j) *reinterpret_cast<float4*>(&S[kq + ks * j][m4]) = r[j];
     } else {
-        const int kq = t & 3, m = t >> 2;
+        constexpr int LPC = KC / 4, CPP = NT / LPC;
+        const int kq = t % LPC, m = t / LPC;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
-            S[4 * kq + 0][m + 64 * j] = r[j].x;
-            S[4 * kq + 1][m + 64 * j] = r[j].y;
-            S[4 * kq + 2][m + 64 * j] = r[j].z;
-            S[4 * kq + 3][m + 64 * j] = r[j].w;
+            S[4 * kq + 0][m + CPP * j] = r[j].x;
+            S[4 * kq + 1][m + CPP * j] = r[j].y;
+            S[4 * kq + 2][m + CPP * j] = r[j].z;
+            S[4 * kq + 3][m + CPP * j] = r[j].w;
         }
     }
 }
 
 // C = beta*C + alpha*op(A) op(B), FP32 storage and FP32 MFMA products.  Tile
-// modes as gemm2_f64.  Each wave owns a (BM/2)^2 quadrant of 32x32 blocks;
-// lane half h = lane/32 takes k = 8h..8h+7 of a 16-wide chunk over eight MFMA
-// steps.  With FOLD > 0 the FP32 accumulators cover at most 16*FOLD
-// consecutive k: they are then added into FP64 accumulators and cleared, and
-// the epilogue forms beta*C + alpha*acc in FP64 with one rounding to FP32.  A
-// long FP32 running sum loses about sqrt(k) ulps of its partial sums; in
-// trailing updates that cancel (A - P P^T with |P P^T| >> |result|) that error
-// dominates log|B|.  FOLD == 0 keeps plain FP32 accumulation.
-template <int TA, int TB, int BM, int FOLD>
-__global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
-                                                const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
-                                                float* C, int64_t ldc, int tiles, int64_t sA, int64_t sB,
-                                                int64_t sC) {
-    constexpr int WT = BM / 2, NBK = WT / 32, AL = BM / 64;
-    constexpr int NF = FOLD > 0 ? NBK : 1;
+// modes as gemm2_f64.  NW waves in a 2 x NW/2 grid each own a (BM/2) x (2BM/NW)
+// block of 32x32 MFMA tiles; lane half h = lane/32 takes k = (KC/2)h + s of a
+// KC-deep chunk over KC/2 MFMA steps.  With FOLD > 0 the FP32 accumulators
+// cover at most KC*FOLD consecutive k: they are then added into FP64
+// accumulators and cleared, and the epilogue forms beta*C + alpha*acc in FP64
+// with one rounding to FP32.  A long FP32 running sum loses about sqrt(k) ulps
+// of its partial sums; in trailing updates that cancel (A - P P^T with
+// |P P^T| >> |result|) that error dominates log|B|.  FOLD == 0 keeps plain
+// FP32 accumulation.
+template <int TA, int TB, int BM, int FOLD, int NW, int KC>
+__global__ __launch_bounds__(64 * NW) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
+                                                    const float* A, int64_t lda, const float* B, int64_t ldb,
+                                                    float beta, float* C, int64_t ldc, int tiles, int64_t sA,
+                                                    int64_t sB, int64_t sC) {
+    constexpr int NT = 64 * NW, WCn = NW / 2, WTR = BM / 2, WTC = BM / WCn;
+    constexpr int NBR = WTR / 32, NBC = WTC / 32, AL = KC * BM / (4 * NT);
+    constexpr int FR = FOLD > 0 ? NBR : 1, FC = FOLD > 0 ? NBC : 1;
+    static_assert(NBR >= 1 && NBC >= 1 && AL >= 1, "tile shape");
     A += blockIdx.y * sA;  // batch: one problem per grid row
     B += blockIdx.y * sB;
     C += blockIdx.y * sC;
-    __shared__ float As[2][16][BM + 4];
-    __shared__ float Bs[2][16][BM + 4];
+    __shared__ float As[2][KC][BM + 4];
+    __shared__ float Bs[2][KC][BM + 4];
     const int64_t bid = (tiles == 2) ? (int64_t)blockIdx.x : xcd_remap32(blockIdx.x, gridDim.x);
     int64_t ti, tj;
     if (tiles == 0 || tiles == 3) {
@@ -101,63 +106,63 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
     const int64_t row0 = ti * BM, col0 = tj * BM;
     const int64_t kbeg = (tiles == 2) ? ti * BM : 0;
     const int64_t kend = (tiles == 3 && (ti + 1) * BM < K) ? (ti + 1) * BM : K;
-    const int64_t nch = kend > kbeg ? (kend - kbeg) / 16 : 0;
+    const int64_t nch = kend > kbeg ? (kend - kbeg) / KC : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l32 = lane & 31, h = lane >> 5;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WCn, wc = wave % WCn;
 
-    f16v acc[NBK][NBK];
-    double acc64[NF][NF][FOLD > 0 ? 16 : 1];
+    f16v acc[NBR][NBC];
+    double acc64[FR][FC][FOLD > 0 ? 16 : 1];
 #pragma unroll
-    for (int i = 0; i < NBK; ++i)
+    for (int i = 0; i < NBR; ++i)
 #pragma unroll
-        for (int j = 0; j < NBK; ++j)
+        for (int j = 0; j < NBC; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
     if constexpr (FOLD > 0) {
 #pragma unroll
-        for (int i = 0; i < NF; ++i)
+        for (int i = 0; i < FR; ++i)
 #pragma unroll
-            for (int j = 0; j < NF; ++j)
+            for (int j = 0; j < FC; ++j)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc64[i][j][v] = 0.0;
     }
 
     float4 ra[AL], rb[AL];
     if (nch > 0) {
-        g32_load<BM, !TA>(A, lda, row0, M, kbeg, ra);
-        g32_load<BM, TB>(B, ldb, col0, N, kbeg, rb);
-        g32_store<BM, !TA>(As[0], ra);
-        g32_store<BM, TB>(Bs[0], rb);
+        g32_load<BM, !TA, NT, KC>(A, lda, row0, M, kbeg, ra);
+        g32_load<BM, TB, NT, KC>(B, ldb, col0, N, kbeg, rb);
+        g32_store<BM, !TA, NT, KC>(As[0], ra);
+        g32_store<BM, TB, NT, KC>(Bs[0], rb);
     }
     __syncthreads();
     for (int64_t c = 0; c < nch; ++c) {
         const int buf = (int)(c & 1);
         const bool more = c + 1 < nch;
         if (more) {
-            g32_load<BM, !TA>(A, lda, row0, M, kbeg + (c + 1) * 16, ra);
-            g32_load<BM, TB>(B, ldb, col0, N, kbeg + (c + 1) * 16, rb);
+            g32_load<BM, !TA, NT, KC>(A, lda, row0, M, kbeg + (c + 1) * KC, ra);
+            g32_load<BM, TB, NT, KC>(B, ldb, col0, N, kbeg + (c + 1) * KC, rb);
         }
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int kk = 8 * h + s;
-            float a[NBK], b[NBK];
+        for (int s = 0; s < KC / 2; ++s) {
+            const int kk = (KC / 2) * h + s;
+            float a[NBR], b[NBC];
 #pragma unroll
-            for (int i = 0; i < NBK; ++i) a[i] = As[buf][kk][wr * WT + i * 32 + l32];
+            for (int i = 0; i < NBR; ++i) a[i] = As[buf][kk][wr * WTR + i * 32 + l32];
 #pragma unroll
-            for (int j = 0; j < NBK; ++j) b[j] = Bs[buf][kk][wc * WT + j * 32 + l32];
+            for (int j = 0; j < NBC; ++j) b[j] = Bs[buf][kk][wc * WTC + j * 32 + l32];
 #pragma unroll
-            for (int i = 0; i < NBK; ++i)
+            for (int i = 0; i < NBR; ++i)
 #pragma unroll
-                for (int j = 0; j < NBK; ++j)
+                for (int j = 0; j < NBC; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if constexpr (FOLD > 0) {
             if ((c % FOLD) == FOLD - 1 || !more) {  // block-uniform branch
 #pragma unroll
-                for (int i = 0; i < NBK; ++i)
+                for (int i = 0; i < NBR; ++i)
 #pragma unroll
-                    for (int j = 0; j < NBK; ++j)
+                    for (int j = 0; j < NBC; ++j)
 #pragma unroll
                         for (int v = 0; v < 16; ++v) {
                             acc64[i][j][v] += (double)acc[i][j][v];
@@ -166,18 +171,18 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
             }
         }
         if (more) {
-            g32_store<BM, !TA>(As[buf ^ 1], ra);
-            g32_store<BM, TB>(Bs[buf ^ 1], rb);
+            g32_store<BM, !TA, NT, KC>(As[buf ^ 1], ra);
+            g32_store<BM, TB, NT, KC>(Bs[buf ^ 1], rb);
         }
         __syncthreads();
     }
     // v_mfma_f32_32x32x2f32 D layout: item v -> row 8*(v/4) + 4*(lane/32) + v%4, column lane%32,
     // so items 4g..4g+3 of a lane are four consecutive rows: one float4 per group
 #pragma unroll
-    for (int i = 0; i < NBK; ++i)
+    for (int i = 0; i < NBR; ++i)
 #pragma unroll
-        for (int j = 0; j < NBK; ++j) {
-            const int64_t rb0 = row0 + wr * WT + i * 32, cb0 = col0 + wc * WT + j * 32;
+        for (int j = 0; j < NBC; ++j) {
+            const int64_t rb0 = row0 + wr * WTR + i * 32, cb0 = col0 + wc * WTC + j * 32;
             if (rb0 >= M || cb0 >= N) continue;
             float* p = C + rb0 + 4 * h + (cb0 + l32) * ldc;
 #pragma unroll
@@ -199,16 +204,16 @@ __global__ __launch_bounds__(256) void k_gemm32(int64_t M, int64_t N, int64_t K,
         }
 }
 
-template <int BM, int FOLD>
+template <int BM, int FOLD, int NW, int KC = 16>
 static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha,
                           const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
                           int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
     const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
-    dim3 grid((unsigned)nblk, (unsigned)batch), block(256);
-#define GPL_G32(TAv, TBv)                                                                                             \
-    hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM, FOLD>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, beta, C, \
-                       ldc, tiles, sA, sB, sC)
+    dim3 grid((unsigned)nblk, (unsigned)batch), block(64 * NW);
+#define GPL_G32(TAv, TBv)                                                                                        \
+    hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM, FOLD, NW, KC>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, \
+                       beta, C, ldc, tiles, sA, sB, sC)
     if (!ta && !tb) GPL_G32(0, 0);
     else if (!ta && tb) GPL_G32(0, 1);
     else if (ta && !tb) GPL_G32(1, 0);
@@ -216,9 +221,10 @@ static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 #undef GPL_G32
 }
 
-// fold != 0: FP32 sums folded into FP64 every 64 k (64-wide tiles unless bm
-// asks for 128: the FP64 accumulators of a 128-wide tile halve occupancy).
-// fold == 0: plain FP32 accumulation, 128-wide tiles when the grid is large.
+// fold != 0: FP32 sums folded into FP64 every 64 k; 128-wide tiles use eight
+// waves there (64x32 per wave keeps the FP64 accumulators at four waves/SIMD).
+// fold == 0: plain FP32 accumulation.  bm == 0 picks 128-wide tiles when the
+// grid holds at least 256 of them.
 void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
                       int batch, int64_t sA, int64_t sB, int64_t sC, int fold) {
@@ -226,18 +232,20 @@ void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64
     if (bm == 0) {
         const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
         const int64_t n128 = ((tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2) * batch;
-        bm = (!fold && n128 >= 256) ? 128 : 64;
+        bm = (n128 >= 256) ? 128 : 64;
     }
     if (fold) {
         if (bm == 128)
-            gemm32_launch<128, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+            gemm32_launch<128, 4, 8>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB,
+                                     sC);
         else
-            gemm32_launch<64, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+            gemm32_launch<64, 4, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
     } else {
         if (bm == 128)
-            gemm32_launch<128, 0>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+            gemm32_launch<128, 0, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB,
+                                     sC);
         else
-            gemm32_launch<64, 0>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+            gemm32_launch<64, 0, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
     }
 }
 
@@ -270,49 +278,70 @@ void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
     set_identity_f32_batched(s, A, ld, N, 1, 0);
 }
 
-// y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32, vectors FP64
-__global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int64_t ld, int64_t T,
-                                                const double* __restrict__ b, double* __restrict__ y, int trans,
+// y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32 (64x64 tiles,
+// column-major), vectors FP64.  Grid (T, nchunk, batch): block (ti, ch) sums
+// the tiles of output block ti that fall in chunk ch; for X^T the stored tile
+// (r, ti) is staged through LDS so every global read is coalesced.  The chunk
+// partials (part: batch * nchunk * N) are added in a fixed order afterwards.
+__global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int64_t ld, int64_t T, int nchunk,
+                                                const double* __restrict__ b, double* __restrict__ part, int trans,
                                                 int64_t sX, int64_t sv) {
+    __shared__ float tile[64][65];
+    __shared__ double bc[64];
     __shared__ double red[256];
-    X += blockIdx.y * sX;
-    b += blockIdx.y * sv;
-    y += blockIdx.y * sv;
+    X += blockIdx.z * sX;
+    b += blockIdx.z * sv;
+    part += blockIdx.z * (int64_t)nchunk * T * 64;
     const int64_t ti = blockIdx.x;
+    const int ch = blockIdx.y;
     const int t = threadIdx.x, e = t & 63, p = t >> 6;
+    // tiles of this output block: tj in [0, ti] (X b) or r in [ti, T) (X^T b), split into nchunk ranges
+    const int64_t lo = trans ? ti : 0, hi = trans ? T : ti + 1, cnt = hi - lo;
+    const int64_t per = (cnt + nchunk - 1) / nchunk, tb = lo + ch * per, te = (tb + per < hi) ? tb + per : hi;
     double s = 0.0;
-    if (!trans) {
-        for (int64_t tj = 0; tj <= ti; ++tj) {
-            const float* Xt = X + ti * 64 + tj * 64 * ld;
+    for (int64_t q = tb; q < te; ++q) {
+        __syncthreads();
+        if (t < 64) bc[t] = b[q * 64 + t];
+        if (trans) {
+            const float* Xt = X + q * 64 + ti * 64 * ld;  // stored tile (q, ti)
 #pragma unroll 4
             for (int u = 0; u < 16; ++u) {
                 const int c = p * 16 + u;
-                s += (double)Xt[e + (int64_t)c * ld] * b[tj * 64 + c];
+                tile[c][e] = Xt[e + (int64_t)c * ld];
             }
         }
-    } else {
-        for (int64_t tr = ti; tr < T; ++tr) {
-            const float* Xt = X + tr * 64 + ti * 64 * ld;
+        __syncthreads();
+        if (!trans) {
+            const float* Xt = X + ti * 64 + q * 64 * ld;
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int c = p * 16 + u;
+                s += (double)Xt[e + (int64_t)c * ld] * bc[c];
+            }
+        } else {
 #pragma unroll 4
             for (int u = 0; u < 16; ++u) {
                 const int r = p * 16 + u;
-                s += (double)Xt[r + (int64_t)e * ld] * b[tr * 64 + r];
+                s += (double)tile[e][r] * bc[r];
             }
         }
     }
     red[t] = s;
     __syncthreads();
-    if (t < 64) y[ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+    if (t < 64) part[(int64_t)ch * T * 64 + ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
 }
 
 void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
-                            int trans, int batch, int64_t sX, int64_t sv) {
+                            int trans, int batch, int64_t sX, int64_t sv, double* part, int nchunk) {
     const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_trmv32, dim3((unsigned)T, (unsigned)batch), dim3(256), 0, s, X, ld, T, b, y, trans, sX, sv);
+    hipLaunchKernelGGL(k_trmv32, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, X, ld, T,
+                       nchunk, b, part, trans, sX, sv);
+    chunk_sum_f64(s, part, N, nchunk, y, batch, sv);
 }
 
-void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans) {
-    trmv_lower_f32_batched(s, X, ld, N, b, y, trans, 1, 0, 0);
+void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans,
+                    double* part, int nchunk) {
+    trmv_lower_f32_batched(s, X, ld, N, b, y, trans, 1, 0, 0, part, nchunk);
 }
 
 __global__ __launch_bounds__(256) void k_colsumsq32(const float* __restrict__ A, int64_t ld, int64_t nrow,

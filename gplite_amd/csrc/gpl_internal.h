@@ -95,7 +95,8 @@ void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, fl
               int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
               int fold = 1);
 void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N);
-void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans);
+void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans,
+                    double* part, int nchunk);
 void colsumsq_f32(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out, int lower);
 void rel_maxabs(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
 // Factor the 64x64 diagonal block at A + k*64*(ld+1) in place (FP64 arithmetic); write its
@@ -122,6 +123,8 @@ void gemv_t_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_
 // out[j] = sum_i A[i + j*ld]^2 for j < ncol (deterministic column sums of squares)
 void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
                   int lower);
+// y[b][i] = sum over c of part[b][c][i] (fixed order)
+void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV);
 // y = A v, A symmetric FP64 held as lower 64x64 tiles (part: batch * nchunk * N)
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV);
@@ -189,7 +192,7 @@ void potrf_diag3_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t k, dou
                              int batch, int64_t sA, int64_t sD, int64_t sL, const double* Dacc);
 void set_identity_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, int batch, int64_t sA);
 void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
-                            int trans, int batch, int64_t sX, int64_t sv);
+                            int trans, int batch, int64_t sX, int64_t sv, double* part, int nchunk);
 void colsumsq_f32_batched(hipStream_t s, const float* A, int64_t ld, int64_t nrow, int64_t ncol, double* out,
                           int lower, int batch, int64_t sA, int64_t sO);
 void seg_scale(hipStream_t s, const double* in, const double* scal, const int64_t* nvec, int64_t N, double* out,

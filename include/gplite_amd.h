@@ -198,7 +198,7 @@ int gpl_batch_energy_grad(gpl_batch* batch, double* log_evidence, double* grad);
 int32_t gpl_batch_grad_len(gpl_batch* batch);
 /* Number of buckets and the largest relative last refinement correction. */
 int gpl_batch_info(gpl_batch* batch, int32_t* nbuckets, double* refine_rel);
-/* Autotune: times GEMM tile edge {64,128} x outer panel {128,256} per bucket,
+/* Autotune: times GEMM tile edge {64,128} x outer panel {128,256,512,1024} per bucket,
  * keeps the fastest; writes up to cap rows (bucket n, tile, panel, ms) and the
  * number of buckets. */
 int gpl_batch_tune(gpl_batch* batch, int32_t cap, int32_t* bucket_n, int32_t* bm, int32_t* nbo,
