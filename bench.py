@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed,mixed_batched")
     ap.add_argument("--restarts", type=int, default=16, help="problems per pass in the batched form")
     ap.add_argument("--refine", type=int, default=3, help="refinement sweeps in the batched form")
+    ap.add_argument("--tuning", default=None,
+                    help="tile,panel for the batched form (skips the autotuner), e.g. 128,512")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -124,7 +126,12 @@ def main():
             ls = w["lscale"] * (np.exp(0.2 * rr.standard_normal(d)) if i else 1.0)
             gps.append(G.gp_init(G.cf_matern52(lscale=ls, magn=w["magn"]), G.lik_gaussian(sigma=w["sigma"])))
         Bt = GPBatch(gps, [w["x"]] * nrest, [y] * nrest, refine_iters=args.refine)
-        table = Bt.tune()
+        if args.tuning:
+            tile, panel = (int(v) for v in args.tuning.split(","))
+            table = [{"n": -(-n // 64) * 64, "tile": tile, "panel": panel}]
+            Bt.load_tuning(table)
+        else:
+            table = Bt.tune()
         log(f"[batched] tuning: {table}")
         ngb = lib.gpl_batch_grad_len(Bt.handle)
         lz = np.empty(nrest)
@@ -160,14 +167,26 @@ def main():
     head = "mixed_batched" if "mixed_batched" in R else ("mixed" if "mixed" in R else forms[0])
     H = R[head]
     prof = H["prof"]
-    mm = {k: v for k, v in prof.items() if v["flops"] > 0}
-    dom = max(mm.items(), key=lambda kv: kv[1]["ms"])
-    ach = dom[1]["flops"] / (dom[1]["ms"] * 1e-3) / 1e12
-    pk = PEAK["f32"] if dom[0].endswith("f32") else PEAK["f64"]
-    roof = {"bound": "mfma", "kernel": dom[0], "achieved": round(ach, 3), "peak": pk, "unit": "TFLOP/s",
-            "frac": round(ach / pk, 4), "traffic": None, "launches": dom[1]["launches"],
-            "avg_launch_us": round(dom[1]["ms"] * 1e3 / max(dom[1]["launches"], 1), 2),
-            "algorithmic_flops_per_launch": dom[1]["flops"] / max(dom[1]["launches"], 1)}
+    # roofline of the B^-1 = X^T X product: the largest single kernel launch of a
+    # step (N^3/3 flops per problem, one launch per pass), timed with HIP events
+    # on the stream it runs on during a separate profiled pass
+    rk = {"mixed_batched": "batch_lauum_f32", "mixed": "lauum_f32", "fp64": "lauum"}[head]
+    rc = prof[rk]
+    ach = rc["flops"] / (rc["ms"] * 1e-3) / 1e12
+    pk = PEAK["f32"] if rk.endswith("f32") else PEAK["f64"]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tt = json.load(fh).get(rk)
+        if tt and tt.get("n") == n and tt.get("problems", 1) == H.get("problems", 1):
+            traffic = tt["bytes_per_launch"]
+    roof = {"bound": "mfma", "kernel": rk, "achieved": round(ach, 3), "peak": pk, "unit": "TFLOP/s",
+            "frac": round(ach / pk, 4), "traffic": traffic, "launches": rc["launches"],
+            "avg_launch_us": round(rc["ms"] * 1e3 / max(rc["launches"], 1), 2),
+            "algorithmic_flops_per_launch": rc["flops"] / max(rc["launches"], 1),
+            "top_classes": {k: {"ms": round(v["ms"], 3), "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}
+                            for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:4]}}
     for f, r in R.items():
         log(f"[{f}] kernel classes:", json.dumps({k: {"ms": round(v["ms"], 3), "n": v["launches"],
                                                        "TF": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}

@@ -103,6 +103,9 @@ static void potrf_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t K0, in
         Prof p(c, "batch_panel_f32", 2.0 * nb * M * W2 * h, 0, s);
         gemm_f32_batched(s, 0, 1, M, W2, h, -1.f, P, N, P, N, 1.f, L + (K0 + h) * (N + 1), N, 0, k.bm, nb, N2, N2,
                          N2);
+    }
+    {
+        Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * h * (W2 / 64), 0, s);
         dacc_update(s, Dacc + ((K0 + h) / 64) * 64 * 64, P, N, W2 / 64, h, nb, N * 64, N2);
     }
     potrf_panel_rec(c, s, k, K0 + h, W2);
@@ -169,6 +172,9 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         {
             Prof p(c, "batch_syrk_f32", 2.0 * nb * R * W1 * W, 0, s);
             gemm_f32_batched(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, bm, nb, N2, N2, N2);
+        }
+        {
+            Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (W1 / 64), 0, s);
             dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, nb, N * 64, N2);
         }
         hipEventRecord(c->ev_a, s);
@@ -178,10 +184,15 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             const float* P2 = L + (K1 + W1) + K0 * N;
-            Prof p(c, "batch_syrk_f32", (double)nb * R2 * R2 * W, 0, s);
-            gemm_f32_batched(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, bm, nb, N2,
-                             N2, N2);
-            dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
+            {
+                Prof p(c, "batch_syrk_f32", (double)nb * R2 * R2 * W, 0, s);
+                gemm_f32_batched(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, bm, nb, N2,
+                                 N2, N2);
+            }
+            {
+                Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (R2 / 64), 0, s);
+                dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
+            }
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }

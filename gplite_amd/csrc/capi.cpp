@@ -382,9 +382,14 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
         }
         const int64_t Wr = K0 + W - (j0 + 64);
         if (Wr > 0) {
-            Prof p(c, "panel_update_f32", 2.0 * M * Wr * 64, 4.0 * (2.0 * M * Wr + M * 64), st);
-            gemm_f32(st, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0);
-            dacc_update(st, Dacc + (k + 1) * 64 * 64, L21, N, Wr / 64, 64, 1, 0, 0);
+            {
+                Prof p(c, "panel_update_f32", 2.0 * M * Wr * 64, 4.0 * (2.0 * M * Wr + M * 64), st);
+                gemm_f32(st, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0);
+            }
+            {
+                Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * 64 * (Wr / 64), 0, st);
+                dacc_update(st, Dacc + (k + 1) * 64 * 64, L21, N, Wr / 64, 64, 1, 0, 0);
+            }
         }
     }
 }
@@ -402,6 +407,9 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         {
             Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W));
             gemm_f32(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, 0);
+        }
+        {
+            Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (W1 / 64), 0, s);
             dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, 1, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
@@ -411,9 +419,14 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             const float* P2 = L + (K1 + W1) + K0 * N;
-            Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W));
-            gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
-            dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, 1, 0, 0);
+            {
+                Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W));
+                gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
+            }
+            {
+                Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (R2 / 64), 0, s);
+                dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, 1, 0, 0);
+            }
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
