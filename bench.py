@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="tile,panel for the batched form (skips the autotuner), e.g. 128,512")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: n=8192 energy+grad (default line); c3: Laplace n=4096; c4: 2048 batched problems; "
+                         "c5: serving sweep")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; how the path would shard is described in "
@@ -58,6 +62,10 @@ def main():
 
     G.require_device()
     log("device:", L.device_name())
+    if args.workload != "c2":
+        out = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args, G, L, api, synth)
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.time()
     w = synth.c2(n=args.n)
     log(f"workload generated in {time.time() - t0:.1f}s")
@@ -252,6 +260,134 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _pct(v, q):
+    return round(float(np.percentile(np.asarray(v), q)), 3)
+
+
+def bench_c3(args, G, L, api, synth):
+    """Laplace binary classification, n=4096, d=4, Bernoulli-logit, Newton to a
+    mode tolerance of 1e-8, in the FP64 and mixed-precision forms."""
+    w = synth.c3()
+    n = w["x"].shape[0]
+    gp = G.gp_init(G.cf_sexp(lscale=w["lscale"], magn=w["magn"]), G.lik_bernoulli("logit"),
+                   approx=G.approx_laplace(maxiter=50, tol=1e-8))
+    steps = max(3, min(args.steps, 10))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        for _ in range(max(args.warmup, 1)):
+            g = G.gp_fit(gp, w["x"], w["y"], precision=prec)
+        lat = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            g = G.gp_fit(gp, w["x"], w["y"], precision=prec)
+            lat.append(1e3 * (time.perf_counter() - t))
+        L.profile(enable=True, reset=True)
+        G.gp_fit(gp, w["x"], w["y"], precision=prec)
+        prof = L.profile(enable=False)
+        res[prec] = {"fit": g.fit, "lat": lat, "prof": prof}
+        log(f"[c3 {prec}] {g.fit.iters} Newton steps, p50 {np.median(lat):.2f} ms per fit")
+    a, b = res["mixed"]["fit"], res["fp64"]["fit"]
+    it = b.iters
+    flops_step = n ** 3 / 3
+    fp = res["fp64"]
+    top = {k: {"ms": round(v["ms"], 3), "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}
+           for k, v in sorted(fp["prof"].items(), key=lambda kv: -kv[1]["ms"])[:5]}
+    return {
+        "metric": "Laplace fits/s (n=4096, d=4, Bernoulli-logit, tol 1e-8)",
+        "value": round(1e3 / np.median(res["mixed"]["lat"]), 3), "unit": "fits/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic (seeded GP sample, Bernoulli labels)",
+        "config": {"workload": "C3 Laplace classification", "n": n, "d": 4, "newton_steps": it},
+        "forms": {p: {"fits_per_s": round(1e3 / np.median(r["lat"]), 3), "p50_ms": _pct(r["lat"], 50),
+                      "p99_ms": _pct(r["lat"], 99), "newton_steps": r["fit"].iters,
+                      "ms_per_newton_step": round(float(np.median(r["lat"])) / r["fit"].iters, 3)}
+                  for p, r in res.items()},
+        "parity": {"rel_logz_mixed_vs_fp64": abs(a.log_evidence / b.log_evidence - 1),
+                   "rel_mode_mixed_vs_fp64": float(np.max(np.abs(a.fmean - b.fmean)) / np.max(np.abs(b.fmean))),
+                   "same_step_count": a.iters == b.iters},
+        "roofline_fp64_classes": top,
+        "algorithmic_flops_per_newton_step": flops_step,
+    }
+
+
+def bench_c4(args, G, L, api, synth):
+    """2048 independent regressions of uneven n in one batched mixed-precision
+    pass (energy + gradient per problem), tiles from the autotuner."""
+    from gplite_amd.batch import GPBatch
+    probs = synth.c4()
+    gps = [G.gp_init(G.cf_sexp(lscale=p["lscale"], magn=p["magn"]), G.lik_gaussian(sigma=p["sigma"])) for p in probs]
+    t = time.perf_counter()
+    Bt = GPBatch(gps, [p["x"] for p in probs], [p["y"] for p in probs], refine_iters=args.refine)
+    t_setup = time.perf_counter() - t
+    table = Bt.tune()
+    for _ in range(max(args.warmup, 1)):
+        Bt.energy_grad(include_prior=False)
+    steps = max(3, min(args.steps, 10))
+    lat = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        e, g = Bt.energy_grad(include_prior=False)
+        lat.append(1e3 * (time.perf_counter() - t))
+    npb = len(probs)
+    # flops per bucket: N^3 per problem (padded) for energy + gradient
+    ns = np.array([p["x"].shape[0] for p in probs])
+    Np = -(-ns // 64) * 64
+    per_bucket = []
+    for row in table:
+        cnt = int(np.sum(Np == row["n"]))
+        fl = cnt * float(row["n"]) ** 3
+        per_bucket.append({"n": row["n"], "problems": cnt, "tile": row["tile"], "panel": row["panel"],
+                           "ms": round(row["ms"], 3), "TFLOP/s": round(fl / (row["ms"] * 1e-3) / 1e12, 2)})
+    # parity sample: 8 problems against the plain FP64 device path
+    rr = np.random.default_rng(1)
+    errs = []
+    for i in rr.choice(npb, 8, replace=False):
+        gi = G.gp_fit(Bt.gps[i], probs[i]["x"], probs[i]["y"])
+        errs.append(abs(-e[i] / gi.fit.log_evidence - 1))
+    med = float(np.median(lat))
+    return {
+        "metric": "batched GP energy+grad evals/s (2048 problems, n in [64, 2048])",
+        "value": round(npb / (med * 1e-3), 1), "unit": "evals/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+f64", "data": "synthetic (seeded, numpy PCG64)",
+        "config": {"workload": "C4 batched", "problems": npb, "buckets": len(table), "refine_sweeps": args.refine,
+                   "sum_n3_flops": float(np.sum(ns.astype(float) ** 3))},
+        "pass_ms": {"p50": _pct(lat, 50), "p99": _pct(lat, 99)}, "setup_s": round(t_setup, 2),
+        "achieved_TFLOP_s": round(float(np.sum(Np.astype(float) ** 3)) / (med * 1e-3) / 1e12, 2),
+        "per_bucket": per_bucket,
+        "parity": {"max_rel_logz_vs_fp64_sample8": float(max(errs)), "last_refine_rel": Bt.info()["last_refine_rel"]},
+    }
+
+
+def bench_c5(args, G, L, api, synth):
+    """Serving: Poisson arrivals, 30% fit / 70% predict (m=1000), fits with
+    n ~ logU[128, 16384], d=6, cf_matern32, Gaussian or Poisson-Laplace 50/50."""
+    from gplite_amd.serve import Server, poisson_load
+    rows = []
+    for prec in ("mixed",):
+        srv = Server(workers=4, mem_gb=128.0)
+        poisson_load(srv, 100.0, 12, seed=99, precision=prec)  # warm the workers
+        for rate in [float(v) for v in args.rates.split(",")]:
+            nreq = int(min(max(2 * rate, 60), 400))
+            t = time.perf_counter()
+            r = poisson_load(srv, rate, nreq, seed=int(rate), precision=prec)
+            r["precision"] = prec
+            st = srv.stats()
+            r["launches"] = st["launches"]
+            r["evictions"] = st["evictions"]
+            log(f"[c5 {prec}] rate {rate:g}: {json.dumps({k: r[k] for k in ('throughput_req_s', 'all', 'fit', 'pred')})}"
+                f" ({time.perf_counter() - t:.1f}s)")
+            rows.append(r)
+        srv.close()
+    best = max(rows, key=lambda r: r["throughput_req_s"])
+    return {
+        "metric": "serving throughput (req/s) and p50/p99 latency under Poisson arrivals",
+        "value": round(best["throughput_req_s"], 2), "unit": "req/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+f64", "data": "synthetic (seeded, numpy PCG64)",
+        "config": {"workload": "C5 serving", "workers": 4, "fit_fraction": 0.3, "m": 1000,
+                   "fit_n": "logU[128,16384]", "liks": "Gaussian(0.1) / Poisson-Laplace 50/50"},
+        "sweep": rows,
+    }
 
 
 if __name__ == "__main__":

@@ -519,7 +519,7 @@ static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool
     ENSURE(m, B_L, mat);
     ENSURE(m, B_DINV, (size_t)N * 64 * 8);
     ENSURE(m, B_LDIAG, vec);
-    if (need_k) ENSURE(m, B_K, mat);
+    if (need_k) ENSURE(m, B_K, (size_t)N * N * 8);  // K stays FP64 in both forms
     ENSURE(m, B_PROG, sizeof(DevProg));
     for (int i : {B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM, B_FOLD, B_FVOLD, B_ZOLD,
                   B_POLD, B_W1, B_W2, B_W3})
@@ -765,6 +765,121 @@ static int fit_laplace(gpl_model* m) {
     return GPL_OK;
 }
 
+// Laplace Newton loop in the mixed-precision form: per step B = I + S K S is
+// formed from the FP64 K in FP64 (kept for residuals) and FP32 (factored with
+// FP64 diagonal tiles), X = L^-1 in FP32, and the Newton step's alpha =
+// S B^-1 S z comes from iterative refinement against the FP64 B, so f = K alpha
+// and the iterate sequence carry FP64 accuracy.  The log-determinant sums the
+// FP64 pivots.  Same iteration, stopping rule and log evidence as fit_laplace.
+static int fit_laplace_mixed(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
+    const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+    ENSURE(m, B_XINV, (size_t)N * N * 4);
+    ENSURE(m, B_DACC, (size_t)N * 64 * 8);
+    ENSURE(m, B_B64, (size_t)N * N * 8);
+    const int nch = kmatvec_chunks(N);
+    ENSURE(m, B_KMV, (size_t)N * nch * 8);
+    float* L = m->b[B_L].as<float>();
+    float* X = m->b[B_XINV].as<float>();
+    double* B64 = m->d(B_B64);
+    {
+        Prof p(c, "build_K", 0, 8.0 * N * (N + 64) / 2);
+        launch_build_sym_f64(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 0, 1.0, nullptr, m->d(B_K), N);
+    }
+    fill_f64(s, m->d(B_FOLD), N, 0.0);  // fhat = 0
+    const int maxiter = m->opts.maxiter > 0 ? m->opts.maxiter : 30;
+    const int maxref = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
+    double* rhs = m->d(B_TERM);
+    double* u = m->d(B_W1);
+    double* r = m->d(B_W2);
+    double* d = m->d(B_W3);
+    double* t1 = m->d(B_CAVM);
+    double diff = INFINITY;
+    int it = 0;
+    for (it = 1; it <= maxiter; ++it) {
+        int* flags = m->b[B_INFO].as<int>();
+        hipMemsetAsync(flags, 0, 8, s);
+        lik_pseudodata_la(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
+                          m->d(B_V), m->d(B_SV), flags + 1);
+        {
+            Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
+            form_B_mixed(s, m->d(B_K), B64, L, m->d(B_DACC), N, n, N, m->d(B_SV));
+        }
+        potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
+        trtri32_blocked(c, L, m->d(B_DINV), N, X);
+        // alpha = S u, B u = S z, by refinement
+        fill_f64(s, rhs, N, 0.0);
+        vec_mul(s, n, m->d(B_SV), m->d(B_Z), rhs);
+        fill_f64(s, u, N, 0.0);
+        HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
+        double rel = INFINITY;
+        for (int k = 1; k <= maxref; ++k) {
+            {
+                Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
+                trmv_lower_f32(s, X, N, N, r, t1, 0, m->d(B_KMV), nch);
+                trmv_lower_f32(s, X, N, N, t1, d, 1, m->d(B_KMV), nch);
+            }
+            vec_axpby(s, n, 1.0, d, 1.0, u, u);
+            {
+                Prof p(c, "symv_b64", 2.0 * N * N, 8.0 * N * (N + 64) / 2);
+                symv_lt_f64(s, B64, N, u, t1, m->d(B_KMV), nch, 1, 0, 0);
+            }
+            vec_axpby(s, n, 1.0, rhs, -1.0, t1, r);
+            rel_maxabs(s, d, u, n, m->d(B_SCAL) + 8);
+            double h;
+            HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            rel = h;
+            if (rel < 1e-14) break;
+        }
+        m->refine_rel = rel;
+        fill_f64(s, m->d(B_ALPHA), N, 0.0);
+        vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
+        {
+            Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
+            symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), nch, 1, 0, 0);
+        }
+        lik_loglik(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
+        double* sc = m->d(B_SCAL);
+        reduce_sum(s, m->d(B_W3), n, sc + 0);
+        reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
+        reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+        reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+        int fl[2];
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        double h[4];
+        int rr = read_scalars(m, h, 4);
+        if (rr) return rr;
+        if (fl[1] & 6) {
+            m->logZ = -INFINITY;
+            return fail((fl[1] & 2) ? GPL_ERR_NAN : GPL_ERR_UNSUPPORTED,
+                        (fl[1] & 2) ? "NaN in Laplace pseudo-data"
+                                    : "non-log-concave pseudo-variance (V <= 0): host fallback branch "
+                                      "R/laplace.R:40-117 is not on the device path");
+        }
+        if (fl[0] > 0) {
+            g_minor = fl[0];
+            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+        }
+        m->logZ = h[0] - 0.5 * h[1] - h[2];
+        diff = h[3];
+        HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
+        if (diff < m->opts.tol) break;
+    }
+    if (it > maxiter) it = maxiter;
+    m->iters = it;
+    m->have_xinv = true;
+    if (maxiter > 1 && it == maxiter && !(diff < m->opts.tol)) {
+        g_err = "Maximum number of iterations in Laplace reached, max delta f = " + std::to_string(diff);
+        return GPL_ERR_NOCONV;
+    }
+    return GPL_OK;
+}
+
 // probabilists' Gauss-Hermite rule: nodes = eigenvalues of the Jacobi matrix of He_k
 // (Golub-Welsch), weights normalised to sum 1 -- the same rule as R/quadrature.R:28-34.
 static void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w) {
@@ -939,8 +1054,8 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
     if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
     const bool mixed = o.precision == GPL_PREC_MIXED;
-    if (mixed && !(lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_LAPLACE))
-        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Gaussian likelihood");
+    if (mixed && o.approx != GPL_APPROX_LAPLACE)
+        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Laplace approximation (and Gaussian)");
     DevProg P;
     int r = prepare_prog(terms, nterms, ncols, P);
     if (r) return r;
@@ -973,7 +1088,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
     if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
     else if (lik->kind == GPL_LIK_GAUSSIAN) r = mixed ? fit_gaussian_mixed(m) : fit_gaussian(m);
-    else r = fit_laplace(m);
+    else r = mixed ? fit_laplace_mixed(m) : fit_laplace(m);
     *inout = m;
     if (log_evidence) *log_evidence = m->logZ;
     if (r && r != GPL_ERR_NOCONV && r != GPL_ERR_NAN) return r;
@@ -1141,7 +1256,15 @@ extern "C" int gpl_model_chol(gpl_model* m, double* out) {
     HIP_TRY(hipSetDevice(m->ctx->device));
     const int64_t n = m->n, N = m->N;
     std::vector<double> V(n);
-    HIP_TRY(hipMemcpy2DAsync(out, n * 8, m->d(B_L), N * 8, n * 8, n, hipMemcpyDeviceToHost, m->ctx->stream));
+    if (m->opts.precision == GPL_PREC_MIXED) {  // FP32 factor
+        std::vector<float> f((size_t)n * n);
+        HIP_TRY(hipMemcpy2DAsync(f.data(), n * 4, m->b[B_L].p, N * 4, n * 4, n, hipMemcpyDeviceToHost,
+                                 m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        for (size_t e = 0; e < f.size(); ++e) out[e] = (double)f[e];
+    } else {
+        HIP_TRY(hipMemcpy2DAsync(out, n * 8, m->d(B_L), N * 8, n * 8, n, hipMemcpyDeviceToHost, m->ctx->stream));
+    }
     HIP_TRY(hipMemcpyAsync(V.data(), m->d(B_V), n * 8, hipMemcpyDeviceToHost, m->ctx->stream));
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     for (int64_t j = 0; j < n; ++j)

@@ -773,25 +773,38 @@ void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t 
                        ld, nrow, ncol, sv);
 }
 
+// B = I + S K S over lower tiles (S = diag(sv), identity on padding).  Optional
+// outputs for the mixed-precision form: B32 = fl32(B) and dacc = the unrounded
+// FP64 diagonal tiles (64x64 column-major, one per block row).
 __global__ void k_form_B(const double* __restrict__ K, double* __restrict__ B, int64_t ld, int64_t n,
-                         const double* __restrict__ sv) {
+                         const double* __restrict__ sv, float* __restrict__ B32, double* __restrict__ dacc) {
     int64_t ti, tj;
     lower_tile(blockIdx.x, ti, tj);
     const int t = threadIdx.x, i = t & 63;
     const int64_t gi = ti * 64 + i;
     const double si = (gi < n) ? sv[gi] : 0.0;
+    double* dt = (dacc && ti == tj) ? dacc + ti * 64 * 64 : nullptr;
     for (int c = t >> 6; c < 64; c += 4) {
         const int64_t gj = tj * 64 + c;
         double v;
         if (gi < n && gj < n) v = si * K[gi + gj * ld] * sv[gj] + (gi == gj ? 1.0 : 0.0);
         else v = (gi == gj) ? 1.0 : 0.0;
         B[gi + gj * ld] = v;
+        if (B32) B32[gi + gj * ld] = (float)v;
+        if (dt) dt[i + c * 64] = v;
     }
 }
 
 void form_B_f64(hipStream_t s, const double* K, double* B, int64_t ld, int64_t n, int64_t N, const double* sv) {
     const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_form_B, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, K, B, ld, n, sv);
+    hipLaunchKernelGGL(k_form_B, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, K, B, ld, n, sv, (float*)nullptr,
+                       (double*)nullptr);
+}
+
+void form_B_mixed(hipStream_t s, const double* K, double* B64, float* B32, double* dacc, int64_t ld, int64_t n,
+                  int64_t N, const double* sv) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_form_B, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, K, B64, ld, n, sv, B32, dacc);
 }
 
 }  // namespace gpl

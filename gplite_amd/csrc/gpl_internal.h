@@ -134,6 +134,9 @@ void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t 
 // B (lower tiles) = I + s_i K_ij s_j for i,j < n; identity on padding
 void form_B_f64(hipStream_t s, const double* K, double* B, int64_t ld, int64_t n, int64_t N,
                 const double* sv);
+// B64 = I + S K S, B32 = fl32(B64) and the FP64 diagonal tiles (mixed-precision form)
+void form_B_mixed(hipStream_t s, const double* K, double* B64, float* B32, double* dacc, int64_t ld, int64_t n,
+                  int64_t N, const double* sv);
 
 // Deterministic reductions (lik.hip)
 void reduce_sum(hipStream_t s, const double* v, int64_t n, double* out);          // out[0] = sum v

@@ -77,16 +77,16 @@ struct Resident {
 };
 
 size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
-    const size_t N = (size_t)pad64(n);
+    const size_t N = (size_t)pad64(n), NN = N * N;
     const bool mixed = o.precision == GPL_PREC_MIXED;
-    size_t mats = 1;                                       // factor of B
-    if (lik.kind != GPL_LIK_GAUSSIAN) mats += 1;           // stored K
-    if (o.approx == GPL_APPROX_EP) mats += 1;              // L^-1
-    if (mixed) mats += 4;                                  // L^-1, B^-1 and the FP64 copy of B
-    else if (o.flags & GPL_FLAG_INVERSE) mats += 1;        // L^-1
-    const size_t esz = mixed ? 4 : 8;
+    const bool gauss = lik.kind == GPL_LIK_GAUSSIAN;
+    size_t b = NN * (mixed ? 4 : 8);                              // factor of B
+    if (!gauss) b += NN * 8;                                      // stored K (FP64)
+    if (o.approx == GPL_APPROX_EP) b += NN * 8;                   // L^-1 (EP)
+    if (mixed) b += NN * 4 + NN * 8 + (gauss ? NN * 4 : 0);       // L^-1, FP64 B, B^-1 (Gaussian)
+    else if (o.flags & GPL_FLAG_INVERSE) b += NN * 8;             // L^-1
     const size_t T = N / 64;
-    return mats * N * N * esz + N * 64 * 8 + N * (MAX_FEAT + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + 4096;
+    return b + 2 * N * 64 * 8 + N * (MAX_FEAT + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + N * 8 * 32 + 4096;
 }
 
 size_t pred_footprint(int64_t n, int64_t m) {
