@@ -43,7 +43,9 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--forms", default="fp64,mixed,mixed_batched")
     ap.add_argument("--restarts", type=int, default=16, help="problems per pass in the batched form")
-    ap.add_argument("--refine", type=int, default=3, help="refinement sweeps in the batched form")
+    ap.add_argument("--refine", type=int, default=None,
+                    help="refinement sweeps in the batched form (default 3 for c2, 6 for c4, whose problems "
+                         "include condition numbers near 1e5)")
     ap.add_argument("--tuning", default=None,
                     help="tile,panel for the batched form (skips the autotuner), e.g. 128,512")
     ap.add_argument("--no-cpu", action="store_true")
@@ -52,6 +54,8 @@ def main():
                          "c5: serving sweep")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
     args = ap.parse_args()
+    if args.refine is None:
+        args.refine = 6 if args.workload == "c4" else 3
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; how the path would shard is described in "
                                    "DESIGN.md but not built", "n_gpus": args.gpus}))

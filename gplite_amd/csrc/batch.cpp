@@ -23,7 +23,7 @@ namespace {
 
 enum {
     BB_X, BB_XOFF, BB_N, BB_PROG, BB_F, BB_Y, BB_SV, BB_SCAL, BB_SIG, BB_WSC, BB_IDX, BB_RHS, BB_U, BB_R, BB_D, BB_T1,
-    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_CORR, BB_DACC, BB_B64, BB_COUNT
+    BB_KU, BB_ALPHA, BB_L, BB_X32, BB_BI, BB_DINV, BB_LDIAG, BB_INFO, BB_PART, BB_KMV, BB_RED, BB_CORR, BB_DACC, BB_B64, BB_RELMAX, BB_COUNT
 };
 
 struct Bucket {
@@ -41,6 +41,14 @@ struct Bucket {
     }
 };
 
+// One stream pair per lane: buckets on different lanes run concurrently (their
+// latency-bound diagonal chains overlap); inside a bucket the side stream
+// carries the look-ahead panel and the refinement.
+struct Lane {
+    hipStream_t s = nullptr, side = nullptr;
+    hipEvent_t ev_a = nullptr, ev_p = nullptr, done = nullptr;
+};
+
 }  // namespace
 
 struct gpl_batch {
@@ -51,16 +59,26 @@ struct gpl_batch {
     std::vector<Bucket*> buckets;
     std::vector<int> bucket_of;  // problem -> bucket index
     std::vector<int> pos_in;     // problem -> position in its bucket
-    DevBuf res_logz, res_grad, relmax;
+    DevBuf res_logz, res_grad;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<Lane> lanes;
+    std::vector<int> lane_of;  // bucket -> lane
+    hipEvent_t fork = nullptr;
     ~gpl_batch() {
         for (auto* b : buckets) delete b;
         res_logz.free_();
         res_grad.free_();
-        relmax.free_();
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
+        for (auto& l : lanes) {
+            if (l.s) hipStreamDestroy(l.s);
+            if (l.side) hipStreamDestroy(l.side);
+            if (l.ev_a) hipEventDestroy(l.ev_a);
+            if (l.ev_p) hipEventDestroy(l.ev_p);
+            if (l.done) hipEventDestroy(l.done);
+        }
+        if (fork) hipEventDestroy(fork);
     }
 };
 
@@ -137,7 +155,8 @@ static void trtri_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t I0, in
 // --------------------------------------------------------------------------
 // the per-bucket pipeline (enqueued on one stream; captured into the graph)
 // --------------------------------------------------------------------------
-static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
+static void bucket_pipeline(gpl_batch* B, Bucket& k, Lane& ln) {
+    hipStream_t s = ln.s;
     gpl_ctx* c = B->ctx;
     const int nb = (int)k.idx.size();
     const int64_t N = k.N, N2 = N * N, T = N / 64;
@@ -161,7 +180,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     // factored on the side stream while the main stream applies the rest of the
     // trailing update.  The fork and join are events, so graph capture records
     // them as dependencies between the two streams.
-    hipStream_t s2 = c->side;
+    hipStream_t s2 = ln.side;
     potrf_panel_rec(c, s, k, 0, std::min<int64_t>(NBo, N));
     for (int64_t K0 = 0; K0 < N; K0 += NBo) {
         const int64_t W = std::min<int64_t>(NBo, N - K0);
@@ -177,10 +196,10 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (W1 / 64), 0, s);
             dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, nb, N * 64, N2);
         }
-        hipEventRecord(c->ev_a, s);
-        hipStreamWaitEvent(s2, c->ev_a, 0);
+        hipEventRecord(ln.ev_a, s);
+        hipStreamWaitEvent(s2, ln.ev_a, 0);
         potrf_panel_rec(c, s2, k, K1, W1);
-        hipEventRecord(c->ev_p, s2);
+        hipEventRecord(ln.ev_p, s2);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             const float* P2 = L + (K1 + W1) + K0 * N;
@@ -194,7 +213,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
                 dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
             }
         }
-        hipStreamWaitEvent(s, c->ev_p, 0);
+        hipStreamWaitEvent(s, ln.ev_p, 0);
     }
     // X = L^-1 (FP32, FP64 diagonal-block applications), same look-ahead over row panels
     set_identity_f32_batched(s, X, N, N, nb, N2);
@@ -209,22 +228,22 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             gemm_f32_batched(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, bm, nb, N2,
                              N2, N2, kFoldInverse);
         }
-        hipEventRecord(c->ev_a, s);
-        hipStreamWaitEvent(s2, c->ev_a, 0);
+        hipEventRecord(ln.ev_a, s);
+        hipStreamWaitEvent(s2, ln.ev_a, 0);
         trtri_panel_rec(c, s2, k, I1, H1);
-        hipEventRecord(c->ev_p, s2);
+        hipEventRecord(ln.ev_p, s2);
         const int64_t R2 = R - H1;
         if (R2 > 0) {
             Prof p(c, "batch_trtri_f32", 2.0 * nb * R2 * Wc * H, 0, s);
             gemm_f32_batched(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0,
                              bm, nb, N2, N2, N2, kFoldInverse);
         }
-        hipStreamWaitEvent(s, c->ev_p, 0);
+        hipStreamWaitEvent(s, ln.ev_p, 0);
     }
     // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is
     // static) on the side stream, while the main stream forms B^-1 = X^T X
-    hipEventRecord(c->ev_a, s);
-    hipStreamWaitEvent(s2, c->ev_a, 0);
+    hipEventRecord(ln.ev_a, s);
+    hipStreamWaitEvent(s2, ln.ev_a, 0);
     const int64_t NV = (int64_t)nb * N;
     double* rhs = k.d(BB_RHS);
     double* u = k.d(BB_U);
@@ -245,7 +264,7 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
             trmv_lower_f32_batched(s2, X, N, N, t1, dd, 1, nb, N2, N, k.d(BB_KMV), k.nchunk);
         }
         vec_axpby(s2, NV, 1.0, dd, 1.0, u, u);
-        if (it == sweeps - 1) relmax_accumulate(s2, dd, u, NV, B->relmax.as<double>());
+        if (it == sweeps - 1) relmax_accumulate(s2, dd, u, NV, k.d(BB_RELMAX));
         {  // r = S y - B u with the FP64 copy of B
             Prof p(c, "batch_symv_b64", 2.0 * nb * N * N, 0, s2);
             symv_lt_f64(s2, k.d(BB_B64), N, u, Ku, k.d(BB_KMV), k.nchunk, nb, N2, N);
@@ -254,13 +273,13 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     }
     double* alpha = k.d(BB_ALPHA);
     vec_mul(s2, NV, sv, u, alpha);
-    hipEventRecord(c->ev_p, s2);
+    hipEventRecord(ln.ev_p, s2);
     // B^-1 = X^T X and the gradient
     {
         Prof p(c, "batch_lauum_f32", (double)nb * N * N * N / 3, 0, s);
         gemm_f32_batched(s, 1, 0, N, N, N, 1.f, X, N, X, N, 0.f, Bi, N, 2, bm, nb, N2, N2, N2, kFoldInverse);
     }
-    hipStreamWaitEvent(s, c->ev_p, 0);
+    hipStreamWaitEvent(s, ln.ev_p, 0);
     const int64_t ntiles = T * (T + 1) / 2;
     {
         Prof p(c, "batch_grad", 0, 0, s);
@@ -282,6 +301,35 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, hipStream_t s) {
     seg_cols(s, k.d(BB_PART), ntiles, B->hp.ngrad, red, rs, 5, nb);
     batch_finish(s, red, rs, nvec, k.d(BB_SIG), k.b[BB_IDX].as<int32_t>(), B->hp.ngrad, nb, B->res_logz.as<double>(),
                  B->res_grad.as<double>(), B->ng);
+}
+
+// Every bucket on its lane: the lanes fork from `root` and join back into it.
+static void run_buckets(gpl_batch* B, hipStream_t root) {
+    hipEventRecord(B->fork, root);
+    std::vector<char> used(B->lanes.size(), 0);
+    for (size_t i = 0; i < B->buckets.size(); ++i) {
+        Lane& ln = B->lanes[B->lane_of[i]];
+        if (!used[B->lane_of[i]]) {
+            hipStreamWaitEvent(ln.s, B->fork, 0);
+            used[B->lane_of[i]] = 1;
+        }
+        bucket_pipeline(B, *B->buckets[i], ln);
+    }
+    for (size_t l = 0; l < B->lanes.size(); ++l)
+        if (used[l]) {
+            hipEventRecord(B->lanes[l].done, B->lanes[l].s);
+            hipStreamWaitEvent(root, B->lanes[l].done, 0);
+        }
+}
+
+// one bucket on lane 0 (autotuner)
+static void run_bucket(gpl_batch* B, Bucket& k, hipStream_t root) {
+    Lane& ln = B->lanes[0];
+    hipEventRecord(B->fork, root);
+    hipStreamWaitEvent(ln.s, B->fork, 0);
+    bucket_pipeline(B, k, ln);
+    hipEventRecord(ln.done, ln.s);
+    hipStreamWaitEvent(root, ln.done, 0);
 }
 
 static int upload_params(gpl_batch* B, const gpl_cf_term* terms, const double* sigma) {
@@ -404,7 +452,7 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
                   k->b[BB_F].ensure((size_t)nb * N * MAX_FEAT * 8) && k->b[BB_SCAL].ensure(nb * 8) &&
                   k->b[BB_SIG].ensure(nb * 8) && k->b[BB_WSC].ensure(nb * 8) && k->b[BB_IDX].ensure(nb * 4) &&
                   k->b[BB_L].ensure(mat32) && k->b[BB_X32].ensure(mat32) && k->b[BB_BI].ensure(mat32) &&
-                  k->b[BB_B64].ensure(mat32 * 2) &&
+                  k->b[BB_B64].ensure(mat32 * 2) && k->b[BB_RELMAX].ensure(64) &&
                   k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_DACC].ensure((size_t)nb * N * 64 * 8) &&
                   k->b[BB_LDIAG].ensure(vec) &&
                   k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&
@@ -430,10 +478,41 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
         HIP_TRY(hipMemcpyAsync(k->b[BB_Y].p, ys.data(), ys.size() * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
-    if (!B->res_logz.ensure((size_t)nprob * 8) || !B->res_grad.ensure((size_t)nprob * B->ng * 8) ||
-        !B->relmax.ensure(64)) {
+    if (!B->res_logz.ensure((size_t)nprob * 8) || !B->res_grad.ensure((size_t)nprob * B->ng * 8)) {
         delete B;
         return fail(GPL_ERR_OOM, "device allocation failed for the batch");
+    }
+    // lanes: up to four stream pairs; buckets go to the least-loaded lane, largest
+    // estimated time first (GEMM work plus a fixed cost per 64-row diagonal step)
+    {
+        const int nl = (int)std::min<size_t>(4, B->buckets.size());
+        B->lanes.resize(nl);
+        hipError_t e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
+        for (auto& l : B->lanes) {
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.side, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&l.ev_a, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&l.ev_p, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
+        }
+        if (e != hipSuccess) {
+            delete B;
+            return fail(GPL_ERR_HIP, std::string("lane streams: ") + hipGetErrorString(e));
+        }
+        std::vector<int> order(B->buckets.size());
+        std::vector<double> cost(B->buckets.size()), load(nl, 0.0);
+        for (size_t i = 0; i < order.size(); ++i) {
+            const double N = (double)B->buckets[i]->N, nbk = (double)B->buckets[i]->idx.size();
+            cost[i] = nbk * N * N * N / 60e9 + (N / 64) * 0.25;  // ms
+            order[i] = (int)i;
+        }
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        B->lane_of.assign(B->buckets.size(), 0);
+        for (int i : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            B->lane_of[i] = l;
+            load[l] += cost[i];
+        }
     }
     r = upload_params(B, terms, sigma);
     if (r) {
@@ -464,7 +543,7 @@ extern "C" int gpl_batch_set_params(gpl_batch* B, const gpl_cf_term* terms, cons
 
 static void reset_flags(gpl_batch* B, hipStream_t s) {
     // outside any graph: status words are cleared with plain memsets before a pass
-    hipMemsetAsync(B->relmax.p, 0, 8, s);
+    for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_RELMAX].p, 0, 8, s);
     for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_INFO].p, 0, k->idx.size() * 4, s);
 }
 
@@ -475,7 +554,7 @@ static int run_once(gpl_batch* B) {
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const bool prof = B->ctx->prof;
         B->ctx->prof = false;  // no event records inside the graph
-        for (auto* k : B->buckets) bucket_pipeline(B, *k, s);
+        run_buckets(B, s);
         B->ctx->prof = prof;
         hipError_t e = hipStreamEndCapture(s, &B->graph);
         if (e != hipSuccess) return fail(GPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -493,7 +572,7 @@ extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double*
     if (B->ctx->prof) {
         // profiled pass: eager launches with per-class events (no graph)
         reset_flags(B, s);
-        for (auto* k : B->buckets) bucket_pipeline(B, *k, s);
+        run_buckets(B, s);
     } else {
         int r = run_once(B);
         if (r) return r;
@@ -521,7 +600,15 @@ extern "C" int32_t gpl_batch_grad_len(gpl_batch* B) { return B ? B->ng : 0; }
 extern "C" int gpl_batch_info(gpl_batch* B, int32_t* nbuckets, double* refine_rel) {
     if (!B) return fail(GPL_ERR_ARG, "NULL batch");
     if (nbuckets) *nbuckets = (int32_t)B->buckets.size();
-    if (refine_rel) HIP_TRY(hipMemcpy(refine_rel, B->relmax.p, 8, hipMemcpyDeviceToHost));
+    if (refine_rel) {
+        double mx = 0.0;
+        for (auto* k : B->buckets) {
+            double v = 0.0;
+            HIP_TRY(hipMemcpy(&v, k->b[BB_RELMAX].p, 8, hipMemcpyDeviceToHost));
+            mx = std::max(mx, v);
+        }
+        *refine_rel = mx;
+    }
     return GPL_OK;
 }
 
@@ -545,9 +632,9 @@ extern "C" int gpl_batch_tune(gpl_batch* B, int32_t cap, int32_t* bucket_n, int3
                 if (cnb > k->N && cnb != 128) continue;
                 k->bm = cbm;
                 k->nbo = cnb;
-                bucket_pipeline(B, *k, s);  // warm
+                run_bucket(B, *k, s);  // warm
                 hipEventRecord(e0, s);
-                bucket_pipeline(B, *k, s);
+                run_bucket(B, *k, s);
                 hipEventRecord(e1, s);
                 hipEventSynchronize(e1);
                 float ms = 0;
