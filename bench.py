@@ -298,11 +298,13 @@ def bench_c3(args, G, L, api, synth):
     fp = res["fp64"]
     top = {k: {"ms": round(v["ms"], 3), "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}
            for k, v in sorted(fp["prof"].items(), key=lambda kv: -kv[1]["ms"])[:5]}
+    best = min(res, key=lambda p: np.median(res[p]["lat"]))
     return {
         "metric": "Laplace fits/s (n=4096, d=4, Bernoulli-logit, tol 1e-8)",
-        "value": round(1e3 / np.median(res["mixed"]["lat"]), 3), "unit": "fits/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic (seeded GP sample, Bernoulli labels)",
-        "config": {"workload": "C3 Laplace classification", "n": n, "d": 4, "newton_steps": it},
+        "value": round(1e3 / np.median(res[best]["lat"]), 3), "unit": "fits/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32+f64" if best == "mixed" else "f64",
+        "data": "synthetic (seeded GP sample, Bernoulli labels)",
+        "config": {"workload": "C3 Laplace classification", "n": n, "d": 4, "newton_steps": it, "form": best},
         "forms": {p: {"fits_per_s": round(1e3 / np.median(r["lat"]), 3), "p50_ms": _pct(r["lat"], 50),
                       "p99_ms": _pct(r["lat"], 99), "newton_steps": r["fit"].iters,
                       "ms_per_newton_step": round(float(np.median(r["lat"])) / r["fit"].iters, 3)}
