@@ -276,6 +276,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// 1/sqrt(d) for d > 0: hardware estimate refined by two Newton steps
+// y <- y (3 - d y^2) / 2 (each roughly doubles the correct bits), a short
+// dependent chain compared with a correctly rounded sqrt followed by a divide.
+// Non-positive or NaN d propagates NaN, which the pivot check reports.
+__device__ __forceinline__ double rsqrt_f64(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 template <int KQ>
 __device__ __forceinline__ double quad_bcast_f64(double v) {
     constexpr int ctrl = KQ | (KQ << 2) | (KQ << 4) | (KQ << 6);  // quad_perm [KQ,KQ,KQ,KQ]
@@ -290,7 +302,12 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, int want_inv, int64_t sA, int64_t sD,
                                                      int64_t sL, const double* __restrict__ Dsrc) {
-    // batch: one problem per workgroup
+    // One 64x64 diagonal block per workgroup, FP64 arithmetic.  Wave w holds
+    // columns 16w..16w+15 of every row (lane = row).  Factor: four phases; in
+    // phase p wave p factors its 16 columns alone (pivots and multipliers are
+    // exchanged with readlane, no barrier), publishes them to LDS, and the
+    // waves to its right apply the rank-16 update.  Inverse: forward
+    // substitution, four lanes per column exchanging values by DPP.
     A += blockIdx.x * sA;
     Dinv += blockIdx.x * sD;
     ldiag += blockIdx.x * sL;
@@ -298,8 +315,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     // Dsrc (optional): the FP64 diagonal tiles (64x64 column-major, one per block
     // row, same batch stride as Dinv) to factor in place of the stored tile of A
     if (Dsrc) Dsrc += blockIdx.x * sD + k * 64 * 64;
-    __shared__ double colbuf[2][64];
-    __shared__ double aL[64][65];  // factor, a[col][row], for the inverse
+    __shared__ double Lp[64][65];  // Lp[col][row] = L(row, col); strictly lower after the factor
     __shared__ double rinv[64];
     __shared__ double piv[64];
     T* blk = A + k * 64 * (ld + 1);
@@ -307,38 +323,52 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     double a[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = w + 4 * q;
+        const int c = 16 * w + q;
         a[q] = (c <= i) ? (Dsrc ? Dsrc[i + c * 64] : (double)blk[i + (int64_t)c * ld]) : 0.0;
     }
 #pragma unroll
-    for (int j = 0; j < 64; ++j) {
-        const int wj = j & 3, qj = j >> 2;
-        if (w == wj) {
-            const double d = readlane_f64(a[qj], j);
-            const double rt = sqrt(d);
-            const double inv = 1.0 / rt;
-            const double l = a[qj] * inv;
-            a[qj] = (i > j) ? l : ((i == j) ? rt : 0.0);
-            colbuf[j & 1][i] = (i > j) ? l : 0.0;
-            if (i == j) {
-                rinv[j] = inv;
-                piv[j] = d;
+    for (int p = 0; p < 4; ++p) {
+        if (w == p) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * p + jj;
+                const double d = readlane_f64(a[jj], j);
+                const double inv = rsqrt_f64(d);
+                const double rt = d * inv;
+                const double l = (i > j) ? a[jj] * inv : ((i == j) ? rt : 0.0);
+                a[jj] = l;
+                if (i == j) {
+                    rinv[j] = inv;
+                    piv[j] = d;
+                }
+                // rows above a column are cleared when that column is reached
+#pragma unroll
+                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= l * readlane_f64(l, 16 * p + cc);
             }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) Lp[16 * p + q][i] = a[q];
         }
         __syncthreads();
-        const double lij = colbuf[j & 1][i];
+        if (w > p) {
+            double li[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            if (w + 4 * q > j) a[q] -= lij * colbuf[j & 1][w + 4 * q];
+            for (int kk = 0; kk < 16; ++kk) li[kk] = Lp[16 * p + kk][i];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int col = 16 * w + q;
+                double sum = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) sum += li[kk] * Lp[16 * p + kk][col];
+                if (col <= i) a[q] -= sum;
+            }
         }
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = w + 4 * q;
-        const double v = (c <= i) ? a[q] : 0.0;
-        blk[i + (int64_t)c * ld] = (T)v;
-        aL[c][i] = (c < i) ? v : 0.0;  // strictly lower part drives the substitution
+        const int c = 16 * w + q;
+        blk[i + (int64_t)c * ld] = (T)((c <= i) ? a[q] : 0.0);
     }
+    if (t < 64) Lp[t][t] = 0.0;  // the substitution uses the strictly lower part
     __syncthreads();
     if (t < 64) {
         ldiag[k * 64 + t] = 0.5 * log(piv[t]);
@@ -356,8 +386,6 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 #pragma unroll
     for (int kk = 0; kk < 64; ++kk) {
         const int kr = kk >> 2;
-        constexpr int dummy = 0;
-        (void)dummy;
         double xk = b[kr] * rinv[kk];
         switch (kk & 3) {
             case 0: xk = quad_bcast_f64<0>(xk); break;
@@ -367,7 +395,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
         }
         if (q == (kk & 3)) b[kr] = xk;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) b[r] -= aL[kk][4 * r + q] * xk;
+        for (int r = 0; r < 16; ++r) b[r] -= Lp[kk][4 * r + q] * xk;
     }
     double* D = Dinv + k * 64 * 64;
 #pragma unroll
