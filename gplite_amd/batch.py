@@ -77,12 +77,15 @@ class GPBatch:
         sig = np.array([g.lik.sigma for g in self.gps])
         L.check(L.lib().gpl_batch_set_params(self.handle, self._flat(terms), L.dptr(sig)))
 
-    def energy_grad(self, include_prior=True):
-        """(energies [nprob], gradients [nprob, P]) in get_param order."""
+    def energy_grad(self, include_prior=True, allow_not_pd=False):
+        """(energies [nprob], gradients [nprob, P]) in get_param order.  With
+        allow_not_pd a problem whose factor fails gets energy +Inf and NaN
+        gradients instead of raising."""
         np_ = len(self.gps)
         lz = np.empty(np_)
         g = np.empty(np_ * self.ng)
-        L.check(L.lib().gpl_batch_energy_grad(self.handle, L.dptr(lz), L.dptr(g)))
+        rc = L.lib().gpl_batch_energy_grad(self.handle, L.dptr(lz), L.dptr(g))
+        L.check(rc, allow=(L.GPL_ERR_NOT_PD,) if allow_not_pd else ())
         g = g.reshape(np_, self.ng)[:, self._mask]
         e = -lz
         if include_prior:

@@ -581,16 +581,29 @@ extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double*
         HIP_TRY(hipMemcpyAsync(log_evidence, B->res_logz.p, (size_t)B->nprob * 8, hipMemcpyDeviceToHost, s));
     if (grad) HIP_TRY(hipMemcpyAsync(grad, B->res_grad.p, (size_t)B->nprob * B->ng * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    // non-PD pivots in any problem
+    // non-PD pivots: those problems get log_evidence = -Inf and NaN gradients, the
+    // others keep their results, and the call reports the first failing problem
+    int first = -1;
+    int64_t first_minor = 0;
     for (auto* k : B->buckets) {
         std::vector<int> info(k->idx.size());
         HIP_TRY(hipMemcpy(info.data(), k->b[BB_INFO].p, info.size() * 4, hipMemcpyDeviceToHost));
         for (size_t q = 0; q < info.size(); ++q)
             if (info[q] > 0) {
-                set_minor(info[q]);
-                return fail(GPL_ERR_NOT_PD, "problem " + std::to_string(k->idx[q]) + ": leading minor of order " +
-                                                std::to_string(info[q]) + " is not positive");
+                const int pidx = k->idx[q];
+                if (log_evidence) log_evidence[pidx] = -INFINITY;
+                if (grad)
+                    for (int j = 0; j < B->ng; ++j) grad[(size_t)pidx * B->ng + j] = NAN;
+                if (first < 0 || pidx < first) {
+                    first = pidx;
+                    first_minor = info[q];
+                }
             }
+    }
+    if (first >= 0) {
+        set_minor(first_minor);
+        return fail(GPL_ERR_NOT_PD, "problem " + std::to_string(first) + ": leading minor of order " +
+                                        std::to_string(first_minor) + " is not positive");
     }
     return GPL_OK;
 }

@@ -194,7 +194,10 @@ int gpl_batch_create(gpl_ctx* ctx, int32_t nprob, const gpl_cf_term* terms, int3
                      gpl_batch** out);
 /* New hyperparameters (same structure) for every problem. */
 int gpl_batch_set_params(gpl_batch* batch, const gpl_cf_term* terms, const double* sigma);
-/* log_evidence[nprob]; grad[nprob * gpl_batch_grad_len()] in the layout of gpl_grad_len. */
+/* log_evidence[nprob]; grad[nprob * gpl_batch_grad_len()] in the layout of gpl_grad_len.
+ * A problem whose factor meets a non-positive pivot gets log_evidence = -Inf and NaN
+ * gradients; the others are complete, and the call returns GPL_ERR_NOT_PD naming the
+ * first such problem. */
 int gpl_batch_energy_grad(gpl_batch* batch, double* log_evidence, double* grad);
 int32_t gpl_batch_grad_len(gpl_batch* batch);
 /* Number of buckets and the largest relative last refinement correction. */

@@ -66,3 +66,25 @@ def test_batch_params_and_tuning(gpl):
         fo = O.gp_fit([c.spec() for c in B.gps[p].cfs], B.gps[p].lik.spec(), xs[p], ys[p])
         eo = O.gp_energy(fo)
         assert abs(e[p] - eo) <= logz_tol(len(ys[p]), B.gps[p].lik.sigma, eo), (p, e[p], eo)
+
+
+def test_batch_reports_failed_factor_per_problem(gpl):
+    """A problem whose B has a negative pivot (negative jitter on a tiny signal)
+    gets energy +Inf while the others keep their results; without allow_not_pd the
+    call raises naming that problem."""
+    G = gpl
+    from gplite_amd._lib import GplError
+    from gplite_amd.batch import GPBatch
+    r = np.random.default_rng(2)
+    x = r.uniform(-2, 2, (100, 2))
+    y = np.sin(x.sum(1))
+    good = G.gp_init(G.cf_sexp(lscale=1.0, magn=1.0), G.lik_gaussian(sigma=1.0))
+    bad = G.gp_init(G.cf_sexp(lscale=1.0, magn=1e-3), G.lik_gaussian(sigma=0.1))
+    B = GPBatch([good, bad, good], [x, x, x], [y, y, y], jitter=-0.5)
+    e, g = B.energy_grad(allow_not_pd=True)
+    assert np.isfinite(e[0]) and np.isfinite(e[2]) and e[1] == np.inf
+    assert np.all(np.isnan(g[1])) and np.all(np.isfinite(g[0]))
+    assert e[0] == e[2]
+    with pytest.raises(GplError) as ei:
+        B.energy_grad()
+    assert "problem 1" in str(ei.value)
