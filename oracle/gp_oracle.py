@@ -39,7 +39,7 @@ __all__ = [
     "hermite_polynomials", "gauss_hermite_points", "logsumexp_w",
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
-    "gp_energy_grad", "gp_pred", "gaussian_logz_closed_form",
+    "gp_energy_grad", "gp_pred", "gp_loo", "gaussian_logz_closed_form",
 ]
 
 JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
@@ -760,6 +760,7 @@ def gp_fit(cfs, lik, x, y, approx="laplace", maxiter=None, tol=1e-4, jitter=None
     else:
         raise ValueError(approx)
     fit["cfs"] = cfs; fit["lik"] = lik; fit["x"] = x; fit["K"] = K; fit["jitter"] = jitter
+    fit["approx"] = approx
     return fit
 
 
@@ -811,6 +812,46 @@ def gp_pred(fit, xt, var=False, jitter=None, prior=False):
     aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
     v = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux * aux, axis=0)
     return mean, v
+
+
+def _logsumexp_rows(a, w=None):
+    """log(sum_j w_j exp(a_ij)) per row, stabilised by the row max (R/gp_loo.R:136-144)."""
+    c = np.max(a, axis=1, keepdims=True)
+    e = np.exp(a - c)
+    if w is not None:
+        e = e * w[None, :]
+    return c[:, 0] + np.log(np.sum(e, axis=1))
+
+
+def gp_loo(fit, lik, x, y, quadrature=True, quad_order=11, draws=4000, seed=0, trials=None, offset=None):
+    """LOO predictive log densities for method_full: R/gp_loo.R:73-110.
+    Laplace LOO posteriors from the posterior at the training inputs
+    (R/gp_loo.R:120-130); EP uses the cavity moments (:132-134)."""
+    y = np.asarray(y, float)
+    n = len(y)
+    if "cavity_mean" in fit and fit.get("approx") == "ep":
+        m, v = fit["cavity_mean"], fit["cavity_var"]
+    else:
+        fhat, z, V = fit["fmean"], fit["z"], fit["V"]
+        grad = (z - fhat) / V
+        _, post_var = gp_pred(fit, x, var=True)
+        v = 1.0 / (1.0 / post_var - 1.0 / V)
+        m = fhat - v * grad
+    sd = np.sqrt(v)
+    if quadrature:
+        xg, wg = gauss_hermite_points(quad_order)
+        fgrid = m[:, None] + sd[:, None] * xg[None, :]
+        ll = loglik(lik, fgrid, y[:, None], None if trials is None else np.asarray(trials, float)[:, None],
+                    None if offset is None else np.asarray(offset, float)[:, None], total=False)
+        loos = _logsumexp_rows(ll, wg / np.sum(wg))
+    else:
+        rng = np.random.default_rng(seed)
+        fs = m[:, None] + sd[:, None] * rng.standard_normal((n, draws))
+        ll = loglik(lik, fs, y[:, None], None if trials is None else np.asarray(trials, float)[:, None],
+                    None if offset is None else np.asarray(offset, float)[:, None], total=False)
+        loos = _logsumexp_rows(ll) - math.log(draws)
+    return {"loo": float(np.sum(loos)), "sd": float(n * np.std(loos, ddof=1) / math.sqrt(n)), "loos": loos,
+            "loo_mean": m, "loo_var": v}
 
 
 def gaussian_logz_closed_form(K_cf, y, sigma, jitter=JITTER):

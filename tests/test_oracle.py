@@ -334,3 +334,41 @@ def test_cpu_baseline_matches_oracle():
     fit = O.gp_fit(cfs, {"kind": "gaussian", "sigma": 0.07}, x, y)
     assert lz == pytest.approx(fit["log_evidence"], rel=1e-10)
     np.testing.assert_allclose(g, O.gp_energy_grad(fit), rtol=1e-8)
+
+
+def test_oracle_loo_gaussian_closed_form():
+    """Gaussian likelihood.
+    * The LOO latent moments of R/gp_loo.R:120-130 must give the closed form
+      s_i^2 = 1/[K_y^-1]_ii and mu_i = y_i - [K_y^-1 y]_i s_i^2 (Rasmussen &
+      Williams eqs. 5.10-5.12): loo_var + sigma^2 = s_i^2, loo_mean = mu_i.
+    * A 200-node Gauss-Hermite integral over those moments must match
+      log N(y_i | mu_i, s_i^2).
+    * The 11-node loos (R/quadrature.R's rule through logsumexp) must match the
+      same integral computed with numpy's hermegauss(11)."""
+    r = np.random.default_rng(8)
+    n = 60
+    x = r.normal(size=(n, 2))
+    y = np.sin(x.sum(1)) + 0.2 * r.normal(size=n)
+    cf = [{"kind": "sexp", "lscale": 0.9, "magn": 1.2}]
+    lik = {"kind": "gaussian", "sigma": 0.2}
+    # jitter 0: with jitter, R's predictive variance at the training inputs uses the
+    # jitter-free prior diagonal while the fit's prior carries the jitter, so the
+    # identity holds only up to O(jitter) amplified by V/(V - post_var)
+    fit = O.gp_fit(cf, lik, x, y, jitter=0.0)
+    out = O.gp_loo(fit, lik, x, y, quad_order=11)
+    Ky = O.eval_cf(fit["cfs"], x, x) + 0.2 ** 2 * np.eye(n)
+    Ki = np.linalg.inv(Ky)
+    s2 = 1.0 / np.diag(Ki)
+    mu = y - (Ki @ y) * s2
+    assert np.max(np.abs(out["loo_var"] + 0.2 ** 2 - s2) / s2) < 1e-8
+    assert np.max(np.abs(out["loo_mean"] - mu)) < 1e-8
+    ref = -0.5 * np.log(2 * np.pi * s2) - 0.5 * (y - mu) ** 2 / s2
+
+    def gh_loos(q):
+        xg, wg = np.polynomial.hermite_e.hermegauss(q)
+        wg = wg / wg.sum()
+        f = out["loo_mean"][:, None] + np.sqrt(out["loo_var"])[:, None] * xg[None, :]
+        ll = -0.5 * np.log(2 * np.pi * 0.04) - 0.5 * (y[:, None] - f) ** 2 / 0.04
+        return np.log(np.sum(wg[None, :] * np.exp(ll), axis=1))
+    assert np.max(np.abs(gh_loos(200) - ref)) < 1e-9
+    assert np.max(np.abs(out["loos"] - gh_loos(11))) < 1e-10
