@@ -104,6 +104,7 @@ def _sig(lib):
         "gpl_server_stats": (C.c_int, [_vp] + [C.POINTER(i64)] * 6),
         "gpl_server_destroy": (None, [_vp]),
         "gpl_pred_ctx": (C.c_int, [_vp, _vp, _dp, i64, i32, _dp, _dp]),
+        "gpl_draw": (C.c_int, [_vp, _dp, i64, C.c_double, _dp, i64, _dp]),
         "gpl_model_bytes": (i64, [_vp]),
     }
     for name, (res, args) in sigs.items():

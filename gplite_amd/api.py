@@ -775,6 +775,48 @@ def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, ji
     return out
 
 
+def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, jitter=None, seed=None,
+            trials=None, offset=None):
+    """Posterior draws at xnew: R/gp_draw.R:42-124 for method_full, returned as
+    a (len(xnew), draws) array.
+
+    marginal=False draws mean + chol(cov + jitter I) z on the device
+    (gpl_draw: full predictive covariance, its Cholesky factor and the triangular
+    product).  marginal=True uses the predictive variances only.  The standard
+    normal draws z come from numpy's Generator(PCG64(seed)); R's rnorm stream
+    cannot be reproduced.  transform maps to the response scale; target draws
+    observations (R/lik.R:502-537).
+    """
+    if not is_fitted(gp):
+        raise NotImplementedError("draws from the prior are not on the device path; fit the model first")
+    xt = _as_matrix(xnew)
+    m = xt.shape[0]
+    jitter = 1e-6 if jitter is None else float(jitter)
+    rng = np.random.default_rng(seed)
+    z = np.asfortranarray(rng.standard_normal((m, draws)))
+    if marginal:
+        pr = gp_pred(gp, xt, var=True)
+        f = pr["mean"][:, None] + np.sqrt(pr["var"])[:, None] * z
+    else:
+        _, _, xtp, _ = _program(gp.cfs, xt)
+        f = np.empty((m, draws), order="F")
+        L.check(L.lib().gpl_draw(gp.fit.handle, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp), draws,
+                                 f.ctypes.data_as(L._dp)))
+    if offset is not None:
+        f = f + np.asarray(offset, dtype=float)[:, None]
+    if target:
+        mu = gp.lik.response(f)
+        if isinstance(gp.lik, lik_gaussian):
+            return f + gp.lik.sigma * rng.standard_normal(f.shape)
+        if isinstance(gp.lik, lik_poisson):
+            return rng.poisson(mu).astype(float)
+        t = np.ones(m) if trials is None else np.asarray(trials, dtype=float)
+        return rng.binomial(t.astype(np.int64)[:, None], mu).astype(float)
+    if transform:
+        return gp.lik.response(f)
+    return f
+
+
 def _loglik_points(lik, f, y, trials=None, offset=None):
     """Pointwise observation log density log p(y_i | f_ik) (R/lik.R:297-366);
     f has shape (n, q) and broadcasts against y[:, None]."""

@@ -1213,6 +1213,90 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
     return GPL_OK;
 }
 
+// Latent posterior draws at mt test points (R/gp_draw.R:107-124, method_full,
+// marginal = FALSE): mean + chol(cov + jitter I) z with the full predictive
+// covariance cov = K_tt - aux^T aux, aux = C^-1 K_t^T (R/gp_pred.R:202-232).
+// z (mt x draws, column-major) holds the caller's standard normal draws.
+static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, double jitter, const double* z,
+                   int64_t draws, double* out) {
+    if (!c || !m || !xt || !z || !out || mt < 1 || draws < 1) return fail(GPL_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N, M = pad64(mt);
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
+    double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
+    double* KtT = (double*)c->buf(2, (size_t)N * M * 8);
+    double* vbuf = (double*)c->buf(3, (size_t)M * 3 * 8);
+    double* A2 = (double*)c->buf(4, (size_t)N * M * 8);
+    double* Cv = (double*)c->buf(5, (size_t)M * M * 8);
+    double* Dv = (double*)c->buf(6, (size_t)M * 64 * 8 + (size_t)M * 8 + 64);
+    const int64_t Dp = pad64(draws);  // the GEMM writes whole 16-column blocks
+    double* Zd = (double*)c->buf(7, (size_t)M * Dp * 8);
+    double* Wd = (double*)c->buf(8, (size_t)M * Dp * 8);
+    if (!xtd || !Ft || !KtT || !vbuf || !A2 || !Cv || !Dv || !Zd || !Wd)
+        return fail(GPL_ERR_OOM, "device allocation failed");
+    double* ld = Dv + M * 64;
+    int* info = (int*)(ld + M);
+    if (mt * m->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, xtd, mt, mt, (int)m->ncols, pd, m->prog, Ft, M);
+    launch_build_cross(s, pd, m->prog, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
+    gemv_t_f64(s, KtT, N, N, mt, m->d(B_ALPHA), vbuf);
+    scale_rows_f64(s, KtT, N, n, M, m->d(B_SV));
+    const int64_t T = N / 64;
+    if (m->have_xinv) {
+        if (m->opts.precision == GPL_PREC_MIXED)
+            gemm_mixed_f32f64f64(s, 0, 0, N, M, N, 1.0, m->b[B_XINV].as<float>(), N, KtT, N, 0.0, A2, N, 3);
+        else
+            gemm2_f64(s, 0, 0, N, M, N, 1.0, m->d(B_XINV), N, KtT, N, 0.0, A2, N, 3, 0);
+    } else {
+        for (int64_t i = 0; i < T; ++i) {
+            gemm2_f64(s, 0, 0, 64, M, 64, 1.0, m->d(B_DINV) + i * 64 * 64, 64, KtT + i * 64, N, 0.0, KtT + i * 64, N,
+                      0, 0);
+            if (i + 1 < T)
+                gemm2_f64(s, 0, 0, (T - 1 - i) * 64, M, 64, -1.0, m->d(B_L) + (i + 1) * 64 + i * 64 * N, N,
+                          KtT + i * 64, N, 1.0, KtT + (i + 1) * 64, N, 0, 0);
+        }
+        A2 = KtT;
+    }
+    // cov = K_tt + jitter I - aux^T aux over lower tiles (padding: identity), then its factor
+    launch_build_sym_f64(s, pd, m->prog, Ft, mt, M, jitter, 0, 1.0, nullptr, Cv, M);
+    {
+        Prof p(c, "draw_cov", (double)M * M * N, 8.0 * N * M);
+        gemm2_f64(s, 1, 0, M, M, N, -1.0, A2, N, A2, N, 1.0, Cv, M, 1, 0);
+    }
+    HIP_TRY(hipMemsetAsync(info, 0, 64, s));
+    potrf_blocked(c, Cv, M, Dv, ld, info);
+    int inf = 0;
+    HIP_TRY(hipMemcpyAsync(&inf, info, 4, hipMemcpyDeviceToHost, s));
+    // W = L Z (Z padded with zero rows); out = mean + W
+    fill_f64(s, Zd, M * Dp, 0.0);
+    HIP_TRY(hipMemcpy2DAsync(Zd, M * 8, z, mt * 8, mt * 8, draws, hipMemcpyHostToDevice, s));
+    {
+        Prof p(c, "draw_trmm", (double)M * M * Dp, 8.0 * M * Dp * 2);
+        gemm2_f64(s, 0, 0, M, Dp, M, 1.0, Cv, M, Zd, M, 0.0, Wd, M, 3, 0);
+    }
+    std::vector<double> mean(mt);
+    HIP_TRY(hipMemcpyAsync(mean.data(), vbuf, mt * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpy2DAsync(out, mt * 8, Wd, M * 8, mt * 8, draws, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (inf > 0) {
+        set_minor(inf);
+        return fail(GPL_ERR_NOT_PD, "predictive covariance: the leading minor of order " + std::to_string(inf) +
+                                        " is not positive (raise jitter)");
+    }
+    for (int64_t j = 0; j < draws; ++j)
+        for (int64_t i = 0; i < mt; ++i) out[i + j * mt] += mean[i];
+    return GPL_OK;
+}
+
+extern "C" int gpl_draw(gpl_model* m, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
+                        double* out) {
+    if (!m) return fail(GPL_ERR_ARG, "NULL model");
+    return draw_on(m->ctx, m, xt, mt, jitter, z, draws, out);
+}
+
 extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var) {
     if (!m) return fail(GPL_ERR_ARG, "NULL model");
     return pred_on(m->ctx, m, xt, mt, want_var, mean, var);

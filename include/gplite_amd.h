@@ -243,6 +243,14 @@ void gpl_server_destroy(gpl_server* srv);
  * same device can be used concurrently by several contexts). */
 int gpl_pred_ctx(gpl_ctx* ctx, gpl_model* model, const double* xt, int64_t m, int32_t want_var,
                  double* mean, double* var);
+/* gp_draw for method_full with marginal = FALSE (R/gp_draw.R:107-124,182-200):
+ * latent draws mean + chol(cov + jitter I) z at the mt test points, where cov is
+ * the full predictive covariance K_tt - aux^T aux, aux = C_chol^-1 K_t^T
+ * (R/gp_pred.R:202-232, cov = TRUE).  z: mt x draws column-major standard normal
+ * draws from the caller's generator (R's rnorm in mvnrnd); out: mt x draws.
+ * GPL_ERR_NOT_PD when cov + jitter I has a non-positive pivot. */
+int gpl_draw(gpl_model* model, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
+             double* out);
 /* Device bytes held by a fitted model. */
 int64_t gpl_model_bytes(gpl_model* model);
 

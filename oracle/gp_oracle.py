@@ -39,7 +39,7 @@ __all__ = [
     "hermite_polynomials", "gauss_hermite_points", "logsumexp_w",
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
-    "gp_energy_grad", "gp_pred", "gp_loo", "gaussian_logz_closed_form",
+    "gp_energy_grad", "gp_pred", "gp_pred_cov", "gp_draw", "gp_loo", "gaussian_logz_closed_form",
 ]
 
 JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
@@ -812,6 +812,28 @@ def gp_pred(fit, xt, var=False, jitter=None, prior=False):
     aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
     v = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux * aux, axis=0)
     return mean, v
+
+
+def gp_pred_cov(fit, xt, jitter=None):
+    """Posterior mean and full covariance + jitter I: R/gp_pred.R:202-227
+    (cov = TRUE branch, K_tt - aux^T aux with aux = C_chol^-1 K_t^T)."""
+    cfs = fit["cfs"]
+    xt = np.atleast_2d(np.asarray(xt, float))
+    jitter = JITTER if jitter is None else jitter
+    Kt = eval_cf(cfs, xt, fit["x"])
+    mean = Kt @ fit["alpha"]
+    aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
+    cov = eval_cf(cfs, xt, xt) - aux.T @ aux + jitter * np.eye(xt.shape[0])
+    return mean, cov
+
+
+def gp_draw(fit, xt, z, jitter=None):
+    """Joint posterior draws of the latent function: R/gp_draw.R:101-118 with
+    mvnrnd (R/gp_draw.R:182-200): mean + t(chol(cov)) z for the given standard
+    normal matrix z of shape (len(xt), draws)."""
+    mean, cov = gp_pred_cov(fit, xt, jitter)
+    Lc = sla.cholesky(cov, lower=True)
+    return mean[:, None] + Lc @ np.asarray(z, float).reshape(len(mean), -1)
 
 
 def _logsumexp_rows(a, w=None):

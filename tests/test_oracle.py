@@ -336,6 +336,39 @@ def test_cpu_baseline_matches_oracle():
     np.testing.assert_allclose(g, O.gp_energy_grad(fit), rtol=1e-8)
 
 
+def test_oracle_draw_gaussian_closed_form():
+    """Joint posterior draws (R/gp_draw.R:101-118).
+    * Gaussian likelihood: mean = K_t K_y^-1 y and cov = K_tt - K_t K_y^-1 K_t^T
+      + jitter I (Rasmussen & Williams eqs. 2.23-2.24), from a direct solve with
+      K_y = K_cf + (sigma^2 + jitter) I.
+    * With z = I the draws minus the mean are the factor itself: lower
+      triangular, positive diagonal and L L^T = cov, which fixes L uniquely.
+    * diag(cov) - jitter equals the predictive variance of gp_pred."""
+    r = np.random.default_rng(11)
+    n, m = 50, 17
+    x = r.normal(size=(n, 2))
+    y = np.cos(x.sum(1)) + 0.1 * r.normal(size=n)
+    xt = r.normal(size=(m, 2))
+    cf = [{"kind": "matern52", "lscale": 0.8, "magn": 1.1}]
+    lik = {"kind": "gaussian", "sigma": 0.3}
+    fit = O.gp_fit(cf, lik, x, y)
+    jit = 1e-6
+    Kc = O.eval_cf(fit["cfs"], x, x)
+    Kt = O.eval_cf(fit["cfs"], xt, x)
+    Ky = Kc + (0.3 ** 2 + jit) * np.eye(n)
+    mean_cf = Kt @ np.linalg.solve(Ky, y)
+    cov_cf = O.eval_cf(fit["cfs"], xt, xt) - Kt @ np.linalg.solve(Ky, Kt.T) + jit * np.eye(m)
+    draws = O.gp_draw(fit, xt, np.eye(m), jitter=jit)
+    Lf = draws - mean_cf[:, None]
+    assert np.max(np.abs(np.triu(Lf, 1))) < 1e-12
+    assert np.all(np.diag(Lf) > 0)
+    np.testing.assert_allclose(Lf @ Lf.T, cov_cf, atol=1e-10)
+    mean, cov = O.gp_pred_cov(fit, xt, jitter=jit)
+    np.testing.assert_allclose(mean, mean_cf, atol=1e-10)
+    _, v = O.gp_pred(fit, xt, var=True)
+    np.testing.assert_allclose(np.diag(cov) - jit, v, atol=1e-12)
+
+
 def test_oracle_loo_gaussian_closed_form():
     """Gaussian likelihood.
     * The LOO latent moments of R/gp_loo.R:120-130 must give the closed form
