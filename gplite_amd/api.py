@@ -817,6 +817,128 @@ def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, jit
     return f
 
 
+# ---------------------------------------------------------------------------
+# gp_save / gp_load (R/saveload.R:32-41)
+# ---------------------------------------------------------------------------
+
+_FILE_FORMAT = "gplite_amd-gp-1"
+
+
+def _saveable():
+    return {c.__name__: c for c in (cf_const, cf_lin, cf_sexp, cf_matern32, cf_matern52, cf_nn, cf_periodic, cf_prod,
+                                    lik_gaussian, lik_bernoulli, lik_binomial, lik_poisson, prior_fixed,
+                                    prior_logunif, prior_lognormal, prior_half_t, approx_laplace, approx_ep,
+                                    method_full)}
+
+
+def _encode(obj, classes):
+    if obj is None or isinstance(obj, (bool, int, float, str)):
+        return obj
+    if isinstance(obj, np.ndarray):
+        return {"__ndarray__": obj.tolist(), "dtype": str(obj.dtype)}
+    if isinstance(obj, np.generic):
+        return obj.item()
+    if isinstance(obj, (list, tuple)):
+        return [_encode(v, classes) for v in obj]
+    if isinstance(obj, dict):
+        return {"__map__": {k: _encode(v, classes) for k, v in obj.items()}}
+    name = type(obj).__name__
+    if classes.get(name) is type(obj):
+        return {"__class__": name, "state": {k: _encode(v, classes) for k, v in vars(obj).items()}}
+    raise TypeError(f"gp_save: cannot store an object of type {name}")
+
+
+def _decode(obj, classes):
+    if isinstance(obj, list):
+        return [_decode(v, classes) for v in obj]
+    if not isinstance(obj, dict):
+        return obj
+    if "__ndarray__" in obj:
+        return np.asarray(obj["__ndarray__"], dtype=np.dtype(obj["dtype"]))
+    if "__map__" in obj:
+        return {k: _decode(v, classes) for k, v in obj["__map__"].items()}
+    cls = classes.get(obj.get("__class__"))
+    if cls is None:
+        raise ValueError(f"gp_load: unknown object class {obj.get('__class__')!r}")
+    inst = cls.__new__(cls)  # state only: no constructor, no code from the file
+    inst.__dict__.update({k: _decode(v, classes) for k, v in obj["state"].items()})
+    return inst
+
+
+def gp_save(gp, filename):
+    """R/saveload.R:32-34: store the model, fitted or not, in one file.
+
+    The model structure (covariance functions with their learned normalisation,
+    likelihood, priors, approximation) is stored as JSON and the numbers as
+    arrays of an .npz archive; nothing is pickled.  A fitted model also stores
+    the gp$fit fields of R/laplace.R:119 and R/ep.R:140-145 (C_chol, fmean,
+    alpha, z, V, fvar, cavity moments, log_evidence), read back from the device,
+    and its training inputs.  A fit made stale by set_param is not stored."""
+    import json
+    classes = _saveable()
+    head = {"format": _FILE_FORMAT, "cfs": _encode(gp.cfs, classes), "lik": _encode(gp.lik, classes),
+            "method": _encode(gp.method, classes), "approx": _encode(gp.approx, classes),
+            "precision": gp.precision, "fitted": bool(gp.fit is not None and gp.fitted)}
+    arrays = {}
+    if head["fitted"]:
+        f = gp.fit
+        head.update(n=int(f.n), log_evidence=float(f.log_evidence), iters=int(f.iters))
+        arrays["x"] = np.asarray(gp.x, dtype=np.float64)
+        arrays["C_chol"] = f.C_chol
+        for nm, w in (("fmean", L.VEC_FMEAN), ("alpha", L.VEC_ALPHA), ("z", L.VEC_Z), ("V", L.VEC_V),
+                      ("fvar", L.VEC_FVAR), ("cavity_mean", L.VEC_CAVITY_MEAN),
+                      ("cavity_var", L.VEC_CAVITY_VAR)):
+            arrays[nm] = f.vector(w)
+        for nm in ("_trials", "_offset"):
+            v = getattr(gp, nm, None)
+            if v is not None:
+                arrays[nm.lstrip("_")] = np.asarray(v, dtype=np.float64)
+    arrays["header"] = np.array(json.dumps(head))
+    with open(filename, "wb") as fh:
+        np.savez(fh, **arrays)
+
+
+def gp_load(filename):
+    """R/saveload.R:37-41: the model stored by gp_save.  A fitted model is
+    restored on the device from its saved fields (gpl_model_restore: no refit),
+    in FP64 whatever precision it was fitted in."""
+    import json
+    classes = _saveable()
+    with np.load(filename, allow_pickle=False) as z:
+        head = json.loads(str(z["header"]))
+        if head.get("format") != _FILE_FORMAT:
+            raise ValueError(f"gp_load: {filename} is not a saved gplite_amd model")
+        arrays = {k: z[k] for k in z.files if k != "header"}
+    g = GP(_decode(head["cfs"], classes), _decode(head["lik"], classes), _decode(head["method"], classes),
+           _decode(head["approx"], classes))
+    g.precision = head["precision"]
+    if not head["fitted"]:
+        return g
+    x = _as_matrix(arrays["x"])
+    n = int(head["n"])
+    terms, nt, xp, _ = _program(g.cfs, x)
+    o = L.Opts()
+    L.lib().gpl_opts_default(C_byref(o))
+    o.approx = L.APPROX_EP if isinstance(g.approx, approx_ep) else L.APPROX_LAPLACE
+    lk = _lik_struct(g.lik)
+    vecs = [np.ascontiguousarray(arrays[nm], dtype=np.float64) for nm in
+            ("fmean", "alpha", "z", "V", "fvar", "cavity_mean", "cavity_var")]
+    vptr = (L._dp * 7)(*[L.dptr(v) for v in vecs])
+    cc = np.asfortranarray(arrays["C_chol"], dtype=np.float64)
+    if cc.shape != (n, n) or x.shape[0] != n or any(v.shape != (n,) for v in vecs):
+        raise ValueError("gp_load: inconsistent sizes in the saved fit")
+    handle = L._vp()
+    L.check(L.lib().gpl_model_restore(L.ctx(), terms, nt, L.dptr(xp), n, xp.shape[1], C_byref(lk), C_byref(o),
+                                      cc.ctypes.data_as(L._dp), vptr, float(head["log_evidence"]),
+                                      int(head["iters"]), C_byref(handle)))
+    g.fit = _Fit(handle, n, float(head["log_evidence"]), int(head["iters"]), o.approx)
+    g.x = x
+    g.fitted = True
+    g._trials = arrays.get("trials")
+    g._offset = arrays.get("offset")
+    return g
+
+
 def _loglik_points(lik, f, y, trials=None, offset=None):
     """Pointwise observation log density log p(y_i | f_ik) (R/lik.R:297-366);
     f has shape (n, q) and broadcasts against y[:, None]."""

@@ -1356,6 +1356,73 @@ extern "C" int gpl_model_chol(gpl_model* m, double* out) {
     return GPL_OK;
 }
 
+// gp_load (R/saveload.R:37-41): a device model from the saved gp$fit fields,
+// without refitting.  The saved C_chol goes to the factor buffer and is
+// rescaled in place; the diagonal-block inverses are recomputed from it.
+extern "C" int gpl_model_restore(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n,
+                                 int64_t ncols, const gpl_lik* lik, const gpl_opts* opts, const double* c_chol,
+                                 const double* const* vectors, double log_evidence, int32_t iters, gpl_model** out) {
+    if (!c || !x || !lik || !c_chol || !vectors || !out) return fail(GPL_ERR_ARG, "NULL argument");
+    if (n < 1 || ncols < 0) return fail(GPL_ERR_ARG, "n must be >= 1");
+    const double* V = vectors[GPL_VEC_V];
+    if (!V || !vectors[GPL_VEC_ALPHA]) return fail(GPL_ERR_ARG, "alpha and V are required");
+    for (int64_t i = 0; i < n; ++i)
+        if (!(V[i] > 0.0) || !std::isfinite(V[i])) return fail(GPL_ERR_ARG, "V must be positive and finite");
+    gpl_opts o;
+    if (opts) o = *opts;
+    else gpl_opts_default(&o);
+    if (lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_EP)
+        return fail(GPL_ERR_ARG, "approx_ep disabled for lik_gaussian");
+    o.precision = GPL_PREC_FP64;  // the saved factor is FP64
+    DevProg P;
+    int r = prepare_prog(terms, nterms, ncols, P);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    gpl_model* m = new gpl_model();
+    m->ctx = c;
+    r = alloc_model(m, n, ncols, false, false);
+    if (r) {
+        delete m;
+        return r;
+    }
+    m->prog = P;
+    m->lik = *lik;
+    m->opts = o;
+    m->logZ = log_evidence;
+    m->iters = iters;
+    const int64_t N = m->N;
+    hipStream_t s = c->stream;
+    int info = 0;
+    auto upload = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(m->b[B_PROG].p, &m->prog, sizeof(DevProg), hipMemcpyHostToDevice, s));
+        if (n * ncols > 0) HIP_TRY(hipMemcpyAsync(m->d(B_X), x, n * ncols * 8, hipMemcpyHostToDevice, s));
+        launch_features(s, m->d(B_X), n, n, (int)ncols, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), N);
+        for (int idx : {B_Y, B_TR, B_OFF, B_SV}) upload_vec(m, idx, nullptr, n);
+        const int map[7] = {B_FMEAN, B_ALPHA, B_Z, B_V, B_FVAR, B_CAVM, B_CAVV};
+        for (int w = 0; w < 7; ++w) upload_vec(m, map[w], vectors[w], n);
+        HIP_TRY(hipMemcpy2DAsync(m->d(B_L), N * 8, c_chol, n * 8, n * 8, n, hipMemcpyHostToDevice, s));
+        restore_factor_f64(s, m->d(B_L), n, N, m->d(B_V), m->d(B_SV));
+        HIP_TRY(hipMemsetAsync(m->b[B_INFO].p, 0, 64, s));
+        trtri_diag64_f64(s, m->d(B_L), N, N / 64, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
+        HIP_TRY(hipMemcpyAsync(&info, m->b[B_INFO].p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return GPL_OK;
+    };
+    r = upload();
+    if (r) {
+        delete m;
+        return r;
+    }
+    if (info > 0) {
+        delete m;
+        set_minor(info);
+        return fail(GPL_ERR_NOT_PD, "saved C_chol: the diagonal entry " + std::to_string(info) + " is not positive");
+    }
+    *out = m;
+    return GPL_OK;
+}
+
 extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence) {
     if (!m) return fail(GPL_ERR_ARG, "NULL model");
     if (n) *n = m->n;

@@ -835,4 +835,63 @@ void form_B_mixed(hipStream_t s, const double* K, double* B64, float* B32, doubl
     hipLaunchKernelGGL(k_form_B, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, K, B64, ld, n, sv, B32, dacc);
 }
 
+// ---------------------------------------------------------------------------
+// Restore a saved factor (gp_load).  In place over L (N x N, column-major): the
+// n x n part holds C_chol = diag(sqrt V) chol(B)^T as saved; it becomes
+// L = diag(1/sqrt V) C_chol with zeros above the diagonal, the padding becomes
+// the identity, and sv[i] = 1/sqrt(V_i).  One workgroup per column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_restore_factor(double* __restrict__ L, int64_t n, int64_t N,
+                                                        const double* __restrict__ V, double* __restrict__ sv) {
+    const int64_t j = blockIdx.x;
+    for (int64_t i = threadIdx.x; i < N; i += 256) {
+        double v;
+        if (i < n && j < n) v = (i >= j) ? L[i + j * N] / sqrt(V[i]) : 0.0;
+        else v = (i == j) ? 1.0 : 0.0;
+        L[i + j * N] = v;
+        if (j == 0 && i < n) sv[i] = 1.0 / sqrt(V[i]);
+    }
+}
+
+void restore_factor_f64(hipStream_t s, double* L, int64_t n, int64_t N, const double* V, double* sv) {
+    hipLaunchKernelGGL(k_restore_factor, dim3((unsigned)N), dim3(256), 0, s, L, n, N, V, sv);
+}
+
+// Inverses of the 64x64 diagonal blocks of an existing lower-triangular factor
+// (the part of k_potrf_diag3 that follows the factorisation): Dinv + k*4096
+// receives L_kk^-1 (column-major, ld 64, zeros above the diagonal) and
+// ldiag[64k + i] = log L_ii.  Lane t forward-substitutes column t.  A pivot
+// that is not positive (or NaN) sets info to its 1-based index if info is 0.
+__global__ __launch_bounds__(64) void k_trtri_diag64(const double* __restrict__ L, int64_t ld,
+                                                     double* __restrict__ Dinv, double* __restrict__ ldiag,
+                                                     int* __restrict__ info) {
+    __shared__ double A[64][65];  // A[col][row] = L_kk(row, col)
+    __shared__ double X[64][65];  // X[col][row] = L_kk^-1(row, col)
+    const int64_t k = blockIdx.x;
+    const int t = threadIdx.x;
+    const double* blk = L + k * 64 * (ld + 1);
+    for (int c = 0; c < 64; ++c) A[c][t] = blk[t + (int64_t)c * ld];
+    __syncthreads();
+    for (int i = 0; i < 64; ++i) {
+        double v = 0.0;
+        if (i >= t) {
+            double acc = (i == t) ? 1.0 : 0.0;
+            for (int q = t; q < i; ++q) acc -= A[q][i] * X[t][q];
+            v = acc / A[i][i];
+        }
+        X[t][i] = v;
+    }
+    __syncthreads();
+    double* D = Dinv + k * 64 * 64;
+    for (int c = 0; c < 64; ++c) D[t + c * 64] = X[c][t];
+    const double p = A[t][t];
+    ldiag[k * 64 + t] = log(p);
+    const unsigned long long bad = __ballot(!(p > 0.0));
+    if (t == 0 && bad) atomicCAS(info, 0, (int)(k * 64 + __ffsll((long long)bad)));
+}
+
+void trtri_diag64_f64(hipStream_t s, const double* L, int64_t ld, int64_t T, double* Dinv, double* ldiag, int* info) {
+    hipLaunchKernelGGL(k_trtri_diag64, dim3((unsigned)T), dim3(64), 0, s, L, ld, Dinv, ldiag, info);
+}
+
 }  // namespace gpl

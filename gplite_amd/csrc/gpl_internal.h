@@ -131,6 +131,10 @@ void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, dou
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
+// gp_load: saved C_chol (in the n x n part of L, ld N) -> device factor and sv = 1/sqrt(V)
+void restore_factor_f64(hipStream_t s, double* L, int64_t n, int64_t N, const double* V, double* sv);
+// inverses and log pivots of the T diagonal 64x64 blocks of a lower-triangular factor
+void trtri_diag64_f64(hipStream_t s, const double* L, int64_t ld, int64_t T, double* Dinv, double* ldiag, int* info);
 // B (lower tiles) = I + s_i K_ij s_j for i,j < n; identity on padding
 void form_B_f64(hipStream_t s, const double* K, double* B, int64_t ld, int64_t n, int64_t N,
                 const double* sv);

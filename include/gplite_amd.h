@@ -176,6 +176,19 @@ enum {
 int gpl_model_vector(gpl_model* model, int32_t which, double* out /* n */);
 /* C_chol = diag(sqrt V) chol(B)^T, n x n column-major lower (for gp_save). */
 int gpl_model_chol(gpl_model* model, double* out);
+/* gp_load (R/saveload.R:37-41) of a method_full fit saved by gp_save
+ * (R/saveload.R:32-34): rebuilds a device model from the gp$fit fields
+ * (R/laplace.R:119, R/ep.R:140-145) without refitting.  terms/x/ncols as for
+ * gpl_fit (with the learned normalisation of the saved model); c_chol is the
+ * saved n x n lower C_chol (as gpl_model_chol returns it; the part above the
+ * diagonal is ignored); vectors[GPL_VEC_*] are the saved n-vectors (alpha and V
+ * required, others may be NULL: zeros).  The restored model is FP64 whatever
+ * precision it was fitted in, predicts, draws and (Gaussian) differentiates
+ * like the original.  GPL_ERR_ARG for V <= 0; GPL_ERR_NOT_PD (minor = index)
+ * for a non-positive diagonal of c_chol. */
+int gpl_model_restore(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n,
+                      int64_t ncols, const gpl_lik* lik, const gpl_opts* opts, const double* c_chol,
+                      const double* const* vectors, double log_evidence, int32_t iters, gpl_model** out);
 int gpl_model_info(gpl_model* model, int64_t* n, int32_t* iters, double* log_evidence);
 void gpl_model_free(gpl_model* model);
 
