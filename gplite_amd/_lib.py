@@ -105,6 +105,8 @@ def _sig(lib):
         "gpl_server_destroy": (None, [_vp]),
         "gpl_pred_ctx": (C.c_int, [_vp, _vp, _dp, i64, i32, _dp, _dp]),
         "gpl_draw": (C.c_int, [_vp, _dp, i64, C.c_double, _dp, i64, _dp]),
+        "gpl_draw_terms": (C.c_int, [_vp, C.POINTER(i32), i32, _dp, i64, C.c_double, _dp, i64, _dp]),
+        "gpl_pred_terms": (C.c_int, [_vp, C.POINTER(i32), i32, _dp, i64, i32, _dp, _dp]),
         "gpl_model_restore": (C.c_int, [_vp, C.POINTER(CfTerm), i32, _dp, i64, i64, C.POINTER(Lik), C.POINTER(Opts),
                                         _dp, C.POINTER(_dp), C.c_double, i32, C.POINTER(_vp)]),
         "gpl_model_bytes": (i64, [_vp]),

@@ -16,6 +16,7 @@ import copy
 import math
 import warnings
 import weakref
+from ctypes import byref as C_byref, c_double as C_double, c_int32 as C_int32
 
 import numpy as np
 
@@ -624,10 +625,34 @@ def _program(cfs, x):
     return arr, len(terms), np.asfortranarray(xp), free_mask
 
 
-def eval_cf(cfs, x1, x2, diag_only=False):
-    """K between two input sets on the device (R/cf.R:402-411)."""
+def _cfind_list(cfs, cfind):
+    """0-based indices of top-level covariance functions (R's cfind, 1-based there)."""
+    sel = [int(i) for i in np.atleast_1d(np.asarray(cfind, dtype=int))]
+    if not sel or min(sel) < 0 or max(sel) >= len(cfs):
+        raise IndexError(f"cfind must index the {len(cfs)} covariance functions (0-based)")
+    return sel
+
+
+def _cfind_terms(cfs, cfind):
+    """The device term indices (products flattened, as _flatten orders them) of cfind."""
+    groups, k = [], 0
+    for cf in cfs:
+        nt = len(cf.cfs) if isinstance(cf, cf_prod) else 1
+        groups.append(list(range(k, k + nt)))
+        k += nt
+    idx = [t for i in _cfind_list(cfs, cfind) for t in groups[i]]
+    if len(idx) > L.MAX_TERMS:
+        raise ValueError("at most 8 covariance terms")
+    return (C_int32 * len(idx))(*idx), len(idx)
+
+
+def eval_cf(cfs, x1, x2, diag_only=False, cfind=None):
+    """K between two input sets on the device (R/cf.R:402-411); cfind keeps the
+    listed covariance functions (0-based)."""
     if isinstance(cfs, Cf):
         cfs = [cfs]
+    if cfind is not None:
+        cfs = [cfs[i] for i in _cfind_list(cfs, cfind)]
     t1, nt, x1p, _ = _program(cfs, x1)
     _, _, x2p, _ = _program(cfs, x2)
     n1, n2 = x1p.shape[0], x2p.shape[0]
@@ -739,21 +764,25 @@ def gp_energy_grad(gp, include_prior=True):
 
 def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, jitter=None, quad_order=15,
             offset=None):
-    """R/gp_pred.R:88-122 for method_full: posterior if fitted, prior otherwise."""
-    if cfind is not None:
-        raise NotImplementedError("cfind is not supported by the device path")
+    """R/gp_pred.R:88-122 for method_full: posterior if fitted, prior otherwise.
+    cfind (0-based) keeps only those covariance functions in K_t and K_tt."""
     if quantiles is not None or transform:
         var = True
     xt = _as_matrix(xnew)
     m = xt.shape[0]
     if not is_fitted(gp):  # R/gp_pred.R:142-156
         mean = np.zeros(m)
-        v = eval_cf(gp.cfs, xt, xt, diag_only=True) if var else None
+        v = eval_cf(gp.cfs, xt, xt, diag_only=True, cfind=cfind) if var else None
     else:
         _, _, xtp, _ = _program(gp.cfs, xt)
         mean = np.empty(m)
         v = np.empty(m) if var else None
-        L.check(L.lib().gpl_pred(gp.fit.handle, L.dptr(xtp), m, int(bool(var)), L.dptr(mean), L.dptr(v)))
+        if cfind is None:
+            L.check(L.lib().gpl_pred(gp.fit.handle, L.dptr(xtp), m, int(bool(var)), L.dptr(mean), L.dptr(v)))
+        else:
+            sel, ns = _cfind_terms(gp.cfs, cfind)
+            L.check(L.lib().gpl_pred_terms(gp.fit.handle, sel, ns, L.dptr(xtp), m, int(bool(var)), L.dptr(mean),
+                                           L.dptr(v)))
     if offset is not None:
         mean = mean + offset
     out = {"mean": mean}
@@ -775,7 +804,7 @@ def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, ji
     return out
 
 
-def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, jitter=None, seed=None,
+def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, cfind=None, jitter=None, seed=None,
             trials=None, offset=None):
     """Posterior draws at xnew: R/gp_draw.R:42-124 for method_full, returned as
     a (len(xnew), draws) array.
@@ -795,13 +824,18 @@ def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, jit
     rng = np.random.default_rng(seed)
     z = np.asfortranarray(rng.standard_normal((m, draws)))
     if marginal:
-        pr = gp_pred(gp, xt, var=True)
+        pr = gp_pred(gp, xt, var=True, cfind=cfind)
         f = pr["mean"][:, None] + np.sqrt(pr["var"])[:, None] * z
     else:
         _, _, xtp, _ = _program(gp.cfs, xt)
         f = np.empty((m, draws), order="F")
-        L.check(L.lib().gpl_draw(gp.fit.handle, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp), draws,
-                                 f.ctypes.data_as(L._dp)))
+        if cfind is None:
+            L.check(L.lib().gpl_draw(gp.fit.handle, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp), draws,
+                                     f.ctypes.data_as(L._dp)))
+        else:
+            sel, ns = _cfind_terms(gp.cfs, cfind)
+            L.check(L.lib().gpl_draw_terms(gp.fit.handle, sel, ns, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp),
+                                           draws, f.ctypes.data_as(L._dp)))
     if offset is not None:
         f = f + np.asarray(offset, dtype=float)[:, None]
     if target:
@@ -1049,4 +1083,3 @@ def gp_optim(gp, x, y, tol=1e-4, tol_param=1e-1, maxiter=500, restarts=1, verbos
     return g
 
 
-from ctypes import byref as C_byref, c_double as C_double, c_int32 as C_int32  # noqa: E402

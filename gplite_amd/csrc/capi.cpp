@@ -1152,27 +1152,57 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
     return GPL_OK;
 }
 
+// The model's covariance program restricted to its terms idx[0..nsel), in that
+// order (cfind of R/gp_pred.R:88 and R/gp_draw.R:3, summed as eval_cf.list does,
+// R/cf.R:402-411).  The feature layout is unchanged, so the model's training
+// features and test features made with the full program serve the subset too.
+// The result is also copied to the context's device slot 9 (on stream s).
+static int sub_program(gpl_ctx* c, hipStream_t s, const gpl_model* m, const int32_t* idx, int32_t nsel,
+                       DevProg& P, const DevProg** dev) {
+    if (!idx || nsel < 1 || nsel > GPL_MAX_TERMS) return fail(GPL_ERR_ARG, "select between 1 and 8 terms");
+    P = m->prog;
+    for (int k = 0; k < nsel; ++k) {
+        if (idx[k] < 0 || idx[k] >= m->prog.nterms) return fail(GPL_ERR_ARG, "term index out of range");
+        P.t[k] = m->prog.t[idx[k]];
+    }
+    P.t[0].op = GPL_OP_SUM;
+    P.nterms = nsel;
+    DevProg* d = (DevProg*)c->buf(9, sizeof(DevProg));
+    if (!d) return fail(GPL_ERR_OOM, "device allocation failed");
+    HIP_TRY(hipMemcpyAsync(d, &P, sizeof(DevProg), hipMemcpyHostToDevice, s));
+    *dev = d;
+    return GPL_OK;
+}
+
 // Prediction on the streams of context c (which may differ from the context
 // that fitted the model: the model's device buffers are read-only here).
 int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean,
-                 double* var) {
+                 double* var, const int32_t* terms, int32_t nsel) {
     if (!c || !m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
     if (mt <= 0) return GPL_OK;
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = m->n, N = m->N, M = pad64(mt);
-    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const DevProg* pf = m->b[B_PROG].as<DevProg>();  // features: always the full program
+    const DevProg* pd = pf;
+    const DevProg* hp = &m->prog;
+    DevProg sub;
+    if (terms) {
+        int r = sub_program(c, s, m, terms, nsel, sub, &pd);
+        if (r) return r;
+        hp = &sub;
+    }
     double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
     double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
     double* KtT = (double*)c->buf(2, (size_t)N * M * 8);
     double* vbuf = (double*)c->buf(3, (size_t)M * 3 * 8);
     if (!xtd || !Ft || !KtT || !vbuf) return fail(GPL_ERR_OOM, "device allocation failed");
     if (mt * m->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
-    launch_features(s, xtd, mt, mt, (int)m->ncols, pd, m->prog, Ft, M);
+    launch_features(s, xtd, mt, mt, (int)m->ncols, pf, m->prog, Ft, M);
     {
         Prof p(c, "pred_cross", 0, 8.0 * N * M);
-        launch_build_cross(s, pd, m->prog, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
+        launch_build_cross(s, pd, *hp, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
     }
     gemv_t_f64(s, KtT, N, N, mt, m->d(B_ALPHA), vbuf);
     if (want_var) {
@@ -1204,7 +1234,7 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
             }
         }
         colsumsq_f64(s, KtT, N, N, mt, vbuf + M, 0);
-        launch_build_diag(s, pd, m->prog, Ft, M, Ft, M, mt, vbuf + 2 * M);
+        launch_build_diag(s, pd, *hp, Ft, M, Ft, M, mt, vbuf + 2 * M);
         vec_axpby(s, mt, 1.0, vbuf + 2 * M, -1.0, vbuf + M, vbuf + M);
         HIP_TRY(hipMemcpyAsync(var, vbuf + M, mt * 8, hipMemcpyDeviceToHost, s));
     }
@@ -1218,13 +1248,21 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
 // covariance cov = K_tt - aux^T aux, aux = C^-1 K_t^T (R/gp_pred.R:202-232).
 // z (mt x draws, column-major) holds the caller's standard normal draws.
 static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, double jitter, const double* z,
-                   int64_t draws, double* out) {
+                   int64_t draws, double* out, const int32_t* terms, int32_t nsel) {
     if (!c || !m || !xt || !z || !out || mt < 1 || draws < 1) return fail(GPL_ERR_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = m->n, N = m->N, M = pad64(mt);
-    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const DevProg* pf = m->b[B_PROG].as<DevProg>();  // features: always the full program
+    const DevProg* pd = pf;
+    const DevProg* hp = &m->prog;
+    DevProg sub;
+    if (terms) {
+        int r = sub_program(c, s, m, terms, nsel, sub, &pd);
+        if (r) return r;
+        hp = &sub;
+    }
     double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
     double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
     double* KtT = (double*)c->buf(2, (size_t)N * M * 8);
@@ -1240,8 +1278,8 @@ static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, doubl
     double* ld = Dv + M * 64;
     int* info = (int*)(ld + M);
     if (mt * m->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
-    launch_features(s, xtd, mt, mt, (int)m->ncols, pd, m->prog, Ft, M);
-    launch_build_cross(s, pd, m->prog, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
+    launch_features(s, xtd, mt, mt, (int)m->ncols, pf, m->prog, Ft, M);
+    launch_build_cross(s, pd, *hp, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
     gemv_t_f64(s, KtT, N, N, mt, m->d(B_ALPHA), vbuf);
     scale_rows_f64(s, KtT, N, n, M, m->d(B_SV));
     const int64_t T = N / 64;
@@ -1261,7 +1299,7 @@ static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, doubl
         A2 = KtT;
     }
     // cov = K_tt + jitter I - aux^T aux over lower tiles (padding: identity), then its factor
-    launch_build_sym_f64(s, pd, m->prog, Ft, mt, M, jitter, 0, 1.0, nullptr, Cv, M);
+    launch_build_sym_f64(s, pd, *hp, Ft, mt, M, jitter, 0, 1.0, nullptr, Cv, M);
     {
         Prof p(c, "draw_cov", (double)M * M * N, 8.0 * N * M);
         gemm2_f64(s, 1, 0, M, M, N, -1.0, A2, N, A2, N, 1.0, Cv, M, 1, 0);
@@ -1294,7 +1332,19 @@ static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, doubl
 extern "C" int gpl_draw(gpl_model* m, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
                         double* out) {
     if (!m) return fail(GPL_ERR_ARG, "NULL model");
-    return draw_on(m->ctx, m, xt, mt, jitter, z, draws, out);
+    return draw_on(m->ctx, m, xt, mt, jitter, z, draws, out, nullptr, 0);
+}
+
+extern "C" int gpl_draw_terms(gpl_model* m, const int32_t* terms, int32_t nsel, const double* xt, int64_t mt,
+                              double jitter, const double* z, int64_t draws, double* out) {
+    if (!m || !terms) return fail(GPL_ERR_ARG, "NULL argument");
+    return draw_on(m->ctx, m, xt, mt, jitter, z, draws, out, terms, nsel);
+}
+
+extern "C" int gpl_pred_terms(gpl_model* m, const int32_t* terms, int32_t nsel, const double* xt, int64_t mt,
+                              int32_t want_var, double* mean, double* var) {
+    if (!m || !terms) return fail(GPL_ERR_ARG, "NULL argument");
+    return pred_on(m->ctx, m, xt, mt, want_var, mean, var, terms, nsel);
 }
 
 extern "C" int gpl_pred(gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var) {

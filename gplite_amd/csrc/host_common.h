@@ -24,7 +24,9 @@ void set_minor(int64_t minor);
     } while (0)
 
 int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P);
-int pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var);
+// terms/nsel (optional): predict with the model's terms terms[0..nsel) only (cfind)
+int pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var,
+            const int32_t* terms = nullptr, int32_t nsel = 0);
 
 }  // namespace gpl
 

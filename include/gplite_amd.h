@@ -264,6 +264,17 @@ int gpl_pred_ctx(gpl_ctx* ctx, gpl_model* model, const double* xt, int64_t m, in
  * GPL_ERR_NOT_PD when cov + jitter I has a non-positive pivot. */
 int gpl_draw(gpl_model* model, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
              double* out);
+/* cfind (R/gp_pred.R:88-122 and R/gp_draw.R:3-35 with cfind; the covariance is
+ * summed over the selected functions as in eval_cf.list, R/cf.R:402-411):
+ * gpl_pred / gpl_draw with only the model's covariance terms terms[0..nsel)
+ * (0-based indices into the gpl_cf_term array the model was fitted with, in
+ * the order given; the first selected term starts a new sum).  Test inputs xt
+ * are prepared as for the full model.  The posterior factor and alpha are the
+ * fitted ones.  GPL_ERR_ARG for an index out of range or nsel outside [1, 8]. */
+int gpl_pred_terms(gpl_model* model, const int32_t* terms, int32_t nsel, const double* xt, int64_t m,
+                   int32_t want_var, double* mean, double* var);
+int gpl_draw_terms(gpl_model* model, const int32_t* terms, int32_t nsel, const double* xt, int64_t mt,
+                   double jitter, const double* z, int64_t draws, double* out);
 /* Device bytes held by a fitted model. */
 int64_t gpl_model_bytes(gpl_model* model);
 

@@ -795,43 +795,45 @@ def gp_energy_grad(fit, include_prior=True):
     return g
 
 
-def gp_pred(fit, xt, var=False, jitter=None, prior=False):
+def gp_pred(fit, xt, var=False, jitter=None, prior=False, cfind=None):
     """Latent predictive mean / variance: R/gp_pred.R:142-156 (prior),
-    :202-232 (posterior; forms K_tt and aux^T aux as the reference does)."""
+    :202-232 (posterior; forms K_tt and aux^T aux as the reference does).
+    cfind: 0-based indices of the covariance functions kept in K_t and K_tt
+    (R/gp_pred.R:206-207); alpha and C_chol stay those of the full fit."""
     cfs = fit["cfs"]
     xt = np.atleast_2d(np.asarray(xt, float))
     if prior:
         mean = np.zeros(xt.shape[0])
         if var:
-            return mean, eval_cf(cfs, xt, xt, diag_only=True)
+            return mean, eval_cf(cfs, xt, xt, diag_only=True, cfind=cfind)
         return mean, None
-    Kt = eval_cf(cfs, xt, fit["x"])
+    Kt = eval_cf(cfs, xt, fit["x"], cfind=cfind)
     mean = Kt @ fit["alpha"]
     if not var:
         return mean, None
     aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
-    v = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux * aux, axis=0)
+    v = eval_cf(cfs, xt, xt, diag_only=True, cfind=cfind) - np.sum(aux * aux, axis=0)
     return mean, v
 
 
-def gp_pred_cov(fit, xt, jitter=None):
+def gp_pred_cov(fit, xt, jitter=None, cfind=None):
     """Posterior mean and full covariance + jitter I: R/gp_pred.R:202-227
     (cov = TRUE branch, K_tt - aux^T aux with aux = C_chol^-1 K_t^T)."""
     cfs = fit["cfs"]
     xt = np.atleast_2d(np.asarray(xt, float))
     jitter = JITTER if jitter is None else jitter
-    Kt = eval_cf(cfs, xt, fit["x"])
+    Kt = eval_cf(cfs, xt, fit["x"], cfind=cfind)
     mean = Kt @ fit["alpha"]
     aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
-    cov = eval_cf(cfs, xt, xt) - aux.T @ aux + jitter * np.eye(xt.shape[0])
+    cov = eval_cf(cfs, xt, xt, cfind=cfind) - aux.T @ aux + jitter * np.eye(xt.shape[0])
     return mean, cov
 
 
-def gp_draw(fit, xt, z, jitter=None):
+def gp_draw(fit, xt, z, jitter=None, cfind=None):
     """Joint posterior draws of the latent function: R/gp_draw.R:101-118 with
     mvnrnd (R/gp_draw.R:182-200): mean + t(chol(cov)) z for the given standard
     normal matrix z of shape (len(xt), draws)."""
-    mean, cov = gp_pred_cov(fit, xt, jitter)
+    mean, cov = gp_pred_cov(fit, xt, jitter, cfind)
     Lc = sla.cholesky(cov, lower=True)
     return mean[:, None] + Lc @ np.asarray(z, float).reshape(len(mean), -1)
 

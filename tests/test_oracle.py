@@ -369,6 +369,40 @@ def test_oracle_draw_gaussian_closed_form():
     np.testing.assert_allclose(np.diag(cov) - jit, v, atol=1e-12)
 
 
+def test_oracle_pred_cfind_closed_form():
+    """cfind (R/gp_pred.R:206-207): for an additive Gaussian model the posterior
+    of one component f_k at xt is mean K_k,t K_y^-1 y and variance
+    diag(K_k,tt - K_k,t K_y^-1 K_k,t^T) (a direct solve against the full K_y);
+    the component means sum to the full mean; cfind over every function equals
+    no cfind."""
+    r = np.random.default_rng(12)
+    n, m = 80, 25
+    x = r.normal(size=(n, 2))
+    y = np.sin(2 * x[:, 0]) + 0.5 * x[:, 1] + 0.2 * r.normal(size=n)
+    xt = r.normal(size=(m, 2))
+    cfs = [{"kind": "sexp", "lscale": 0.7, "magn": 1.0, "vars": [0]},
+           {"kind": "matern32", "lscale": 1.5, "magn": 0.8, "vars": [1]},
+           {"kind": "const", "magn": 0.4}]
+    lik = {"kind": "gaussian", "sigma": 0.2}
+    fit = O.gp_fit(cfs, lik, x, y)
+    c2 = fit["cfs"]
+    Ky = O.eval_cf(c2, x, x) + (0.2 ** 2 + 1e-6) * np.eye(n)
+    means = []
+    for k in range(3):
+        Kt = O.eval_cf(c2, xt, x, cfind=[k])
+        mu = Kt @ np.linalg.solve(Ky, y)
+        v = np.diag(O.eval_cf(c2, xt, xt, cfind=[k]) - Kt @ np.linalg.solve(Ky, Kt.T))
+        mo, vo = O.gp_pred(fit, xt, var=True, cfind=[k])
+        np.testing.assert_allclose(mo, mu, atol=1e-10)
+        np.testing.assert_allclose(vo, v, atol=1e-10)
+        means.append(mo)
+    mfull, vfull = O.gp_pred(fit, xt, var=True)
+    np.testing.assert_allclose(sum(means), mfull, atol=1e-10)
+    ma, va = O.gp_pred(fit, xt, var=True, cfind=[0, 1, 2])
+    np.testing.assert_allclose(ma, mfull, atol=1e-12)
+    np.testing.assert_allclose(va, vfull, atol=1e-12)
+
+
 def test_oracle_loo_gaussian_closed_form():
     """Gaussian likelihood.
     * The LOO latent moments of R/gp_loo.R:120-130 must give the closed form
