@@ -384,6 +384,23 @@ class method_full:
     name = "full"
 
 
+class method_fitc:
+    """FITC with inducing inputs (R/method.R:113-124).  Without `inducing`, gp_fit
+    places num_inducing points by k-means on the scaled inputs, optionally per bin
+    of one input (R/gp_fit.R:195-240)."""
+    name = "fitc"
+
+    def __init__(self, inducing=None, num_inducing=100, bin_along=None, bin_count=10, seed=12345):
+        if inducing is not None:
+            inducing = _as_matrix(inducing)
+            num_inducing = inducing.shape[0]
+        self.inducing = inducing
+        self.num_inducing = int(num_inducing)
+        self.bin_along = bin_along
+        self.bin_count = min(int(bin_count), self.num_inducing)
+        self.seed = seed
+
+
 # ---------------------------------------------------------------------------
 # the GP object (R/gp.R)
 # ---------------------------------------------------------------------------
@@ -437,6 +454,38 @@ def _free_model(ptr):
         L._lib.gpl_model_free(ptr)
 
 
+def _free_fitc(ptr):
+    if ptr and L._lib is not None:
+        L._lib.gpl_fitc_free(ptr)
+
+
+class _FitcFit:
+    """Device-resident FITC fit (gp$fit of R/laplace.R:177-180: fmean, z, V,
+    alpha, the FITC diagonal and the log evidence)."""
+
+    def __init__(self, handle, n, m, log_evidence, iters):
+        self.handle = handle
+        self.n, self.m = n, m
+        self.log_evidence = log_evidence
+        self.iters = iters
+        self.type = "analytic"
+        self.approx = L.APPROX_LAPLACE
+        self._fin = weakref.finalize(self, _free_fitc, handle.value if handle is not None else None)
+
+    def vector(self, which):
+        if self.handle is None:
+            raise RuntimeError("the FITC fit failed (log_evidence = -Inf); there is nothing to read")
+        out = np.empty(self.n)
+        L.check(L.lib().gpl_fitc_vector(self.handle, which, L.dptr(out)))
+        return out
+
+    fmean = property(lambda self: self.vector(L.VEC_FMEAN))
+    alpha = property(lambda self: self.vector(L.VEC_ALPHA))
+    z = property(lambda self: self.vector(L.VEC_Z))
+    V = property(lambda self: self.vector(L.VEC_V))
+    diag = property(lambda self: self.vector(7))
+
+
 class GP:
     def __init__(self, cfs, lik, method, approx):
         self.cfs = cfs
@@ -463,8 +512,8 @@ def gp_init(cfs=None, lik=None, method=None, approx=None):
     if isinstance(lik, lik_gaussian) and isinstance(approx, approx_ep):
         raise ValueError("approx_ep disabled for lik_gaussian: use approx_laplace instead, because it gives exact "
                          "results")
-    if not isinstance(method, method_full):
-        raise NotImplementedError("only method_full (the dense core) is provided")
+    if not isinstance(method, (method_full, method_fitc)):
+        raise NotImplementedError("method_full and method_fitc are provided")
     if isinstance(cfs, Cf):
         cfs = [cfs]
     return GP(list(cfs), lik, method, approx)
@@ -679,6 +728,111 @@ def _lik_struct(lik):
     return s
 
 
+def _scale_cols(x):
+    """R's scale(): centre by the column means, divide by the column sd (n - 1)."""
+    mu = x.mean(axis=0)
+    sd = x.std(axis=0, ddof=1) if x.shape[0] > 1 else np.ones(x.shape[1])
+    sd = np.where(sd > 0, sd, 1.0)
+    return (x - mu) / sd, mu, sd
+
+
+def _kmeans_centres(xs, k, rng, iters=100):
+    """k-means centres of the rows of xs: k-means++ seeding, then Lloyd
+    iterations until the assignment is stable.  The reference calls
+    stats::kmeans (Hartigan-Wong) with starts from R's generator, so the
+    centres are not the reference's; FITC given the same inducing inputs is."""
+    n = xs.shape[0]
+    if k > n:
+        raise ValueError("more cluster centers than distinct data points")
+    sq = np.sum(xs * xs, axis=1)
+    first = int(rng.integers(n))
+    C = [xs[first]]
+    d2 = np.sum((xs - xs[first]) ** 2, axis=1)
+    for _ in range(1, k):
+        tot = d2.sum()
+        i = int(rng.choice(n, p=d2 / tot)) if tot > 0 else int(rng.integers(n))
+        C.append(xs[i])
+        d2 = np.minimum(d2, np.sum((xs - xs[i]) ** 2, axis=1))
+    C = np.array(C)
+    lab = None
+    for _ in range(iters):
+        new = np.empty(n, dtype=np.int64)
+        for a in range(0, n, 8192):  # distances in row chunks
+            b = min(n, a + 8192)
+            d = sq[a:b, None] - 2.0 * xs[a:b] @ C.T + np.sum(C * C, axis=1)[None, :]
+            new[a:b] = np.argmin(d, axis=1)
+        if lab is not None and np.array_equal(new, lab):
+            break
+        lab = new
+        cnt = np.bincount(lab, minlength=k)
+        for j in range(xs.shape[1]):
+            s_ = np.bincount(lab, weights=xs[:, j], minlength=k)
+            C[cnt > 0, j] = s_[cnt > 0] / cnt[cnt > 0]
+    return C
+
+
+def get_inducing(gp, x):
+    """Inducing inputs of a method_fitc model (R/gp_fit.R:195-240): the given
+    ones, or k-means centres of the scaled inputs, per bin of input column
+    bin_along (0-based) when it is set."""
+    meth = gp.method
+    if meth.inducing is not None:
+        return _as_matrix(meth.inducing)
+    x = _as_matrix(x)
+    rng = np.random.default_rng(meth.seed)
+    k = meth.num_inducing
+    if meth.bin_along is None:
+        xs, mu, sd = _scale_cols(x)
+        return _kmeans_centres(xs, k, rng) * sd + mu
+    v = x[:, meth.bin_along]
+    eps = 2 * np.finfo(float).eps
+    cuts = np.linspace(v.min() - eps, v.max() + eps, meth.bin_count + 1)  # R/gp_fit.R:243-259
+    bins = [x[(v > cuts[i]) & (v < cuts[i + 1])] for i in range(meth.bin_count)]
+    order = np.argsort([-len(b) for b in bins], kind="stable")
+    bins = [bins[i] for i in order]
+    per = np.full(meth.bin_count, k // meth.bin_count)
+    per[: k - per.sum()] += 1
+    out = []
+    for b, kb in zip(bins, per):
+        bs, mu, sd = _scale_cols(b)
+        out.append(_kmeans_centres(bs, int(kb), rng) * sd + mu)
+    return np.vstack(out)
+
+
+def _fit_fitc(g, x, y, terms, nt, xp, tr, off, lk, o):
+    """fit_laplace.method_fitc (R/gp_fit.R:101-123) through gpl_fitc_fit."""
+    if isinstance(g.approx, approx_ep):
+        raise NotImplementedError("FITC with approx_ep (R/ep.R:148-280) is not on the device path")
+    if g.precision == "mixed":
+        raise NotImplementedError("FITC runs in FP64")
+    g.method = copy.copy(g.method)
+    z = get_inducing(g, x)
+    g.method.inducing = z
+    _, _, zp, _ = _program(g.cfs, z)
+    n, m = y.shape[0], z.shape[0]
+    handle = L._vp()
+    logz = C_double()
+    rc = L.lib().gpl_fitc_fit(L.ctx(), terms, nt, L.dptr(xp), n, xp.shape[1], L.dptr(zp), m, L.dptr(y), L.dptr(tr),
+                              L.dptr(off), C_byref(lk), C_byref(o), C_byref(handle), C_byref(logz))
+    if rc in (L.GPL_ERR_NOT_PD, L.GPL_ERR_NAN):
+        # the reference catches the error, prints it and keeps log_evidence = -Inf (R/gp_fit.R:113-121)
+        print(L.lib().gpl_last_error().decode())
+        fit = _FitcFit(None, n, m, -math.inf, 0)
+    else:
+        if rc not in (L.GPL_OK, L.GPL_ERR_NOCONV):
+            L.check(rc)
+        iters = C_int32()
+        L.lib().gpl_fitc_info(handle, None, None, C_byref(iters), None)
+        fit = _FitcFit(handle, n, m, float(logz.value), int(iters.value))
+        if rc == L.GPL_ERR_NOCONV:
+            warnings.warn(L.lib().gpl_last_error().decode())
+    g.fit = fit
+    g.x = x
+    g.fitted = True
+    g._trials, g._offset = tr, off
+    return g
+
+
 def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want_grad=False):
     """R/gp_fit.R:54-60: learn input scales, build K + jitter I on the device and
     run the Laplace Newton loop or EP; returns a new fitted GP.  want_grad forms
@@ -714,6 +868,8 @@ def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want
     o.precision = L.PREC_MIXED if prec == "mixed" else L.PREC_FP64
     g.precision = prec
     lk = _lik_struct(g.lik)
+    if isinstance(g.method, method_fitc):
+        return _fit_fitc(g, x, y, terms, nt, xp, tr, off, lk, o)
     handle = L._vp()
     logz = C_double()
     n = y.shape[0]
@@ -751,6 +907,8 @@ def gp_energy_grad(gp, include_prior=True):
     """d gp_energy / d get_param(gp) (Gaussian likelihood), computed on the device."""
     if not is_fitted(gp):
         raise RuntimeError("The GP must be fitted. Call gp_fit first.")
+    if isinstance(gp.method, method_fitc):
+        raise NotImplementedError("the energy gradient is provided for method_full")
     terms, nt, _, mask = _program(gp.cfs, gp.x)
     ng = L.lib().gpl_grad_len(terms, nt, _LIKMAP[gp.lik.kind])
     full = np.empty(ng)
@@ -770,9 +928,21 @@ def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, ji
         var = True
     xt = _as_matrix(xnew)
     m = xt.shape[0]
-    if not is_fitted(gp):  # R/gp_pred.R:142-156
+    fitc = isinstance(gp.method, method_fitc)
+    if fitc and cfind is not None:
+        raise NotImplementedError("cfind with method_fitc is not on the device path")
+    if not is_fitted(gp):  # R/gp_pred.R:142-184
+        if fitc and gp.method.inducing is None:
+            raise ValueError("Inducing points not set yet.")
         mean = np.zeros(m)
         v = eval_cf(gp.cfs, xt, xt, diag_only=True, cfind=cfind) if var else None
+    elif fitc:  # R/gp_pred.R:234-290
+        if gp.fit.handle is None:
+            raise RuntimeError("the FITC fit failed (log_evidence = -Inf)")
+        _, _, xtp, _ = _program(gp.cfs, xt)
+        mean = np.empty(m)
+        v = np.empty(m) if var else None
+        L.check(L.lib().gpl_fitc_pred(gp.fit.handle, L.dptr(xtp), m, int(bool(var)), L.dptr(mean), L.dptr(v)))
     else:
         _, _, xtp, _ = _program(gp.cfs, xt)
         mean = np.empty(m)
@@ -818,6 +988,8 @@ def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, cfi
     """
     if not is_fitted(gp):
         raise NotImplementedError("draws from the prior are not on the device path; fit the model first")
+    if isinstance(gp.method, method_fitc):
+        raise NotImplementedError("gp_draw for method_fitc is not on the device path")
     xt = _as_matrix(xnew)
     m = xt.shape[0]
     jitter = 1e-6 if jitter is None else float(jitter)
@@ -862,7 +1034,7 @@ def _saveable():
     return {c.__name__: c for c in (cf_const, cf_lin, cf_sexp, cf_matern32, cf_matern52, cf_nn, cf_periodic, cf_prod,
                                     lik_gaussian, lik_bernoulli, lik_binomial, lik_poisson, prior_fixed,
                                     prior_logunif, prior_lognormal, prior_half_t, approx_laplace, approx_ep,
-                                    method_full)}
+                                    method_full, method_fitc)}
 
 
 def _encode(obj, classes):
@@ -914,6 +1086,8 @@ def gp_save(gp, filename):
             "method": _encode(gp.method, classes), "approx": _encode(gp.approx, classes),
             "precision": gp.precision, "fitted": bool(gp.fit is not None and gp.fitted)}
     arrays = {}
+    if head["fitted"] and isinstance(gp.method, method_fitc):
+        raise NotImplementedError("saving a fitted method_fitc model is not provided")
     if head["fitted"]:
         f = gp.fit
         head.update(n=int(f.n), log_evidence=float(f.log_evidence), iters=int(f.iters))
@@ -1006,6 +1180,8 @@ def gp_loo(gp, x, y, quadrature=True, quad_order=11, draws=4000, seed=None, tria
     Monte Carlo with `draws` samples."""
     if not is_fitted(gp):
         raise ValueError("The provied GP model is not fitted. Please call gp_fit (or gp_optim) first.")
+    if isinstance(gp.method, method_fitc):
+        raise NotImplementedError("gp_loo for method_fitc is not on the device path")
     y = np.asarray(y, dtype=float).ravel()
     n = len(y)
     if isinstance(gp.approx, approx_ep):

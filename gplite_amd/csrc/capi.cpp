@@ -285,7 +285,7 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
 // Two-level blocked right-looking Cholesky with one-panel look-ahead: while the
 // main stream applies the rank-NB update to the trailing matrix, the side
 // stream factors the next panel (whose columns are updated first).
-static void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
     hipStream_t s = c->stream, s2 = c->side;
     potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
@@ -332,7 +332,7 @@ static void trtri_panel(gpl_ctx* c, hipStream_t st, const double* L, const doubl
 // X = L^-1 by blocked forward substitution against the identity, two levels with
 // look-ahead: the next row panel is updated first and solved on the side stream
 // while the main stream updates the remaining rows.
-static void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
+void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
     hipStream_t s = c->stream, s2 = c->side;
     set_identity_f64(s, X, N, N);
     trtri_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);

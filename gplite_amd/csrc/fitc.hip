@@ -1,0 +1,101 @@
+// Elementwise and column-reduction kernels of the FITC path (fitc_host.cpp): the
+// per-site combination S = D + V of the inversion lemma, the row-scaled copy
+// of K_xz, the alpha assembly, the FITC diagonal, and column dot products for
+// the predictive variance.  Everything else in FITC runs on the dense kernels.
+#include "gpl_internal.h"
+
+#include <cmath>
+
+namespace gpl {
+
+// i < n: S = D + V, rs = 1/sqrt(S), zs = z / S; padding (n <= i < N): zeros
+__global__ void k_fitc_site(int64_t n, int64_t N, const double* __restrict__ D, const double* __restrict__ V,
+                            const double* __restrict__ z, double* __restrict__ S, double* __restrict__ rs,
+                            double* __restrict__ zs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (i < n) {
+        const double s = D[i] + V[i];
+        S[i] = s;
+        rs[i] = 1.0 / sqrt(s);
+        zs[i] = z[i] / s;
+    } else {
+        S[i] = 0.0;
+        rs[i] = 0.0;
+        zs[i] = 0.0;
+    }
+}
+
+void fitc_site(hipStream_t s, int64_t n, int64_t N, const double* D, const double* V, const double* z, double* S,
+               double* rs, double* zs) {
+    hipLaunchKernelGGL(k_fitc_site, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, n, N, D, V, z, S, rs, zs);
+}
+
+// W[i + j*ld] = r[i] * U[i + j*ld] over nrow x ncol (one workgroup per column)
+__global__ __launch_bounds__(256) void k_scale_rows_copy(const double* __restrict__ U, int64_t ld, int64_t nrow,
+                                                         const double* __restrict__ r, double* __restrict__ W) {
+    const int64_t j = blockIdx.x;
+    for (int64_t i = threadIdx.x; i < nrow; i += 256) W[i + j * ld] = r[i] * U[i + j * ld];
+}
+
+void scale_rows_copy_f64(hipStream_t s, const double* U, int64_t ld, int64_t nrow, int64_t ncol, const double* r,
+                         double* W) {
+    hipLaunchKernelGGL(k_scale_rows_copy, dim3((unsigned)ncol), dim3(256), 0, s, U, ld, nrow, r, W);
+}
+
+// i < n: alpha = zs - u / S (R/linear_algebra.R inv_lemma_solve, vector rhs);
+// padding: 0
+__global__ void k_fitc_alpha(int64_t n, int64_t N, const double* __restrict__ zs, const double* __restrict__ u,
+                             const double* __restrict__ S, double* __restrict__ alpha) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    alpha[i] = (i < n) ? zs[i] - u[i] / S[i] : 0.0;
+}
+
+void fitc_alpha(hipStream_t s, int64_t n, int64_t N, const double* zs, const double* u, const double* S,
+                double* alpha) {
+    hipLaunchKernelGGL(k_fitc_alpha, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, n, N, zs, u, S, alpha);
+}
+
+// out[j] = sum_i A[i + j*ld] B[i + j*ld], i < nrow (one wavefront per column, fixed order)
+__global__ __launch_bounds__(64) void k_coldot(const double* __restrict__ A, const double* __restrict__ B, int64_t ld,
+                                               int64_t nrow, double* __restrict__ out) {
+    const int64_t j = blockIdx.x;
+    const int l = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t i = l; i < nrow; i += 64) acc += A[i + j * ld] * B[i + j * ld];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (l == 0) out[j] = acc;
+}
+
+void coldot_f64(hipStream_t s, const double* A, const double* B, int64_t ld, int64_t nrow, int64_t ncol, double* out) {
+    if (ncol <= 0) return;
+    hipLaunchKernelGGL(k_coldot, dim3((unsigned)ncol), dim3(64), 0, s, A, B, ld, nrow, out);
+}
+
+// FITC diagonal (R/gp_fit.R:107-108): D = (k(x_i, x_i) + jitter) - cs_i for
+// i < n, 0 on padding
+__global__ void k_fitc_diag(int64_t n, int64_t N, const double* __restrict__ kd, const double* __restrict__ cs,
+                            double jitter, double* __restrict__ D) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    D[i] = (i < n) ? (kd[i] + jitter) - cs[i] : 0.0;
+}
+
+void fitc_diag(hipStream_t s, int64_t n, int64_t N, const double* kd, const double* cs, double jitter, double* D) {
+    hipLaunchKernelGGL(k_fitc_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, n, N, kd, cs, jitter, D);
+}
+
+// out[i] = log a[i], i < n
+__global__ void k_vec_log(int64_t n, const double* __restrict__ a, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = log(a[i]);
+}
+
+void vec_log(hipStream_t s, int64_t n, const double* a, double* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_vec_log, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, out);
+}
+
+}  // namespace gpl

@@ -168,6 +168,17 @@ void vec_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, co
 void vec_mul(hipStream_t s, int64_t n, const double* a, const double* b, double* out);
 void vec_recip(hipStream_t s, int64_t n, const double* a, double* out);
 
+// FITC (fitc.hip)
+void fitc_site(hipStream_t s, int64_t n, int64_t N, const double* D, const double* V, const double* z, double* S,
+               double* rs, double* zs);
+void scale_rows_copy_f64(hipStream_t s, const double* U, int64_t ld, int64_t nrow, int64_t ncol, const double* r,
+                         double* W);
+void fitc_alpha(hipStream_t s, int64_t n, int64_t N, const double* zs, const double* u, const double* S,
+                double* alpha);
+void coldot_f64(hipStream_t s, const double* A, const double* B, int64_t ld, int64_t nrow, int64_t ncol, double* out);
+void fitc_diag(hipStream_t s, int64_t n, int64_t N, const double* kd, const double* cs, double jitter, double* D);
+void vec_log(hipStream_t s, int64_t n, const double* a, double* out);
+
 // ---- batched forms (one problem per grid row / workgroup, stride per problem) ----
 void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,
                              const DevProg* progs, double* F, int64_t N, int64_t sF, int batch);

@@ -24,6 +24,11 @@ void set_minor(int64_t minor);
     } while (0)
 
 int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P);
+// Blocked FP64 Cholesky of the N x N lower tiles of L in place (Dinv: the 64x64
+// diagonal-block inverses, ldiag: log L_ii, info: first failing row + 1) and the
+// triangular inverse X = L^-1 (all tiles written)
+void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info);
+void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X);
 // terms/nsel (optional): predict with the model's terms terms[0..nsel) only (cfind)
 int pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var,
             const int32_t* terms = nullptr, int32_t nsel = 0);

@@ -192,6 +192,28 @@ int gpl_model_restore(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, co
 int gpl_model_info(gpl_model* model, int64_t* n, int32_t* iters, double* log_evidence);
 void gpl_model_free(gpl_model* model);
 
+/* ---- FITC (method_fitc) ------------------------------------------------- */
+/* Fully independent training conditional with m inducing inputs z:
+ * prior covariance Q + diag(D), Q = K_xz Kz^-1 K_zx, Kz = K(z, z) + jitter I,
+ * D = diag K(x, x) + jitter - diag Q (R/gp_fit.R:101-123).  Laplace Newton loop
+ * for log-concave likelihoods on the inversion lemma (R/laplace.R:122-180,
+ * 300-322, R/linear_algebra.R:3-172), one step for the Gaussian likelihood,
+ * stopping when the unnormalised log posterior changes by less than opts->tol.
+ * FP64 only; approx_ep returns GPL_ERR_UNSUPPORTED.  z: m x ncols inducing
+ * inputs prepared like x (column-major).  Status codes as gpl_fit. */
+typedef struct gpl_fitc gpl_fitc;
+int gpl_fitc_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n, int64_t ncols,
+                 const double* z, int64_t m, const double* y, const double* trials, const double* offset,
+                 const gpl_lik* lik, const gpl_opts* opts, gpl_fitc** out, double* log_evidence);
+/* gp_pred_post.method_fitc (R/gp_pred.R:234-290, train = FALSE): mean
+ * K_tz Kz^-1 K_zx alpha and variance prior_var - diag(A_t^T M A_t), with
+ * A_t = Lz^-1 K_zt and M = A C^-1 A^T formed once after the fit. */
+int gpl_fitc_pred(gpl_fitc* model, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var);
+/* fit vectors fmean, alpha, z, V (GPL_VEC_*), and the FITC diagonal D (which = 7) */
+int gpl_fitc_vector(gpl_fitc* model, int32_t which, double* out);
+int gpl_fitc_info(gpl_fitc* model, int64_t* n, int64_t* m, int32_t* iters, double* log_evidence);
+void gpl_fitc_free(gpl_fitc* model);
+
 /* ---- batched form: many independent Gaussian-likelihood GPs ------------- */
 /* Energy and gradient of `nprob` independent GPs of uneven n in one device
  * pass (new: the reference fits one GP per R call).  All problems share the

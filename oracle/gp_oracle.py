@@ -40,6 +40,7 @@ __all__ = [
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
     "gp_energy_grad", "gp_pred", "gp_pred_cov", "gp_draw", "gp_loo", "gaussian_logz_closed_form",
+    "inv_lemma_get", "inv_lemma_solve", "fitc_fit", "fitc_pred",
 ]
 
 JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
@@ -876,6 +877,104 @@ def gp_loo(fit, lik, x, y, quadrature=True, quad_order=11, draws=4000, seed=0, t
         loos = _logsumexp_rows(ll) - math.log(draws)
     return {"loo": float(np.sum(loos)), "sd": float(n * np.std(loos, ddof=1) / math.sqrt(n)), "loos": loos,
             "loo_mean": m, "loo_var": v}
+
+
+# ---------------------------------------------------------------------------
+# FITC (method_fitc): R/gp_fit.R:101-123, R/laplace.R:122-152,300-322,
+# R/linear_algebra.R:3-172, R/gp_pred.R:234-290
+# ---------------------------------------------------------------------------
+
+def inv_lemma_get(K, U, D, logdet=True):
+    """inv(U K^-1 U^T + diag(D)) by the inversion lemma: R/linear_algebra.R:3-47
+    (V = t(U), symmetrised V D^-1 U)."""
+    D = np.asarray(D, float)
+    inv_D_U = U / D[:, None]
+    VDU = U.T @ inv_D_U
+    VDU = 0.5 * (VDU + VDU.T)
+    L = sla.cholesky(K + VDU, lower=True)
+    ld = None
+    if logdet:
+        ld = float(np.sum(np.log(D)) - 2 * np.sum(np.log(np.diag(sla.cholesky(K, lower=True))))
+                   + 2 * np.sum(np.log(np.diag(L))))
+    return {"chol": L, "D": D, "inv_D_U": inv_D_U, "logdet": ld}
+
+
+def inv_lemma_solve(obj, rhs, lhs=None):
+    """Dense rhs (vector or matrix), optional dense lhs: R/linear_algebra.R:112-170
+    (the non-diagonal branches)."""
+    D, IDU, L = obj["D"], obj["inv_D_U"], obj["chol"]
+    rhs = np.asarray(rhs, float)
+    core = sla.cho_solve((L, True), IDU.T @ rhs)  # backsolve(t(L), forwardsolve(L, aux_right))
+    scaled = rhs / D[:, None] if rhs.ndim == 2 else rhs / D
+    if lhs is None:
+        return scaled - IDU @ core
+    return lhs @ scaled - (lhs @ IDU) @ core
+
+
+def fitc_fit(cfs, lik, x, y, z, maxiter=30, tol=1e-4, jitter=None, normalize_learn=True, **kw):
+    """fit_laplace.method_fitc (R/gp_fit.R:101-123) for given inducing inputs z,
+    Newton loop of laplace.method_fitc (R/laplace.R:300-322, stopping on the
+    change of the unnormalised log posterior) over the log-concave branch of
+    laplace_iter.method_fitc (R/laplace.R:122-152)."""
+    x = np.atleast_2d(np.asarray(x, float))
+    if x.shape[0] == 1 and len(y) != 1:
+        x = x.T
+    z = np.atleast_2d(np.asarray(z, float))
+    if normalize_learn:
+        cfs = learn_scales(cfs, x)
+    jitter = JITTER if jitter is None else jitter
+    y = np.asarray(y, float)
+    n, m = len(y), z.shape[0]
+    Kz = eval_cf(cfs, z, z) + jitter * np.eye(m)
+    Kxz = eval_cf(cfs, x, z)
+    Kz_chol = sla.cholesky(Kz, lower=True)
+    D = eval_cf(cfs, x, x, diag_only=True) + jitter
+    D = D - np.sum(sla.solve_triangular(Kz_chol, Kxz.T, lower=True) ** 2, axis=0)
+    fhat = np.zeros(n)
+    if lik["kind"] == "gaussian":
+        maxiter = 1
+    log_post_old = -np.inf
+    fit = None
+    it = 0
+    for it in range(1, maxiter + 1):
+        pz, V = pseudodata_la(lik, fhat, y, **kw)
+        S = D + V
+        C_inv = inv_lemma_get(Kz, Kxz, S)
+        alpha = inv_lemma_solve(C_inv, pz)
+        fnew = Kxz @ sla.cho_solve((Kz_chol, True), Kxz.T @ alpha) + D * alpha
+        f_invK_f = float(fnew @ alpha)
+        ll = loglik(lik, fnew, y, **kw)
+        logZ = ll - 0.5 * f_invK_f + 0.5 * (float(np.sum(np.log(V))) - C_inv["logdet"])
+        log_post = ll - 0.5 * f_invK_f
+        fit = dict(fmean=fnew, z=pz, V=V, Kz=Kz, Kxz=Kxz, Kz_chol=Kz_chol, C_inv=C_inv, D=D, alpha=alpha,
+                   log_evidence=float(logZ), log_post_unnorm=float(log_post))
+        fhat = fnew
+        diff = log_post - log_post_old
+        log_post_old = log_post
+        if abs(diff) < tol:
+            break
+    if maxiter > 1 and it == maxiter:
+        warnings.warn("Maximum number of iterations in Laplace reached")
+    fit.update(iters=it, cfs=cfs, lik=lik, x=x, inducing=z, jitter=jitter, approx="laplace")
+    return fit
+
+
+def fitc_pred(fit, xt, var=False):
+    """gp_pred_post.method_fitc with train = FALSE: R/gp_pred.R:234-290."""
+    cfs, z = fit["cfs"], fit["inducing"]
+    xt = np.atleast_2d(np.asarray(xt, float))
+    Ktz = eval_cf(cfs, xt, z)
+    Kz_chol, Kxz = fit["Kz_chol"], fit["Kxz"]
+    aux1 = sla.solve_triangular(Kz_chol, Kxz.T, lower=True)
+    aux2 = sla.solve_triangular(Kz_chol, Ktz.T, lower=True)
+    mean = aux2.T @ (aux1 @ fit["alpha"])
+    if not var:
+        return mean, None
+    Dt = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux2 ** 2, axis=0)
+    Kz_inv_Ktz = sla.cho_solve((Kz_chol, True), Ktz.T)
+    prior_var = np.sum(Ktz * Kz_inv_Ktz.T, axis=1) + Dt
+    aux = inv_lemma_solve(fit["C_inv"], aux1.T, aux1) @ aux2
+    return mean, prior_var - np.sum(aux2 * aux, axis=0)
 
 
 def gaussian_logz_closed_form(K_cf, y, sigma, jitter=JITTER):

@@ -1,0 +1,91 @@
+"""FITC (method_fitc: R/gp_fit.R:101-123, R/laplace.R:122-152,300-322,
+R/gp_pred.R:234-290) on the device against the oracle's restatement with the
+same inducing inputs: log evidence, posterior mean, alpha, pseudo-data and the
+FITC diagonal at relative 1e-8, the Newton step count, and predictive mean and
+variance at 1e-8, for Gaussian, Bernoulli-logit, binomial (trials) and Poisson
+(offset) likelihoods over ragged n and m.  Inducing inputs placed by the
+package's k-means are checked the same way (the oracle reads them back).  A
+factor failure gives log_evidence = -Inf as in the reference."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _data(kind, n, m, seed):
+    r = np.random.default_rng(seed)
+    x = r.normal(size=(n, 2))
+    z = r.normal(size=(m, 2)) * 1.3
+    f = np.sin(x.sum(1))
+    kw = {}
+    if kind == "gaussian":
+        y = f + 0.3 * r.normal(size=n)
+    elif kind == "logit":
+        y = (r.uniform(size=n) < 1 / (1 + np.exp(-2 * f))).astype(float)
+    elif kind == "binomial":
+        t = r.integers(1, 6, size=n).astype(float)
+        y = r.binomial(t.astype(int), 1 / (1 + np.exp(-f))).astype(float)
+        kw["trials"] = t
+    else:
+        off = 0.3 * r.normal(size=n)
+        y = r.poisson(np.exp(0.5 * f + off)).astype(float)
+        kw["offset"] = off
+    return x, y, z, kw
+
+
+def _lik(G, kind):
+    return {"gaussian": G.lik_gaussian(sigma=0.3), "logit": G.lik_bernoulli("logit"),
+            "binomial": G.lik_binomial("logit"), "poisson": G.lik_poisson()}[kind]
+
+
+@pytest.mark.parametrize("kind,n,m", [("gaussian", 1, 1), ("gaussian", 150, 20), ("gaussian", 333, 70),
+                                      ("logit", 200, 30), ("binomial", 150, 25), ("poisson", 130, 65)])
+def test_fitc_matches_oracle(gpl, kind, n, m):
+    G = gpl
+    x, y, z, kw = _data(kind, n, m, 100 + n + m)
+    lk = _lik(G, kind)
+    gp = G.gp_init(G.cf_matern52(lscale=[0.9, 1.2], magn=1.1), lk, method=G.method_fitc(inducing=z),
+                   approx=G.approx_laplace(maxiter=50, tol=1e-8))
+    gf = G.gp_fit(gp, x, y, **kw)
+    fo = O.fitc_fit([c.spec() for c in gf.cfs], lk.spec(), x, y, z, maxiter=50, tol=1e-8, **kw)
+    assert gf.fit.iters == fo["iters"]
+    assert abs(gf.fit.log_evidence - fo["log_evidence"]) <= 1e-8 * abs(fo["log_evidence"])
+    for nm, ref in (("fmean", fo["fmean"]), ("alpha", fo["alpha"]), ("z", fo["z"]), ("V", fo["V"]),
+                    ("diag", fo["D"])):
+        assert _rel(getattr(gf.fit, nm), ref) < 1e-8, nm
+    xt = np.random.default_rng(5).normal(size=(77, 2))
+    pr = G.gp_pred(gf, xt, var=True)
+    mo, vo = O.fitc_pred(fo, xt, var=True)
+    assert _rel(pr["mean"], mo) < 1e-8
+    assert _rel(pr["var"], vo) < 1e-8
+
+
+def test_fitc_kmeans_inducing(gpl):
+    G = gpl
+    x, y, _, _ = _data("gaussian", 400, 1, 9)
+    lk = _lik(G, "gaussian")
+    gp = G.gp_init(G.cf_sexp(lscale=0.8), lk, method=G.method_fitc(num_inducing=24, bin_along=0, bin_count=4))
+    gf = G.gp_fit(gp, x, y)
+    z = gf.method.inducing
+    assert z.shape == (24, 2) and gp.method.inducing is None  # the input model is not modified
+    fo = O.fitc_fit([c.spec() for c in gf.cfs], lk.spec(), x, y, z)
+    assert abs(gf.fit.log_evidence - fo["log_evidence"]) <= 1e-8 * abs(fo["log_evidence"])
+    pr = G.gp_pred(gf, x[:50], var=True)
+    mo, vo = O.fitc_pred(fo, x[:50], var=True)
+    assert _rel(pr["mean"], mo) < 1e-8 and _rel(pr["var"], vo) < 1e-8
+
+
+def test_fitc_failed_factor_gives_minus_inf(gpl, capsys):
+    G = gpl
+    x, y, z, _ = _data("gaussian", 60, 8, 3)
+    gp = G.gp_init(G.cf_sexp(magn=0.5), G.lik_gaussian(sigma=0.3), method=G.method_fitc(inducing=z))
+    gf = G.gp_fit(gp, x, y, jitter=-1.0)  # Kz = K_zz - I is indefinite
+    assert gf.fit.log_evidence == -np.inf
+    assert "not positive" in capsys.readouterr().out

@@ -403,6 +403,82 @@ def test_oracle_pred_cfind_closed_form():
     np.testing.assert_allclose(va, vfull, atol=1e-12)
 
 
+def _fitc_case(seed=14, n=90, m=12, kind="gaussian"):
+    r = np.random.default_rng(seed)
+    x = r.normal(size=(n, 2))
+    f = np.sin(x.sum(1))
+    z = r.normal(size=(m, 2)) * 1.2
+    if kind == "gaussian":
+        y = f + 0.3 * r.normal(size=n)
+        lik = {"kind": "gaussian", "sigma": 0.3}
+    elif kind == "bernoulli":
+        y = (r.uniform(size=n) < 1 / (1 + np.exp(-2 * f))).astype(float)
+        lik = {"kind": "bernoulli", "link": "logit"}
+    else:
+        y = r.poisson(np.exp(0.6 * f)).astype(float)
+        lik = {"kind": "poisson", "link": "log"}
+    cf = [{"kind": "sexp", "lscale": 0.9, "magn": 1.1}]
+    return x, y, z, cf, lik
+
+
+def test_oracle_fitc_gaussian_closed_form():
+    """FITC Gaussian (R/gp_fit.R:101-123, R/laplace.R:122-152): the log evidence
+    must be log N(y | 0, Q + diag(D) + sigma^2 I) with Q = K_xz Kz^-1 K_zx,
+    Kz = K_zz + jitter I and D = diag K + jitter - diag Q, from a direct n x n
+    Cholesky; the predictive mean and variance (train = FALSE) must be
+    Q_tx (Q + L)^-1 y and k_tt - diag(Q_tx (Q + L)^-1 Q_xt), L = diag(D) + sigma^2 I
+    (Quinonero-Candela & Rasmussen 2005, eqs. 16 and 24 with the FITC prior)."""
+    x, y, z, cf, lik = _fitc_case()
+    fit = O.fitc_fit(cf, lik, x, y, z)
+    c2 = fit["cfs"]
+    n, m, jit = len(y), z.shape[0], 1e-6
+    Kz = O.eval_cf(c2, z, z) + jit * np.eye(m)
+    Kxz = O.eval_cf(c2, x, z)
+    Q = Kxz @ np.linalg.solve(Kz, Kxz.T)
+    D = O.eval_cf(c2, x, x, diag_only=True) + jit - np.diag(Q)
+    Lam = np.diag(D + 0.09)
+    Ky = Q + Lam
+    Lc = np.linalg.cholesky(Ky)
+    a = np.linalg.solve(Lc, y)
+    ref = -0.5 * a @ a - np.sum(np.log(np.diag(Lc))) - 0.5 * n * np.log(2 * np.pi)
+    assert abs(fit["log_evidence"] - ref) < 1e-10 * abs(ref)
+    xt = np.random.default_rng(2).normal(size=(30, 2))
+    Ktz = O.eval_cf(c2, xt, z)
+    Qtx = Ktz @ np.linalg.solve(Kz, Kxz.T)
+    mu = Qtx @ np.linalg.solve(Ky, y)
+    v = O.eval_cf(c2, xt, xt, diag_only=True) - np.sum(Qtx * np.linalg.solve(Ky, Qtx.T).T, axis=1)
+    mo, vo = O.fitc_pred(fit, xt, var=True)
+    np.testing.assert_allclose(mo, mu, atol=1e-9)
+    np.testing.assert_allclose(vo, v, atol=1e-9)
+
+
+@pytest.mark.parametrize("kind", ["bernoulli", "poisson"])
+def test_oracle_fitc_laplace_mode(kind):
+    """FITC Laplace: at the mode f = (Q + diag D) grad log p(y | f), i.e. alpha
+    equals the likelihood gradient (the Newton fixed point, for any V > 0), and
+    the log evidence equals the Laplace formula with the analytic W at the mode
+    (Rasmussen & Williams eq. 3.32 with the FITC prior covariance), up to the
+    finite-difference curvature of R/lik.R:449-454."""
+    x, y, z, cf, lik = _fitc_case(kind=kind)
+    fit = O.fitc_fit(cf, lik, x, y, z, maxiter=100, tol=1e-14)
+    f = fit["fmean"]
+    g = O.loglik_d(lik, f, y)
+    np.testing.assert_allclose(fit["alpha"], g, atol=1e-8)
+    c2 = fit["cfs"]
+    m, jit = z.shape[0], 1e-6
+    Kz = O.eval_cf(c2, z, z) + jit * np.eye(m)
+    Kxz = O.eval_cf(c2, x, z)
+    Q = Kxz @ np.linalg.solve(Kz, Kxz.T)
+    Kf = Q + np.diag(O.eval_cf(c2, x, x, diag_only=True) + jit - np.diag(Q))
+    np.testing.assert_allclose(Kf @ g, f, atol=1e-8)
+    mu = 1 / (1 + np.exp(-f)) if kind == "bernoulli" else np.exp(f)
+    W = mu * (1 - mu) if kind == "bernoulli" else mu
+    sw = np.sqrt(W)
+    B = np.eye(len(y)) + sw[:, None] * Kf * sw[None, :]
+    ref = O.loglik(lik, f, y) - 0.5 * f @ g - np.sum(np.log(np.diag(np.linalg.cholesky(B))))
+    assert abs(fit["log_evidence"] - ref) < 1e-5 * abs(ref)
+
+
 def test_oracle_loo_gaussian_closed_form():
     """Gaussian likelihood.
     * The LOO latent moments of R/gp_loo.R:120-130 must give the closed form
