@@ -98,4 +98,24 @@ void vec_log(hipStream_t s, int64_t n, const double* a, double* out) {
     hipLaunchKernelGGL(k_vec_log, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, out);
 }
 
+// B (ncol x nrow, ld ldb) = A^T for A (nrow x ncol, ld lda); both dimensions
+// multiples of 64.  64x64 tiles through LDS so reads and writes are coalesced.
+__global__ __launch_bounds__(256) void k_transpose64(const double* __restrict__ A, int64_t lda, int64_t tiles_r,
+                                                     double* __restrict__ B, int64_t ldb) {
+    __shared__ double t[64][65];
+    const int64_t ti = blockIdx.x % tiles_r, tj = blockIdx.x / tiles_r;
+    const int e = threadIdx.x & 63, p = threadIdx.x >> 6;
+    const double* a = A + ti * 64 + tj * 64 * lda;
+    for (int c = p; c < 64; c += 4) t[c][e] = a[e + (int64_t)c * lda];  // t[col][row]
+    __syncthreads();
+    double* b = B + tj * 64 + ti * 64 * ldb;
+    for (int c = p; c < 64; c += 4) b[e + (int64_t)c * ldb] = t[e][c];  // B(tj*64+e, ti*64+c) = A(ti*64+c, tj*64+e)
+}
+
+void transpose_f64(hipStream_t s, const double* A, int64_t lda, int64_t nrow, int64_t ncol, double* B, int64_t ldb) {
+    const int64_t tr = nrow / 64, tc = ncol / 64;
+    if (tr <= 0 || tc <= 0) return;
+    hipLaunchKernelGGL(k_transpose64, dim3((unsigned)(tr * tc)), dim3(256), 0, s, A, lda, tr, B, ldb);
+}
+
 }  // namespace gpl

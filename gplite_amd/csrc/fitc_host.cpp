@@ -130,11 +130,17 @@ static int fitc_run(gpl_fitc* f) {
     }
     trtri_blocked(c, f->d(F_LZ), f->d(F_DINVZ), Mz, f->d(F_XZ));
     // K_xz in both orientations (zero outside n x m)
-    launch_build_cross(s, pd, f->prog, f->d(F_F), N, n, N, f->d(F_FZ), Mz, m, Mz, f->d(F_U), N);
-    launch_build_cross(s, pd, f->prog, f->d(F_FZ), Mz, m, Mz, f->d(F_F), N, n, N, f->d(F_UT), Mz);
+    {
+        Prof p(c, "fitc_cross", 0, 16.0 * N * Mz);
+        launch_build_cross(s, pd, f->prog, f->d(F_F), N, n, N, f->d(F_FZ), Mz, m, Mz, f->d(F_U), N);
+        transpose_f64(s, f->d(F_U), N, N, Mz, f->d(F_UT), Mz);
+    }
     // D = diag K + jitter - colSums((Lz^-1 K_zx)^2)  (R/gp_fit.R:107-108); F_W holds Lz^-1 K_zx here
-    gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, f->d(F_XZ), Mz, f->d(F_UT), Mz, 0.0, f->d(F_W), Mz, 3, 0);
-    colsumsq_f64(s, f->d(F_W), Mz, Mz, n, f->d(F_V1), 0);
+    {
+        Prof p(c, "fitc_diag_trmm", (double)Mz * Mz * N, 16.0 * N * Mz);
+        gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, f->d(F_XZ), Mz, f->d(F_UT), Mz, 0.0, f->d(F_W), Mz, 3, 0);
+        colsumsq_f64(s, f->d(F_W), Mz, Mz, n, f->d(F_V1), 0);
+    }
     launch_build_diag(s, pd, f->prog, f->d(F_F), N, f->d(F_F), N, n, f->d(F_V2));
     fitc_diag(s, n, N, f->d(F_V2), f->d(F_V1), jitter, f->d(F_D));
 
@@ -163,6 +169,7 @@ static int fitc_run(gpl_fitc* f) {
         vec_axpby(s, Mz * Mz, 1.0, f->d(F_KZ), 1.0, f->d(F_H), f->d(F_G));
         potrf_blocked(c, f->d(F_G), Mz, f->d(F_DINVG), f->d(F_LDG), flags);
         // alpha = z/S - (K_xz/S) G^-1 (K_zx z/S)  (inv_lemma_solve, vector rhs)
+        Prof pv(c, "fitc_gemv", 0, 32.0 * N * Mz);
         gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ZS), f->d(F_Q1));
         trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q1), f->d(F_Q2), 0);
         trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q2), f->d(F_Q1), 1);
@@ -215,7 +222,10 @@ static int fitc_run(gpl_fitc* f) {
     // prediction state: w = Lz^-1 K_zx alpha and M (R/gp_pred.R:249-252,279)
     gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ALPHA), f->d(F_Q1));
     trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_WZ), 0);
-    fitc_predictive_matrix(f);
+    {
+        Prof p(c, "fitc_pred_matrix", 0, 0);
+        fitc_predictive_matrix(f);
+    }
     HIP_TRY(hipStreamSynchronize(s));
     if (maxiter > 1 && it == maxiter) {  // R/laplace.R:318-320 warns whenever the cap is reached
         fail(GPL_ERR_NOCONV, "Maximum number of iterations in Laplace reached, delta log post = " +
