@@ -105,9 +105,11 @@ def _sig(lib):
         "gpl_server_destroy": (None, [_vp]),
         "gpl_pred_ctx": (C.c_int, [_vp, _vp, _dp, i64, i32, _dp, _dp]),
         "gpl_draw": (C.c_int, [_vp, _dp, i64, C.c_double, _dp, i64, _dp]),
+        "gpl_draw_prior": (C.c_int, [_vp, C.POINTER(CfTerm), i32, _dp, i64, i64, C.c_double, _dp, i64, _dp]),
         "gpl_fitc_fit": (C.c_int, [_vp, C.POINTER(CfTerm), i32, _dp, i64, i64, _dp, i64, _dp, _dp, _dp,
                                    C.POINTER(Lik), C.POINTER(Opts), C.POINTER(_vp), C.POINTER(C.c_double)]),
         "gpl_fitc_pred": (C.c_int, [_vp, _dp, i64, i32, _dp, _dp]),
+        "gpl_fitc_draw": (C.c_int, [_vp, _dp, i64, C.c_double, _dp, i64, _dp]),
         "gpl_fitc_vector": (C.c_int, [_vp, i32, _dp]),
         "gpl_fitc_info": (C.c_int, [_vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(C.c_double)]),
         "gpl_fitc_free": (None, [_vp]),

@@ -932,7 +932,7 @@ def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, ji
     if fitc and cfind is not None:
         raise NotImplementedError("cfind with method_fitc is not on the device path")
     if not is_fitted(gp):  # R/gp_pred.R:142-184
-        if fitc and gp.method.inducing is None:
+        if fitc and var and gp.method.inducing is None:
             raise ValueError("Inducing points not set yet.")
         mean = np.zeros(m)
         v = eval_cf(gp.cfs, xt, xt, diag_only=True, cfind=cfind) if var else None
@@ -976,8 +976,8 @@ def gp_pred(gp, xnew, var=False, quantiles=None, transform=False, cfind=None, ji
 
 def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, cfind=None, jitter=None, seed=None,
             trials=None, offset=None):
-    """Posterior draws at xnew: R/gp_draw.R:42-124 for method_full, returned as
-    a (len(xnew), draws) array.
+    """Posterior draws at xnew: R/gp_draw.R:42-140 for method_full and
+    method_fitc, returned as a (len(xnew), draws) array.
 
     marginal=False draws mean + chol(cov + jitter I) z on the device
     (gpl_draw: full predictive covariance, its Cholesky factor and the triangular
@@ -986,22 +986,37 @@ def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, cfi
     cannot be reproduced.  transform maps to the response scale; target draws
     observations (R/lik.R:502-537).
     """
-    if not is_fitted(gp):
-        raise NotImplementedError("draws from the prior are not on the device path; fit the model first")
-    if isinstance(gp.method, method_fitc):
-        raise NotImplementedError("gp_draw for method_fitc is not on the device path")
+    fitted = is_fitted(gp)
+    fitc = isinstance(gp.method, method_fitc)
+    if fitc and cfind is not None:
+        raise NotImplementedError("cfind with method_fitc is not on the device path")
     xt = _as_matrix(xnew)
     m = xt.shape[0]
     jitter = 1e-6 if jitter is None else float(jitter)
     rng = np.random.default_rng(seed)
     z = np.asfortranarray(rng.standard_normal((m, draws)))
-    if marginal:
+    if not fitted:  # gp_draw_prior (R/gp_draw.R:56-67)
+        if fitc:
+            if gp.method.inducing is None:
+                raise ValueError("Inducing points not set yet.")
+            raise NotImplementedError("prior draws for method_fitc are not on the device path")
+        cfs = gp.cfs if cfind is None else [gp.cfs[i] for i in _cfind_list(gp.cfs, cfind)]
+        terms, nt, xtp, _ = _program(cfs, xt)
+        f = np.empty((m, draws), order="F")
+        L.check(L.lib().gpl_draw_prior(L.ctx(), terms, nt, L.dptr(xtp), m, xtp.shape[1], jitter,
+                                       z.ctypes.data_as(L._dp), draws, f.ctypes.data_as(L._dp)))
+    elif marginal:
         pr = gp_pred(gp, xt, var=True, cfind=cfind)
         f = pr["mean"][:, None] + np.sqrt(pr["var"])[:, None] * z
     else:
         _, _, xtp, _ = _program(gp.cfs, xt)
         f = np.empty((m, draws), order="F")
-        if cfind is None:
+        if fitc:  # R/gp_draw.R:126-140
+            if gp.fit.handle is None:
+                raise RuntimeError("the FITC fit failed (log_evidence = -Inf)")
+            L.check(L.lib().gpl_fitc_draw(gp.fit.handle, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp), draws,
+                                          f.ctypes.data_as(L._dp)))
+        elif cfind is None:
             L.check(L.lib().gpl_draw(gp.fit.handle, L.dptr(xtp), m, jitter, z.ctypes.data_as(L._dp), draws,
                                      f.ctypes.data_as(L._dp)))
         else:

@@ -1310,14 +1310,14 @@ static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, doubl
     HIP_TRY(hipMemcpyAsync(&inf, info, 4, hipMemcpyDeviceToHost, s));
     // W = L Z (Z padded with zero rows); out = mean + W
     fill_f64(s, Zd, M * Dp, 0.0);
-    HIP_TRY(hipMemcpy2DAsync(Zd, M * 8, z, mt * 8, mt * 8, draws, hipMemcpyHostToDevice, s));
+    HIP_TRY(upload_2d(c, s, Zd, M, z, mt, draws, 20));
     {
         Prof p(c, "draw_trmm", (double)M * M * Dp, 8.0 * M * Dp * 2);
         gemm2_f64(s, 0, 0, M, Dp, M, 1.0, Cv, M, Zd, M, 0.0, Wd, M, 3, 0);
     }
     std::vector<double> mean(mt);
     HIP_TRY(hipMemcpyAsync(mean.data(), vbuf, mt * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpy2DAsync(out, mt * 8, Wd, M * 8, mt * 8, draws, hipMemcpyDeviceToHost, s));
+    HIP_TRY(download_2d(c, s, out, Wd, M, mt, draws, 21));
     HIP_TRY(hipStreamSynchronize(s));
     if (inf > 0) {
         set_minor(inf);
@@ -1326,6 +1326,49 @@ static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, doubl
     }
     for (int64_t j = 0; j < draws; ++j)
         for (int64_t i = 0; i < mt; ++i) out[i + j * mt] += mean[i];
+    return GPL_OK;
+}
+
+// Draws from the prior (gp_draw_prior.method_full, R/gp_draw.R:56-67): chol(K_tt + jitter I) z
+// (gp_pred_prior with cov = TRUE, R/gp_pred.R:142-151; the prior mean is 0).
+extern "C" int gpl_draw_prior(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, const double* xt, int64_t mt,
+                              int64_t ncols, double jitter, const double* z, int64_t draws, double* out) {
+    if (!c || !xt || !z || !out || mt < 1 || draws < 1 || ncols < 0) return fail(GPL_ERR_ARG, "bad argument");
+    DevProg P;
+    int r = prepare_prog(terms, nterms, ncols, P);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t M = pad64(mt), Dp = pad64(draws);
+    DevProg* pd = (DevProg*)c->buf(9, sizeof(DevProg));
+    double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(ncols, 1) * 8);
+    double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
+    double* Cv = (double*)c->buf(5, (size_t)M * M * 8);
+    double* Dv = (double*)c->buf(6, (size_t)M * 64 * 8 + (size_t)M * 8 + 64);
+    double* Zd = (double*)c->buf(7, (size_t)M * Dp * 8);
+    double* Wd = (double*)c->buf(8, (size_t)M * Dp * 8);
+    if (!pd || !xtd || !Ft || !Cv || !Dv || !Zd || !Wd) return fail(GPL_ERR_OOM, "device allocation failed");
+    double* ld = Dv + M * 64;
+    int* info = (int*)(ld + M);
+    HIP_TRY(hipMemcpyAsync(pd, &P, sizeof(DevProg), hipMemcpyHostToDevice, s));
+    if (mt * ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, xtd, mt, mt, (int)ncols, pd, P, Ft, M);
+    launch_build_sym_f64(s, pd, P, Ft, mt, M, jitter, 0, 1.0, nullptr, Cv, M);
+    HIP_TRY(hipMemsetAsync(info, 0, 64, s));
+    potrf_blocked(c, Cv, M, Dv, ld, info);
+    int inf = 0;
+    HIP_TRY(hipMemcpyAsync(&inf, info, 4, hipMemcpyDeviceToHost, s));
+    fill_f64(s, Zd, M * Dp, 0.0);
+    HIP_TRY(upload_2d(c, s, Zd, M, z, mt, draws, 20));
+    gemm2_f64(s, 0, 0, M, Dp, M, 1.0, Cv, M, Zd, M, 0.0, Wd, M, 3, 0);
+    HIP_TRY(download_2d(c, s, out, Wd, M, mt, draws, 21));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (inf > 0) {
+        set_minor(inf);
+        return fail(GPL_ERR_NOT_PD, "prior covariance: the leading minor of order " + std::to_string(inf) +
+                                        " is not positive (raise jitter)");
+    }
     return GPL_OK;
 }
 

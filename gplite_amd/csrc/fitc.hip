@@ -118,4 +118,17 @@ void transpose_f64(hipStream_t s, const double* A, int64_t lda, int64_t nrow, in
     hipLaunchKernelGGL(k_transpose64, dim3((unsigned)(tr * tc)), dim3(256), 0, s, A, lda, tr, B, ldb);
 }
 
+// C[i + i*ld] += v[i] + jitter for i < n; the padded diagonal (n <= i < N) becomes 1
+__global__ void k_add_diag(double* __restrict__ C, int64_t ld, int64_t n, int64_t N, const double* __restrict__ v,
+                           double jitter) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (i < n) C[i + i * ld] += v[i] + jitter;
+    else C[i + i * ld] = 1.0;
+}
+
+void add_diag_f64(hipStream_t s, double* C, int64_t ld, int64_t n, int64_t N, const double* v, double jitter) {
+    hipLaunchKernelGGL(k_add_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, C, ld, n, N, v, jitter);
+}
+
 }  // namespace gpl

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "gplite_amd.h"
 #include "host_common.h"
@@ -335,6 +336,68 @@ extern "C" int gpl_fitc_pred(gpl_fitc* f, const double* xt, int64_t mt, int32_t 
     }
     HIP_TRY(hipMemcpyAsync(mean, vb, mt * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return GPL_OK;
+}
+
+// gp_draw_analytic.method_fitc with marginal = FALSE (R/gp_draw.R:126-140):
+// mean + chol(cov + jitter I) z, cov = A_t^T A_t + diag(Dt) - A_t^T M A_t
+// (R/gp_pred.R:258-268, train = FALSE).
+extern "C" int gpl_fitc_draw(gpl_fitc* f, const double* xt, int64_t mt, double jitter, const double* z,
+                             int64_t draws, double* out) {
+    if (!f || !xt || !z || !out || mt < 1 || draws < 1) return fail(GPL_ERR_ARG, "bad argument");
+    gpl_ctx* c = f->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t m = f->m, Mz = f->Mz, Mt = pad64(mt), Dp = pad64(draws);
+    const DevProg* pd = f->b[F_PROG].as<DevProg>();
+    double* xtd = (double*)c->buf(10, (size_t)mt * std::max<int64_t>(f->ncols, 1) * 8);
+    double* Ft = (double*)c->buf(11, (size_t)Mt * MAX_FEAT * 8);
+    double* KzT = (double*)c->buf(12, (size_t)Mz * Mt * 8);
+    double* A2 = (double*)c->buf(13, (size_t)Mz * Mt * 8);
+    double* Yv = (double*)c->buf(14, (size_t)Mz * Mt * 8);
+    double* vb = (double*)c->buf(15, (size_t)Mt * 6 * 8);
+    double* Cv = (double*)c->buf(16, (size_t)Mt * Mt * 8);
+    double* Dv = (double*)c->buf(17, (size_t)Mt * 64 * 8 + (size_t)Mt * 8 + 64);
+    double* Zd = (double*)c->buf(18, (size_t)Mt * Dp * 8);
+    double* Wd = (double*)c->buf(19, (size_t)Mt * Dp * 8);
+    if (!xtd || !Ft || !KzT || !A2 || !Yv || !vb || !Cv || !Dv || !Zd || !Wd)
+        return fail(GPL_ERR_OOM, "device allocation failed");
+    double* ld = Dv + Mt * 64;
+    int* info = (int*)(ld + Mt);
+    if (mt * f->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * f->ncols * 8, hipMemcpyHostToDevice, s));
+    launch_features(s, xtd, mt, mt, (int)f->ncols, pd, f->prog, Ft, Mt);
+    launch_build_cross(s, pd, f->prog, f->d(F_FZ), Mz, m, Mz, Ft, Mt, mt, Mt, KzT, Mz);
+    gemm2_f64(s, 0, 0, Mz, Mt, Mz, 1.0, f->d(F_XZ), Mz, KzT, Mz, 0.0, A2, Mz, 3, 0);
+    gemv_t_f64(s, A2, Mz, Mz, mt, f->d(F_WZ), vb);
+    // Dt = diag K_tt - colSums(A_t^2)
+    double* dt = vb + Mt;
+    colsumsq_f64(s, A2, Mz, Mz, mt, dt, 0);
+    launch_build_diag(s, pd, f->prog, Ft, Mt, Ft, Mt, mt, vb + 2 * Mt);
+    vec_axpby(s, mt, 1.0, vb + 2 * Mt, -1.0, dt, dt);
+    // lower tiles of A_t^T A_t - A_t^T (M A_t), then the diagonal terms
+    gemm2_f64(s, 0, 0, Mz, Mt, Mz, 1.0, f->d(F_MM), Mz, A2, Mz, 0.0, Yv, Mz, 0, 0);
+    gemm2_f64(s, 1, 0, Mt, Mt, Mz, 1.0, A2, Mz, A2, Mz, 0.0, Cv, Mt, 1, 0);
+    gemm2_f64(s, 1, 0, Mt, Mt, Mz, -1.0, A2, Mz, Yv, Mz, 1.0, Cv, Mt, 1, 0);
+    add_diag_f64(s, Cv, Mt, mt, Mt, dt, jitter);
+    HIP_TRY(hipMemsetAsync(info, 0, 64, s));
+    potrf_blocked(c, Cv, Mt, Dv, ld, info);
+    int inf = 0;
+    HIP_TRY(hipMemcpyAsync(&inf, info, 4, hipMemcpyDeviceToHost, s));
+    fill_f64(s, Zd, Mt * Dp, 0.0);
+    HIP_TRY(upload_2d(c, s, Zd, Mt, z, mt, draws, 20));
+    gemm2_f64(s, 0, 0, Mt, Dp, Mt, 1.0, Cv, Mt, Zd, Mt, 0.0, Wd, Mt, 3, 0);
+    std::vector<double> mean(mt);
+    HIP_TRY(hipMemcpyAsync(mean.data(), vb, mt * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(download_2d(c, s, out, Wd, Mt, mt, draws, 21));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (inf > 0) {
+        set_minor(inf);
+        return fail(GPL_ERR_NOT_PD, "predictive covariance: the leading minor of order " + std::to_string(inf) +
+                                        " is not positive (raise jitter)");
+    }
+    for (int64_t j = 0; j < draws; ++j)
+        for (int64_t i = 0; i < mt; ++i) out[i + j * mt] += mean[i];
     return GPL_OK;
 }
 

@@ -178,6 +178,8 @@ void fitc_alpha(hipStream_t s, int64_t n, int64_t N, const double* zs, const dou
 void coldot_f64(hipStream_t s, const double* A, const double* B, int64_t ld, int64_t nrow, int64_t ncol, double* out);
 void fitc_diag(hipStream_t s, int64_t n, int64_t N, const double* kd, const double* cs, double jitter, double* D);
 void vec_log(hipStream_t s, int64_t n, const double* a, double* out);
+// C_ii += v_i + jitter (i < n), padded diagonal set to 1
+void add_diag_f64(hipStream_t s, double* C, int64_t ld, int64_t n, int64_t N, const double* v, double jitter);
 // B = A^T (dimensions multiples of 64)
 void transpose_f64(hipStream_t s, const double* A, int64_t lda, int64_t nrow, int64_t ncol, double* B, int64_t ldb);
 

@@ -118,6 +118,28 @@ struct gpl_ctx {
     }
 };
 
+// Column-major host matrix (rows x cols, leading dimension rows) to a device
+// matrix of leading dimension dld, and back.  The host side goes through one
+// contiguous copy into a staging buffer (context slot `slot`): a pageable 2D
+// copy with short rows is carried out row by row.
+inline hipError_t upload_2d(gpl_ctx* c, hipStream_t s, double* dst, int64_t dld, const double* src, int64_t rows,
+                            int64_t cols, int slot) {
+    double* st = (double*)c->buf(slot, (size_t)rows * cols * 8);
+    if (!st) return hipErrorOutOfMemory;
+    hipError_t e = hipMemcpyAsync(st, src, (size_t)rows * cols * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpy2DAsync(dst, dld * 8, st, rows * 8, rows * 8, cols, hipMemcpyDeviceToDevice, s);
+}
+
+inline hipError_t download_2d(gpl_ctx* c, hipStream_t s, double* dst, const double* src, int64_t sld, int64_t rows,
+                              int64_t cols, int slot) {
+    double* st = (double*)c->buf(slot, (size_t)rows * cols * 8);
+    if (!st) return hipErrorOutOfMemory;
+    hipError_t e = hipMemcpy2DAsync(st, rows * 8, src, sld * 8, rows * 8, cols, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(dst, st, (size_t)rows * cols * 8, hipMemcpyDeviceToHost, s);
+}
+
 // Times one launch group on the library stream when profiling is on.
 struct Prof {
     gpl_ctx* c;

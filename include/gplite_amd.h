@@ -209,6 +209,11 @@ int gpl_fitc_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const d
  * K_tz Kz^-1 K_zx alpha and variance prior_var - diag(A_t^T M A_t), with
  * A_t = Lz^-1 K_zt and M = A C^-1 A^T formed once after the fit. */
 int gpl_fitc_pred(gpl_fitc* model, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var);
+/* gp_draw for method_fitc, marginal = FALSE (R/gp_draw.R:126-140): latent draws
+ * mean + chol(cov + jitter I) z with the full FITC predictive covariance
+ * A_t^T A_t + diag(Dt) - A_t^T M A_t (R/gp_pred.R:258-268); z and out as gpl_draw. */
+int gpl_fitc_draw(gpl_fitc* model, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
+                  double* out);
 /* fit vectors fmean, alpha, z, V (GPL_VEC_*), and the FITC diagonal D (which = 7) */
 int gpl_fitc_vector(gpl_fitc* model, int32_t which, double* out);
 int gpl_fitc_info(gpl_fitc* model, int64_t* n, int64_t* m, int32_t* iters, double* log_evidence);
@@ -297,6 +302,11 @@ int gpl_pred_terms(gpl_model* model, const int32_t* terms, int32_t nsel, const d
                    int32_t want_var, double* mean, double* var);
 int gpl_draw_terms(gpl_model* model, const int32_t* terms, int32_t nsel, const double* xt, int64_t mt,
                    double jitter, const double* z, int64_t draws, double* out);
+/* gp_draw_prior.method_full (R/gp_draw.R:56-67): draws chol(K_tt + jitter I) z from
+ * the prior of an unfitted model (gp_pred_prior with cov = TRUE, R/gp_pred.R:142-151;
+ * mean 0).  terms / xt / ncols as for gpl_eval_cf, z and out as for gpl_draw. */
+int gpl_draw_prior(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const double* xt, int64_t mt,
+                   int64_t ncols, double jitter, const double* z, int64_t draws, double* out);
 /* Device bytes held by a fitted model. */
 int64_t gpl_model_bytes(gpl_model* model);
 

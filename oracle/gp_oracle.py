@@ -40,7 +40,7 @@ __all__ = [
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
     "gp_energy_grad", "gp_pred", "gp_pred_cov", "gp_draw", "gp_loo", "gaussian_logz_closed_form",
-    "inv_lemma_get", "inv_lemma_solve", "fitc_fit", "fitc_pred",
+    "inv_lemma_get", "inv_lemma_solve", "fitc_fit", "fitc_pred", "fitc_pred_cov", "fitc_draw", "prior_draw",
 ]
 
 JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
@@ -975,6 +975,39 @@ def fitc_pred(fit, xt, var=False):
     prior_var = np.sum(Ktz * Kz_inv_Ktz.T, axis=1) + Dt
     aux = inv_lemma_solve(fit["C_inv"], aux1.T, aux1) @ aux2
     return mean, prior_var - np.sum(aux2 * aux, axis=0)
+
+
+def fitc_pred_cov(fit, xt, jitter=None):
+    """Mean and full predictive covariance + jitter I of a FITC fit, train = FALSE:
+    R/gp_pred.R:258-268 (prior_cov - A_t^T A C^-1 A^T A_t)."""
+    cfs, z = fit["cfs"], fit["inducing"]
+    xt = np.atleast_2d(np.asarray(xt, float))
+    jitter = JITTER if jitter is None else jitter
+    Ktz = eval_cf(cfs, xt, z)
+    Kz_chol, Kxz = fit["Kz_chol"], fit["Kxz"]
+    aux1 = sla.solve_triangular(Kz_chol, Kxz.T, lower=True)
+    aux2 = sla.solve_triangular(Kz_chol, Ktz.T, lower=True)
+    mean = aux2.T @ (aux1 @ fit["alpha"])
+    Dt = eval_cf(cfs, xt, xt, diag_only=True) - np.sum(aux2 ** 2, axis=0)
+    prior_cov = aux2.T @ aux2 + np.diag(Dt)
+    cov = prior_cov - aux2.T @ (aux1 @ inv_lemma_solve(fit["C_inv"], aux1.T)) @ aux2
+    return mean, cov + jitter * np.eye(xt.shape[0])
+
+
+def fitc_draw(fit, xt, z, jitter=None):
+    """FITC joint draws, R/gp_draw.R:126-140 with mvnrnd (:182-200)."""
+    mean, cov = fitc_pred_cov(fit, xt, jitter)
+    Lc = sla.cholesky(cov, lower=True)
+    return mean[:, None] + Lc @ np.asarray(z, float).reshape(len(mean), -1)
+
+
+def prior_draw(cfs, xt, z, jitter=None):
+    """Prior draws of method_full: R/gp_draw.R:56-67 with gp_pred_prior(cov = TRUE),
+    R/gp_pred.R:142-151: t(chol(K_tt + jitter I)) z (mean 0)."""
+    xt = np.atleast_2d(np.asarray(xt, float))
+    jitter = JITTER if jitter is None else jitter
+    Lc = sla.cholesky(eval_cf(cfs, xt, xt) + jitter * np.eye(xt.shape[0]), lower=True)
+    return Lc @ np.asarray(z, float).reshape(xt.shape[0], -1)
 
 
 def gaussian_logz_closed_form(K_cf, y, sigma, jitter=JITTER):

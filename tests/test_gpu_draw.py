@@ -101,3 +101,14 @@ def test_draw_not_pd_reported(gpl):
     with pytest.raises(L.GplError) as ei:
         G.gp_draw(gf, xt, draws=2, jitter=-10.0, seed=1)
     assert ei.value.code == L.GPL_ERR_NOT_PD
+
+
+def test_prior_draw_matches_oracle(gpl):
+    """gp_draw on an unfitted model (gp_draw_prior, R/gp_draw.R:56-67): chol(K_tt + jitter I) z."""
+    G = gpl
+    gp = G.gp_init([G.cf_sexp(lscale=0.7, magn=1.3), G.cf_matern32(lscale=1.1, magn=0.4)], G.lik_gaussian())
+    xt = np.random.default_rng(8).normal(size=(70, 2))
+    d = G.gp_draw(gp, xt, draws=3, transform=False, seed=6)
+    z = np.random.default_rng(6).standard_normal((70, 3))
+    ref = O.prior_draw([c.spec() for c in gp.cfs], xt, z)
+    assert np.max(np.abs(d - ref)) <= 1e-8 * np.max(np.abs(ref))

@@ -1,8 +1,9 @@
 """FITC (method_fitc: R/gp_fit.R:101-123, R/laplace.R:122-152,300-322,
 R/gp_pred.R:234-290) on the device against the oracle's restatement with the
 same inducing inputs: log evidence, posterior mean, alpha, pseudo-data and the
-FITC diagonal at relative 1e-8, the Newton step count, and predictive mean and
-variance at 1e-8, for Gaussian, Bernoulli-logit, binomial (trials) and Poisson
+FITC diagonal at relative 1e-8, the Newton step count, predictive mean and
+variance at 1e-8 and joint draws at 1e-7 (a Cholesky of the predictive
+covariance, whose smallest eigenvalues are near the jitter), for Gaussian, Bernoulli-logit, binomial (trials) and Poisson
 (offset) likelihoods over ragged n and m.  Inducing inputs placed by the
 package's k-means are checked the same way (the oracle reads them back).  A
 factor failure gives log_evidence = -Inf as in the reference."""
@@ -65,6 +66,11 @@ def test_fitc_matches_oracle(gpl, kind, n, m):
     mo, vo = O.fitc_pred(fo, xt, var=True)
     assert _rel(pr["mean"], mo) < 1e-8
     assert _rel(pr["var"], vo) < 1e-8
+    # joint draws (R/gp_draw.R:126-140) on the same standard normal matrix
+    d = G.gp_draw(gf, xt, draws=3, transform=False, seed=11)
+    zz = np.random.default_rng(11).standard_normal((77, 3))
+    ref = O.fitc_draw(fo, xt, zz)
+    assert _rel(d, ref) < 1e-7
 
 
 def test_fitc_kmeans_inducing(gpl):

@@ -450,6 +450,16 @@ def test_oracle_fitc_gaussian_closed_form():
     mo, vo = O.fitc_pred(fit, xt, var=True)
     np.testing.assert_allclose(mo, mu, atol=1e-9)
     np.testing.assert_allclose(vo, v, atol=1e-9)
+    # full predictive covariance (R/gp_pred.R:258-268): Q_tt + diag(k_tt - diag Q_tt) - Q_tx (Q + L)^-1 Q_xt
+    Qtt = Ktz @ np.linalg.solve(Kz, Ktz.T)
+    cov_cf = Qtt + np.diag(O.eval_cf(c2, xt, xt, diag_only=True) - np.diag(Qtt)) - Qtx @ np.linalg.solve(Ky, Qtx.T)
+    mc, cc = O.fitc_pred_cov(fit, xt, jitter=0.0)
+    np.testing.assert_allclose(mc, mu, atol=1e-9)
+    np.testing.assert_allclose(cc, cov_cf, atol=1e-9)
+    # draws with z = I give the factor of cov + jitter I
+    Lf = O.fitc_draw(fit, xt, np.eye(30), jitter=1e-6) - mo[:, None]
+    np.testing.assert_allclose(Lf @ Lf.T, cov_cf + 1e-6 * np.eye(30), atol=1e-9)
+    assert np.max(np.abs(np.triu(Lf, 1))) < 1e-12
 
 
 @pytest.mark.parametrize("kind", ["bernoulli", "poisson"])
