@@ -111,6 +111,7 @@ def _sig(lib):
         "gpl_fitc_pred": (C.c_int, [_vp, _dp, i64, i32, _dp, _dp]),
         "gpl_fitc_draw": (C.c_int, [_vp, _dp, i64, C.c_double, _dp, i64, _dp]),
         "gpl_fitc_vector": (C.c_int, [_vp, i32, _dp]),
+        "gpl_fitc_train_var": (C.c_int, [_vp, _dp]),
         "gpl_fitc_info": (C.c_int, [_vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(C.c_double)]),
         "gpl_fitc_free": (None, [_vp]),
         "gpl_draw_terms": (C.c_int, [_vp, C.POINTER(i32), i32, _dp, i64, C.c_double, _dp, i64, _dp]),

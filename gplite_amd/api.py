@@ -460,16 +460,17 @@ def _free_fitc(ptr):
 
 
 class _FitcFit:
-    """Device-resident FITC fit (gp$fit of R/laplace.R:177-180: fmean, z, V,
-    alpha, the FITC diagonal and the log evidence)."""
+    """Device-resident FITC fit (gp$fit of R/laplace.R:177-180 and
+    R/ep.R:271-278: fmean, z, V, alpha, fvar and cavity moments for EP, the FITC
+    diagonal and the log evidence)."""
 
-    def __init__(self, handle, n, m, log_evidence, iters):
+    def __init__(self, handle, n, m, log_evidence, iters, approx):
         self.handle = handle
         self.n, self.m = n, m
         self.log_evidence = log_evidence
         self.iters = iters
         self.type = "analytic"
-        self.approx = L.APPROX_LAPLACE
+        self.approx = approx
         self._fin = weakref.finalize(self, _free_fitc, handle.value if handle is not None else None)
 
     def vector(self, which):
@@ -483,6 +484,9 @@ class _FitcFit:
     alpha = property(lambda self: self.vector(L.VEC_ALPHA))
     z = property(lambda self: self.vector(L.VEC_Z))
     V = property(lambda self: self.vector(L.VEC_V))
+    fvar = property(lambda self: self.vector(L.VEC_FVAR))
+    cavity_mean = property(lambda self: self.vector(L.VEC_CAVITY_MEAN))
+    cavity_var = property(lambda self: self.vector(L.VEC_CAVITY_VAR))
     diag = property(lambda self: self.vector(7))
 
 
@@ -801,8 +805,6 @@ def get_inducing(gp, x):
 
 def _fit_fitc(g, x, y, terms, nt, xp, tr, off, lk, o):
     """fit_laplace.method_fitc (R/gp_fit.R:101-123) through gpl_fitc_fit."""
-    if isinstance(g.approx, approx_ep):
-        raise NotImplementedError("FITC with approx_ep (R/ep.R:148-280) is not on the device path")
     if g.precision == "mixed":
         raise NotImplementedError("FITC runs in FP64")
     g.method = copy.copy(g.method)
@@ -817,13 +819,13 @@ def _fit_fitc(g, x, y, terms, nt, xp, tr, off, lk, o):
     if rc in (L.GPL_ERR_NOT_PD, L.GPL_ERR_NAN):
         # the reference catches the error, prints it and keeps log_evidence = -Inf (R/gp_fit.R:113-121)
         print(L.lib().gpl_last_error().decode())
-        fit = _FitcFit(None, n, m, -math.inf, 0)
+        fit = _FitcFit(None, n, m, -math.inf, 0, o.approx)
     else:
         if rc not in (L.GPL_OK, L.GPL_ERR_NOCONV):
             L.check(rc)
         iters = C_int32()
         L.lib().gpl_fitc_info(handle, None, None, C_byref(iters), None)
-        fit = _FitcFit(handle, n, m, float(logz.value), int(iters.value))
+        fit = _FitcFit(handle, n, m, float(logz.value), int(iters.value), o.approx)
         if rc == L.GPL_ERR_NOCONV:
             warnings.warn(L.lib().gpl_last_error().decode())
     g.fit = fit
@@ -1188,15 +1190,17 @@ def _loglik_points(lik, f, y, trials=None, offset=None):
 
 def gp_loo(gp, x, y, quadrature=True, quad_order=11, draws=4000, seed=None, trials=None, offset=None):
     """Approximate leave-one-out predictive log densities: R/gp_loo.R:73-110
-    (method_full).  The LOO latent posteriors come from the fitted model:
-    Laplace uses the device predictive variance at the training inputs
-    (R/gp_loo.R:120-130); EP uses the cavity moments (:132-134).  The
+    (method_full and method_fitc).  The LOO latent posteriors come from the
+    fitted model: Laplace uses the device predictive variance at the training
+    inputs (R/gp_loo.R:120-130; for FITC with the diagonal correction,
+    train = TRUE); EP uses the cavity moments (:132-134).  The
     predictive density is integrated with Gauss-Hermite quadrature, or by
     Monte Carlo with `draws` samples."""
     if not is_fitted(gp):
         raise ValueError("The provied GP model is not fitted. Please call gp_fit (or gp_optim) first.")
-    if isinstance(gp.method, method_fitc):
-        raise NotImplementedError("gp_loo for method_fitc is not on the device path")
+    fitc = isinstance(gp.method, method_fitc)
+    if fitc and gp.fit.handle is None:
+        raise RuntimeError("the FITC fit failed (log_evidence = -Inf)")
     y = np.asarray(y, dtype=float).ravel()
     n = len(y)
     if isinstance(gp.approx, approx_ep):
@@ -1204,7 +1208,11 @@ def gp_loo(gp, x, y, quadrature=True, quad_order=11, draws=4000, seed=None, tria
         v = gp.fit.vector(L.VEC_CAVITY_VAR)
     else:
         fhat, z, V = gp.fit.fmean, gp.fit.z, gp.fit.V
-        post_var = gp_pred(gp, x, var=True)["var"]
+        if fitc:  # gp_pred_post.method_fitc with train = TRUE (R/gp_pred.R:270-285)
+            post_var = np.empty(n)
+            L.check(L.lib().gpl_fitc_train_var(gp.fit.handle, L.dptr(post_var)))
+        else:
+            post_var = gp_pred(gp, x, var=True)["var"]
         v = 1.0 / (1.0 / post_var - 1.0 / V)
         m = fhat - v * (z - fhat) / V
     sd = np.sqrt(v)

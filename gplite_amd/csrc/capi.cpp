@@ -882,7 +882,7 @@ static int fit_laplace_mixed(gpl_model* m) {
 
 // probabilists' Gauss-Hermite rule: nodes = eigenvalues of the Jacobi matrix of He_k
 // (Golub-Welsch), weights normalised to sum 1 -- the same rule as R/quadrature.R:28-34.
-static void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w) {
+void gpl::gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w) {
     // symmetric tridiagonal: diag 0, off-diagonal sqrt(k), k = 1..q-1; Jacobi eigen-iteration
     std::vector<double> A(q * q, 0.0), V(q * q, 0.0);
     for (int k = 1; k < q; ++k) A[k * q + k - 1] = A[(k - 1) * q + k] = std::sqrt((double)k);

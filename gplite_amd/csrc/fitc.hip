@@ -131,4 +131,36 @@ void add_diag_f64(hipStream_t s, double* C, int64_t ld, int64_t n, int64_t N, co
     hipLaunchKernelGGL(k_add_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, C, ld, n, N, v, jitter);
 }
 
+// EP posterior variance of FITC (R/ep.R:170-174 through the inversion lemma):
+// fvar = V [K_f C^-1]_ii = V (q/S + D (1/S - q/S^2)), q_i = |Lc^-1 u_i|^2
+__global__ void k_fitc_ep_var(int64_t n, const double* __restrict__ V, const double* __restrict__ S,
+                              const double* __restrict__ D, const double* __restrict__ q, double* __restrict__ fvar) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double si = S[i], qi = q[i];
+    fvar[i] = V[i] * (qi / si + D[i] * (1.0 / si - qi / (si * si)));
+}
+
+void fitc_ep_var(hipStream_t s, int64_t n, const double* V, const double* S, const double* D, const double* q,
+                 double* fvar) {
+    hipLaunchKernelGGL(k_fitc_ep_var, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, V, S, D, q, fvar);
+}
+
+// Training-input variance (R/gp_pred.R:270-285): prior - (t1 + 2 Dt q/S + Dt^2 (1/S - q/S^2)),
+// t1 = (A^T M A)_ii, the three terms diag1 + diag2 + diag3 of the reference
+__global__ void k_fitc_train_var(int64_t n, const double* __restrict__ prior, const double* __restrict__ t1,
+                                 const double* __restrict__ Dt, const double* __restrict__ q,
+                                 const double* __restrict__ S, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double si = S[i], qi = q[i], d = Dt[i];
+    out[i] = prior[i] - (t1[i] + 2.0 * d * qi / si + d * d * (1.0 / si - qi / (si * si)));
+}
+
+void fitc_train_var(hipStream_t s, int64_t n, const double* prior, const double* t1, const double* Dt,
+                    const double* q, const double* S, double* out) {
+    hipLaunchKernelGGL(k_fitc_train_var, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, prior, t1, Dt, q, S,
+                       out);
+}
+
 }  // namespace gpl

@@ -30,6 +30,7 @@ enum {
     F_H, F_G, F_DINVG, F_LDG, F_XG, F_MM, F_T1,  // H = K_zx S^-1 K_xz, G = Kz + H -> Lc, Lc^-1, M, scratch
     F_WZ,                                        // w = Lz^-1 K_zx alpha
     F_D, F_Z, F_V, F_SV, F_S, F_RS, F_ZS, F_ALPHA, F_FMEAN, F_FOLD, F_V1, F_V2, F_V3, F_V4,
+    F_FVAR, F_FVOLD, F_ZOLD, F_POLD, F_CAVM, F_CAVV, F_LOGZS, F_TERM, F_GH,  // EP state
     F_Q1, F_Q2, F_SCAL, F_INFO,
     F_COUNT
 };
@@ -70,8 +71,9 @@ static int fitc_alloc(gpl_fitc* f) {
     for (int i : {F_LDZ, F_LDG, F_WZ, F_Q1, F_Q2}) FENSURE(f, i, vM);
     for (int i : {F_U, F_UT, F_W}) FENSURE(f, i, nm);
     for (int i : {F_Y, F_TR, F_OFF, F_D, F_Z, F_V, F_SV, F_S, F_RS, F_ZS, F_ALPHA, F_FMEAN, F_FOLD, F_V1, F_V2, F_V3,
-                  F_V4})
+                  F_V4, F_FVAR, F_FVOLD, F_ZOLD, F_POLD, F_CAVM, F_CAVV, F_LOGZS, F_TERM})
         FENSURE(f, i, vN);
+    FENSURE(f, F_GH, 64 * 8);
     FENSURE(f, F_SCAL, 64 * 8);
     FENSURE(f, F_INFO, 64);
     return GPL_OK;
@@ -106,6 +108,143 @@ static void fitc_predictive_matrix(gpl_fitc* f) {
     gemm2_f64(s, 1, 0, Mz, Mz, Mz, -1.0, Y, Mz, Y, Mz, 1.0, f->d(F_T1), Mz, 0, 0);  // H - H G^-1 H
     gemm2_f64(s, 0, 0, Mz, Mz, Mz, 1.0, f->d(F_XZ), Mz, f->d(F_T1), Mz, 0.0, f->d(F_XG), Mz, 3, 0);
     gemm2_f64(s, 0, 1, Mz, Mz, Mz, 1.0, f->d(F_XG), Mz, f->d(F_XZ), Mz, 0.0, Y, Mz, 0, 0);
+}
+
+// One inversion-lemma solve for the current sites: S = D + V, G = Kz + K_zx S^-1 K_xz
+// and its factor, alpha = C^-1 z and the posterior mean K_f alpha into F_FMEAN
+// (R/laplace.R:134-141, R/ep.R:166-176).  flags[0] receives G's failing minor.
+static void fitc_lemma_step(gpl_fitc* f, int* flags) {
+    gpl_ctx* c = f->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = f->n, N = f->N, Mz = f->Mz;
+    fitc_site(s, n, N, f->d(F_D), f->d(F_V), f->d(F_Z), f->d(F_S), f->d(F_RS), f->d(F_ZS));
+    scale_rows_copy_f64(s, f->d(F_U), N, N, Mz, f->d(F_RS), f->d(F_W));
+    {
+        Prof p(c, "fitc_syrk", 2.0 * Mz * Mz * N, 8.0 * N * Mz);
+        gemm2_f64(s, 1, 0, Mz, Mz, N, 1.0, f->d(F_W), N, f->d(F_W), N, 0.0, f->d(F_H), Mz, 0, 0);
+    }
+    vec_axpby(s, Mz * Mz, 1.0, f->d(F_KZ), 1.0, f->d(F_H), f->d(F_G));
+    potrf_blocked(c, f->d(F_G), Mz, f->d(F_DINVG), f->d(F_LDG), flags);
+    Prof pv(c, "fitc_gemv", 0, 32.0 * N * Mz);
+    gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ZS), f->d(F_Q1));
+    trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q1), f->d(F_Q2), 0);
+    trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q2), f->d(F_Q1), 1);
+    gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V1));
+    fitc_alpha(s, n, N, f->d(F_ZS), f->d(F_V1), f->d(F_S), f->d(F_ALPHA));
+    gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ALPHA), f->d(F_Q1));
+    trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_Q2), 0);
+    trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q2), f->d(F_Q1), 1);
+    gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V2));
+    vec_mul(s, n, f->d(F_D), f->d(F_ALPHA), f->d(F_V3));
+    fill_f64(s, f->d(F_FMEAN), N, 0.0);
+    vec_axpby(s, n, 1.0, f->d(F_V2), 1.0, f->d(F_V3), f->d(F_FMEAN));
+}
+
+// q_i = |Lc^-1 u_i|^2 for the current G (into `out`), through Lc^-1 (F_XG) and F_W
+static void fitc_q(gpl_fitc* f, double* out) {
+    gpl_ctx* c = f->ctx;
+    hipStream_t s = c->stream;
+    const int64_t N = f->N, Mz = f->Mz;
+    trtri_blocked(c, f->d(F_G), f->d(F_DINVG), Mz, f->d(F_XG));
+    Prof p(c, "fitc_q", (double)Mz * Mz * N, 16.0 * N * Mz);
+    gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, f->d(F_XG), Mz, f->d(F_UT), Mz, 0.0, f->d(F_W), Mz, 3, 0);
+    colsumsq_f64(s, f->d(F_W), Mz, Mz, f->n, out, 0);
+}
+
+// Damped parallel EP over the FITC prior: ep.method_fitc (R/ep.R:332-370; its
+// tolerance is 1e-3) around the log-concave branch of ep_iter.method_fitc
+// (R/ep.R:148-185,271-278).
+static int fitc_ep_loop(gpl_fitc* f, const double* tr, const double* off) {
+    gpl_ctx* c = f->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = f->n, N = f->N, Mz = f->Mz;
+    const int q = f->opts.quad_order > 0 ? f->opts.quad_order : 11;
+    if (q > 32) return fail(GPL_ERR_ARG, "quad_order must be <= 32");
+    std::vector<double> gx, gw;
+    gauss_hermite(q, gx, gw);
+    HIP_TRY(hipMemcpyAsync(f->d(F_GH), gx.data(), q * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(f->d(F_GH) + 32, gw.data(), q * 8, hipMemcpyHostToDevice, s));
+    // P_old = 0, z_old = 0, fmean_old = 0, fvar_old = diag K + jitter (R/gp_fit.R:171, R/ep.R:339)
+    fill_f64(s, f->d(F_POLD), N, 0.0);
+    fill_f64(s, f->d(F_ZOLD), N, 0.0);
+    fill_f64(s, f->d(F_FOLD), N, 0.0);
+    fill_f64(s, f->d(F_V3), N, 0.0);
+    launch_build_diag(s, f->b[F_PROG].as<DevProg>(), f->prog, f->d(F_F), N, f->d(F_F), N, n, f->d(F_V2));
+    fitc_diag(s, n, N, f->d(F_V2), f->d(F_V3), f->opts.jitter, f->d(F_FVOLD));
+    const int maxiter = f->opts.maxiter > 0 ? f->opts.maxiter : 100;
+    const double tol = 1e-3, damping = f->opts.damping;
+    const double dmin = f->opts.damping_min > 0 ? f->opts.damping_min : 0.1;
+    double damp = damping, diff_old = INFINITY, diff_new = INFINITY;
+    bool quad_ok = true;
+    int* flags = f->b[F_INFO].as<int>();
+    int fl[2];
+    double sc[8];
+    int it;
+    for (it = 1; it <= maxiter; ++it) {
+        HIP_TRY(hipMemsetAsync(flags, 0, 8, s));
+        ep_sites(s, f->lik.kind, f->lik.link, n, f->d(F_FOLD), f->d(F_FVOLD), f->d(F_ZOLD), f->d(F_POLD), f->d(F_Y),
+                 tr, off, q, f->d(F_GH), f->d(F_GH) + 32, damp, f->d(F_Z), f->d(F_V), f->d(F_SV), f->d(F_LOGZS),
+                 f->d(F_CAVM), f->d(F_CAVV), flags + 1);
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (fl[1] & 2) {
+            f->logZ = -INFINITY;
+            f->iters = it;
+            return fail(GPL_ERR_NAN, "NaN in EP site parameters (R/ep.R:163-165 returns log_evidence = -Inf)");
+        }
+        if (fl[1] & 4) {
+            f->logZ = -INFINITY;
+            return fail(GPL_ERR_UNSUPPORTED, "EP site variance <= 0: R/ep.R:186-270 is not on the device path");
+        }
+        if (fl[1] & 1) quad_ok = false;
+        fitc_lemma_step(f, flags);
+        fitc_q(f, f->d(F_V1));
+        fitc_ep_var(s, n, f->d(F_V), f->d(F_S), f->d(F_D), f->d(F_V1), f->d(F_FVAR));
+        // per-site evidence terms (the binv argument only feeds an unused output here)
+        ep_post(s, n, f->d(F_V), f->d(F_Z), f->d(F_V1), f->d(F_CAVM), f->d(F_CAVV), f->d(F_LOGZS), f->d(F_V4),
+                f->d(F_TERM));
+        double* scd = f->d(F_SCAL);
+        reduce_sum(s, f->d(F_TERM), n, scd + 0);
+        reduce_dot(s, f->d(F_Z), f->d(F_ALPHA), n, scd + 1);
+        vec_log(s, n, f->d(F_V), f->d(F_V3));
+        reduce_sum(s, f->d(F_V3), n, scd + 2);
+        vec_log(s, n, f->d(F_S), f->d(F_V4));
+        reduce_sum(s, f->d(F_V4), n, scd + 3);
+        reduce_sum(s, f->d(F_LDZ), Mz, scd + 4);
+        reduce_sum(s, f->d(F_LDG), Mz, scd + 5);
+        reduce_maxabsdiff(s, f->d(F_FOLD), f->d(F_FMEAN), n, scd + 6);
+        int r = fitc_scalars(f, sc, 7, fl);
+        if (r) return r;
+        if (fl[0] > 0) {
+            set_minor(fl[0]);
+            return fail(GPL_ERR_NOT_PD, "inversion lemma: the leading minor of order " + std::to_string(fl[0]) +
+                                            " of Kz + K_zx S^-1 K_xz is not positive");
+        }
+        // log Z_EP (R/ep.R:179-181); the site terms carry -1/2 log V, added back here
+        const double logdet = sc[3] - 2.0 * sc[4] + 2.0 * sc[5];
+        f->logZ = sc[0] + 0.5 * sc[2] - 0.5 * logdet - 0.5 * sc[1];
+        diff_new = sc[6];
+        if (diff_new > diff_old) {
+            damp = damp * damping;
+            continue;
+        }
+        damp = std::max(damp, dmin);
+        HIP_TRY(hipMemcpyAsync(f->d(F_FOLD), f->d(F_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(f->d(F_FVOLD), f->d(F_FVAR), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(f->d(F_ZOLD), f->d(F_Z), N * 8, hipMemcpyDeviceToDevice, s));
+        vec_recip(s, n, f->d(F_V), f->d(F_POLD));
+        diff_old = diff_new;
+        if (diff_new < tol) break;
+    }
+    if (it > maxiter) it = maxiter;
+    f->iters = it;
+    if (!quad_ok) f->logZ = -INFINITY;
+    if (maxiter > 1 && it == maxiter) {  // R/ep.R:362-365
+        f->logZ = -INFINITY;
+        fail(GPL_ERR_NOCONV, "Maximum number of iterations in EP reached, max delta f = " + std::to_string(diff_new));
+        return GPL_ERR_NOCONV;
+    }
+    return GPL_OK;
 }
 
 static int fitc_run(gpl_fitc* f) {
@@ -147,6 +286,18 @@ static int fitc_run(gpl_fitc* f) {
 
     const double* tr = f->has_trials ? f->d(F_TR) : nullptr;
     const double* off = f->has_offset ? f->d(F_OFF) : nullptr;
+    if (f->opts.approx == GPL_APPROX_EP) {
+        r = fitc_ep_loop(f, tr, off);
+        if (r && r != GPL_ERR_NOCONV) return r;
+        gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ALPHA), f->d(F_Q1));
+        trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_WZ), 0);
+        {
+            Prof p(c, "fitc_pred_matrix", 0, 0);
+            fitc_predictive_matrix(f);
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        return r;
+    }
     const bool gauss = f->lik.kind == GPL_LIK_GAUSSIAN;
     const int maxiter = gauss ? 1 : (f->opts.maxiter > 0 ? f->opts.maxiter : 30);  // R/laplace.R:303-305
     double log_post_old = -INFINITY, diff = INFINITY;
@@ -161,29 +312,7 @@ static int fitc_run(gpl_fitc* f) {
             lik_pseudodata_la(s, f->lik.kind, f->lik.link, 0.0, n, f->d(F_FOLD), f->d(F_Y), tr, off, f->d(F_Z),
                               f->d(F_V), f->d(F_SV), flags + 1);
         // inversion lemma for C = K_xz Kz^-1 K_zx + S, S = D + V (R/linear_algebra.R:3-47)
-        fitc_site(s, n, N, f->d(F_D), f->d(F_V), f->d(F_Z), f->d(F_S), f->d(F_RS), f->d(F_ZS));
-        scale_rows_copy_f64(s, f->d(F_U), N, N, Mz, f->d(F_RS), f->d(F_W));
-        {
-            Prof p(c, "fitc_syrk", 2.0 * Mz * Mz * N, 8.0 * N * Mz);
-            gemm2_f64(s, 1, 0, Mz, Mz, N, 1.0, f->d(F_W), N, f->d(F_W), N, 0.0, f->d(F_H), Mz, 0, 0);
-        }
-        vec_axpby(s, Mz * Mz, 1.0, f->d(F_KZ), 1.0, f->d(F_H), f->d(F_G));
-        potrf_blocked(c, f->d(F_G), Mz, f->d(F_DINVG), f->d(F_LDG), flags);
-        // alpha = z/S - (K_xz/S) G^-1 (K_zx z/S)  (inv_lemma_solve, vector rhs)
-        Prof pv(c, "fitc_gemv", 0, 32.0 * N * Mz);
-        gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ZS), f->d(F_Q1));
-        trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q1), f->d(F_Q2), 0);
-        trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q2), f->d(F_Q1), 1);
-        gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V1));
-        fitc_alpha(s, n, N, f->d(F_ZS), f->d(F_V1), f->d(F_S), f->d(F_ALPHA));
-        // fhat = K_xz Kz^-1 (K_zx alpha) + D alpha  (R/laplace.R:141)
-        gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ALPHA), f->d(F_Q1));
-        trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_Q2), 0);
-        trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q2), f->d(F_Q1), 1);
-        gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V2));
-        vec_mul(s, n, f->d(F_D), f->d(F_ALPHA), f->d(F_V3));
-        fill_f64(s, f->d(F_FMEAN), N, 0.0);
-        vec_axpby(s, n, 1.0, f->d(F_V2), 1.0, f->d(F_V3), f->d(F_FMEAN));
+        fitc_lemma_step(f, flags);
         // log evidence (R/laplace.R:143-151): logdet C - log|V| from the lemma
         lik_loglik(s, f->lik.kind, f->lik.link, gauss ? f->lik.sigma : 0.0, n, f->d(F_FMEAN), f->d(F_Y), tr, off,
                    f->d(F_V3));
@@ -245,8 +374,8 @@ extern "C" int gpl_fitc_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms
     gpl_opts o;
     if (opts) o = *opts;
     else gpl_opts_default(&o);
-    if (o.approx != GPL_APPROX_LAPLACE)
-        return fail(GPL_ERR_UNSUPPORTED, "FITC on the device covers approx_laplace (EP: R/ep.R:148-280 is not built)");
+    if (lik->kind == GPL_LIK_GAUSSIAN && o.approx == GPL_APPROX_EP)
+        return fail(GPL_ERR_ARG, "approx_ep disabled for lik_gaussian");  // R/gp.R:54-56
     if (o.precision != GPL_PREC_FP64) return fail(GPL_ERR_UNSUPPORTED, "FITC runs in FP64");
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
     DevProg P;
@@ -401,6 +530,42 @@ extern "C" int gpl_fitc_draw(gpl_fitc* f, const double* xt, int64_t mt, double j
     return GPL_OK;
 }
 
+// Predictive variance at the training inputs with the FITC diagonal
+// (gp_pred_post.method_fitc, train = TRUE: R/gp_pred.R:270-285), the input of
+// loo_posteriors.approx_laplace (R/gp_loo.R:120-130).  With A = Lz^-1 K_zx,
+// Dt = diag K - colSums(A^2) and q_i = |Lc^-1 u_i|^2 the reference's three terms
+// are (A^T M A)_ii, 2 Dt_i q_i / S_i and Dt_i^2 [C^-1]_ii.
+extern "C" int gpl_fitc_train_var(gpl_fitc* f, double* out) {
+    if (!f || !out) return fail(GPL_ERR_ARG, "NULL argument");
+    gpl_ctx* c = f->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = f->n, N = f->N, Mz = f->Mz;
+    double* A = (double*)c->buf(22, (size_t)Mz * N * 8);
+    double* B = (double*)c->buf(23, (size_t)Mz * N * 8);
+    double* v = (double*)c->buf(24, (size_t)N * 6 * 8);
+    double* X = (double*)c->buf(25, (size_t)Mz * Mz * 8);
+    if (!A || !B || !v || !X) return fail(GPL_ERR_OOM, "device allocation failed");
+    double *c2 = v, *kd = v + N, *c1 = v + 2 * N, *t1 = v + 3 * N, *q = v + 4 * N, *res = v + 5 * N;
+    gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, f->d(F_XZ), Mz, f->d(F_UT), Mz, 0.0, A, Mz, 3, 0);  // A = Lz^-1 K_zx
+    colsumsq_f64(s, A, Mz, Mz, n, c2, 0);
+    launch_build_diag(s, f->b[F_PROG].as<DevProg>(), f->prog, f->d(F_F), N, f->d(F_F), N, n, kd);
+    vec_axpby(s, n, 1.0, kd, -1.0, c2, c2);  // Dt
+    gemm2_f64(s, 1, 0, Mz, N, Mz, 1.0, f->d(F_XZ), Mz, A, Mz, 0.0, B, Mz, 0, 0);  // Kz^-1 K_zx
+    coldot_f64(s, f->d(F_UT), B, Mz, Mz, n, c1);
+    vec_axpby(s, n, 1.0, c1, 1.0, c2, c1);  // prior_var
+    gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, f->d(F_MM), Mz, A, Mz, 0.0, B, Mz, 0, 0);  // M A
+    coldot_f64(s, A, B, Mz, Mz, n, t1);
+    trtri_blocked(c, f->d(F_G), f->d(F_DINVG), Mz, X);
+    gemm2_f64(s, 0, 0, Mz, N, Mz, 1.0, X, Mz, f->d(F_UT), Mz, 0.0, B, Mz, 3, 0);  // Lc^-1 K_zx
+    colsumsq_f64(s, B, Mz, Mz, n, q, 0);
+    fitc_train_var(s, n, c1, t1, c2, q, f->d(F_S), res);
+    HIP_TRY(hipMemcpyAsync(out, res, n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GPL_OK;
+}
+
 extern "C" int gpl_fitc_vector(gpl_fitc* f, int32_t which, double* out) {
     if (!f || !out) return fail(GPL_ERR_ARG, "NULL argument");
     int idx;
@@ -409,6 +574,9 @@ extern "C" int gpl_fitc_vector(gpl_fitc* f, int32_t which, double* out) {
         case GPL_VEC_ALPHA: idx = F_ALPHA; break;
         case GPL_VEC_Z: idx = F_Z; break;
         case GPL_VEC_V: idx = F_V; break;
+        case GPL_VEC_FVAR: idx = F_FVAR; break;
+        case GPL_VEC_CAVITY_MEAN: idx = F_CAVM; break;
+        case GPL_VEC_CAVITY_VAR: idx = F_CAVV; break;
         case 7: idx = F_D; break;
         default: return fail(GPL_ERR_ARG, "unknown FITC vector");
     }

@@ -180,6 +180,10 @@ void fitc_diag(hipStream_t s, int64_t n, int64_t N, const double* kd, const doub
 void vec_log(hipStream_t s, int64_t n, const double* a, double* out);
 // C_ii += v_i + jitter (i < n), padded diagonal set to 1
 void add_diag_f64(hipStream_t s, double* C, int64_t ld, int64_t n, int64_t N, const double* v, double jitter);
+void fitc_ep_var(hipStream_t s, int64_t n, const double* V, const double* S, const double* D, const double* q,
+                 double* fvar);
+void fitc_train_var(hipStream_t s, int64_t n, const double* prior, const double* t1, const double* Dt,
+                    const double* q, const double* S, double* out);
 // B = A^T (dimensions multiples of 64)
 void transpose_f64(hipStream_t s, const double* A, int64_t lda, int64_t nrow, int64_t ncol, double* B, int64_t ldb);
 

@@ -29,6 +29,8 @@ int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevPro
 // triangular inverse X = L^-1 (all tiles written)
 void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info);
 void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X);
+// probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
+void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
 // terms/nsel (optional): predict with the model's terms terms[0..nsel) only (cfind)
 int pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var,
             const int32_t* terms = nullptr, int32_t nsel = 0);

@@ -198,8 +198,9 @@ void gpl_model_free(gpl_model* model);
  * D = diag K(x, x) + jitter - diag Q (R/gp_fit.R:101-123).  Laplace Newton loop
  * for log-concave likelihoods on the inversion lemma (R/laplace.R:122-180,
  * 300-322, R/linear_algebra.R:3-172), one step for the Gaussian likelihood,
- * stopping when the unnormalised log posterior changes by less than opts->tol.
- * FP64 only; approx_ep returns GPL_ERR_UNSUPPORTED.  z: m x ncols inducing
+ * stopping when the unnormalised log posterior changes by less than opts->tol;
+ * or damped parallel EP (approx EP: R/ep.R:148-185,332-370, tolerance 1e-3 as
+ * there).  FP64 only.  z: m x ncols inducing
  * inputs prepared like x (column-major).  Status codes as gpl_fit. */
 typedef struct gpl_fitc gpl_fitc;
 int gpl_fitc_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n, int64_t ncols,
@@ -214,7 +215,12 @@ int gpl_fitc_pred(gpl_fitc* model, const double* xt, int64_t mt, int32_t want_va
  * A_t^T A_t + diag(Dt) - A_t^T M A_t (R/gp_pred.R:258-268); z and out as gpl_draw. */
 int gpl_fitc_draw(gpl_fitc* model, const double* xt, int64_t mt, double jitter, const double* z, int64_t draws,
                   double* out);
-/* fit vectors fmean, alpha, z, V (GPL_VEC_*), and the FITC diagonal D (which = 7) */
+/* Predictive variance at the training inputs with the FITC diagonal
+ * (gp_pred_post.method_fitc with train = TRUE, R/gp_pred.R:270-285), as the
+ * Laplace LOO uses it (R/gp_loo.R:120-130); out: n. */
+int gpl_fitc_train_var(gpl_fitc* model, double* out);
+/* fit vectors fmean, alpha, z, V, fvar and the EP cavity moments (GPL_VEC_*),
+ * and the FITC diagonal D (which = 7) */
 int gpl_fitc_vector(gpl_fitc* model, int32_t which, double* out);
 int gpl_fitc_info(gpl_fitc* model, int64_t* n, int64_t* m, int32_t* iters, double* log_evidence);
 void gpl_fitc_free(gpl_fitc* model);

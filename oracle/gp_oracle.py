@@ -40,7 +40,8 @@ __all__ = [
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
     "gp_energy_grad", "gp_pred", "gp_pred_cov", "gp_draw", "gp_loo", "gaussian_logz_closed_form",
-    "inv_lemma_get", "inv_lemma_solve", "fitc_fit", "fitc_pred", "fitc_pred_cov", "fitc_draw", "prior_draw",
+    "inv_lemma_get", "inv_lemma_solve", "fitc_fit", "fitc_ep_iter", "fitc_ep", "fitc_pred", "fitc_pred_cov",
+    "fitc_train_var", "fitc_draw", "prior_draw",
 ]
 
 JITTER = 1e-6  # R/gp.R:276-281 get_jitter default
@@ -859,7 +860,10 @@ def gp_loo(fit, lik, x, y, quadrature=True, quad_order=11, draws=4000, seed=0, t
     else:
         fhat, z, V = fit["fmean"], fit["z"], fit["V"]
         grad = (z - fhat) / V
-        _, post_var = gp_pred(fit, x, var=True)
+        if "inducing" in fit:  # method_fitc: train = TRUE
+            post_var = fitc_train_var(fit, x)
+        else:
+            _, post_var = gp_pred(fit, x, var=True)
         v = 1.0 / (1.0 / post_var - 1.0 / V)
         m = fhat - v * grad
     sd = np.sqrt(v)
@@ -899,19 +903,89 @@ def inv_lemma_get(K, U, D, logdet=True):
     return {"chol": L, "D": D, "inv_D_U": inv_D_U, "logdet": ld}
 
 
-def inv_lemma_solve(obj, rhs, lhs=None):
-    """Dense rhs (vector or matrix), optional dense lhs: R/linear_algebra.R:112-170
-    (the non-diagonal branches)."""
+def inv_lemma_solve(obj, rhs, lhs=None, rhs_diag=False, lhs_diag=False, diag_only=False):
+    """R/linear_algebra.R:49-172: lhs C^-1 rhs with C^-1 from inv_lemma_get; a
+    diagonal rhs or lhs is given as its vector, diag_only (both diagonal) returns
+    the diagonal of the product."""
     D, IDU, L = obj["D"], obj["inv_D_U"], obj["chol"]
     rhs = np.asarray(rhs, float)
+    if rhs_diag:
+        core = sla.cho_solve((L, True), IDU.T * rhs[None, :])  # diag_times_dense(t(inv_D_U), rhs, diag_right)
+        if lhs is None:
+            return np.diag(rhs / D) - IDU @ core
+        lhs = np.asarray(lhs, float)
+        if lhs_diag:
+            if diag_only:
+                return lhs / D * rhs - np.sum((lhs[:, None] * IDU).T * core, axis=0)
+            return np.diag(lhs / D * rhs) - (lhs[:, None] * IDU) @ core
+        return lhs * (rhs / D)[None, :] - (lhs @ IDU) @ core
     core = sla.cho_solve((L, True), IDU.T @ rhs)  # backsolve(t(L), forwardsolve(L, aux_right))
-    scaled = rhs / D[:, None] if rhs.ndim == 2 else rhs / D
     if lhs is None:
+        scaled = rhs / D[:, None] if rhs.ndim == 2 else rhs / D
         return scaled - IDU @ core
-    return lhs @ scaled - (lhs @ IDU) @ core
+    lhs = np.asarray(lhs, float)
+    if lhs_diag:
+        return (lhs / D)[:, None] * rhs - (lhs[:, None] * IDU) @ core
+    return lhs @ (rhs / D[:, None]) - (lhs @ IDU) @ core
 
 
-def fitc_fit(cfs, lik, x, y, z, maxiter=30, tol=1e-4, jitter=None, normalize_learn=True, **kw):
+def fitc_ep_iter(lik, Kz, Kz_chol, Kxz, D, y, fmean_old, fvar_old, z_old, P_old, quad_order=11, damping=0.9,
+                 **kw):
+    """ep_iter.method_fitc, log-concave branch: R/ep.R:148-185,271-278."""
+    z, V, logZ_site, cm, cv, ok = pseudodata_ep(lik, fmean_old, 1 / fvar_old, z_old, P_old, y,
+                                                quad_order, damping, **kw)
+    if np.any(np.isnan(V)) or np.any(np.isnan(z)):
+        return dict(fmean=fmean_old, log_evidence=-np.inf, nan=True)
+    if not np.all(V > 0):
+        raise NotImplementedError("non-log-concave EP branch (R/ep.R:186-270) is a host fallback")
+    S = D + V
+    C_inv = inv_lemma_get(Kz, Kxz, S)
+    alpha = inv_lemma_solve(C_inv, z)
+    inv_Kz_Kzx = sla.cho_solve((Kz_chol, True), Kxz.T)
+    aux = inv_lemma_solve(C_inv, V, inv_Kz_Kzx, rhs_diag=True)
+    diag1 = np.sum(Kxz.T * aux, axis=0)
+    diag2 = inv_lemma_solve(C_inv, V, D, rhs_diag=True, lhs_diag=True, diag_only=True)
+    fmean = Kxz @ sla.cho_solve((Kz_chol, True), Kxz.T @ alpha) + D * alpha
+    logZ = (-0.5 * C_inv["logdet"] - 0.5 * float(z @ alpha) + np.sum(logZ_site)
+            + 0.5 * np.sum(np.log(cv + V)) + 0.5 * np.sum((cm - z) ** 2 / (cv + V)))
+    return dict(fmean=fmean, fvar=diag1 + diag2, cavity_mean=cm, cavity_var=cv, z=z, V=V, P=1 / V, Kz=Kz,
+                Kxz=Kxz, Kz_chol=Kz_chol, C_inv=C_inv, D=D, alpha=alpha, log_evidence=float(logZ), quad_ok=ok)
+
+
+def fitc_ep(lik, Kz, Kz_chol, Kxz, K_diag, D, y, damping=0.9, damping_min=0.1, maxiter=100, tol=1e-3,
+            quad_order=11, **kw):
+    """ep.method_fitc: R/ep.R:332-370 (its tolerance defaults to 1e-3 and
+    fit_ep.method_fitc, R/gp_fit.R:163-187, passes none)."""
+    n = len(y)
+    P_old = np.zeros(n); z_old = np.zeros(n)
+    fmean_old = np.zeros(n); fvar_old = np.asarray(K_diag, float).copy()
+    diff_old = np.inf
+    damp = damping
+    it = 0
+    fit = None
+    diff_new = np.inf
+    for it in range(1, maxiter + 1):
+        fit = fitc_ep_iter(lik, Kz, Kz_chol, Kxz, D, y, fmean_old, fvar_old, z_old, P_old, quad_order, damp, **kw)
+        diff_new = float(np.max(np.abs(fmean_old - fit["fmean"])))
+        if diff_new > diff_old:
+            damp = damp * damping
+            continue
+        damp = max(damp, damping_min)
+        fmean_old = fit["fmean"]; fvar_old = fit["fvar"]; z_old = fit["z"]; P_old = fit["P"]
+        diff_old = diff_new
+        if diff_new < tol:
+            break
+    if maxiter > 1 and it == maxiter:
+        fit["log_evidence"] = -np.inf
+        warnings.warn(f"Maximum number of iterations in EP reached, max delta f = {diff_new}")
+    if fit.get("quad_ok") is False:
+        fit["log_evidence"] = -np.inf
+    fit["iters"] = it
+    return fit
+
+
+def fitc_fit(cfs, lik, x, y, z, maxiter=30, tol=1e-4, jitter=None, normalize_learn=True, approx="laplace",
+             damping=0.9, quad_order=11, **kw):
     """fit_laplace.method_fitc (R/gp_fit.R:101-123) for given inducing inputs z,
     Newton loop of laplace.method_fitc (R/laplace.R:300-322, stopping on the
     change of the unnormalised log posterior) over the log-concave branch of
@@ -930,6 +1004,14 @@ def fitc_fit(cfs, lik, x, y, z, maxiter=30, tol=1e-4, jitter=None, normalize_lea
     Kz_chol = sla.cholesky(Kz, lower=True)
     D = eval_cf(cfs, x, x, diag_only=True) + jitter
     D = D - np.sum(sla.solve_triangular(Kz_chol, Kxz.T, lower=True) ** 2, axis=0)
+    if approx == "ep":  # fit_ep.method_fitc, R/gp_fit.R:163-187
+        if lik["kind"] == "gaussian":
+            raise ValueError("approx_ep disabled for lik_gaussian")
+        K_diag = eval_cf(cfs, x, x, diag_only=True) + jitter
+        fit = fitc_ep(lik, Kz, Kz_chol, Kxz, K_diag, D, y, damping=damping, damping_min=0.1,
+                      maxiter=100 if maxiter is None else maxiter, quad_order=quad_order, **kw)
+        fit.update(cfs=cfs, lik=lik, x=x, inducing=z, jitter=jitter, approx="ep")
+        return fit
     fhat = np.zeros(n)
     if lik["kind"] == "gaussian":
         maxiter = 1
@@ -975,6 +1057,26 @@ def fitc_pred(fit, xt, var=False):
     prior_var = np.sum(Ktz * Kz_inv_Ktz.T, axis=1) + Dt
     aux = inv_lemma_solve(fit["C_inv"], aux1.T, aux1) @ aux2
     return mean, prior_var - np.sum(aux2 * aux, axis=0)
+
+
+def fitc_train_var(fit, x):
+    """Predictive variance at the training inputs with the FITC diagonal
+    (gp_pred_post.method_fitc with train = TRUE, R/gp_pred.R:270-285), as
+    loo_posteriors.approx_laplace uses it (R/gp_loo.R:120-130)."""
+    cfs, z = fit["cfs"], fit["inducing"]
+    x = np.atleast_2d(np.asarray(x, float))
+    Ktz = eval_cf(cfs, x, z)
+    Kz_chol, Kxz, C_inv = fit["Kz_chol"], fit["Kxz"], fit["C_inv"]
+    aux1 = sla.solve_triangular(Kz_chol, Kxz.T, lower=True)
+    aux2 = sla.solve_triangular(Kz_chol, Ktz.T, lower=True)
+    Dt = eval_cf(cfs, x, x, diag_only=True) - np.sum(aux2 ** 2, axis=0)
+    Kz_inv_Ktz = sla.cho_solve((Kz_chol, True), Ktz.T)
+    prior_var = np.sum(Ktz * Kz_inv_Ktz.T, axis=1) + Dt
+    aux = inv_lemma_solve(C_inv, aux1.T, aux1) @ aux2 + inv_lemma_solve(C_inv, Dt, aux1, rhs_diag=True)
+    diag1 = np.sum(aux2 * aux, axis=0)
+    diag2 = np.sum(inv_lemma_solve(C_inv, aux1.T, Dt, lhs_diag=True) * aux2.T, axis=1)
+    diag3 = inv_lemma_solve(C_inv, Dt, Dt, rhs_diag=True, lhs_diag=True, diag_only=True)
+    return prior_var - (diag1 + diag2 + diag3)
 
 
 def fitc_pred_cov(fit, xt, jitter=None):

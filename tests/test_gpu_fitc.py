@@ -95,3 +95,44 @@ def test_fitc_failed_factor_gives_minus_inf(gpl, capsys):
     gf = G.gp_fit(gp, x, y, jitter=-1.0)  # Kz = K_zz - I is indefinite
     assert gf.fit.log_evidence == -np.inf
     assert "not positive" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("kind,n,m", [("logit", 120, 15), ("poisson", 140, 30), ("binomial", 100, 20)])
+def test_fitc_ep_matches_oracle(gpl, kind, n, m):
+    """FITC with approx_ep (R/ep.R:148-185,332-370): log evidence, posterior mean and
+    variance, cavity moments and the step count against the oracle at 1e-8."""
+    G = gpl
+    x, y, z, kw = _data(kind, n, m, 300 + n)
+    lk = _lik(G, kind)
+    gp = G.gp_init(G.cf_sexp(lscale=[0.9, 1.2], magn=1.1), lk, method=G.method_fitc(inducing=z),
+                   approx=G.approx_ep(damping=0.8))
+    gf = G.gp_fit(gp, x, y, **kw)
+    fo = O.fitc_fit([c.spec() for c in gf.cfs], lk.spec(), x, y, z, approx="ep", damping=0.8, **kw)
+    assert gf.fit.iters == fo["iters"]
+    assert abs(gf.fit.log_evidence - fo["log_evidence"]) <= 1e-8 * abs(fo["log_evidence"])
+    for nm, ref in (("fmean", fo["fmean"]), ("fvar", fo["fvar"]), ("cavity_mean", fo["cavity_mean"]),
+                    ("cavity_var", fo["cavity_var"]), ("alpha", fo["alpha"])):
+        assert _rel(getattr(gf.fit, nm), ref) < 1e-8, nm
+    xt = np.random.default_rng(6).normal(size=(40, 2))
+    pr = G.gp_pred(gf, xt, var=True)
+    mo, vo = O.fitc_pred(fo, xt, var=True)
+    assert _rel(pr["mean"], mo) < 1e-8 and _rel(pr["var"], vo) < 1e-8
+
+
+@pytest.mark.parametrize("kind,approx", [("gaussian", "laplace"), ("logit", "laplace"), ("poisson", "laplace"),
+                                         ("logit", "ep"), ("poisson", "ep")])
+def test_fitc_loo_matches_oracle(gpl, kind, approx):
+    """gp_loo for method_fitc (R/gp_loo.R:73-134): the Laplace LOO moments use the
+    training-input variance with the FITC diagonal (train = TRUE), EP the cavities."""
+    G = gpl
+    x, y, z, kw = _data(kind, 130, 25, 500)
+    lk = _lik(G, kind)
+    ap = G.approx_ep() if approx == "ep" else G.approx_laplace(maxiter=50, tol=1e-8)
+    gp = G.gp_init(G.cf_matern32(lscale=1.0, magn=1.0), lk, method=G.method_fitc(inducing=z), approx=ap)
+    gf = G.gp_fit(gp, x, y, **kw)
+    okw = dict(approx="ep") if approx == "ep" else dict(maxiter=50, tol=1e-8)
+    fo = O.fitc_fit([c.spec() for c in gf.cfs], lk.spec(), x, y, z, **okw, **kw)
+    ld = G.gp_loo(gf, x, y, quad_order=11, **kw)
+    lo = O.gp_loo(fo, lk.spec(), x, y, quad_order=11, **kw)
+    assert _rel(ld["loos"], lo["loos"]) < 1e-8
+    assert ld["loo"] == pytest.approx(lo["loo"], rel=1e-8)

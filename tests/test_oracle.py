@@ -489,6 +489,56 @@ def test_oracle_fitc_laplace_mode(kind):
     assert abs(fit["log_evidence"] - ref) < 1e-5 * abs(ref)
 
 
+def test_oracle_fitc_train_var_closed_form():
+    """Training-input variance of FITC (train = TRUE, R/gp_pred.R:270-285): with
+    K_t = Q + diag(Dt), Dt = diag K - diag Q (no jitter) and C = Q + diag(D) + sigma^2 I,
+    it is diag(K_t - K_t C^-1 K_t) from dense solves."""
+    x, y, z, cf, lik = _fitc_case(n=70, m=15)
+    fit = O.fitc_fit(cf, lik, x, y, z)
+    c2 = fit["cfs"]
+    n, m, jit = len(y), z.shape[0], 1e-6
+    Kz = O.eval_cf(c2, z, z) + jit * np.eye(m)
+    Kxz = O.eval_cf(c2, x, z)
+    Q = Kxz @ np.linalg.solve(Kz, Kxz.T)
+    kd = O.eval_cf(c2, x, x, diag_only=True)
+    Kt = Q + np.diag(kd - np.diag(Q))
+    C = Q + np.diag(kd + jit - np.diag(Q) + 0.09)
+    ref = np.diag(Kt - Kt @ np.linalg.solve(C, Kt))
+    np.testing.assert_allclose(O.fitc_train_var(fit, x), ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("kind", ["bernoulli", "poisson"])
+def test_oracle_fitc_ep_fixed_point(kind):
+    """FITC EP (R/ep.R:148-185,332-370): with K_f = Q + diag(D) and the final sites
+    (z, V), the posterior mean and variance are K_f (K_f + V)^-1 z and
+    diag(K_f - K_f (K_f + V)^-1 K_f), and the log evidence is
+    -1/2 log|K_f + V| - 1/2 z^T (K_f + V)^-1 z + sum log Z_i
+    + 1/2 sum log(v_cav + V) + 1/2 sum (m_cav - z)^2 / (v_cav + V), all from dense
+    solves; at convergence the tilted means match the posterior means (the EP
+    fixed point) to the loop tolerance."""
+    x, y, z, cf, lik = _fitc_case(kind=kind, n=60, m=10)
+    fit = O.fitc_fit(cf, lik, x, y, z, approx="ep", maxiter=200)
+    c2 = fit["cfs"]
+    m, jit = z.shape[0], 1e-6
+    Kz = O.eval_cf(c2, z, z) + jit * np.eye(m)
+    Kxz = O.eval_cf(c2, x, z)
+    Q = Kxz @ np.linalg.solve(Kz, Kxz.T)
+    Kf = Q + np.diag(O.eval_cf(c2, x, x, diag_only=True) + jit - np.diag(Q))
+    zz, V = fit["z"], fit["V"]
+    Cm = Kf + np.diag(V)
+    np.testing.assert_allclose(fit["fmean"], Kf @ np.linalg.solve(Cm, zz), atol=1e-9)
+    np.testing.assert_allclose(fit["fvar"], np.diag(Kf - Kf @ np.linalg.solve(Cm, Kf)), atol=1e-9)
+    cm, cv = fit["cavity_mean"], fit["cavity_var"]
+    zero = np.zeros(len(y))
+    mt, _, logZ_site, _, _, _ = O.tilted_moments(lik, cm, 1 / cv, zero, zero, y)  # moments of lik x N(cm, cv)
+    sign, ld = np.linalg.slogdet(Cm)
+    assert sign > 0
+    ref = (-0.5 * ld - 0.5 * zz @ np.linalg.solve(Cm, zz) + np.sum(logZ_site) + 0.5 * np.sum(np.log(cv + V))
+           + 0.5 * np.sum((cm - zz) ** 2 / (cv + V)))
+    assert abs(fit["log_evidence"] - ref) < 1e-9 * abs(ref)
+    np.testing.assert_allclose(mt, fit["fmean"], atol=5e-3)
+
+
 def test_oracle_loo_gaussian_closed_form():
     """Gaussian likelihood.
     * The LOO latent moments of R/gp_loo.R:120-130 must give the closed form
