@@ -258,8 +258,9 @@ class cf_prod(Cf):
     def __init__(self, *cfs):
         super().__init__()
         self.cfs = list(cfs)
-        nfree = sum(1 for c in self.cfs if "magn" in c.param_names and not _fixed(c.priors.get("magn")))
-        if nfree > 1:  # R/misc.R:58-81
+        # free parameters named magn, the base kernel of cf_periodic included (R/misc.R:58-81)
+        nfree = sum(1 for c in self.cfs for (_o, nm, _i) in c.slots() if nm == "magn")
+        if nfree > 1:
             warnings.warn("Detected a product of covariance functions with more than one non-fixed magnitude "
                           "parameters. Consider setting prior_magn = prior_fixed() for all except one of the "
                           "kernels in the product.")

@@ -111,3 +111,44 @@ def test_golden_c1_reproduced_by_oracle():
     new = build()
     for k in ref.files:
         np.testing.assert_allclose(new[k], ref[k], rtol=1e-12, atol=1e-14, err_msg=k)
+
+
+def test_product_magnitude_warnings():
+    """tests/testthat/test_warnings.R: a product of two covariance functions with
+    non-fixed magnitudes warns (cf_prod and *, both orders); with one magnitude
+    fixed it is silent (R/misc.R:58-81)."""
+    import itertools
+    import warnings as W
+
+    free = [lambda: G.cf_const(), lambda: G.cf_lin(), lambda: G.cf_sexp(), lambda: G.cf_matern32(),
+            lambda: G.cf_matern52(), lambda: G.cf_nn(), lambda: G.cf_periodic()]
+    fixed = [lambda: G.cf_const(prior_magn=G.prior_fixed()), lambda: G.cf_lin(prior_magn=G.prior_fixed()),
+             lambda: G.cf_sexp(prior_magn=G.prior_fixed()), lambda: G.cf_matern32(prior_magn=G.prior_fixed()),
+             lambda: G.cf_matern52(prior_magn=G.prior_fixed()), lambda: G.cf_nn(prior_magn=G.prior_fixed()),
+             lambda: G.cf_periodic(cf_base=G.cf_sexp(prior_magn=G.prior_fixed()))]
+    for a, b in itertools.combinations(range(len(free)), 2):
+        for make in (lambda p, q: G.cf_prod(p, q), lambda p, q: p * q):
+            for p, q in ((free[a](), free[b]()), (free[b](), free[a]())):
+                with pytest.warns(UserWarning, match="more than one non-fixed magnitude"):
+                    make(p, q)
+    for i in range(len(free)):
+        for j in range(len(fixed)):
+            for make in (lambda p, q: G.cf_prod(p, q), lambda p, q: p * q):
+                with W.catch_warnings():
+                    W.simplefilter("error")
+                    make(free[i](), fixed[j]())
+                    make(fixed[j](), free[i]())
+            # products of three: adding a fixed magnitude stays silent, a free one warns
+            for k in range(len(free)):
+                with W.catch_warnings():
+                    W.simplefilter("error")
+                    tmp = free[i]() * fixed[j]()
+                    G.cf_prod(tmp, fixed[k]())
+                    G.cf_prod(fixed[k](), tmp)
+                    tmp * fixed[j]()
+                    fixed[j]() * tmp
+                for make in (lambda p, q: G.cf_prod(p, q), lambda p, q: p * q):
+                    with pytest.warns(UserWarning, match="more than one non-fixed magnitude"):
+                        make(tmp, free[k]())
+                    with pytest.warns(UserWarning, match="more than one non-fixed magnitude"):
+                        make(free[k](), tmp)
