@@ -619,14 +619,21 @@ _KIND = {"const": L.CF_CONST, "lin": L.CF_LIN, "sexp": L.CF_SEXP, "matern32": L.
          "matern52": L.CF_MATERN52, "nn": L.CF_NN, "periodic": L.CF_PERIODIC}
 
 
+def _prod_factors(cf):
+    """The factors of a product, nested products expanded in order (a product of
+    products is one product: .cf_prod, R/cf.R:491-497)."""
+    out = []
+    for c in cf.cfs:
+        out.extend(_prod_factors(c) if isinstance(c, cf_prod) else [c])
+    return out
+
+
 def _flatten(cfs):
     """[(cf, op)] with products flattened; op = SUM starts a summand."""
     out = []
     for cf in cfs:
         if isinstance(cf, cf_prod):
-            for k, c in enumerate(cf.cfs):
-                if isinstance(c, cf_prod):
-                    raise ValueError("nested cf_prod")
+            for k, c in enumerate(_prod_factors(cf)):
                 out.append((c, L.OP_SUM if k == 0 else L.OP_PROD))
         else:
             out.append((cf, L.OP_SUM))
@@ -691,7 +698,7 @@ def _cfind_terms(cfs, cfind):
     """The device term indices (products flattened, as _flatten orders them) of cfind."""
     groups, k = [], 0
     for cf in cfs:
-        nt = len(cf.cfs) if isinstance(cf, cf_prod) else 1
+        nt = len(_prod_factors(cf)) if isinstance(cf, cf_prod) else 1
         groups.append(list(range(k, k + nt)))
         k += nt
     idx = [t for i in _cfind_list(cfs, cfind) for t in groups[i]]

@@ -145,6 +145,9 @@ GRAD_CASES = [
     lambda G: [G.cf_lin(magn=0.6), G.cf_sexp(lscale=1.0)],
     lambda G: [G.cf_periodic(vars=[0], period=2.0, cf_base=G.cf_sexp(lscale=0.8))],
     lambda G: [G.cf_prod(G.cf_sexp(lscale=0.9), G.cf_matern32(lscale=[1.1, 0.6], magn=0.7, prior_magn=G.prior_fixed()))],
+    # a product of a product (R/cf.R:189-215 allow nesting; it is one product)
+    lambda G: [G.cf_prod(G.cf_prod(G.cf_sexp(vars=[0], lscale=0.9), G.cf_lin(vars=[1], prior_magn=G.prior_fixed())),
+                         G.cf_matern52(lscale=1.2, prior_magn=G.prior_fixed())), G.cf_const(magn=0.3)],
 ]
 
 
