@@ -221,6 +221,28 @@ void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, d
     else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
+// `batch` independent FP64 GEMMs (strides in elements), one grid row per problem
+void gemm2_f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                       int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles,
+                       int batch, int64_t sA, int64_t sB, int64_t sC) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+}
+
+// out[i] = sum over p < parts of W[p * stride + i], in order of p (deterministic)
+__global__ void k_sum_parts(const double* __restrict__ W, int64_t stride, int parts, int64_t n,
+                            double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = W[i];
+    for (int p = 1; p < parts; ++p) acc += W[p * stride + i];
+    out[i] = acc;
+}
+
+void sum_parts_f64(hipStream_t s, const double* W, int64_t stride, int parts, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, W, stride, parts, n, out);
+}
+
 // FP64-compute GEMM on FP32 storage for A/C with an FP64 B (panel solves of the mixed path)
 void gemm_mixed_f32f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
                        int64_t lda, const double* B, int64_t ldb, double beta, float* C, int64_t ldc, int tiles) {

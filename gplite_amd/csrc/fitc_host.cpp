@@ -31,7 +31,7 @@ enum {
     F_WZ,                                        // w = Lz^-1 K_zx alpha
     F_D, F_Z, F_V, F_SV, F_S, F_RS, F_ZS, F_ALPHA, F_FMEAN, F_FOLD, F_V1, F_V2, F_V3, F_V4,
     F_FVAR, F_FVOLD, F_ZOLD, F_POLD, F_CAVM, F_CAVV, F_LOGZS, F_TERM, F_GH,  // EP state
-    F_Q1, F_Q2, F_SCAL, F_INFO,
+    F_Q1, F_Q2, F_SCAL, F_INFO, F_SPLIT,
     F_COUNT
 };
 
@@ -58,6 +58,16 @@ struct gpl_fitc {
         if (!(fm)->b[idx].ensure(bytes)) return fail(GPL_ERR_OOM, "device allocation failed"); \
     } while (0)
 
+// K-split of the K_zx S^-1 K_xz product: with few m x m output tiles and a long
+// reduction over n, P partial products (rows split in P ranges) fill the chip and
+// are summed in a fixed order.
+static int fitc_split(const gpl_fitc* f) {
+    const int64_t tiles = (f->Mz / 64) * (f->Mz / 64);
+    int P = 1;
+    while (P < 16 && tiles * P < 1024 && (f->N / 64) % (2 * P) == 0 && f->N / (2 * P) >= 512) P *= 2;
+    return P;
+}
+
 static int fitc_alloc(gpl_fitc* f) {
     const int64_t N = f->N, Mz = f->Mz;
     const size_t vN = (size_t)N * 8, vM = (size_t)Mz * 8, mm = (size_t)Mz * Mz * 8, nm = (size_t)N * Mz * 8;
@@ -76,6 +86,8 @@ static int fitc_alloc(gpl_fitc* f) {
     FENSURE(f, F_GH, 64 * 8);
     FENSURE(f, F_SCAL, 64 * 8);
     FENSURE(f, F_INFO, 64);
+    const int P = fitc_split(f);
+    if (P > 1) FENSURE(f, F_SPLIT, (size_t)P * Mz * Mz * 8);
     return GPL_OK;
 }
 
@@ -121,7 +133,15 @@ static void fitc_lemma_step(gpl_fitc* f, int* flags) {
     scale_rows_copy_f64(s, f->d(F_U), N, N, Mz, f->d(F_RS), f->d(F_W));
     {
         Prof p(c, "fitc_syrk", 2.0 * Mz * Mz * N, 8.0 * N * Mz);
-        gemm2_f64(s, 1, 0, Mz, Mz, N, 1.0, f->d(F_W), N, f->d(F_W), N, 0.0, f->d(F_H), Mz, 0, 0);
+        const int P = fitc_split(f);
+        if (P == 1) {
+            gemm2_f64(s, 1, 0, Mz, Mz, N, 1.0, f->d(F_W), N, f->d(F_W), N, 0.0, f->d(F_H), Mz, 0, 0);
+        } else {
+            const int64_t Kp = N / P;
+            gemm2_f64_batched(s, 1, 0, Mz, Mz, Kp, 1.0, f->d(F_W), N, f->d(F_W), N, 0.0, f->d(F_SPLIT), Mz, 0, P, Kp,
+                              Kp, Mz * Mz);
+            sum_parts_f64(s, f->d(F_SPLIT), Mz * Mz, P, Mz * Mz, f->d(F_H));
+        }
     }
     vec_axpby(s, Mz * Mz, 1.0, f->d(F_KZ), 1.0, f->d(F_H), f->d(F_G));
     potrf_blocked(c, f->d(F_G), Mz, f->d(F_DINVG), f->d(F_LDG), flags);

@@ -81,6 +81,11 @@ void launch_grad_tiles_f32(hipStream_t s, const DevProg* prog, const DevProg& hp
 // bm: output tile edge (64 or 128; 0 = choose by grid size).
 void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles, int bm);
+void gemm2_f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                       int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles,
+                       int batch, int64_t sA, int64_t sB, int64_t sC);
+// out[i] = sum_p W[p * stride + i] for i < n (fixed order)
+void sum_parts_f64(hipStream_t s, const double* W, int64_t stride, int parts, int64_t n, double* out);
 // FP64 MFMA on FP32 storage: C(f32) = beta*C + alpha op(A(f32)) op(B(f64)) and the mirror case.
 void gemm_mixed_f32f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
                        int64_t lda, const double* B, int64_t ldb, double beta, float* C, int64_t ldc, int tiles);

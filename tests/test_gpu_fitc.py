@@ -47,7 +47,9 @@ def _lik(G, kind):
 
 
 @pytest.mark.parametrize("kind,n,m", [("gaussian", 1, 1), ("gaussian", 150, 20), ("gaussian", 333, 70),
-                                      ("logit", 200, 30), ("binomial", 150, 25), ("poisson", 130, 65)])
+                                      ("logit", 200, 30), ("binomial", 150, 25), ("poisson", 130, 65),
+                                      # long reductions: the K_zx S^-1 K_xz product is split over n (4 and 2 parts)
+                                      ("gaussian", 4096, 64), ("logit", 2048, 40)])
 def test_fitc_matches_oracle(gpl, kind, n, m):
     G = gpl
     x, y, z, kw = _data(kind, n, m, 100 + n + m)
