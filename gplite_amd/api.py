@@ -1237,6 +1237,27 @@ def gp_loo(gp, x, y, quadrature=True, quad_order=11, draws=4000, seed=None, tria
     return {"loo": float(np.sum(loos)), "sd": float(n * np.std(loos, ddof=1) / math.sqrt(n)), "loos": loos}
 
 
+def gp_compare(*loos, ref=None):
+    """R/gp_loo.R:147-175: compare gp_loo results.  Row i holds
+    n * mean(loos_i - loos_ref) and n * sd(loos_i - loos_ref) / sqrt(n); ref
+    (0-based) defaults to the model with the largest loo.  Returns a (k, 2)
+    array with columns loo-diff and se."""
+    for a in loos:
+        if not (isinstance(a, dict) and "loo" in a and "loos" in a):
+            raise TypeError(f"Expected objects of type 'loores', but found an object with type '{type(a).__name__}'")
+    loo = np.array([a["loo"] for a in loos], dtype=float)
+    if ref is None:
+        ref = int(np.argmax(loo))
+    base = np.asarray(loos[ref]["loos"], dtype=float)
+    n = len(base)
+    res = np.empty((len(loos), 2))
+    for i, a in enumerate(loos):
+        d = np.asarray(a["loos"], dtype=float) - base
+        res[i, 0] = n * d.mean()
+        res[i, 1] = n * (d.std(ddof=1) / math.sqrt(n)) if n > 1 else float("nan")
+    return res
+
+
 def _gh(q):
     x, w = np.polynomial.hermite_e.hermegauss(q)
     return x, w / w.sum()

@@ -152,3 +152,23 @@ def test_product_magnitude_warnings():
                         make(tmp, free[k]())
                     with pytest.warns(UserWarning, match="more than one non-fixed magnitude"):
                         make(free[k](), tmp)
+
+
+def test_gp_compare():
+    """R/gp_loo.R:147-175: differences against the best model (or ref), n * mean and
+    n * sd / sqrt(n) of the pointwise differences; other objects are rejected."""
+    r = np.random.default_rng(3)
+    a = r.normal(size=20) - 1.0
+    b = a + r.normal(scale=0.1, size=20) - 0.05
+    la = {"loo": float(a.sum()), "loos": a}
+    lb = {"loo": float(b.sum()), "loos": b}
+    res = G.gp_compare(la, lb)
+    best = 0 if a.sum() > b.sum() else 1
+    d = [a, b][1 - best] - [a, b][best]
+    assert np.allclose(res[best], [0.0, 0.0])
+    assert np.isclose(res[1 - best, 0], d.sum())
+    assert np.isclose(res[1 - best, 1], 20 * d.std(ddof=1) / np.sqrt(20))
+    res1 = G.gp_compare(la, lb, ref=1)
+    assert np.isclose(res1[0, 0], (a - b).sum())
+    with pytest.raises(TypeError):
+        G.gp_compare(la, [1, 2])
