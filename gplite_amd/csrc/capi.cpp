@@ -1060,8 +1060,9 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     int r = prepare_prog(terms, nterms, ncols, P);
     if (r) return r;
     gpl_model* m = *inout;
+    if (m && m->ctx != c) return fail(GPL_ERR_ARG, "the model to refit belongs to another context");
     bool fresh = false;
-    if (!m || m->ctx != c) {
+    if (!m) {
         m = new gpl_model();
         m->ctx = c;
         fresh = true;

@@ -149,9 +149,10 @@ int gpl_eval_cf(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms,
  * K = eval_cf(x,x) + jitter*I on the device, runs the Laplace Newton loop
  * (R/laplace.R:274-297; one step for a Gaussian likelihood) or parallel EP
  * (R/ep.R:286-330), and leaves the factor of B = I + S K S, alpha and the
- * posterior vectors resident in *inout.  If *inout is an existing model of
- * the same n it is reused (its buffers are recycled), otherwise a new one is
- * allocated.  trials/offset may be NULL. */
+ * posterior vectors resident in *inout.  If *inout is an existing model it is
+ * refitted in place (its buffers are recycled when large enough); it must belong
+ * to ctx (GPL_ERR_ARG otherwise).  NULL allocates a new model.  trials/offset
+ * may be NULL. */
 int gpl_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms,
             const double* x, int64_t n, int64_t ncols, const double* y,
             const double* trials, const double* offset, const gpl_lik* lik,
