@@ -223,7 +223,7 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
 enum {
     B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
     B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
-    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC, B_B64,
+    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_W4, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC, B_B64,
     B_COUNT
 };
 
@@ -522,7 +522,7 @@ static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool
     if (need_k) ENSURE(m, B_K, (size_t)N * N * 8);  // K stays FP64 in both forms
     ENSURE(m, B_PROG, sizeof(DevProg));
     for (int i : {B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM, B_FOLD, B_FVOLD, B_ZOLD,
-                  B_POLD, B_W1, B_W2, B_W3})
+                  B_POLD, B_W1, B_W2, B_W3, B_W4})
         ENSURE(m, i, vec);
     ENSURE(m, B_SCAL, 256 * 8);
     ENSURE(m, B_KMV, (size_t)N * kmatvec_chunks(N) * 8);
@@ -765,6 +765,50 @@ static int fit_laplace(gpl_model* m) {
     return GPL_OK;
 }
 
+// alpha = S u with B u = S z by iterative refinement (mixed form): corrections
+// from the FP32 inverse X (B^-1 ~ X^T X), residuals against the FP64 copy of B.
+// Stops when the correction is below 1e-14 relative or after maxref sweeps.
+static int refine_alpha_mixed(gpl_model* m, int maxref, int nch) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const float* X = m->b[B_XINV].as<float>();
+    const double* B64 = m->d(B_B64);
+    double* rhs = m->d(B_TERM);
+    double* u = m->d(B_W1);
+    double* r = m->d(B_W2);
+    double* d = m->d(B_W3);
+    double* t1 = m->d(B_W4);
+    fill_f64(s, rhs, N, 0.0);
+    vec_mul(s, n, m->d(B_SV), m->d(B_Z), rhs);
+    fill_f64(s, u, N, 0.0);
+    HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
+    double rel = INFINITY;
+    for (int k = 1; k <= maxref; ++k) {
+        {
+            Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
+            trmv_lower_f32(s, X, N, N, r, t1, 0, m->d(B_KMV), nch);
+            trmv_lower_f32(s, X, N, N, t1, d, 1, m->d(B_KMV), nch);
+        }
+        vec_axpby(s, n, 1.0, d, 1.0, u, u);
+        {
+            Prof p(c, "symv_b64", 2.0 * N * N, 8.0 * N * (N + 64) / 2);
+            symv_lt_f64(s, B64, N, u, t1, m->d(B_KMV), nch, 1, 0, 0);
+        }
+        vec_axpby(s, n, 1.0, rhs, -1.0, t1, r);
+        rel_maxabs(s, d, u, n, m->d(B_SCAL) + 8);
+        double h;
+        HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        rel = h;
+        if (rel < 1e-14) break;
+    }
+    m->refine_rel = rel;
+    fill_f64(s, m->d(B_ALPHA), N, 0.0);
+    vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
+    return GPL_OK;
+}
+
 // Laplace Newton loop in the mixed-precision form: per step B = I + S K S is
 // formed from the FP64 K in FP64 (kept for residuals) and FP32 (factored with
 // FP64 diagonal tiles), X = L^-1 in FP32, and the Newton step's alpha =
@@ -793,11 +837,6 @@ static int fit_laplace_mixed(gpl_model* m) {
     fill_f64(s, m->d(B_FOLD), N, 0.0);  // fhat = 0
     const int maxiter = m->opts.maxiter > 0 ? m->opts.maxiter : 30;
     const int maxref = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
-    double* rhs = m->d(B_TERM);
-    double* u = m->d(B_W1);
-    double* r = m->d(B_W2);
-    double* d = m->d(B_W3);
-    double* t1 = m->d(B_CAVM);
     double diff = INFINITY;
     int it = 0;
     for (it = 1; it <= maxiter; ++it) {
@@ -811,34 +850,8 @@ static int fit_laplace_mixed(gpl_model* m) {
         }
         potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
         trtri32_blocked(c, L, m->d(B_DINV), N, X);
-        // alpha = S u, B u = S z, by refinement
-        fill_f64(s, rhs, N, 0.0);
-        vec_mul(s, n, m->d(B_SV), m->d(B_Z), rhs);
-        fill_f64(s, u, N, 0.0);
-        HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
-        double rel = INFINITY;
-        for (int k = 1; k <= maxref; ++k) {
-            {
-                Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
-                trmv_lower_f32(s, X, N, N, r, t1, 0, m->d(B_KMV), nch);
-                trmv_lower_f32(s, X, N, N, t1, d, 1, m->d(B_KMV), nch);
-            }
-            vec_axpby(s, n, 1.0, d, 1.0, u, u);
-            {
-                Prof p(c, "symv_b64", 2.0 * N * N, 8.0 * N * (N + 64) / 2);
-                symv_lt_f64(s, B64, N, u, t1, m->d(B_KMV), nch, 1, 0, 0);
-            }
-            vec_axpby(s, n, 1.0, rhs, -1.0, t1, r);
-            rel_maxabs(s, d, u, n, m->d(B_SCAL) + 8);
-            double h;
-            HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-            rel = h;
-            if (rel < 1e-14) break;
-        }
-        m->refine_rel = rel;
-        fill_f64(s, m->d(B_ALPHA), N, 0.0);
-        vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
+        int rr0 = refine_alpha_mixed(m, maxref, nch);
+        if (rr0) return rr0;
         {
             Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
             symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), nch, 1, 0, 0);
@@ -1038,6 +1051,129 @@ static int fit_ep(gpl_model* m) {
     return GPL_OK;
 }
 
+// Parallel EP in the mixed-precision form: per sweep B = I + S K S is formed in
+// FP64 (kept for residuals) and FP32 (factored with FP64 diagonal tiles), X = L^-1
+// in FP32 gives diag(B^-1) for the posterior variances, and alpha comes from
+// iterative refinement against the FP64 B.  Loop control, site updates and the
+// evidence terms are those of fit_ep.
+static int fit_ep_mixed(gpl_model* m) {
+    gpl_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    const DevProg* pd = m->b[B_PROG].as<DevProg>();
+    const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
+    const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+    const int q = m->opts.quad_order > 0 ? m->opts.quad_order : 11;
+    if (q > 32) return fail(GPL_ERR_ARG, "quad_order must be <= 32");
+    std::vector<double> gx, gw;
+    gauss_hermite(q, gx, gw);
+    HIP_TRY(hipMemcpyAsync(m->d(B_GH), gx.data(), q * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(m->d(B_GH) + 32, gw.data(), q * 8, hipMemcpyHostToDevice, s));
+    ENSURE(m, B_XINV, (size_t)N * N * 4);
+    ENSURE(m, B_DACC, (size_t)N * 64 * 8);
+    ENSURE(m, B_B64, (size_t)N * N * 8);
+    const int nch = kmatvec_chunks(N);
+    ENSURE(m, B_KMV, (size_t)N * nch * 8);
+    const int maxref = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
+    float* L32 = m->b[B_L].as<float>();
+    float* X32 = m->b[B_XINV].as<float>();
+    {
+        Prof p(c, "build_K", 0, 8.0 * N * (N + 64) / 2);
+        launch_build_sym_f64(s, pd, m->prog, m->d(B_F), n, N, m->opts.jitter, 0, 1.0, nullptr, m->d(B_K), N);
+    }
+    // P_old = 0, z_old = 0, fmean_old = 0, fvar_old = diag(K)
+    fill_f64(s, m->d(B_POLD), N, 0.0);
+    fill_f64(s, m->d(B_ZOLD), N, 0.0);
+    fill_f64(s, m->d(B_FOLD), N, 0.0);
+    launch_build_diag(s, pd, m->prog, m->d(B_F), N, m->d(B_F), N, n, m->d(B_FVOLD));
+    // fvar_old = diag(K) with K = eval_cf + jitter I (R/gp_fit.R:154, R/ep.R:298)
+    fill_f64(s, m->d(B_W3), N, m->opts.jitter);
+    vec_axpby(s, n, 1.0, m->d(B_FVOLD), 1.0, m->d(B_W3), m->d(B_FVOLD));
+
+    const int maxiter = m->opts.maxiter > 0 ? m->opts.maxiter : 100;
+    const double tol = 1e-4, damping = m->opts.damping, dmin = m->opts.damping_min > 0 ? m->opts.damping_min : 0.1;
+    double damp = damping, diff_old = INFINITY, diff_new = INFINITY;
+    bool quad_ok = true;
+    int it;
+    for (it = 1; it <= maxiter; ++it) {
+        int* flags = m->b[B_INFO].as<int>();
+        hipMemsetAsync(flags, 0, 8, s);
+        {
+            Prof p(c, "ep_sites", 0, 0);
+            ep_sites(s, m->lik.kind, m->lik.link, n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
+                     m->d(B_Y), tr, off, q, m->d(B_GH), m->d(B_GH) + 32, damp, m->d(B_Z), m->d(B_V), m->d(B_SV),
+                     m->d(B_LOGZS), m->d(B_CAVM), m->d(B_CAVV), flags + 1);
+        }
+        int fl[2];
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (fl[1] & 2) {
+            m->logZ = -INFINITY;
+            m->iters = it;
+            return fail(GPL_ERR_NAN, "NaN in EP site parameters (R/ep.R:30-32 returns log_evidence = -Inf)");
+        }
+        if (fl[1] & 4) {
+            m->logZ = -INFINITY;
+            return fail(GPL_ERR_UNSUPPORTED,
+                        "EP site variance <= 0: host fallback branch R/ep.R:50-138 is not on the device path");
+        }
+        if (fl[1] & 1) quad_ok = false;
+        {
+            Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
+            form_B_mixed(s, m->d(B_K), m->d(B_B64), L32, m->d(B_DACC), N, n, N, m->d(B_SV));
+        }
+        potrf32_blocked(c, L32, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
+        trtri32_blocked(c, L32, m->d(B_DINV), N, X32);
+        {
+            int rr0 = refine_alpha_mixed(m, maxref, nch);
+            if (rr0) return rr0;
+        }
+        {
+            Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
+            symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), nch, 1, 0, 0);
+        }
+        colsumsq_f32(s, X32, N, N, n, m->d(B_W3), 1);
+        ep_post(s, n, m->d(B_V), m->d(B_Z), m->d(B_W3), m->d(B_CAVM), m->d(B_CAVV), m->d(B_LOGZS), m->d(B_FVAR),
+                m->d(B_TERM));
+        double* sc = m->d(B_SCAL);
+        reduce_sum(s, m->d(B_TERM), n, sc + 0);
+        reduce_dot(s, m->d(B_Z), m->d(B_ALPHA), n, sc + 1);
+        reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
+        reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+        HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
+        double h[4];
+        int r = read_scalars(m, h, 4);
+        if (r) return r;
+        if (fl[0] > 0) {
+            g_minor = fl[0];
+            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+        }
+        m->logZ = h[0] - h[2] - 0.5 * h[1];
+        diff_new = h[3];
+        if (diff_new > diff_old) {
+            damp = damp * damping;
+            continue;
+        }
+        damp = std::max(damp, dmin);
+        HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(m->d(B_FVOLD), m->d(B_FVAR), N * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(m->d(B_ZOLD), m->d(B_Z), N * 8, hipMemcpyDeviceToDevice, s));
+        vec_recip(s, n, m->d(B_V), m->d(B_POLD));  // P_old = fit$P = 1 / V
+        diff_old = diff_new;
+        if (diff_new < tol) break;
+    }
+    if (it > maxiter) it = maxiter;
+    m->iters = it;
+    if (maxiter > 1 && it == maxiter && !(diff_new < tol)) {
+        m->logZ = -INFINITY;
+        g_err = "Maximum number of iterations in EP reached, max delta f = " + std::to_string(diff_new);
+        return GPL_ERR_NOCONV;
+    }
+    if (!quad_ok) m->logZ = -INFINITY;
+    m->have_xinv = true;
+    return GPL_OK;
+}
+
 extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, const double* x, int64_t n,
                        int64_t ncols, const double* y, const double* trials, const double* offset, const gpl_lik* lik,
                        const gpl_opts* opts, gpl_model** inout, double* log_evidence) {
@@ -1054,8 +1190,6 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
     if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
     const bool mixed = o.precision == GPL_PREC_MIXED;
-    if (mixed && o.approx != GPL_APPROX_LAPLACE)
-        return fail(GPL_ERR_UNSUPPORTED, "the mixed-precision form covers the Laplace approximation (and Gaussian)");
     DevProg P;
     int r = prepare_prog(terms, nterms, ncols, P);
     if (r) return r;
@@ -1087,7 +1221,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     upload_vec(m, B_TR, trials, n);
     upload_vec(m, B_OFF, offset, n);
     hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
-    if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
+    if (o.approx == GPL_APPROX_EP) r = mixed ? fit_ep_mixed(m) : fit_ep(m);
     else if (lik->kind == GPL_LIK_GAUSSIAN) r = mixed ? fit_gaussian_mixed(m) : fit_gaussian(m);
     else r = mixed ? fit_laplace_mixed(m) : fit_laplace(m);
     *inout = m;

@@ -91,7 +91,7 @@ typedef struct gpl_opts {
     double damping_min;   /* 0.1 (R/gp_fit.R:158) */
     int32_t quad_order;   /* approx_ep(quad_order=11) */
     int32_t precision;    /* GPL_PREC_FP64 (plain) or GPL_PREC_MIXED (FP32 MFMA + FP64 diagonal tiles and
-                             refinement; Gaussian and Laplace fits, GPL_ERR_UNSUPPORTED for EP) */
+                             refinement; Gaussian, Laplace and EP fits) */
     double jitter;        /* 1e-6 (R/gp.R:276-281) */
     int32_t refine_iters; /* mixed precision: max iterative-refinement sweeps (default 8) */
     int32_t flags;        /* GPL_FLAG_* bits */

@@ -87,6 +87,36 @@ def test_mixed_n8192_workload_meets_1e6(gpl):
     assert rel(G.gp_pred(g32, xt)["mean"], G.gp_pred(g64, xt)["mean"]) <= 1e-6
 
 
+@pytest.mark.parametrize("lik_name,n", [("logit", 400), ("probit", 700), ("poisson", 1000)])
+def test_mixed_ep_parity(gpl, lik_name, n):
+    """Parallel EP in the mixed form (FP32 factor and L^-1, FP64 diagonal tiles and
+    refinement): the sweep count, posterior mean and predictive mean follow the FP64
+    oracle to 1e-6 relative, the posterior variance to 1e-5, the log evidence to
+    1e-6 relative to the size of its summands (n)."""
+    G = gpl
+    r = np.random.default_rng(n + 17)
+    x = r.normal(size=(n, 3))
+    f = np.sin(2 * x[:, 0]) + 0.5 * x[:, 1]
+    if lik_name == "poisson":
+        lk = G.lik_poisson()
+        y = r.poisson(np.exp(0.5 * f)).astype(float)
+    else:
+        lk = G.lik_bernoulli(lik_name)
+        y = (r.uniform(size=n) < 1 / (1 + np.exp(-f))).astype(float)
+    gp = G.gp_init(G.cf_sexp(lscale=[0.8, 0.9, 1.2], magn=1.5), lk, approx=G.approx_ep())
+    gm = G.gp_fit(gp, x, y, precision="mixed")
+    fo = O.gp_fit([c.spec() for c in gm.cfs], lk.spec(), x, y, approx="ep")
+    assert gm.fit.iters == fo["iters"]
+    assert abs(gm.fit.log_evidence - fo["log_evidence"]) <= 1e-6 * max(abs(fo["log_evidence"]), n)
+    assert rel(gm.fit.fmean, fo["fmean"]) < 1e-6
+    assert rel(gm.fit.fvar, fo["fvar"]) < 1e-5
+    xt = r.normal(size=(64, 3))
+    pr = G.gp_pred(gm, xt, var=True)
+    mo, vo = O.gp_pred(fo, xt, var=True)
+    assert rel(pr["mean"], mo) < 1e-6
+    assert rel(pr["var"], vo) < 1e-3
+
+
 @pytest.mark.parametrize("lik_name,n", [("logit", 400), ("poisson", 1000)])
 def test_mixed_laplace_parity(gpl, lik_name, n):
     """Laplace Newton loop in the mixed form: the iterate sequence (step count),
