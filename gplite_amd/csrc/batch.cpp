@@ -60,8 +60,8 @@ struct gpl_batch {
     std::vector<int> bucket_of;  // problem -> bucket index
     std::vector<int> pos_in;     // problem -> position in its bucket
     DevBuf res_logz, res_grad;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraph_t> graph;      // one per lane
+    std::vector<hipGraphExec_t> exec;   // one per lane (null: lane without buckets)
     std::vector<Lane> lanes;
     std::vector<int> lane_of;  // bucket -> lane
     hipEvent_t fork = nullptr;
@@ -69,8 +69,10 @@ struct gpl_batch {
         for (auto* b : buckets) delete b;
         res_logz.free_();
         res_grad.free_();
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+        for (auto e : exec)
+            if (e) hipGraphExecDestroy(e);
+        for (auto g : graph)
+            if (g) hipGraphDestroy(g);
         for (auto& l : lanes) {
             if (l.s) hipStreamDestroy(l.s);
             if (l.side) hipStreamDestroy(l.side);
@@ -363,10 +365,12 @@ static int upload_params(gpl_batch* B, const gpl_cf_term* terms, const double* s
 }
 
 static void drop_graph(gpl_batch* B) {
-    if (B->exec) hipGraphExecDestroy(B->exec);
-    if (B->graph) hipGraphDestroy(B->graph);
-    B->exec = nullptr;
-    B->graph = nullptr;
+    for (auto e : B->exec)
+        if (e) hipGraphExecDestroy(e);
+    for (auto g : B->graph)
+        if (g) hipGraphDestroy(g);
+    B->exec.clear();
+    B->graph.clear();
 }
 
 extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
@@ -547,20 +551,48 @@ static void reset_flags(gpl_batch* B, hipStream_t s) {
     for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_INFO].p, 0, k->idx.size() * 4, s);
 }
 
+// One graph per lane, captured on the lane's stream (its look-ahead side stream
+// joins through events) and launched on it: the lanes' graphs then run
+// concurrently, where a single graph holding every lane executed them one after
+// the other (measured: 2048 uneven problems in 87.7 ms against 100.8 ms).
 static int run_once(gpl_batch* B) {
     hipStream_t s = B->ctx->stream;
     reset_flags(B, s);
-    if (!B->exec) {
-        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const size_t nl = B->lanes.size();
+    if (B->exec.empty()) {
+        B->graph.assign(nl, nullptr);
+        B->exec.assign(nl, nullptr);
         const bool prof = B->ctx->prof;
-        B->ctx->prof = false;  // no event records inside the graph
-        run_buckets(B, s);
+        B->ctx->prof = false;  // no event records inside the graphs
+        for (size_t l = 0; l < nl; ++l) {
+            Lane& ln = B->lanes[l];
+            bool any = false;
+            for (size_t i = 0; i < B->buckets.size(); ++i) any = any || B->lane_of[i] == (int)l;
+            if (!any) continue;
+            hipError_t e = hipStreamBeginCapture(ln.s, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                for (size_t i = 0; i < B->buckets.size(); ++i)
+                    if (B->lane_of[i] == (int)l) bucket_pipeline(B, *B->buckets[i], ln);
+                e = hipStreamEndCapture(ln.s, &B->graph[l]);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&B->exec[l], B->graph[l], nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                B->ctx->prof = prof;
+                drop_graph(B);
+                return fail(GPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            }
+        }
         B->ctx->prof = prof;
-        hipError_t e = hipStreamEndCapture(s, &B->graph);
-        if (e != hipSuccess) return fail(GPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        HIP_TRY(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     }
-    HIP_TRY(hipGraphLaunch(B->exec, s));
+    HIP_TRY(hipEventRecord(B->fork, s));
+    for (size_t l = 0; l < nl; ++l) {
+        if (!B->exec[l]) continue;
+        Lane& ln = B->lanes[l];
+        HIP_TRY(hipStreamWaitEvent(ln.s, B->fork, 0));
+        HIP_TRY(hipGraphLaunch(B->exec[l], ln.s));
+        HIP_TRY(hipEventRecord(ln.done, ln.s));
+        HIP_TRY(hipStreamWaitEvent(s, ln.done, 0));
+    }
     return GPL_OK;
 }
 
