@@ -288,8 +288,8 @@ void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 // broadcasts the pivot with readlane, scales its column and publishes the
 // multipliers through a double-buffered LDS vector; after one barrier every
 // thread applies the rank-1 update to its own 16 elements.  The inverse
-// (written to Dinv, FP64) is then formed column-parallel with four lanes per
-// column exchanging pivots by DPP quad broadcasts.
+// (written to Dinv, FP64) is then formed by 16x16 blocks: DPP forward
+// substitution inside the diagonal blocks, f64 MFMA for the block rows.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
     const long long b = __double_as_longlong(v);
@@ -328,8 +328,8 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     // columns 16w..16w+15 of every row (lane = row).  Factor: four phases; in
     // phase p wave p factors its 16 columns alone (pivots and multipliers are
     // exchanged with readlane, no barrier), publishes them to LDS, and the
-    // waves to its right apply the rank-16 update.  Inverse: forward
-    // substitution, four lanes per column exchanging values by DPP.
+    // waves to its right apply the rank-16 update.  Inverse: by 16x16 blocks
+    // (below).
     A += blockIdx.x * sA;
     Dinv += blockIdx.x * sD;
     ldiag += blockIdx.x * sL;
@@ -398,30 +398,58 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
         if (t == 0 && bad) atomicCAS(info, 0, (int)(k * 64 + __ffsll((long long)bad)));
     }
     if (!want_inv) return;
-    // inverse: column c = 16*w + lane/4, this lane owns rows 4r + q
-    const int lane = t & 63;
-    const int c = w * 16 + (lane >> 2);
-    const int q = lane & 3;
-    double b[16];
+    // Inverse by 16x16 blocks.  (1) Each wave inverts its diagonal block by
+    // forward substitution, four lanes per column exchanging values by DPP (16
+    // dependent steps).  (2) Block rows bi = 1..3 in turn, wave j < bi forming
+    // X_{bi,j} = -X_{bi,bi} T, T = sum_{j <= kb < bi} L_{bi,kb} X_{kb,j}, on
+    // v_mfma_f64_16x16x4.  T is left in the D layout (lane (li, g) holds rows
+    // g + 4v of column li), which is the A-operand layout of T^T with k = g + 4v,
+    // so the product is taken as Y^T = T^T X_{bi,bi}^T without leaving registers.
+    __shared__ double Xs[64][65];  // Xs[col][row] = (L_kk^-1)(row, col)
+    const int lane = t & 63, li = lane & 15, g = lane >> 4;
+    {
+        const int c = lane >> 2, q = lane & 3, base = 16 * w;
+        double b[4];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) b[r] = (4 * r + q == c) ? 1.0 : 0.0;
+        for (int r = 0; r < 4; ++r) b[r] = (4 * r + q == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int kk = 0; kk < 64; ++kk) {
-        const int kr = kk >> 2;
-        double xk = b[kr] * rinv[kk];
-        switch (kk & 3) {
-            case 0: xk = quad_bcast_f64<0>(xk); break;
-            case 1: xk = quad_bcast_f64<1>(xk); break;
-            case 2: xk = quad_bcast_f64<2>(xk); break;
-            default: xk = quad_bcast_f64<3>(xk); break;
+        for (int kk = 0; kk < 16; ++kk) {
+            const int kr = kk >> 2;
+            double xk = b[kr] * rinv[base + kk];
+            switch (kk & 3) {
+                case 0: xk = quad_bcast_f64<0>(xk); break;
+                case 1: xk = quad_bcast_f64<1>(xk); break;
+                case 2: xk = quad_bcast_f64<2>(xk); break;
+                default: xk = quad_bcast_f64<3>(xk); break;
+            }
+            if (q == (kk & 3)) b[kr] = xk;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b[r] -= Lp[base + kk][base + 4 * r + q] * xk;
         }
-        if (q == (kk & 3)) b[kr] = xk;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) b[r] -= Lp[kk][4 * r + q] * xk;
+        for (int r = 0; r < 4; ++r) Xs[base + c][base + 4 * r + q] = b[r];
+        for (int r = q; r < base; r += 4) Xs[base + c][r] = 0.0;  // above the diagonal block
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bi = 1; bi < 4; ++bi) {
+        if (w < bi) {
+            const int r0 = 16 * bi, c0 = 16 * w;
+            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kb = 0; kb < r0; kb += 4)  // X is zero above its diagonal blocks
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lp[kb + g][r0 + li], Xs[c0 + li][kb + g], acc, 0, 0, 0);
+            d4 y = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                y = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[v], Xs[r0 + g + 4 * v][r0 + li], y, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Xs[c0 + g + 4 * v][r0 + li] = -y[v];
+        }
+        __syncthreads();
     }
     double* D = Dinv + k * 64 * 64;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) D[(4 * r + q) + c * 64] = b[r];
+    for (int e = t; e < 64 * 64; e += 256) D[(e & 63) + (e >> 6) * 64] = Xs[e >> 6][e & 63];
 }
 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
