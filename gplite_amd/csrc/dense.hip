@@ -283,11 +283,10 @@ void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 
 // ---------------------------------------------------------------------------
 // Diagonal block (64x64), FP64 arithmetic on FP64 or FP32 storage.
-// Thread (row i = t%64, wave w = t/64) keeps elements (i, w+4q), q < 16, in
-// registers for the whole factorisation.  Step j: the wave owning column j
-// broadcasts the pivot with readlane, scales its column and publishes the
-// multipliers through a double-buffered LDS vector; after one barrier every
-// thread applies the rank-1 update to its own 16 elements.  The inverse
+// Thread (row i = t%64, wave w = t/64) keeps elements (i, 16w+q), q < 16, in
+// registers for the whole factorisation, which runs in four phases of 16
+// columns (one wave each, exchanging pivots by readlane) with a rank-16 update
+// of the waves to the right between phases.  The inverse
 // (written to Dinv, FP64) is then formed by 16x16 blocks: DPP forward
 // substitution inside the diagonal blocks, f64 MFMA for the block rows.
 // ---------------------------------------------------------------------------
