@@ -250,19 +250,32 @@ def test_prior_prediction_and_refit_guard(gpl):
 
 
 def test_closed_form_gaussian_suite(gpl):
-    """R tests/testthat/test_basic.R:64-132 restated at 1e-8: all single cfs and
-    pairs, 3 perturbations each, logZ and pred vs the closed form."""
+    """R tests/testthat/test_basic.R:1-132 restated at 1e-8: every one of the seven
+    covariance functions alone, every pair as a sum and every triple as a product
+    (63 models), ten hyperparameter perturbations each, logZ and pred vs the
+    closed form."""
+    import copy
+    import itertools
     G = gpl
     r = np.random.default_rng(1234)
     n, nt = 20, 30
     x = r.normal(size=n); xt = r.normal(size=nt); y = x ** 2 + 0.2 * r.normal(size=n)
     base = [G.cf_const(), G.cf_lin(), G.cf_sexp(), G.cf_matern32(), G.cf_matern52(), G.cf_nn(), G.cf_periodic()]
-    models = [[c] for c in base] + [[base[i], base[j]] for i in range(len(base)) for j in range(i + 1, len(base))]
+    cp = copy.deepcopy
+    models = [[cp(c)] for c in base]
+    models += [[cp(base[i]), cp(base[j])] for i, j in itertools.combinations(range(len(base)), 2)]
+    with warnings.catch_warnings():  # products warn about several magnitudes (R's SWO)
+        warnings.simplefilter("ignore")
+        models += [[cp(base[i]) * cp(base[j]) * cp(base[k])]
+                   for i, j, k in itertools.combinations(range(len(base)), 3)]
+    assert len(models) == 63
     for cfs in models:
-        gp = G.gp_init([__import__("copy").deepcopy(c) for c in cfs], G.lik_gaussian())
+        gp = G.gp_init(cfs, G.lik_gaussian())
         p0 = np.array(list(G.get_param(gp).values()))
-        for _ in range(3):
-            gpp = G.gp_fit(G.set_param(gp, p0 + 0.3 * r.normal(size=p0.size)), x, y)
+        for _ in range(10):
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                gpp = G.gp_fit(G.set_param(gp, p0 + 0.3 * r.normal(size=p0.size)), x, y)
             spec = _spec(gpp.cfs)
             K = O.eval_cf(spec, x[:, None], x[:, None])
             lz = O.gaussian_logz_closed_form(K, y, gpp.lik.sigma)
