@@ -59,8 +59,8 @@ def main():
     if args.refine is None:
         args.refine = 6 if args.workload == "c4" else 3
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; how the path would shard is described in "
-                                   "DESIGN.md but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: the sharded path is designed but not built "
+                                   "(--gpus 1 only)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import gplite_amd as G
