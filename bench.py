@@ -236,6 +236,21 @@ def main():
                          f"restatement of the reference sequence ({secs:.1f}s)",
                "rel_logz_fp64_gpu_vs_cpu": abs(ref["logz"] / lz - 1),
                "rel_grad_fp64_gpu_vs_cpu": float(np.max(np.abs(ref["grad"] - gcpu)) / np.max(np.abs(gcpu)))}
+        # single-thread figure beside it, on a bounded sample (the first half of the
+        # inputs), with the all-threads rate on the same sample for the ratio
+        try:
+            from threadpoolctl import threadpool_limits
+            ns = n // 2
+            xs, ysm = np.ascontiguousarray(w["x"][:ns]), np.ascontiguousarray(y[:ns])
+            _, _, s_all = energy_grad_matern52_ard(xs, ysm, w["lscale"], w["magn"], w["sigma"])
+            with threadpool_limits(limits=1):
+                _, _, s_one = energy_grad_matern52_ard(xs, ysm, w["lscale"], w["magn"], w["sigma"])
+            cpu["single_thread"] = {"value": round(1.0 / s_one, 5), "unit": "evals/s", "cores": 1,
+                                    "all_threads_value": round(1.0 / s_all, 5),
+                                    "sample": f"one energy+gradient evaluation at n={ns} (first {ns} points), "
+                                              f"{s_one:.1f}s on one thread, {s_all:.1f}s on {cores}"}
+        except ImportError:
+            pass
         log("cpu baseline:", cpu)
 
     out = {
