@@ -233,7 +233,8 @@ def main():
         ref = R.get("fp64", H)
         cpu = {"value": round(1.0 / secs, 5), "unit": "evals/s", "cores": cores, "kind": "port",
                "sample": f"one energy+gradient evaluation at n={n}, d={d}: numpy/scipy LAPACK dpotrf + dpotri "
-                         f"restatement of the reference sequence ({secs:.1f}s)",
+                         f"restatement of the reference sequence, kernel build and gradient contraction over "
+                         f"row chunks on {cores} threads ({secs:.1f}s)",
                "rel_logz_fp64_gpu_vs_cpu": abs(ref["logz"] / lz - 1),
                "rel_grad_fp64_gpu_vs_cpu": float(np.max(np.abs(ref["grad"] - gcpu)) / np.max(np.abs(gcpu)))}
         # single-thread figure beside it, on a bounded sample (the first half of the
