@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--fitc-n", type=int, default=131072)
     ap.add_argument("--fitc-m", type=int, default=1024)
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
+    ap.add_argument("--serve-precision", default="auto", choices=["auto", "mixed", "fp64"],
+                    help="c5 fit precision: auto = mixed for Gaussian fits, FP64 for Laplace fits")
     args = ap.parse_args()
     if args.refine is None:
         args.refine = 6 if args.workload == "c4" else 3
@@ -433,7 +435,7 @@ def bench_c5(args, G, L, api, synth):
     n ~ logU[128, 16384], d=6, cf_matern32, Gaussian or Poisson-Laplace 50/50."""
     from gplite_amd.serve import Server, poisson_load
     rows = []
-    for prec in ("mixed",):
+    for prec in (args.serve_precision,):
         srv = Server(workers=4, mem_gb=128.0)
         poisson_load(srv, 100.0, 12, seed=99, precision=prec)  # warm the workers
         for rate in [float(v) for v in args.rates.split(",")]:
@@ -454,7 +456,8 @@ def bench_c5(args, G, L, api, synth):
         "value": round(best["throughput_req_s"], 2), "unit": "req/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32+f64", "data": "synthetic (seeded, numpy PCG64)",
         "config": {"workload": "C5 serving", "workers": 4, "fit_fraction": 0.3, "m": 1000,
-                   "fit_n": "logU[128,16384]", "liks": "Gaussian(0.1) / Poisson-Laplace 50/50"},
+                   "fit_n": "logU[128,16384]", "liks": "Gaussian(0.1) / Poisson-Laplace 50/50",
+                   "fit_precision": args.serve_precision},
         "sweep": rows,
     }
 

