@@ -120,8 +120,10 @@ static void potrf_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t K0, in
     potrf_panel_rec(c, s, k, K0, h);
     const float* P = L + (K0 + h) + K0 * N;
     {
+        // k is at most half a panel here: 64-wide tiles (shorter prologue and
+        // epilogue per k, four times the tiles) whatever the bucket's tile
         Prof p(c, "batch_panel_f32", 2.0 * nb * M * W2 * h, 0, s);
-        gemm_f32_batched(s, 0, 1, M, W2, h, -1.f, P, N, P, N, 1.f, L + (K0 + h) * (N + 1), N, 0, k.bm, nb, N2, N2,
+        gemm_f32_batched(s, 0, 1, M, W2, h, -1.f, P, N, P, N, 1.f, L + (K0 + h) * (N + 1), N, 0, 64, nb, N2, N2,
                          N2);
     }
     {
@@ -146,9 +148,9 @@ static void trtri_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t I0, in
     }
     const int64_t h = (H / 128) * 64, H2 = H - h, Wc = I0 + h;
     trtri_panel_rec(c, s, k, I0, h);
-    {
+    {  // 64-wide tiles, as in potrf_panel_rec
         Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * H2 * Wc * h, 0, s);
-        gemm_f32_batched(s, 0, 0, H2, Wc, h, -1.f, L + (I0 + h) + I0 * N, N, X + I0, N, 1.f, X + I0 + h, N, 0, k.bm, nb,
+        gemm_f32_batched(s, 0, 0, H2, Wc, h, -1.f, L + (I0 + h) + I0 * N, N, X + I0, N, 1.f, X + I0 + h, N, 0, 64, nb,
                          N2, N2, N2, kFoldInverse);
     }
     trtri_panel_rec(c, s, k, I0 + h, H2);
