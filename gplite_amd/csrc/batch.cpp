@@ -192,20 +192,13 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, Lane& ln) {
         if (R <= 0) break;
         const int64_t W1 = std::min<int64_t>(NBo, R);
         const float* P = L + K1 + K0 * N;
-        // The FP64 diagonal tiles take the factored panel's update on the side
-        // stream: the next panel's tiles beside the first trailing GEMM (they
-        // read only the finished panel and write only Dacc), the rest after the
-        // next panel's factor, beside the second trailing GEMM.  The main stream
-        // never reads Dacc; the side stream keeps the updates in order.
-        hipEventRecord(ln.ev_a, s);  // the panel at K0 is final
-        hipStreamWaitEvent(s2, ln.ev_a, 0);
-        {
-            Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (W1 / 64), 0, s2);
-            dacc_update(s2, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, nb, N * 64, N2);
-        }
         {
             Prof p(c, "batch_syrk_f32", 2.0 * nb * R * W1 * W, 0, s);
             gemm_f32_batched(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, bm, nb, N2, N2, N2);
+        }
+        {
+            Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (W1 / 64), 0, s);
+            dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, nb, N * 64, N2);
         }
         hipEventRecord(ln.ev_a, s);
         hipStreamWaitEvent(s2, ln.ev_a, 0);
@@ -220,8 +213,8 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, Lane& ln) {
                                  N2, N2);
             }
             {
-                Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (R2 / 64), 0, s2);
-                dacc_update(s2, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
+                Prof p(c, "batch_dacc_f64", 2.0 * nb * 4096.0 * W * (R2 / 64), 0, s);
+                dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, nb, N * 64, N2);
             }
         }
         hipStreamWaitEvent(s, ln.ev_p, 0);
