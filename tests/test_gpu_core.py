@@ -286,3 +286,27 @@ def test_closed_form_gaussian_suite(gpl):
             assert rel(pr["mean"], Kt @ np.linalg.solve(C, y)) < 1e-7
             vv = O.eval_cf(spec, xt[:, None], xt[:, None], diag_only=True) - np.sum(Kt.T * np.linalg.solve(C, Kt.T), 0)
             assert np.max(np.abs(pr["var"] - vv)) <= 1e-7 * max(1.0, np.max(np.abs(vv)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pred_ragged_test_sizes(gpl, prec):
+    """gp_pred at m = 0, 1 and around the 64-row tile edges against the oracle:
+    the mean to 1e-8 (FP64) or 1e-6 (mixed), the variance to 1e-8 or 1e-3,
+    relative; m = 0 gives empty results."""
+    G = gpl
+    r = np.random.default_rng(77)
+    n = 150
+    x = r.normal(size=(n, 2)); y = np.sin(x.sum(1)) + 0.1 * r.normal(size=n)
+    gp = G.gp_init(G.cf_matern32(lscale=[0.8, 1.1], magn=1.1), G.lik_gaussian(sigma=0.1))
+    gf = G.gp_fit(gp, x, y, precision=prec)
+    fo = O.gp_fit(_spec(gf.cfs), gf.lik.spec(), x, y)
+    tm, tv = (1e-8, 1e-8) if prec == "fp64" else (1e-6, 1e-3)
+    for m in (0, 1, 63, 64, 65, 129, 300):
+        xt = r.normal(size=(m, 2))
+        pr = G.gp_pred(gf, xt, var=True)
+        assert pr["mean"].shape == (m,) and pr["var"].shape == (m,)
+        if m == 0:
+            continue
+        mo, vo = O.gp_pred(fo, xt, var=True)
+        assert rel(pr["mean"], mo) < tm, m
+        assert rel(pr["var"], vo) < tv, m
