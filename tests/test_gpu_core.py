@@ -310,3 +310,42 @@ def test_pred_ragged_test_sizes(gpl, prec):
         mo, vo = O.gp_pred(fo, xt, var=True)
         assert rel(pr["mean"], mo) < tm, m
         assert rel(pr["var"], vo) < tv, m
+
+
+@pytest.mark.parametrize("case", ["ep_binomial_trials", "ep_poisson_offset", "gaussian_offset"])
+def test_trials_and_offset(gpl, case):
+    """trials and offset through the device fits (R/lik.R:297-366,188-270):
+    EP with binomial trials, EP with a Poisson offset, and the Gaussian
+    likelihood with an offset, whose pseudo-data ignore it (z = y,
+    R/lik.R:191) while its log-likelihood adds it (R/lik.R:302).  logZ, the
+    posterior mean and the predictive mean/variance to 1e-8 of the oracle."""
+    G = gpl
+    r = np.random.default_rng(21)
+    n = 150
+    x = r.normal(size=(n, 2))
+    f = np.sin(2 * x[:, 0]) + 0.5 * x[:, 1]
+    kw = {}
+    if case == "ep_binomial_trials":
+        lk, ap = G.lik_binomial("logit"), G.approx_ep()
+        kw["trials"] = r.integers(1, 8, n).astype(float)
+        y = r.binomial(kw["trials"].astype(int), 1 / (1 + np.exp(-f))).astype(float)
+    elif case == "ep_poisson_offset":
+        lk, ap = G.lik_poisson(), G.approx_ep()
+        kw["offset"] = 0.3 * r.normal(size=n)
+        y = r.poisson(np.exp(0.5 * f + kw["offset"])).astype(float)
+    else:
+        lk, ap = G.lik_gaussian(sigma=0.3), G.approx_laplace()
+        kw["offset"] = 0.5 + 0.2 * r.normal(size=n)
+        y = f + kw["offset"] + 0.3 * r.normal(size=n)
+    gp = G.gp_init(G.cf_sexp(lscale=[0.8, 1.0], magn=1.2), lk, approx=ap)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        gf = G.gp_fit(gp, x, y, **kw)
+        fo = O.gp_fit(_spec(gf.cfs), lk.spec(), x, y, approx="ep" if case.startswith("ep") else "laplace", **kw)
+    assert gf.fit.iters == fo["iters"]
+    assert gf.fit.log_evidence == pytest.approx(fo["log_evidence"], rel=RTOL)
+    assert rel(gf.fit.fmean, fo["fmean"]) < RTOL
+    xt = r.normal(size=(40, 2))
+    pr = G.gp_pred(gf, xt, var=True)
+    mo, vo = O.gp_pred(fo, xt, var=True)
+    assert rel(pr["mean"], mo) < RTOL and rel(pr["var"], vo) < RTOL
