@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from tests.test_gpu_core import GRAD_CASES
 
 pytestmark = pytest.mark.gpu
 
@@ -143,3 +144,21 @@ def test_mixed_laplace_parity(gpl, lik_name, n):
     mo, vo = O.gp_pred(fo, xt, var=True)
     assert rel(pr["mean"], mo) < 1e-6
     assert rel(pr["var"], vo) < 1e-3
+
+
+@pytest.mark.parametrize("ci", range(len(GRAD_CASES)))
+def test_mixed_gradient_composites(gpl, ci):
+    """The mixed form's gradient for every covariance structure of the FP64
+    gradient tests (sums, products, nested products, periodic, nn, lin), which
+    go through the generic gradient kernel on the FP32 B^-1: 1e-3 relative to
+    the oracle, the log evidence to 1e-6 of its summands."""
+    G = gpl
+    r = np.random.default_rng(100 + ci)
+    n = 700
+    x = r.normal(size=(n, 2))
+    y = np.sin(x[:, 0]) * np.cos(0.5 * x[:, 1]) + 0.25 * r.normal(size=n)
+    gp = G.gp_init(GRAD_CASES[ci](G), G.lik_gaussian(sigma=0.25, prior_sigma=G.prior_lognormal(-1.0, 0.5)))
+    gm = G.gp_fit(gp, x, y, precision="mixed")
+    fo = O.gp_fit([c.spec() for c in gm.cfs], gm.lik.spec(), x, y)
+    assert abs(gm.fit.log_evidence - fo["log_evidence"]) <= logz_tol(n, 0.25, fo["log_evidence"])
+    assert rel(G.gp_energy_grad(gm), O.gp_energy_grad(fo)) < 1e-3
