@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from tests.test_gpu_core import GRAD_CASES
 
 
 def logz_tol(n, sigma, logz, rtol=1e-6):
@@ -88,3 +89,30 @@ def test_batch_reports_failed_factor_per_problem(gpl):
     with pytest.raises(GplError) as ei:
         B.energy_grad()
     assert "problem 1" in str(ei.value)
+
+
+@pytest.mark.parametrize("ci", range(len(GRAD_CASES)))
+def test_batch_covariance_structures(gpl, ci):
+    """Every covariance structure of the FP64 gradient tests (sums, products,
+    nested products, periodic, nn, lin) in one batch of three problems of
+    uneven n with their own hyperparameters: energy and gradient per problem
+    against the oracle, as in test_batch_matches_oracle_per_problem."""
+    G = gpl
+    from gplite_amd.batch import GPBatch
+    r = np.random.default_rng(300 + ci)
+    gps, xs, ys = [], [], []
+    for p, n in enumerate((37, 150, 301)):
+        x = r.normal(size=(n, 2))
+        ys.append(np.sin(x[:, 0]) * np.cos(0.5 * x[:, 1]) + 0.1 * r.normal(size=n))
+        xs.append(x)
+        gp = G.gp_init(GRAD_CASES[ci](G), G.lik_gaussian(sigma=0.1 + 0.05 * p))
+        pp = np.array(list(G.get_param(gp).values()))
+        gps.append(G.set_param(gp, pp + 0.1 * r.normal(size=pp.size)))
+    B = GPBatch(gps, xs, ys, refine_iters=6)
+    e, g = B.energy_grad()
+    for p in range(3):
+        fo = O.gp_fit([c.spec() for c in B.gps[p].cfs], B.gps[p].lik.spec(), xs[p], ys[p])
+        eo = O.gp_energy(fo)
+        assert abs(e[p] - eo) <= logz_tol(len(ys[p]), B.gps[p].lik.sigma, eo), (ci, p, e[p], eo)
+        go = O.gp_energy_grad(fo)
+        assert np.max(np.abs(g[p] - go)) <= 1e-3 * np.max(np.abs(go)), (ci, p)
