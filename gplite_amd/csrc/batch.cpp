@@ -488,10 +488,14 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
         delete B;
         return fail(GPL_ERR_OOM, "device allocation failed for the batch");
     }
-    // lanes: up to four stream pairs; buckets go to the least-loaded lane, largest
-    // estimated time first (GEMM work plus a fixed cost per 64-row diagonal step)
+    // lanes: up to sixteen stream pairs; buckets go to the least-loaded lane,
+    // largest estimated time first (GEMM work plus a fixed cost per 64-row
+    // diagonal step).  Most buckets are chains of small launches, so more lanes
+    // overlap more of them: 2048 problems in 32 buckets take 99 ms per pass on 2
+    // lanes, 86 on 4, 79 on 8, 76 on 16 (HIP multiplexes the streams onto its
+    // hardware queues).
     {
-        const int nl = (int)std::min<size_t>(4, B->buckets.size());
+        const int nl = (int)std::min<size_t>(16, B->buckets.size());
         B->lanes.resize(nl);
         hipError_t e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
         for (auto& l : B->lanes) {
