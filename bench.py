@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--fitc-n", type=int, default=131072)
     ap.add_argument("--fitc-m", type=int, default=1024)
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
+    ap.add_argument("--serve-workers", type=int, default=4, help="c5 server workers (one context each)")
     ap.add_argument("--serve-precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="c5 fit precision: auto = mixed for Gaussian fits, FP64 for Laplace fits")
     args = ap.parse_args()
@@ -436,7 +437,7 @@ def bench_c5(args, G, L, api, synth):
     from gplite_amd.serve import Server, poisson_load
     rows = []
     for prec in (args.serve_precision,):
-        srv = Server(workers=4, mem_gb=128.0)
+        srv = Server(workers=args.serve_workers, mem_gb=128.0)
         poisson_load(srv, 100.0, 12, seed=99, precision=prec)  # warm the workers
         for rate in [float(v) for v in args.rates.split(",")]:
             nreq = int(min(max(2 * rate, 60), 400))
@@ -455,7 +456,7 @@ def bench_c5(args, G, L, api, synth):
         "metric": "serving throughput (req/s) and p50/p99 latency under Poisson arrivals",
         "value": round(best["throughput_req_s"], 2), "unit": "req/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32+f64", "data": "synthetic (seeded, numpy PCG64)",
-        "config": {"workload": "C5 serving", "workers": 4, "fit_fraction": 0.3, "m": 1000,
+        "config": {"workload": "C5 serving", "workers": args.serve_workers, "fit_fraction": 0.3, "m": 1000,
                    "fit_n": "logU[128,16384]", "liks": "Gaussian(0.1) / Poisson-Laplace 50/50",
                    "fit_precision": args.serve_precision},
         "sweep": rows,
