@@ -226,15 +226,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle.cpu_baseline import energy_grad_matern52_ard
-        try:
-            from threadpoolctl import threadpool_info
-            cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
-        except Exception:
-            cores = os.cpu_count()
-        lz, gcpu, secs = energy_grad_matern52_ard(w["x"], y, w["lscale"], w["magn"], w["sigma"])
+        from oracle.cpu_baseline import energy_grad_matern52_ard, host_threads
+        from threadpoolctl import threadpool_limits
+        nproc, cores = host_threads()
+        with threadpool_limits(limits=cores):  # BLAS and the chunk pool on every host thread used
+            lz, gcpu, secs = energy_grad_matern52_ard(w["x"], y, w["lscale"], w["magn"], w["sigma"])
         ref = R.get("fp64", H)
-        cpu = {"value": round(1.0 / secs, 5), "unit": "evals/s", "cores": cores, "kind": "port",
+        cpu = {"value": round(1.0 / secs, 5), "unit": "evals/s", "cores": cores, "nproc": nproc, "kind": "port",
                "sample": f"one energy+gradient evaluation at n={n}, d={d}: numpy/scipy LAPACK dpotrf + dpotri "
                          f"restatement of the reference sequence, kernel build and gradient contraction over "
                          f"row chunks on {cores} threads ({secs:.1f}s)",
@@ -242,19 +240,16 @@ def main():
                "rel_grad_fp64_gpu_vs_cpu": float(np.max(np.abs(ref["grad"] - gcpu)) / np.max(np.abs(gcpu)))}
         # single-thread figure beside it, on a bounded sample (the first half of the
         # inputs), with the all-threads rate on the same sample for the ratio
-        try:
-            from threadpoolctl import threadpool_limits
-            ns = n // 2
-            xs, ysm = np.ascontiguousarray(w["x"][:ns]), np.ascontiguousarray(y[:ns])
+        ns = n // 2
+        xs, ysm = np.ascontiguousarray(w["x"][:ns]), np.ascontiguousarray(y[:ns])
+        with threadpool_limits(limits=cores):
             _, _, s_all = energy_grad_matern52_ard(xs, ysm, w["lscale"], w["magn"], w["sigma"])
-            with threadpool_limits(limits=1):
-                _, _, s_one = energy_grad_matern52_ard(xs, ysm, w["lscale"], w["magn"], w["sigma"])
-            cpu["single_thread"] = {"value": round(1.0 / s_one, 5), "unit": "evals/s", "cores": 1,
-                                    "all_threads_value": round(1.0 / s_all, 5),
-                                    "sample": f"one energy+gradient evaluation at n={ns} (first {ns} points), "
-                                              f"{s_one:.1f}s on one thread, {s_all:.1f}s on {cores}"}
-        except ImportError:
-            pass
+        with threadpool_limits(limits=1):
+            _, _, s_one = energy_grad_matern52_ard(xs, ysm, w["lscale"], w["magn"], w["sigma"])
+        cpu["single_thread"] = {"value": round(1.0 / s_one, 5), "unit": "evals/s", "cores": 1,
+                                "all_threads_value": round(1.0 / s_all, 5),
+                                "sample": f"one energy+gradient evaluation at n={ns} (first {ns} points), "
+                                          f"{s_one:.1f}s on one thread, {s_all:.1f}s on {cores}"}
         log("cpu baseline:", cpu)
 
     out = {

@@ -444,6 +444,16 @@ class _Fit:
         return self.vector(L.VEC_FVAR)
 
     @property
+    def precision_info(self):
+        """The precision the fit's results come from ("mixed" or "fp64"), whether a
+        mixed fit was redone in FP64, and the last refinement's sweeps and relative
+        correction."""
+        p, f, k, r = C_int32(), C_int32(), C_int32(), C_double()
+        L.check(L.lib().gpl_model_precision(self.handle, C_byref(p), C_byref(f), C_byref(k), C_byref(r)))
+        return {"precision": "mixed" if p.value == L.PREC_MIXED else "fp64", "refit_fp64": bool(f.value),
+                "refine_sweeps": int(k.value), "refine_rel": float(r.value)}
+
+    @property
     def C_chol(self):
         out = np.empty((self.n, self.n), order="F")
         L.check(L.lib().gpl_model_chol(self.handle, out.ctypes.data_as(L._dp)))
@@ -843,10 +853,12 @@ def _fit_fitc(g, x, y, terms, nt, xp, tr, off, lk, o):
     return g
 
 
-def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want_grad=False):
+def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want_grad=False, refine_iters=None):
     """R/gp_fit.R:54-60: learn input scales, build K + jitter I on the device and
     run the Laplace Newton loop or EP; returns a new fitted GP.  want_grad forms
-    L^-1 during the fit (the gradient needs it anyway)."""
+    L^-1 during the fit (the gradient needs it anyway).  refine_iters caps the
+    mixed form's refinement sweeps per solve (default 8); a solve that does not
+    converge within them makes the device redo the fit in FP64."""
     g = gp.copy()
     x = _as_matrix(x)
     y = np.ascontiguousarray(np.asarray(y, dtype=np.float64).ravel())
@@ -874,6 +886,8 @@ def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want
         o.tol = g.approx.tol
     o.jitter = 1e-6 if jitter is None else float(jitter)
     o.flags = 1 if want_grad else 0
+    if refine_iters is not None:
+        o.refine_iters = int(refine_iters)
     prec = precision or g.precision
     o.precision = L.PREC_MIXED if prec == "mixed" else L.PREC_FP64
     g.precision = prec
@@ -1290,8 +1304,18 @@ def gp_optim(gp, x, y, tol=1e-4, tol_param=1e-1, maxiter=500, restarts=1, verbos
             return e, gp_energy_grad(g2) / n
         res = minimize(fg, p0, jac=True, method="L-BFGS-B", options={"maxiter": maxiter, "ftol": tol})
     else:
+        # stats::optim's Nelder-Mead as R/gp_optim.R:87-95 calls it: fnscale = n, stop
+        # when the simplex's energies lie within reltol * (|f(p0)| + reltol) of each
+        # other (no parameter tolerance), maxit counts energy evaluations, and the
+        # first simplex steps every coordinate by 0.1 max|p0| (0.1 if p0 = 0).
+        f0 = energy(p0)[0]
+        step = 0.1 * float(np.max(np.abs(p0))) if np.any(p0 != 0) else 0.1
+        simplex = np.vstack([p0] + [p0 + step * np.eye(len(p0))[k] for k in range(len(p0))])
         res = minimize(lambda p: energy(p)[0], p0, method="Nelder-Mead",
-                       options={"maxiter": maxiter, "fatol": tol, "xatol": 1e-8})
+                       options={"maxfev": maxiter, "maxiter": 10 * maxiter + 1000, "fatol": tol * (abs(f0) + tol),
+                                "xatol": np.inf, "initial_simplex": simplex})
+    if res.status == 1 and warnings_:  # evaluation or iteration cap (R/gp_optim.R:97-99)
+        warnings.warn("Maximum number of iterations reached, optimization may not have converged.")
     popt = res.x
     g = gp_fit(set_param(gp, popt), x, y, **fit_args)
     if warnings_:

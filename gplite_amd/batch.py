@@ -79,19 +79,29 @@ class GPBatch:
 
     def energy_grad(self, include_prior=True, allow_not_pd=False):
         """(energies [nprob], gradients [nprob, P]) in get_param order.  With
-        allow_not_pd a problem whose factor fails gets energy +Inf and NaN
-        gradients instead of raising."""
+        allow_not_pd a problem whose factor fails, or whose refinement does not
+        converge, gets energy +Inf and NaN gradients instead of raising
+        (status() tells which)."""
         np_ = len(self.gps)
         lz = np.empty(np_)
         g = np.empty(np_ * self.ng)
         rc = L.lib().gpl_batch_energy_grad(self.handle, L.dptr(lz), L.dptr(g))
-        L.check(rc, allow=(L.GPL_ERR_NOT_PD,) if allow_not_pd else ())
+        L.check(rc, allow=(L.GPL_ERR_NOT_PD, L.GPL_ERR_NOCONV) if allow_not_pd else ())
         g = g.reshape(np_, self.ng)[:, self._mask]
         e = -lz
         if include_prior:
             e = e - np.array([api.lpdf_prior(gp) for gp in self.gps])
             g = g - np.array([api._lpdf_prior_grad(gp) for gp in self.gps])
         return e, g
+
+    def status(self):
+        """Per-problem status of the last pass (GPL_OK, GPL_ERR_NOT_PD, GPL_ERR_NOCONV)
+        and the last refinement correction relative to each problem's solution."""
+        np_ = len(self.gps)
+        st = np.empty(np_, dtype=np.int32)
+        rel = np.empty(np_)
+        L.check(L.lib().gpl_batch_status(self.handle, st.ctypes.data_as(C.POINTER(C.c_int32)), L.dptr(rel)))
+        return st, rel
 
     def info(self):
         nb = C.c_int32()

@@ -11,6 +11,7 @@
 // candidates) or gpl_batch_set_tuning (a persisted table).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -64,6 +65,8 @@ struct gpl_batch {
     std::vector<hipGraphExec_t> exec;   // one per lane (null: lane without buckets)
     std::vector<Lane> lanes;
     std::vector<int> lane_of;  // bucket -> lane
+    std::vector<int32_t> status;       // per problem, from the last pass
+    std::vector<double> refine_rel;    // per problem, last refinement correction of the last pass
     hipEvent_t fork = nullptr;
     ~gpl_batch() {
         for (auto* b : buckets) delete b;
@@ -83,6 +86,26 @@ struct gpl_batch {
         if (fork) hipEventDestroy(fork);
     }
 };
+
+// A pass runs a fixed number of refinement sweeps.  From the last two corrections
+// c1, c2 (relative to the solution) the contraction is rho = c2 / c1 and the error
+// left after the last sweep about c2 rho / (1 - rho); it must be below this.  A
+// last correction under 1e-11 is at the FP64 residual's rounding floor, where the
+// corrections stop shrinking (rho ~ 1) and the solve is converged.
+static const double kBatchRefineTol = 1e-8;
+
+static std::string fmt_e(double v) {
+    char b[32];
+    snprintf(b, sizeof b, "%.3e", v);
+    return b;
+}
+
+static double refine_error_estimate(double last, double prev) {
+    if (last != last || prev != prev) return NAN;
+    if (last <= 1e-3 * kBatchRefineTol) return last;
+    const double rho = prev > 0.0 ? last / prev : 1.0;
+    return rho < 1.0 ? last * rho / (1.0 - rho) : INFINITY;
+}
 
 // scalars per problem in BB_RED: five fixed slots and the gradient partials
 static int red_stride(const gpl_batch* B) { return 6 + std::max(1, B->hp.ngrad); }
@@ -268,7 +291,9 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, Lane& ln) {
             trmv_lower_f32_batched(s2, X, N, N, t1, dd, 1, nb, N2, N, k.d(BB_KMV), k.nchunk);
         }
         vec_axpby(s2, NV, 1.0, dd, 1.0, u, u);
-        if (it == sweeps - 1) relmax_accumulate(s2, dd, u, NV, k.d(BB_RELMAX));
+        // the last two corrections relative to the solution: their ratio is the contraction
+        if (it == sweeps - 2) seg_relmax(s2, dd, u, nvec, N, k.d(BB_RELMAX) + nb, nb);
+        if (it == sweeps - 1) seg_relmax(s2, dd, u, nvec, N, k.d(BB_RELMAX), nb);
         {  // r = S y - B u with the FP64 copy of B
             Prof p(c, "batch_symv_b64", 2.0 * nb * N * N, 0, s2);
             symv_lt_f64(s2, k.d(BB_B64), N, u, Ku, k.d(BB_KMV), k.nchunk, nb, N2, N);
@@ -458,7 +483,7 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
                   k->b[BB_F].ensure((size_t)nb * N * MAX_FEAT * 8) && k->b[BB_SCAL].ensure(nb * 8) &&
                   k->b[BB_SIG].ensure(nb * 8) && k->b[BB_WSC].ensure(nb * 8) && k->b[BB_IDX].ensure(nb * 4) &&
                   k->b[BB_L].ensure(mat32) && k->b[BB_X32].ensure(mat32) && k->b[BB_BI].ensure(mat32) &&
-                  k->b[BB_B64].ensure(mat32 * 2) && k->b[BB_RELMAX].ensure(64) &&
+                  k->b[BB_B64].ensure(mat32 * 2) && k->b[BB_RELMAX].ensure((size_t)nb * 16) &&
                   k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_DACC].ensure((size_t)nb * N * 64 * 8) &&
                   k->b[BB_LDIAG].ensure(vec) &&
                   k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&
@@ -553,7 +578,7 @@ extern "C" int gpl_batch_set_params(gpl_batch* B, const gpl_cf_term* terms, cons
 
 static void reset_flags(gpl_batch* B, hipStream_t s) {
     // outside any graph: status words are cleared with plain memsets before a pass
-    for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_RELMAX].p, 0, 8, s);
+    for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_RELMAX].p, 0, k->idx.size() * 16, s);
     for (auto* k : B->buckets) hipMemsetAsync(k->b[BB_INFO].p, 0, k->idx.size() * 4, s);
 }
 
@@ -619,30 +644,56 @@ extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double*
         HIP_TRY(hipMemcpyAsync(log_evidence, B->res_logz.p, (size_t)B->nprob * 8, hipMemcpyDeviceToHost, s));
     if (grad) HIP_TRY(hipMemcpyAsync(grad, B->res_grad.p, (size_t)B->nprob * B->ng * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    // non-PD pivots: those problems get log_evidence = -Inf and NaN gradients, the
+    // Non-PD pivots, and refinements whose last correction stays above 1e-8 of the
+    // solution (cond(B) too large for the FP32 inverse factor with the pass's fixed
+    // sweep count): those problems get log_evidence = -Inf and NaN gradients, the
     // others keep their results, and the call reports the first failing problem
-    int first = -1;
+    // (non-PD first).  gpl_batch_status gives every problem's status.
+    int first = -1, first_nc = -1;
     int64_t first_minor = 0;
+    B->status.assign(B->nprob, GPL_OK);
+    B->refine_rel.assign(B->nprob, 0.0);
     for (auto* k : B->buckets) {
-        std::vector<int> info(k->idx.size());
-        HIP_TRY(hipMemcpy(info.data(), k->b[BB_INFO].p, info.size() * 4, hipMemcpyDeviceToHost));
-        for (size_t q = 0; q < info.size(); ++q)
-            if (info[q] > 0) {
-                const int pidx = k->idx[q];
-                if (log_evidence) log_evidence[pidx] = -INFINITY;
-                if (grad)
-                    for (int j = 0; j < B->ng; ++j) grad[(size_t)pidx * B->ng + j] = NAN;
-                if (first < 0 || pidx < first) {
-                    first = pidx;
-                    first_minor = info[q];
-                }
+        const size_t nb = k->idx.size();
+        std::vector<int> info(nb);
+        std::vector<double> rr(2 * nb);
+        HIP_TRY(hipMemcpy(info.data(), k->b[BB_INFO].p, nb * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rr.data(), k->b[BB_RELMAX].p, nb * 16, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < nb; ++q) {
+            const int pidx = k->idx[q];
+            B->refine_rel[pidx] = rr[q];
+            const bool notpd = info[q] > 0;
+            const bool noconv = !(refine_error_estimate(rr[q], rr[nb + q]) <= kBatchRefineTol);
+            if (!notpd && !noconv) continue;
+            B->status[pidx] = notpd ? GPL_ERR_NOT_PD : GPL_ERR_NOCONV;
+            if (log_evidence) log_evidence[pidx] = -INFINITY;
+            if (grad)
+                for (int j = 0; j < B->ng; ++j) grad[(size_t)pidx * B->ng + j] = NAN;
+            if (notpd && (first < 0 || pidx < first)) {
+                first = pidx;
+                first_minor = info[q];
             }
+            if (!notpd && (first_nc < 0 || pidx < first_nc)) first_nc = pidx;
+        }
     }
     if (first >= 0) {
         set_minor(first_minor);
         return fail(GPL_ERR_NOT_PD, "problem " + std::to_string(first) + ": leading minor of order " +
                                         std::to_string(first_minor) + " is not positive");
     }
+    if (first_nc >= 0)
+        return fail(GPL_ERR_NOCONV, "problem " + std::to_string(first_nc) +
+                                        ": iterative refinement did not converge to 1e-8 (last relative "
+                                        "correction " + fmt_e(B->refine_rel[first_nc]) +
+                                        "); refit it in FP64 or raise refine_iters");
+    return GPL_OK;
+}
+
+extern "C" int gpl_batch_status(gpl_batch* B, int32_t* status, double* refine_rel) {
+    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
+    if (B->status.empty()) return fail(GPL_ERR_ARG, "no pass has run yet");
+    if (status) memcpy(status, B->status.data(), B->status.size() * 4);
+    if (refine_rel) memcpy(refine_rel, B->refine_rel.data(), B->refine_rel.size() * 8);
     return GPL_OK;
 }
 
@@ -653,11 +704,7 @@ extern "C" int gpl_batch_info(gpl_batch* B, int32_t* nbuckets, double* refine_re
     if (nbuckets) *nbuckets = (int32_t)B->buckets.size();
     if (refine_rel) {
         double mx = 0.0;
-        for (auto* k : B->buckets) {
-            double v = 0.0;
-            HIP_TRY(hipMemcpy(&v, k->b[BB_RELMAX].p, 8, hipMemcpyDeviceToHost));
-            mx = std::max(mx, v);
-        }
+        for (double v : B->refine_rel) mx = (v != v || mx != mx) ? NAN : std::max(mx, v);
         *refine_rel = mx;
     }
     return GPL_OK;

@@ -240,6 +240,7 @@ struct gpl_model {
     int32_t iters = 0;
     int32_t refine_iters_done = 0;
     double refine_rel = 0;
+    bool refit_fp64 = false;  // mixed form redone in FP64 (refinement did not contract / FP32 pivot failed)
     DevBuf b[B_COUNT];
     ~gpl_model() {
         for (auto& x : b) x.free_();
@@ -596,6 +597,19 @@ static int fit_gaussian(gpl_model* m) {
     return GPL_OK;
 }
 
+// Iterative refinement of the mixed form must contract: cond(B) * 2^-24 well
+// below 1.  A last relative correction above kRefineTol, a correction that grows
+// between sweeps while still above it, or a non-finite one means the FP32
+// inverse factor cannot carry the solve, and the fit is redone in FP64
+// (internal status kRefineFail, never returned to the caller).
+static const double kRefineTol = 1e-9;
+static const int kRefineFail = 1000;
+
+static bool refine_diverging(int sweep, double rel, double prev) {
+    if (!std::isfinite(rel)) return true;
+    return sweep >= 2 && rel >= prev && rel > kRefineTol;
+}
+
 // Gaussian fit in the mixed-precision form.  alpha = S B^-1 S y is obtained by
 // iterative refinement: corrections come from the FP32 inverse factor
 // (d = X^T X r), residuals r = S y - B u are formed in FP64 from the FP64 copy
@@ -653,7 +667,7 @@ static int fit_gaussian_mixed(gpl_model* m) {
     HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
     const int maxit = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
     int it = 0;
-    double rel = INFINITY;
+    double rel = INFINITY, prev = INFINITY;
     for (it = 1; it <= maxit; ++it) {
         {
             Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
@@ -673,9 +687,15 @@ static int fit_gaussian_mixed(gpl_model* m) {
         HIP_TRY(hipStreamSynchronize(s));
         rel = h;
         if (rel < 1e-14) break;
+        if (refine_diverging(it, rel, prev)) break;
+        prev = rel;
     }
     m->refine_iters_done = std::min(it, maxit);
     m->refine_rel = rel;
+    if (!(rel <= kRefineTol)) {
+        int rr = check_info(m);
+        return rr ? rr : kRefineFail;
+    }
     fill_f64(s, m->d(B_ALPHA), N, 0.0);
     vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
     // f = (K + jitter I) alpha = sigma (B u - u), B u from the last sweep (u updated before it)
@@ -757,7 +777,7 @@ static int fit_laplace(gpl_model* m) {
     }
     if (it > maxiter) it = maxiter;
     m->iters = it;
-    if (maxiter > 1 && it == maxiter && !(diff < m->opts.tol)) {
+    if (maxiter > 1 && it == maxiter) {  // also when the last allowed step converged (R/laplace.R:293)
         // the reference only warns here (R/laplace.R:293-295): keep the fit, report NOCONV
         g_err = "Maximum number of iterations in Laplace reached, max delta f = " + std::to_string(diff);
         return GPL_ERR_NOCONV;
@@ -783,8 +803,9 @@ static int refine_alpha_mixed(gpl_model* m, int maxref, int nch) {
     vec_mul(s, n, m->d(B_SV), m->d(B_Z), rhs);
     fill_f64(s, u, N, 0.0);
     HIP_TRY(hipMemcpyAsync(r, rhs, N * 8, hipMemcpyDeviceToDevice, s));
-    double rel = INFINITY;
-    for (int k = 1; k <= maxref; ++k) {
+    double rel = INFINITY, prev = INFINITY;
+    int k;
+    for (k = 1; k <= maxref; ++k) {
         {
             Prof p(c, "refine_solve_f32", 4.0 * N * N / 2, 4.0 * N * N);
             trmv_lower_f32(s, X, N, N, r, t1, 0, m->d(B_KMV), nch);
@@ -802,8 +823,12 @@ static int refine_alpha_mixed(gpl_model* m, int maxref, int nch) {
         HIP_TRY(hipStreamSynchronize(s));
         rel = h;
         if (rel < 1e-14) break;
+        if (refine_diverging(k, rel, prev)) break;
+        prev = rel;
     }
+    m->refine_iters_done = std::min(k, maxref);
     m->refine_rel = rel;
+    if (!(rel <= kRefineTol)) return kRefineFail;
     fill_f64(s, m->d(B_ALPHA), N, 0.0);
     vec_mul(s, n, m->d(B_SV), u, m->d(B_ALPHA));
     return GPL_OK;
@@ -886,7 +911,7 @@ static int fit_laplace_mixed(gpl_model* m) {
     if (it > maxiter) it = maxiter;
     m->iters = it;
     m->have_xinv = true;
-    if (maxiter > 1 && it == maxiter && !(diff < m->opts.tol)) {
+    if (maxiter > 1 && it == maxiter) {  // also when the last allowed step converged (R/laplace.R:293)
         g_err = "Maximum number of iterations in Laplace reached, max delta f = " + std::to_string(diff);
         return GPL_ERR_NOCONV;
     }
@@ -1041,7 +1066,7 @@ static int fit_ep(gpl_model* m) {
     }
     if (it > maxiter) it = maxiter;
     m->iters = it;
-    if (maxiter > 1 && it == maxiter && !(diff_new < tol)) {
+    if (maxiter > 1 && it == maxiter) {  // also when the last allowed sweep converged (R/ep.R:322)
         m->logZ = -INFINITY;
         g_err = "Maximum number of iterations in EP reached, max delta f = " + std::to_string(diff_new);
         return GPL_ERR_NOCONV;
@@ -1164,7 +1189,7 @@ static int fit_ep_mixed(gpl_model* m) {
     }
     if (it > maxiter) it = maxiter;
     m->iters = it;
-    if (maxiter > 1 && it == maxiter && !(diff_new < tol)) {
+    if (maxiter > 1 && it == maxiter) {  // also when the last allowed sweep converged (R/ep.R:322)
         m->logZ = -INFINITY;
         g_err = "Maximum number of iterations in EP reached, max delta f = " + std::to_string(diff_new);
         return GPL_ERR_NOCONV;
@@ -1221,9 +1246,34 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     upload_vec(m, B_TR, trials, n);
     upload_vec(m, B_OFF, offset, n);
     hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
+    m->refit_fp64 = false;
+    m->refine_iters_done = 0;
+    m->refine_rel = 0;
     if (o.approx == GPL_APPROX_EP) r = mixed ? fit_ep_mixed(m) : fit_ep(m);
     else if (lik->kind == GPL_LIK_GAUSSIAN) r = mixed ? fit_gaussian_mixed(m) : fit_gaussian(m);
     else r = mixed ? fit_laplace_mixed(m) : fit_laplace(m);
+    if (mixed && (r == kRefineFail || r == GPL_ERR_NOT_PD)) {
+        // The FP32 factor could not carry this problem (refinement did not contract,
+        // or rounding B to FP32 made a pivot non-positive): redo the fit in FP64,
+        // whose result (or failing minor) is the one the reference would give.
+        hipStreamSynchronize(s);
+        for (int i : {B_B64, B_DACC, B_XINV, B_BINV}) m->b[i].free_();
+        m->opts.precision = GPL_PREC_FP64;
+        m->have_xinv = m->have_binv = false;
+        m->round_corr = 0.0;
+        m->logZ = 0.0;
+        r = alloc_model(m, n, ncols, need_k, false);
+        if (r) {
+            if (fresh) delete m;
+            else *inout = m;
+            return r;
+        }
+        hipMemsetAsync(m->b[B_INFO].p, 0, 64, s);
+        m->refit_fp64 = true;
+        if (o.approx == GPL_APPROX_EP) r = fit_ep(m);
+        else if (lik->kind == GPL_LIK_GAUSSIAN) r = fit_gaussian(m);
+        else r = fit_laplace(m);
+    }
     *inout = m;
     if (log_evidence) *log_evidence = m->logZ;
     if (r && r != GPL_ERR_NOCONV && r != GPL_ERR_NAN) return r;
@@ -1648,6 +1698,16 @@ extern "C" int gpl_model_restore(gpl_ctx* c, const gpl_cf_term* terms, int32_t n
         return fail(GPL_ERR_NOT_PD, "saved C_chol: the diagonal entry " + std::to_string(info) + " is not positive");
     }
     *out = m;
+    return GPL_OK;
+}
+
+extern "C" int gpl_model_precision(gpl_model* m, int32_t* precision, int32_t* refit_fp64, int32_t* refine_sweeps,
+                                   double* refine_rel) {
+    if (!m) return fail(GPL_ERR_ARG, "NULL model");
+    if (precision) *precision = m->opts.precision;
+    if (refit_fp64) *refit_fp64 = m->refit_fp64 ? 1 : 0;
+    if (refine_sweeps) *refine_sweeps = m->refine_iters_done;
+    if (refine_rel) *refine_rel = m->refine_rel;
     return GPL_OK;
 }
 

@@ -232,7 +232,9 @@ void seg_dot(hipStream_t s, const double* a, const double* b, const int64_t* nve
              int ostride, int slot, int batch);
 void seg_cols(hipStream_t s, const double* part, int64_t ntiles, int ng, double* out, int ostride, int slot0,
               int batch);
-void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t n, double* out);
+// per problem: last refinement correction relative to the solution (NaN if non-finite)
+void seg_relmax(hipStream_t s, const double* d, const double* u, const int64_t* nvec, int64_t N, double* out,
+                int batch);
 void batch_finish(hipStream_t s, const double* red, int rs, const int64_t* nvec, const double* sigma,
                   const int32_t* idx, int ng, int nb, double* logz, double* grad, int gstride);
 

@@ -348,31 +348,44 @@ void seg_cols(hipStream_t s, const double* part, int64_t ntiles, int ng, double*
                        slot0);
 }
 
-// max over all entries of |d| / max |u| (whole batch)
-__global__ __launch_bounds__(1024) void k_relmax_all(const double* __restrict__ d, const double* __restrict__ u,
-                                                     int64_t n, double* __restrict__ out) {
-    __shared__ double rd[1024], ru[1024];
+// per problem b: out[b] = max_i |d_i| / max_i |u_i| over its i < n_b (the last
+// refinement correction relative to the solution); NaN when any entry is not a number
+__global__ __launch_bounds__(256) void k_seg_relmax(const double* __restrict__ d, const double* __restrict__ u,
+                                                    const int64_t* __restrict__ nvec, int64_t N,
+                                                    double* __restrict__ out) {
+    __shared__ double rd[256], ru[256];
+    __shared__ int bad[256];
+    const int64_t b = blockIdx.x;
     const int t = threadIdx.x;
+    const int64_t n = nvec[b];
+    const double* pd = d + b * N;
+    const double* pu = u + b * N;
     double a = 0.0, c = 0.0;
-    for (int64_t i = t; i < n; i += 1024) {
-        a = fmax(a, fabs(d[i]));
-        c = fmax(c, fabs(u[i]));
+    int nan = 0;
+    for (int64_t i = t; i < n; i += 256) {
+        const double x = pd[i], y = pu[i];
+        nan |= (x != x) | (y != y);
+        a = fmax(a, fabs(x));
+        c = fmax(c, fabs(y));
     }
     rd[t] = a;
     ru[t] = c;
+    bad[t] = nan;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (t < o) {
             rd[t] = fmax(rd[t], rd[t + o]);
             ru[t] = fmax(ru[t], ru[t + o]);
+            bad[t] |= bad[t + o];
         }
         __syncthreads();
     }
-    if (t == 0) out[0] = fmax(out[0], (ru[0] > 0.0) ? rd[0] / ru[0] : rd[0]);
+    if (t == 0) out[b] = bad[0] ? __longlong_as_double(0x7ff8000000000000LL) : ((ru[0] > 0.0) ? rd[0] / ru[0] : rd[0]);
 }
 
-void relmax_accumulate(hipStream_t s, const double* d, const double* u, int64_t n, double* out) {
-    hipLaunchKernelGGL(k_relmax_all, dim3(1), dim3(1024), 0, s, d, u, n, out);
+void seg_relmax(hipStream_t s, const double* d, const double* u, const int64_t* nvec, int64_t N, double* out,
+                int batch) {
+    hipLaunchKernelGGL(k_seg_relmax, dim3((unsigned)batch), dim3(256), 0, s, d, u, nvec, N, out);
 }
 
 // Per problem: log evidence and gradient from the segment sums.

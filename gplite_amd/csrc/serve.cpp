@@ -5,7 +5,9 @@
 // request that can run now: a prediction needs its model fitted, and every
 // request needs its estimated HBM footprint to fit in the budget next to the
 // resident models and the work in flight (idle resident models are evicted,
-// least recently used first, to make room).  Several workers keep requests
+// least recently used first, to make room, but only for a request that is then
+// admitted; an evicted model keeps its fit inputs on the host and is refitted
+// when a prediction asks for it again).  Several workers keep requests
 // overlapping on the device.  Queued predictions against the same resident
 // model are coalesced into one launch (their test points are concatenated;
 // every output column is computed independently of the others, so a result is
@@ -39,16 +41,23 @@ double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-struct Req {
-    int64_t ticket = 0;
-    int type = REQ_FIT;
-    int64_t model_id = 0;
-    // fit
+// The inputs of a fit, kept on the host while the model exists (an evicted model
+// is refitted from them).
+struct FitSpec {
     std::vector<gpl_cf_term> terms;
     std::vector<double> x, y, trials, offset;
     int64_t n = 0, ncols = 0;
     gpl_lik lik{};
     gpl_opts opts{};
+};
+
+struct Req {
+    int64_t ticket = 0;
+    int type = REQ_FIT;
+    int64_t model_id = 0;
+    bool internal = false;  // refit of an evicted model: nobody waits for it
+    // fit
+    std::shared_ptr<const FitSpec> spec;
     // prediction
     std::vector<double> xt;
     int64_t m = 0;
@@ -69,11 +78,15 @@ struct Resident {
     gpl_model* model = nullptr;
     int64_t n = 0, ncols = 0;
     size_t bytes = 0;
-    bool ready = false;
+    bool ready = false;      // fitted and resident
+    bool fitting = false;    // a fit (first or after eviction) is queued or running
+    bool evicted = false;    // fitted once, factors freed; refitted on the next prediction
+    bool releasing = false;  // gpl_server_release_model is waiting for it
     bool failed = false;
     int status = GPL_OK;
     double last_use = 0;
     int refs = 0;  // queued or running predictions
+    std::shared_ptr<const FitSpec> spec;
 };
 
 size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
@@ -108,67 +121,143 @@ struct gpl_server {
     std::deque<Req*> queue;
     std::map<int64_t, std::unique_ptr<Req>> reqs;
     std::map<int64_t, Resident> models;
+    // Device models no longer resident, kept for reuse by later fits (gpl_fit
+    // recycles their buffers): freeing device memory waits for the whole device,
+    // which would stall every worker.  Their bytes stay counted in in_use.
+    struct Spare {
+        gpl_model* model;
+        size_t bytes;
+        int worker;  // a model is refitted only on the context that made it
+    };
+    std::vector<Spare> spare;
     bool stop = false;
     int64_t next_ticket = 1, next_model = 1;
     int64_t done_fits = 0, done_preds = 0, launches = 0, evictions = 0;
 };
 
-// Must hold s->mu.  Pick the oldest runnable request (and, for a prediction,
-// the queued predictions on the same model that coalesce with it).
-// Evict idle resident models (no queued or running predictions), least recently
-// used first, until `need` more bytes fit in the budget; `keep` is never evicted.
-// Caller holds s->mu.
+static void release_bytes(gpl_server* s, size_t b) { s->in_use -= std::min(s->in_use, b); }
+
+// Caller holds s->mu.  A model leaves residency: its device memory goes to the
+// spare list (still counted) for a later fit on the same worker to reuse.
+static void retire_model(gpl_server* s, gpl_model* m, size_t bytes, int worker) {
+    if (m) s->spare.push_back({m, bytes, worker});
+}
+
+static bool evictable(const std::pair<const int64_t, Resident>& kv, int64_t keep) {
+    const Resident& md = kv.second;
+    return md.ready && md.refs == 0 && md.model && !md.releasing && kv.first != keep;
+}
+
+// Bytes that spare models and evictable idle resident models would give back.
+static size_t reclaimable(gpl_server* s, int64_t keep) {
+    size_t b = 0;
+    for (auto& sp : s->spare) b += sp.bytes;
+    for (auto& kv : s->models)
+        if (evictable(kv, keep)) b += kv.second.bytes;
+    return b;
+}
+
+// Caller holds s->mu.  Free spare models, then evict idle resident models (least
+// recently used first), until `need` more bytes fit in the budget; `keep` is never
+// evicted.  An evicted model keeps its entry and fit inputs.
 static void evict_until(gpl_server* s, size_t need, int64_t keep) {
+    while (s->in_use + need > s->budget && !s->spare.empty()) {
+        gpl_model_free(s->spare.back().model);
+        release_bytes(s, s->spare.back().bytes);
+        s->spare.pop_back();
+    }
     while (s->in_use + need > s->budget) {
         int64_t victim = -1;
         double oldest = 1e300;
-        for (auto& kv : s->models) {
-            const Resident& md = kv.second;
-            if (md.ready && md.refs == 0 && md.model && md.last_use < oldest && kv.first != keep) {
-                oldest = md.last_use;
+        for (auto& kv : s->models)
+            if (evictable(kv, keep) && kv.second.last_use < oldest) {
+                oldest = kv.second.last_use;
                 victim = kv.first;
             }
-        }
         if (victim < 0) return;
         Resident& md = s->models[victim];
         gpl_model_free(md.model);
-        s->in_use -= std::min(s->in_use, md.bytes);
-        s->models.erase(victim);
+        release_bytes(s, md.bytes);
+        md.model = nullptr;
+        md.bytes = 0;
+        md.ready = false;
+        md.evicted = true;
         s->evictions++;
     }
 }
 
+// Caller holds s->mu.  A spare device model of worker w to refit into (the
+// smallest that holds `need` bytes, else the largest), or nullptr.  Its bytes
+// leave in_use: the fit's footprint, counted from admission, covers them.
+static gpl_model* take_spare(gpl_server* s, size_t need, int w) {
+    int best = -1;
+    for (int i = 0; i < (int)s->spare.size(); ++i) {
+        if (s->spare[i].worker != w) continue;
+        if (best < 0) {
+            best = i;
+            continue;
+        }
+        const size_t b = s->spare[i].bytes, bb = s->spare[best].bytes;
+        const bool fits = b >= need, bfits = bb >= need;
+        if ((fits && (!bfits || b < bb)) || (!fits && !bfits && b > bb)) best = i;
+    }
+    if (best < 0) return nullptr;
+    gpl_model* m = s->spare[best].model;
+    release_bytes(s, s->spare[best].bytes);
+    s->spare.erase(s->spare.begin() + best);
+    return m;
+}
+
+static void complete_with(gpl_server* s, Req* r, int status) {
+    r->status = status;
+    r->state = ST_DONE;
+    r->t_start = r->t_done = now_ms();
+    s->cv_done.notify_all();
+}
+
+// Caller holds s->mu.  Pick the oldest runnable request (and, for a prediction,
+// the queued predictions on the same model that coalesce with it).  A request is
+// admitted when its footprint fits the budget, after reclaiming spare and idle
+// memory if that suffices; with nothing running it is admitted anyway, so the
+// queue cannot stall.  Nothing is evicted for a request that is not admitted.
 static std::vector<Req*> take_runnable(gpl_server* s) {
     std::vector<Req*> out;
-    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
         Req* r = *it;
         if (r->type == REQ_PRED) {
             auto mit = s->models.find(r->model_id);
             if (mit == s->models.end() || mit->second.failed) {  // unknown or failed model: complete with an error
-                r->status = (mit == s->models.end()) ? GPL_ERR_ARG : mit->second.status;
-                r->state = ST_DONE;
-                r->t_start = r->t_done = now_ms();
                 if (mit != s->models.end()) mit->second.refs--;
-                s->queue.erase(it);
-                s->cv_done.notify_all();
-                return take_runnable(s);
+                it = s->queue.erase(it);
+                complete_with(s, r, mit == s->models.end() ? GPL_ERR_ARG : mit->second.status);
+                continue;
             }
-            if (!mit->second.ready) continue;
+            if (!mit->second.ready) {
+                ++it;
+                continue;
+            }
         }
-        // admission by footprint, evicting idle resident models (LRU) if needed
-        evict_until(s, r->footprint, r->type == REQ_PRED ? r->model_id : -1);
-        // wait for memory while running work will free some; with nothing running and
-        // nothing left to evict the request is admitted anyway, so the queue cannot stall
-        if (s->in_use + r->footprint > s->budget && s->inflight > 0) continue;
+        const int64_t keep = r->type == REQ_PRED ? r->model_id : -1;
+        if (s->in_use + r->footprint > s->budget) {
+            const size_t rec = reclaimable(s, keep);
+            const size_t floor = s->in_use - std::min(s->in_use, rec);
+            if (floor + r->footprint > s->budget && s->inflight > 0) {
+                ++it;
+                continue;
+            }
+            evict_until(s, r->footprint, keep);
+        }
         out.push_back(r);
         s->queue.erase(it);
         if (r->type == REQ_PRED) {
             int64_t tot = r->m;
+            size_t fp = r->footprint;
             for (auto jt = s->queue.begin(); jt != s->queue.end();) {
                 Req* q = *jt;
                 if (q->type == REQ_PRED && q->model_id == r->model_id && q->want_var == r->want_var &&
-                    tot + q->m <= s->max_coalesce && s->in_use + r->footprint + q->footprint <= s->budget) {
+                    tot + q->m <= s->max_coalesce && s->in_use + fp + q->footprint <= s->budget) {
                     tot += q->m;
+                    fp += q->footprint;
                     out.push_back(q);
                     jt = s->queue.erase(jt);
                 } else {
@@ -186,6 +275,7 @@ static void worker_main(gpl_server* s, int w) {
     for (;;) {
         std::vector<Req*> job;
         size_t fp = 0;
+        gpl_model* m = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -204,31 +294,35 @@ static void worker_main(gpl_server* s, int w) {
             s->in_use += fp;
             s->inflight += fp;
             s->launches++;
+            if (job[0]->type == REQ_FIT) m = take_spare(s, job[0]->footprint, w);
+            else m = s->models.find(job[0]->model_id)->second.model;  // refs > 0: resident
         }
         Req* r0 = job[0];
         if (r0->type == REQ_FIT) {
-            gpl_model* m = nullptr;
+            const FitSpec& f = *r0->spec;
             double lz = 0;
-            int st = gpl_fit(c, r0->terms.data(), (int32_t)r0->terms.size(), r0->x.data(), r0->n, r0->ncols,
-                             r0->y.data(), r0->trials.empty() ? nullptr : r0->trials.data(),
-                             r0->offset.empty() ? nullptr : r0->offset.data(), &r0->lik, &r0->opts, &m, &lz);
+            int st = gpl_fit(c, f.terms.data(), (int32_t)f.terms.size(), f.x.data(), f.n, f.ncols, f.y.data(),
+                             f.trials.empty() ? nullptr : f.trials.data(),
+                             f.offset.empty() ? nullptr : f.offset.data(), &f.lik, &f.opts, &m, &lz);
             std::unique_lock<std::mutex> lk(s->mu);
             r0->status = st;
             r0->log_evidence = lz;
-            Resident& md = s->models[r0->model_id];
-            s->in_use -= std::min(s->in_use, fp);
+            release_bytes(s, fp);
             s->inflight -= std::min(s->inflight, fp);
+            Resident& md = s->models.find(r0->model_id)->second;  // release waits while fitting
+            md.fitting = false;
             if (m && (st == GPL_OK || st == GPL_ERR_NOCONV)) {
                 gpl_model_info(m, nullptr, &r0->iters, nullptr);
                 md.model = m;
                 md.ready = true;
+                md.evicted = false;
                 md.bytes = (size_t)gpl_model_bytes(m);
                 s->in_use += md.bytes;
-                md.last_use = now_ms();
                 // the resident model may be larger than the admission estimate
                 evict_until(s, 0, r0->model_id);
             } else {
-                if (m) gpl_model_free(m);
+                retire_model(s, m, m ? (size_t)gpl_model_bytes(m) : 0, w);
+                if (m) s->in_use += (size_t)gpl_model_bytes(m);
                 md.failed = true;
                 md.status = st;
             }
@@ -236,12 +330,8 @@ static void worker_main(gpl_server* s, int w) {
             r0->state = ST_DONE;
             r0->t_done = now_ms();
             s->done_fits++;
+            if (r0->internal) s->reqs.erase(r0->ticket);
         } else {
-            gpl_model* m;
-            {
-                std::unique_lock<std::mutex> lk(s->mu);
-                m = s->models[r0->model_id].model;
-            }
             int64_t tot = 0;
             for (Req* r : job) tot += r->m;
             const int64_t ncols = (int64_t)(r0->xt.size() / std::max<int64_t>(r0->m, 1));
@@ -262,9 +352,9 @@ static void worker_main(gpl_server* s, int w) {
                 off += r->m;
             }
             std::unique_lock<std::mutex> lk(s->mu);
-            s->in_use -= std::min(s->in_use, fp);
+            release_bytes(s, fp);
             s->inflight -= std::min(s->inflight, fp);
-            Resident& md = s->models[r0->model_id];
+            Resident& md = s->models.find(r0->model_id)->second;
             md.refs -= (int)job.size();
             md.last_use = now_ms();
             const double t = now_ms();
@@ -307,20 +397,22 @@ extern "C" int gpl_server_submit_fit(gpl_server* s, const gpl_cf_term* terms, in
                                      int64_t* model_id, int64_t* ticket) {
     if (!s || !terms || !x || !y || !lik || !model_id || !ticket || n < 1 || nterms < 1)
         return fail(GPL_ERR_ARG, "bad fit request");
+    auto spec = std::make_shared<FitSpec>();
+    spec->terms.assign(terms, terms + nterms);
+    spec->x.assign(x, x + n * ncols);
+    spec->y.assign(y, y + n);
+    if (trials) spec->trials.assign(trials, trials + n);
+    if (offset) spec->offset.assign(offset, offset + n);
+    spec->n = n;
+    spec->ncols = ncols;
+    spec->lik = *lik;
+    if (opts) spec->opts = *opts;
+    else gpl_opts_default(&spec->opts);
     auto r = std::make_unique<Req>();
     r->type = REQ_FIT;
-    r->terms.assign(terms, terms + nterms);
-    r->x.assign(x, x + n * ncols);
-    r->y.assign(y, y + n);
-    if (trials) r->trials.assign(trials, trials + n);
-    if (offset) r->offset.assign(offset, offset + n);
-    r->n = n;
-    r->ncols = ncols;
-    r->lik = *lik;
-    if (opts) r->opts = *opts;
-    else gpl_opts_default(&r->opts);
-    r->footprint = fit_footprint(n, r->lik, r->opts);
+    r->footprint = fit_footprint(n, spec->lik, spec->opts);
     if (r->footprint > s->budget) return fail(GPL_ERR_OOM, "fit request exceeds the memory budget");
+    r->spec = spec;
     r->t_submit = now_ms();
     std::lock_guard<std::mutex> lk(s->mu);
     r->ticket = s->next_ticket++;
@@ -328,6 +420,8 @@ extern "C" int gpl_server_submit_fit(gpl_server* s, const gpl_cf_term* terms, in
     Resident& md = s->models[r->model_id];
     md.n = n;
     md.ncols = ncols;
+    md.fitting = true;
+    md.spec = spec;
     *model_id = r->model_id;
     *ticket = r->ticket;
     s->queue.push_back(r.get());
@@ -348,8 +442,24 @@ extern "C" int gpl_server_submit_pred(gpl_server* s, int64_t model_id, const dou
     r->var = var;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->models.find(model_id);
-    if (it == s->models.end()) return fail(GPL_ERR_ARG, "unknown model id");
-    r->xt.assign(xt, xt + m * it->second.ncols);
+    if (it == s->models.end() || it->second.releasing) return fail(GPL_ERR_ARG, "unknown model id");
+    Resident& md = it->second;
+    if (md.evicted && !md.fitting) {
+        // refit the evicted model from its kept inputs; this prediction waits for it
+        auto f = std::make_unique<Req>();
+        f->type = REQ_FIT;
+        f->internal = true;
+        f->spec = md.spec;
+        f->model_id = model_id;
+        f->footprint = fit_footprint(md.spec->n, md.spec->lik, md.spec->opts);
+        f->t_submit = now_ms();
+        f->ticket = s->next_ticket++;
+        md.fitting = true;
+        md.evicted = false;
+        s->queue.push_back(f.get());
+        s->reqs[f->ticket] = std::move(f);
+    }
+    r->xt.assign(xt, xt + m * md.ncols);
     r->footprint = pred_footprint(it->second.n, m);
     r->t_submit = now_ms();
     r->ticket = s->next_ticket++;
@@ -385,11 +495,18 @@ extern "C" int gpl_server_release_model(gpl_server* s, int64_t model_id) {
     if (!s) return fail(GPL_ERR_ARG, "NULL server");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->models.find(model_id);
-    if (it == s->models.end()) return fail(GPL_ERR_ARG, "unknown model id");
-    s->cv_done.wait(lk, [&] { return it->second.refs == 0 && (it->second.ready || it->second.failed); });
+    if (it == s->models.end() || it->second.releasing) return fail(GPL_ERR_ARG, "unknown model id");
+    it->second.releasing = true;  // no new predictions, never evicted, no second release
+    // wait for queued and running predictions and any fit in progress; the entry
+    // is looked up again after every wake (the map may have changed meanwhile)
+    s->cv_done.wait(lk, [&] {
+        const Resident& md = s->models.find(model_id)->second;
+        return md.refs == 0 && !md.fitting;
+    });
+    it = s->models.find(model_id);
     if (it->second.model) {
         gpl_model_free(it->second.model);
-        s->in_use -= std::min(s->in_use, it->second.bytes);
+        release_bytes(s, it->second.bytes);
     }
     s->models.erase(it);
     s->cv_work.notify_all();
@@ -424,6 +541,7 @@ extern "C" void gpl_server_destroy(gpl_server* s) {
     for (auto& t : s->workers) t.join();
     for (auto& kv : s->models)
         if (kv.second.model) gpl_model_free(kv.second.model);
+    for (auto& sp : s->spare) gpl_model_free(sp.model);
     for (auto* c : s->ctxs) gpl_ctx_destroy(c);
     delete s;
 }

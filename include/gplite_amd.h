@@ -191,6 +191,15 @@ int gpl_model_restore(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, co
                       int64_t ncols, const gpl_lik* lik, const gpl_opts* opts, const double* c_chol,
                       const double* const* vectors, double log_evidence, int32_t iters, gpl_model** out);
 int gpl_model_info(gpl_model* model, int64_t* n, int32_t* iters, double* log_evidence);
+/* Precision bookkeeping of the last fit (new: the reference has one precision).
+ * precision: GPL_PREC_MIXED, or GPL_PREC_FP64 for a plain fit and for a mixed fit
+ * that was redone in FP64 (refit_fp64 = 1) because its iterative refinement did
+ * not contract to a last relative correction of 1e-9 (cond(B) too large for the
+ * FP32 factor) or the FP32 factor met a non-positive pivot; refine_sweeps and
+ * refine_rel: sweeps and last relative correction of the mixed form's last
+ * refinement (0 for a plain fit). */
+int gpl_model_precision(gpl_model* model, int32_t* precision, int32_t* refit_fp64, int32_t* refine_sweeps,
+                        double* refine_rel);
 void gpl_model_free(gpl_model* model);
 
 /* ---- FITC (method_fitc) ------------------------------------------------- */
@@ -242,13 +251,20 @@ int gpl_batch_create(gpl_ctx* ctx, int32_t nprob, const gpl_cf_term* terms, int3
 /* New hyperparameters (same structure) for every problem. */
 int gpl_batch_set_params(gpl_batch* batch, const gpl_cf_term* terms, const double* sigma);
 /* log_evidence[nprob]; grad[nprob * gpl_batch_grad_len()] in the layout of gpl_grad_len.
- * A problem whose factor meets a non-positive pivot gets log_evidence = -Inf and NaN
- * gradients; the others are complete, and the call returns GPL_ERR_NOT_PD naming the
- * first such problem. */
+ * A problem whose factor meets a non-positive pivot, or whose refinement does not
+ * converge, gets log_evidence = -Inf and NaN gradients; the others are complete, and
+ * the call returns GPL_ERR_NOT_PD (or, with no such problem, GPL_ERR_NOCONV) naming
+ * the first such problem. */
 int gpl_batch_energy_grad(gpl_batch* batch, double* log_evidence, double* grad);
 int32_t gpl_batch_grad_len(gpl_batch* batch);
-/* Number of buckets and the largest relative last refinement correction. */
+/* Number of buckets and the largest relative last refinement correction of the last pass. */
 int gpl_batch_info(gpl_batch* batch, int32_t* nbuckets, double* refine_rel);
+/* Per-problem outcome of the last gpl_batch_energy_grad: status[nprob] (GPL_OK,
+ * GPL_ERR_NOT_PD, or GPL_ERR_NOCONV when the fixed refinement sweeps leave an
+ * estimated error above 1e-8 of the solution -- last correction c2 times
+ * rho / (1 - rho), rho = c2 / c1 the contraction of the last two sweeps; cond(B)
+ * too large for the FP32 factor) and refine_rel[nprob] = c2; either may be NULL. */
+int gpl_batch_status(gpl_batch* batch, int32_t* status, double* refine_rel);
 /* Autotune: times GEMM tile edge {64,128} x outer panel {128,256,512,1024} per bucket,
  * keeps the fastest; writes up to cap rows (bucket n, tile, panel, ms) and the
  * number of buckets. */

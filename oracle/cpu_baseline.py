@@ -21,6 +21,19 @@ import scipy.linalg as sla
 from scipy.linalg import lapack
 
 
+def host_threads():
+    """(nproc, threads): the CPUs this process may run on, and the thread count the
+    baseline uses -- all of them, unless OMP_NUM_THREADS names the host's share
+    (a machine that gives each GPU a fixed share of its cores sets it)."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(nproc, int(env)) if env.isdigit() and int(env) > 0 else nproc
+    return nproc, threads
+
+
 def _threads():
     """Worker count for the elementwise passes: the BLAS thread count in force
     (so threadpool_limits(1) gives a single-thread run), else the CPU count."""

@@ -719,6 +719,10 @@ def ep(lik, K, y, maxiter=100, tol=1e-4, damping=0.9, damping_min=0.1, quad_orde
     diff_new = np.inf
     for it in range(1, maxiter + 1):
         fit = ep_iter(lik, K, y, fmean_old, fvar_old, z_old, P_old, quad_order, damp, **kw)
+        if fit.get("nan"):
+            # R/ep.R:26-27 returns list(fmean_new = fmean_old, log_evidence = -Inf); `fit$fmean`
+            # then partially matches fmean_new, so diff_new = 0 < tol and the loop ends here
+            break
         diff_new = float(np.max(np.abs(fmean_old - fit["fmean"])))
         if diff_new > diff_old:
             damp = damp * damping

@@ -59,3 +59,22 @@ def test_multistart_reaches_a_stationary_point(gpl):
     e_multi, e_single = G.gp_energy(gbest), G.gp_energy(single)
     assert e_multi <= e_single + 1e-6 * abs(e_single)
     assert info["passes"] >= info["iterations"].max()
+
+
+def test_nelder_mead_stops_by_the_r_rule(gpl):
+    """gp_optim's default Nelder-Mead (stats::optim's stopping rule: energies of the
+    simplex within reltol (|f0| + reltol) with fnscale = n, evaluations capped at
+    maxiter) reaches the optimum that L-BFGS-B on the device gradient finds."""
+    import warnings
+    G = gpl
+    r = np.random.default_rng(12)
+    x = r.uniform(-2, 2, (120, 1))
+    y = np.sin(2 * x[:, 0]) + 0.2 * r.standard_normal(120)
+    gp = G.gp_init(G.cf_sexp(lscale=1.0, magn=1.0), G.lik_gaussian(sigma=0.5))
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        nm = G.gp_optim(gp, x, y, verbose=False, tol=1e-8)
+    lb = G.gp_optim(gp, x, y, method="L-BFGS-B", verbose=False, warnings_=False, tol=1e-12)
+    assert G.gp_energy(nm) == pytest.approx(G.gp_energy(lb), abs=1e-3)
+    with pytest.warns(UserWarning, match="Maximum number of iterations"):
+        G.gp_optim(gp, x, y, verbose=False, maxiter=5, restarts=0)

@@ -288,6 +288,42 @@ def test_laplace_mode_fixed_point(lik):
     assert fit["log_evidence"] == pytest.approx(lz, rel=1e-6)
 
 
+@pytest.mark.parametrize("lik", [{"kind": "bernoulli", "link": "logit"}, {"kind": "poisson", "link": "log"}])
+def test_laplace_evidence_uses_the_forward_difference_curvature(lik):
+    """The reference's W is a forward difference of the analytic gradient with
+    eps = 1e-6 (R/lik.R:449-454): W_fd = W + dW with dW_i = -(eps/2) g'''(f_i) to
+    first order.  At the mode (W-independent, converged to 1e-11 so that W of the
+    previous iterate equals W at the mode), the evidence then differs from the one
+    with the analytic W by -1/2 [log|B(W_fd)| - log|B(W)|] = -1/2 sum_i S_ii dW_i,
+    S = (K^-1 + W)^-1 the Laplace posterior covariance.  The observed gap must
+    match that prediction, which is far from zero: an oracle using the analytic
+    W, a central difference, or W of the new iterate fails here."""
+    x, y = _class_data(n=80, seed=21, kind="poisson" if lik["kind"] == "poisson" else "bernoulli")
+    cfs = [{"kind": "sexp", "lscale": 0.8, "magn": 2.0 if lik["kind"] != "poisson" else 0.7}]
+    fit = O.gp_fit(cfs, lik, x, y, maxiter=200, tol=1e-11)
+    f, K, n = fit["fmean"], fit["K"], len(y)
+    # analytic derivatives of log p(y|f) in f: logit g'' = -mu(1-mu), g''' = -mu(1-mu)(1-2mu);
+    # log link g'' = g''' = -exp(f)
+    if lik["kind"] == "poisson":
+        g2 = -np.exp(f)
+        g3 = -np.exp(f)
+    else:
+        mu = 1 / (1 + np.exp(-f))
+        g2 = -mu * (1 - mu)
+        g3 = -mu * (1 - mu) * (1 - 2 * mu)
+    W = -g2
+    sW = np.sqrt(W)
+    Lb = sla.cholesky(np.eye(n) + sW[:, None] * K * sW[None, :], lower=True)
+    a = O.loglik_d(lik, f, y)
+    lz_analytic = O.loglik(lik, f, y) - 0.5 * a @ f - np.sum(np.log(np.diag(Lb)))
+    S = K - K @ (sW[:, None] * sla.cho_solve((Lb, True), sW[:, None] * K))
+    dW = -0.5e-6 * g3
+    predicted = -0.5 * np.sum(np.diag(S) * dW)
+    observed = fit["log_evidence"] - lz_analytic
+    assert abs(predicted) > 1e-8
+    assert observed == pytest.approx(predicted, rel=2e-2, abs=1e-11)
+
+
 def test_ep_single_site_is_exact():
     """n=1: the EP evidence equals log int N(f|0,k) p(y|f) df."""
     for lik, y in (({"kind": "bernoulli", "link": "probit"}, 1.0), ({"kind": "bernoulli", "link": "logit"}, 0.0),
