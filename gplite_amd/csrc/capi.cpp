@@ -46,6 +46,10 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     HIP_TRY(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(GPL_ERR_ARG, "no such device");
     HIP_TRY(hipSetDevice(device));
+    {
+        const int r = noalloc_init(device);
+        if (r) return r;
+    }
     gpl_ctx* c = new gpl_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1359,10 +1363,39 @@ static int sub_program(gpl_ctx* c, hipStream_t s, const gpl_model* m, const int3
     return GPL_OK;
 }
 
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t gpl::pred_bufs_bytes(int64_t N, int64_t ncols, int64_t M) {
+    return align256((size_t)M * std::max<int64_t>(ncols, 1) * 8) + align256((size_t)M * MAX_FEAT * 8) +
+           2 * align256((size_t)N * M * 8) + align256((size_t)M * 3 * 8);
+}
+
+PredBufs gpl::pred_bufs_carve(void* base, int64_t N, int64_t ncols, int64_t M) {
+    char* p = (char*)base;
+    PredBufs b;
+    b.xt = (double*)p;
+    p += align256((size_t)M * std::max<int64_t>(ncols, 1) * 8);
+    b.Ft = (double*)p;
+    p += align256((size_t)M * MAX_FEAT * 8);
+    b.KtT = (double*)p;
+    p += align256((size_t)N * M * 8);
+    b.A2 = (double*)p;
+    p += align256((size_t)N * M * 8);
+    b.v = (double*)p;
+    return b;
+}
+
+void gpl::model_dims(const gpl_model* m, int64_t* N, int64_t* ncols) {
+    *N = m->N;
+    *ncols = m->ncols;
+}
+
+gpl_ctx* gpl::model_ctx(const gpl_model* m) { return m->ctx; }
+
 // Prediction on the streams of context c (which may differ from the context
 // that fitted the model: the model's device buffers are read-only here).
 int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean,
-                 double* var, const int32_t* terms, int32_t nsel) {
+                 double* var, const int32_t* terms, int32_t nsel, const PredBufs* pb) {
     if (!c || !m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
     if (mt <= 0) return GPL_OK;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1378,12 +1411,13 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
         if (r) return r;
         hp = &sub;
     }
-    double* xtd = (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
-    double* Ft = (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
-    double* KtT = (double*)c->buf(2, (size_t)N * M * 8);
-    double* vbuf = (double*)c->buf(3, (size_t)M * 3 * 8);
+    double* xtd = pb ? pb->xt : (double*)c->buf(0, (size_t)mt * std::max<int64_t>(m->ncols, 1) * 8);
+    double* Ft = pb ? pb->Ft : (double*)c->buf(1, (size_t)M * MAX_FEAT * 8);
+    double* KtT = pb ? pb->KtT : (double*)c->buf(2, (size_t)N * M * 8);
+    double* vbuf = pb ? pb->v : (double*)c->buf(3, (size_t)M * 3 * 8);
     if (!xtd || !Ft || !KtT || !vbuf) return fail(GPL_ERR_OOM, "device allocation failed");
-    if (mt * m->ncols > 0) HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
+    if (mt * m->ncols > 0 && !(pb && pb->staged))
+        HIP_TRY(hipMemcpyAsync(xtd, xt, mt * m->ncols * 8, hipMemcpyHostToDevice, s));
     launch_features(s, xtd, mt, mt, (int)m->ncols, pf, m->prog, Ft, M);
     {
         Prof p(c, "pred_cross", 0, 8.0 * N * M);
@@ -1395,7 +1429,7 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
         scale_rows_f64(s, KtT, N, n, M, m->d(B_SV));
         const int64_t T = N / 64;
         if (m->have_xinv) {
-            double* A2 = (double*)c->buf(4, (size_t)N * M * 8);
+            double* A2 = pb ? pb->A2 : (double*)c->buf(4, (size_t)N * M * 8);
             if (!A2) return fail(GPL_ERR_OOM, "device allocation failed");
             Prof p(c, "pred_trmm", (double)N * N * M, 8.0 * N * M * 2);
             if (m->opts.precision == GPL_PREC_MIXED)

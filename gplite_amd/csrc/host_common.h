@@ -31,9 +31,25 @@ void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag
 void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X);
 // probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
 void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
-// terms/nsel (optional): predict with the model's terms terms[0..nsel) only (cfind)
+// Device buffers of one prediction launch over M = pad64(mt) test points (see
+// pred_bufs_bytes); when given to pred_on they replace the context's own scratch.
+struct PredBufs {
+    double *xt, *Ft, *KtT, *v, *A2;
+    bool staged = false;  // the test inputs are already in xt (leading dimension mt)
+};
+// Bytes of the PredBufs for a model of padded size N with ncols input columns and
+// M test-point rows, each part aligned to 256 bytes; carve them out of `base`.
+size_t pred_bufs_bytes(int64_t N, int64_t ncols, int64_t M);
+PredBufs pred_bufs_carve(void* base, int64_t N, int64_t ncols, int64_t M);
+// padded size and input columns of a fitted model, and the context that owns it
+void model_dims(const gpl_model* m, int64_t* N, int64_t* ncols);
+gpl_ctx* model_ctx(const gpl_model* m);
+// device address of gpl_pred_noalloc's scratch array on `device` (current), once
+int noalloc_init(int device);
+// terms/nsel (optional): predict with the model's terms terms[0..nsel) only (cfind);
+// pb (optional): caller-provided device buffers instead of the context's scratch
 int pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean, double* var,
-            const int32_t* terms = nullptr, int32_t nsel = 0);
+            const int32_t* terms = nullptr, int32_t nsel = 0, const PredBufs* pb = nullptr);
 
 }  // namespace gpl
 

@@ -333,6 +333,19 @@ int gpl_draw_prior(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const
 /* Device bytes held by a fitted model. */
 int64_t gpl_model_bytes(gpl_model* model);
 
+/* No counterpart in the reference: gpl_pred for callers that cannot allocate
+ * device memory (serving loops that must not stall on hipMalloc / hipFree,
+ * harnesses that forbid allocation).  Same arguments and results as gpl_pred
+ * (bit for bit), with every scratch buffer taken from one static device array of
+ * GPL_NOALLOC_SCRATCH_MB MiB per device that the library holds until the process
+ * ends; the test points are processed in chunks that fit it.  A test point
+ * needs 16 N bytes (N = n rounded up to 64) and a chunk at least 64 points, so
+ * models up to n = 65408 are accepted; larger ones give GPL_ERR_ARG (use
+ * gpl_pred).  Calls on one device are serialised by the library (they share the
+ * array); the array keeps the last call's contents. */
+#define GPL_NOALLOC_SCRATCH_MB 64
+int gpl_pred_noalloc(gpl_model* model, const double* xt, int64_t m, int32_t want_var, double* mean, double* var);
+
 #ifdef __cplusplus
 }
 #endif
