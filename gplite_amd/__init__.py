@@ -10,7 +10,7 @@ from . import _lib
 from .api import (  # noqa: F401
     GP, approx_ep, approx_laplace, cf_const, cf_lin, cf_matern32, cf_matern52, cf_nn, cf_periodic, cf_prod,
     cf_sexp, eval_cf, get_param, gp_compare, gp_draw, gp_energy, gp_energy_grad, gp_fit, gp_init, gp_load, gp_loo, gp_optim,
-    gp_pred, gp_save, is_fitted, lik_bernoulli, lik_binomial, lik_gaussian, lik_poisson, lpdf_prior, method_fitc,
+    gp_pred, gp_save, is_fitted, lik_bernoulli, lik_betabinom, lik_binomial, lik_gaussian, lik_poisson, lpdf_prior, method_fitc,
     method_full, prior_fixed, prior_half_t, prior_lognormal, prior_logunif, set_param,
 )
 

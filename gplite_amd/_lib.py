@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libgplite_amd.so")
 GPL_OK, GPL_ERR_ARG, GPL_ERR_NOT_PD, GPL_ERR_OOM, GPL_ERR_NOCONV, GPL_ERR_HIP, GPL_ERR_NAN, GPL_ERR_UNSUPPORTED = range(8)
 CF_CONST, CF_LIN, CF_SEXP, CF_MATERN32, CF_MATERN52, CF_NN, CF_PERIODIC = range(7)
 OP_SUM, OP_PROD = 0, 1
-LIK_GAUSSIAN, LIK_BERNOULLI, LIK_BINOMIAL, LIK_POISSON = range(4)
+LIK_GAUSSIAN, LIK_BERNOULLI, LIK_BINOMIAL, LIK_POISSON, LIK_BETABINOM = range(5)
 LINK_IDENTITY, LINK_LOGIT, LINK_PROBIT, LINK_LOG = range(4)
 APPROX_LAPLACE, APPROX_EP = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -35,7 +35,7 @@ class CfTerm(C.Structure):
 
 
 class Lik(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("link", C.c_int32), ("sigma", C.c_double)]
+    _fields_ = [("kind", C.c_int32), ("link", C.c_int32), ("sigma", C.c_double), ("phi", C.c_double)]
 
 
 class Opts(C.Structure):

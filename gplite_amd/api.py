@@ -355,6 +355,21 @@ class lik_binomial(lik_bernoulli):
     kind = "binomial"
 
 
+class lik_betabinom(Lik):
+    """lik_betabinom(link, phi, prior_phi) (R/lik.R:74-80): counts y of `trials` with a
+    beta-distributed success probability of mean mu = link^-1(f) and overdispersion phi."""
+    kind = "betabinom"
+    param_names = ("phi",)
+
+    def __init__(self, link="logit", phi=1.0, prior_phi=None):
+        super().__init__()
+        if link not in ("logit", "probit"):
+            raise ValueError("Unknown link: " + str(link))
+        self.link = link
+        self.phi = phi
+        self.priors = {"phi": prior_phi or prior_logunif()}
+
+
 class lik_poisson(Lik):
     kind = "poisson"
 
@@ -736,7 +751,7 @@ def eval_cf(cfs, x1, x2, diag_only=False, cfind=None):
 # ---- gp_fit / gp_energy / gp_pred ---------------------------------------
 
 _LIKMAP = {"gaussian": L.LIK_GAUSSIAN, "bernoulli": L.LIK_BERNOULLI, "binomial": L.LIK_BINOMIAL,
-           "poisson": L.LIK_POISSON}
+           "poisson": L.LIK_POISSON, "betabinom": L.LIK_BETABINOM}
 _LINKMAP = {"identity": L.LINK_IDENTITY, "logit": L.LINK_LOGIT, "probit": L.LINK_PROBIT, "log": L.LINK_LOG}
 
 
@@ -747,6 +762,7 @@ def _lik_struct(lik):
     s.kind = _LIKMAP[lik.kind]
     s.link = _LINKMAP[lik.link]
     s.sigma = getattr(lik, "sigma", 0.0)
+    s.phi = getattr(lik, "phi", 0.0)
     return s
 
 
@@ -869,6 +885,8 @@ def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want
     terms, nt, xp, _ = _program(g.cfs, x)
     if isinstance(g.lik, lik_binomial) and trials is None:
         raise ValueError("trials must be provided for the binomial likelihood.")
+    if isinstance(g.lik, lik_betabinom) and trials is None:
+        raise ValueError("trials must be provided for the beta binomial likelihood.")
     if isinstance(g.lik, lik_bernoulli) and not isinstance(g.lik, lik_binomial):
         trials = None
     tr = None if trials is None else np.ascontiguousarray(np.asarray(trials, dtype=np.float64).ravel())
@@ -1056,6 +1074,8 @@ def gp_draw(gp, xnew, draws=1, transform=True, target=False, marginal=False, cfi
         if isinstance(gp.lik, lik_poisson):
             return rng.poisson(mu).astype(float)
         t = np.ones(m) if trials is None else np.asarray(trials, dtype=float)
+        if isinstance(gp.lik, lik_betabinom):  # generate_target.lik_betabinom, R/lik.R:526-533
+            mu = rng.beta(mu / gp.lik.phi, (1 - mu) / gp.lik.phi)
         return rng.binomial(t.astype(np.int64)[:, None], mu).astype(float)
     if transform:
         return gp.lik.response(f)
@@ -1071,7 +1091,7 @@ _FILE_FORMAT = "gplite_amd-gp-1"
 
 def _saveable():
     return {c.__name__: c for c in (cf_const, cf_lin, cf_sexp, cf_matern32, cf_matern52, cf_nn, cf_periodic, cf_prod,
-                                    lik_gaussian, lik_bernoulli, lik_binomial, lik_poisson, prior_fixed,
+                                    lik_gaussian, lik_bernoulli, lik_binomial, lik_betabinom, lik_poisson, prior_fixed,
                                     prior_logunif, prior_lognormal, prior_half_t, approx_laplace, approx_ep,
                                     method_full, method_fitc)}
 
@@ -1207,6 +1227,12 @@ def _loglik_points(lik, f, y, trials=None, offset=None):
     if isinstance(lik, lik_poisson):
         mu = np.exp(f)
         return np.where(yy > 0, yy * f, 0.0) - mu - gammaln(yy + 1)
+    if isinstance(lik, lik_betabinom):  # R/lik.R:332-352
+        t = np.asarray(trials, dtype=float)[:, None]
+        mu = ndtr(f) if lik.link == "probit" else 1.0 / (1.0 + np.exp(-f))
+        a, b = mu / lik.phi, (1 - mu) / lik.phi
+        return (gammaln(t + 1) - gammaln(yy + 1) - gammaln(t - yy + 1) + gammaln(yy + a) + gammaln(t - yy + b)
+                - gammaln(t + a + b) + gammaln(a + b) - gammaln(a) - gammaln(b))
     raise NotImplementedError(type(lik).__name__)
 
 

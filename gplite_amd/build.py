@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgplite_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["dense.hip", "cov.hip", "lik.hip", "dense32.hip", "fitc.hip", "capi.cpp", "batch.cpp", "serve.cpp", "fitc_host.cpp", "noalloc.cpp"]
+SOURCES = ["dense.hip", "cov.hip", "lik.hip", "dense32.hip", "fitc.hip", "split_kernels.hip", "capi.cpp", "batch.cpp", "serve.cpp", "fitc_host.cpp", "noalloc.cpp", "split.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

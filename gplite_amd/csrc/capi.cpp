@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "host_common.h"
+#include "model.h"
 
 using namespace gpl;
 
@@ -223,39 +224,6 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
     P.ngrad = g;
     return GPL_OK;
 }
-
-enum {
-    B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
-    B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
-    B_FOLD, B_FVOLD, B_ZOLD, B_POLD, B_W1, B_W2, B_W3, B_W4, B_SCAL, B_PART, B_KMV, B_GH, B_INFO, B_DACC, B_B64,
-    B_COUNT
-};
-
-struct gpl_model {
-    gpl_ctx* ctx = nullptr;
-    int64_t n = 0, N = 0, ncols = 0;
-    DevProg prog{};
-    gpl_lik lik{};
-    gpl_opts opts{};
-    bool has_trials = false, has_offset = false;
-    bool have_xinv = false, have_binv = false;
-    double round_corr = 0.0;  // mixed form: first-order log|B| correction applied to logZ
-    double logZ = 0;
-    int32_t iters = 0;
-    int32_t refine_iters_done = 0;
-    double refine_rel = 0;
-    bool refit_fp64 = false;  // mixed form redone in FP64 (refinement did not contract / FP32 pivot failed)
-    DevBuf b[B_COUNT];
-    ~gpl_model() {
-        for (auto& x : b) x.free_();
-    }
-    double* d(int i) const { return b[i].as<double>(); }
-};
-
-#define ENSURE(m, idx, bytes)                                                             \
-    do {                                                                                  \
-        if (!(m)->b[idx].ensure(bytes)) return fail(GPL_ERR_OOM, "device allocation failed"); \
-    } while (0)
 
 // Two-level blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
 // Outer panels of NB columns are factored with 64-column inner steps (diagonal
@@ -740,7 +708,7 @@ static int fit_laplace(gpl_model* m) {
     for (it = 1; it <= maxiter; ++it) {
         int* flags = m->b[B_INFO].as<int>();
         hipMemsetAsync(flags, 0, 8, s);
-        lik_pseudodata_la(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
+        lik_pseudodata_la(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
                           m->d(B_V), m->d(B_SV), flags + 1);
         {
             Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
@@ -752,7 +720,7 @@ static int fit_laplace(gpl_model* m) {
             Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
             symv_lower_f64(s, m->d(B_K), N, N, m->d(B_ALPHA), m->d(B_FMEAN), nullptr);
         }
-        lik_loglik(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
+        lik_loglik(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
         double* sc = m->d(B_SCAL);
         reduce_sum(s, m->d(B_W3), n, sc + 0);
         reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
@@ -763,24 +731,42 @@ static int fit_laplace(gpl_model* m) {
         double h[4];
         int r = read_scalars(m, h, 4);
         if (r) return r;
-        if (fl[1] & 6) {
+        if (fl[1] & 2) {
             m->logZ = -INFINITY;
-            return fail((fl[1] & 2) ? GPL_ERR_NAN : GPL_ERR_UNSUPPORTED,
-                        (fl[1] & 2) ? "NaN in Laplace pseudo-data"
-                                    : "non-log-concave pseudo-variance (V <= 0): host fallback branch "
-                                      "R/laplace.R:40-117 is not on the device path");
+            return fail(GPL_ERR_NAN, "NaN in Laplace pseudo-data");
         }
-        if (fl[0] > 0) {
-            g_minor = fl[0];
-            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+        if (fl[1] & 4) {
+            // some V <= 0: the split step of R/laplace.R:40-117 replaces this one
+            double lz = 0;
+            r = split_step(m, false, &lz);
+            if (r) return r;
+            m->logZ = lz;
+            if (m->no_posterior) {  // fit = list(fmean = fhat_old, log_evidence = -Inf): diff 0, the loop ends
+                HIP_TRY(hipMemcpyAsync(m->d(B_FMEAN), m->d(B_FOLD), N * 8, hipMemcpyDeviceToDevice, s));
+                break;
+            }
+            reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+            r = read_scalars(m, h, 4);
+            if (r) return r;
+        } else {
+            if (fl[0] > 0) {
+                g_minor = fl[0];
+                return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
+            }
+            m->split = false;
+            m->logZ = h[0] - 0.5 * h[1] - h[2];
         }
-        m->logZ = h[0] - 0.5 * h[1] - h[2];
         diff = h[3];
         HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
         if (diff < m->opts.tol) break;
     }
     if (it > maxiter) it = maxiter;
     m->iters = it;
+    if (m->no_posterior) {
+        m->logZ = -INFINITY;
+        return fail(GPL_ERR_NAN, "the posterior of all the data is not proper (I + Lk^T V^-1 Lk is not positive "
+                                 "definite, R/laplace.R:64-72): log evidence -Inf");
+    }
     if (maxiter > 1 && it == maxiter) {  // also when the last allowed step converged (R/laplace.R:293)
         // the reference only warns here (R/laplace.R:293-295): keep the fit, report NOCONV
         g_err = "Maximum number of iterations in Laplace reached, max delta f = " + std::to_string(diff);
@@ -871,7 +857,7 @@ static int fit_laplace_mixed(gpl_model* m) {
     for (it = 1; it <= maxiter; ++it) {
         int* flags = m->b[B_INFO].as<int>();
         hipMemsetAsync(flags, 0, 8, s);
-        lik_pseudodata_la(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
+        lik_pseudodata_la(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
                           m->d(B_V), m->d(B_SV), flags + 1);
         {
             Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
@@ -885,7 +871,7 @@ static int fit_laplace_mixed(gpl_model* m) {
             Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
             symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), nch, 1, 0, 0);
         }
-        lik_loglik(s, m->lik.kind, m->lik.link, 0.0, n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
+        lik_loglik(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
         double* sc = m->d(B_SCAL);
         reduce_sum(s, m->d(B_W3), n, sc + 0);
         reduce_dot(s, m->d(B_FMEAN), m->d(B_ALPHA), n, sc + 1);
@@ -896,13 +882,11 @@ static int fit_laplace_mixed(gpl_model* m) {
         double h[4];
         int rr = read_scalars(m, h, 4);
         if (rr) return rr;
-        if (fl[1] & 6) {
+        if (fl[1] & 2) {
             m->logZ = -INFINITY;
-            return fail((fl[1] & 2) ? GPL_ERR_NAN : GPL_ERR_UNSUPPORTED,
-                        (fl[1] & 2) ? "NaN in Laplace pseudo-data"
-                                    : "non-log-concave pseudo-variance (V <= 0): host fallback branch "
-                                      "R/laplace.R:40-117 is not on the device path");
+            return fail(GPL_ERR_NAN, "NaN in Laplace pseudo-data");
         }
+        if (fl[1] & 4) return kRefineFail;  // some V <= 0: the split step runs in the FP64 refit
         if (fl[0] > 0) {
             g_minor = fl[0];
             return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
@@ -1009,7 +993,7 @@ static int fit_ep(gpl_model* m) {
         hipMemsetAsync(flags, 0, 8, s);
         {
             Prof p(c, "ep_sites", 0, 0);
-            ep_sites(s, m->lik.kind, m->lik.link, n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
+            ep_sites(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
                      m->d(B_Y), tr, off, q, m->d(B_GH), m->d(B_GH) + 32, damp, m->d(B_Z), m->d(B_V), m->d(B_SV),
                      m->d(B_LOGZS), m->d(B_CAVM), m->d(B_CAVV), flags + 1);
         }
@@ -1021,12 +1005,34 @@ static int fit_ep(gpl_model* m) {
             m->iters = it;
             return fail(GPL_ERR_NAN, "NaN in EP site parameters (R/ep.R:30-32 returns log_evidence = -Inf)");
         }
+        quad_ok = !(fl[1] & 1);  // the last sweep's flag decides (R/ep.R:326-328)
+        double* sc = m->d(B_SCAL);
+        double h[4];
         if (fl[1] & 4) {
-            m->logZ = -INFINITY;
-            return fail(GPL_ERR_UNSUPPORTED,
-                        "EP site variance <= 0: host fallback branch R/ep.R:50-138 is not on the device path");
-        }
-        if (fl[1] & 1) quad_ok = false;
+            // some site variance <= 0: the split sweep of R/ep.R:50-138
+            double lz = 0;
+            int r = split_step(m, true, &lz);
+            if (r) return r;
+            if (m->no_posterior) {
+                // The reference returns fmean = Inf (R/ep.R:90-92): diff_new = Inf, so after an
+                // accepted sweep the sweep is rejected with more damping; on the first sweep it
+                // would carry Inf into the next one, where it fails.
+                m->logZ = -INFINITY;
+                if (!(diff_old < INFINITY)) {
+                    m->iters = it;
+                    return fail(GPL_ERR_NAN, "EP: the posterior of all the data is not proper (I + Lk^T V^-1 Lk is "
+                                             "not positive definite, R/ep.R:83-92) on the first sweep");
+                }
+                diff_new = INFINITY;
+                damp = damp * damping;
+                continue;
+            }
+            reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
+            r = read_scalars(m, h, 4);
+            if (r) return r;
+            m->logZ = lz;
+            m->have_xinv = false;
+        } else {
         {
             Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
             form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
@@ -1041,20 +1047,22 @@ static int fit_ep(gpl_model* m) {
         colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);
         ep_post(s, n, m->d(B_V), m->d(B_Z), m->d(B_W3), m->d(B_CAVM), m->d(B_CAVV), m->d(B_LOGZS), m->d(B_FVAR),
                 m->d(B_TERM));
-        double* sc = m->d(B_SCAL);
         reduce_sum(s, m->d(B_TERM), n, sc + 0);
         reduce_dot(s, m->d(B_Z), m->d(B_ALPHA), n, sc + 1);
         reduce_sum(s, m->d(B_LDIAG), n, sc + 2);
         reduce_maxabsdiff(s, m->d(B_FOLD), m->d(B_FMEAN), n, sc + 3);
         HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
-        double h[4];
         int r = read_scalars(m, h, 4);
         if (r) return r;
         if (fl[0] > 0) {
             g_minor = fl[0];
             return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
         }
+        m->split = false;
+        m->no_posterior = false;
+        m->have_xinv = true;
         m->logZ = h[0] - h[2] - 0.5 * h[1];
+        }
         diff_new = h[3];
         if (diff_new > diff_old) {
             damp = damp * damping;
@@ -1076,7 +1084,6 @@ static int fit_ep(gpl_model* m) {
         return GPL_ERR_NOCONV;
     }
     if (!quad_ok) m->logZ = -INFINITY;
-    m->have_xinv = true;
     return GPL_OK;
 }
 
@@ -1129,7 +1136,7 @@ static int fit_ep_mixed(gpl_model* m) {
         hipMemsetAsync(flags, 0, 8, s);
         {
             Prof p(c, "ep_sites", 0, 0);
-            ep_sites(s, m->lik.kind, m->lik.link, n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
+            ep_sites(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_FVOLD), m->d(B_ZOLD), m->d(B_POLD),
                      m->d(B_Y), tr, off, q, m->d(B_GH), m->d(B_GH) + 32, damp, m->d(B_Z), m->d(B_V), m->d(B_SV),
                      m->d(B_LOGZS), m->d(B_CAVM), m->d(B_CAVV), flags + 1);
         }
@@ -1141,12 +1148,8 @@ static int fit_ep_mixed(gpl_model* m) {
             m->iters = it;
             return fail(GPL_ERR_NAN, "NaN in EP site parameters (R/ep.R:30-32 returns log_evidence = -Inf)");
         }
-        if (fl[1] & 4) {
-            m->logZ = -INFINITY;
-            return fail(GPL_ERR_UNSUPPORTED,
-                        "EP site variance <= 0: host fallback branch R/ep.R:50-138 is not on the device path");
-        }
-        if (fl[1] & 1) quad_ok = false;
+        if (fl[1] & 4) return kRefineFail;  // some site variance <= 0: the split sweep runs in the FP64 refit
+        quad_ok = !(fl[1] & 1);  // the last sweep's flag decides (R/ep.R:326-328)
         {
             Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
             form_B_mixed(s, m->d(B_K), m->d(B_B64), L32, m->d(B_DACC), N, n, N, m->d(B_SV));
@@ -1217,6 +1220,10 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
         return fail(GPL_ERR_ARG, "approx_ep disabled for lik_gaussian: use approx_laplace instead, because it gives "
                                  "exact results");  // R/gp.R:54-56
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
+    if (lik->kind < GPL_LIK_GAUSSIAN || lik->kind > GPL_LIK_BETABINOM) return fail(GPL_ERR_ARG, "unknown likelihood");
+    if (lik->kind == GPL_LIK_BETABINOM && !(lik->phi > 0)) return fail(GPL_ERR_ARG, "phi must be positive");
+    if ((lik->kind == GPL_LIK_BINOMIAL || lik->kind == GPL_LIK_BETABINOM) && !trials)
+        return fail(GPL_ERR_ARG, "trials must be provided for the binomial and beta binomial likelihoods");
     if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
     const bool mixed = o.precision == GPL_PREC_MIXED;
     DevProg P;
@@ -1242,6 +1249,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     m->has_trials = trials != nullptr;
     m->has_offset = offset != nullptr;
     m->have_xinv = m->have_binv = false;
+    m->split = m->no_posterior = false;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(m->b[B_PROG].p, &m->prog, sizeof(DevProg), hipMemcpyHostToDevice, s));
     if (n * ncols > 0) HIP_TRY(hipMemcpyAsync(m->d(B_X), x, n * ncols * 8, hipMemcpyHostToDevice, s));
@@ -1397,6 +1405,7 @@ gpl_ctx* gpl::model_ctx(const gpl_model* m) { return m->ctx; }
 int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t want_var, double* mean,
                  double* var, const int32_t* terms, int32_t nsel, const PredBufs* pb) {
     if (!c || !m || (!xt && mt > 0) || !mean || (want_var && !var)) return fail(GPL_ERR_ARG, "NULL argument");
+    if (m->no_posterior) return fail(GPL_ERR_ARG, "the fit has no posterior (its log evidence is -Inf)");
     if (mt <= 0) return GPL_OK;
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
@@ -1424,7 +1433,15 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
         launch_build_cross(s, pd, *hp, m->d(B_F), N, n, N, Ft, M, mt, M, KtT, N);
     }
     gemv_t_f64(s, KtT, N, N, mt, m->d(B_ALPHA), vbuf);
-    if (want_var) {
+    if (want_var && m->split) {
+        // var_reduction = K_t (K + V)^-1 K_t^T of the reference's C_lu branch (R/gp_pred.R:220-223)
+        if (pb) return fail(GPL_ERR_UNSUPPORTED, "a fit with split sites (pseudo-variances <= 0) needs gpl_pred");
+        int r = split_var_reduction(c, m, KtT, M, vbuf + M);
+        if (r) return r;
+        launch_build_diag(s, pd, *hp, Ft, M, Ft, M, mt, vbuf + 2 * M);
+        vec_axpby(s, mt, 1.0, vbuf + 2 * M, -1.0, vbuf + M, vbuf + M);
+        HIP_TRY(hipMemcpyAsync(var, vbuf + M, mt * 8, hipMemcpyDeviceToHost, s));
+    } else if (want_var) {
         // aux = C^-1 K_t^T = L^-1 (sv .* K_t^T)  (R/gp_pred.R:218)
         scale_rows_f64(s, KtT, N, n, M, m->d(B_SV));
         const int64_t T = N / 64;
@@ -1469,6 +1486,9 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
 static int draw_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, double jitter, const double* z,
                    int64_t draws, double* out, const int32_t* terms, int32_t nsel) {
     if (!c || !m || !xt || !z || !out || mt < 1 || draws < 1) return fail(GPL_ERR_ARG, "bad argument");
+    if (m->split || m->no_posterior)
+        return fail(GPL_ERR_UNSUPPORTED, "draws from a fit with split sites (pseudo-variances <= 0) are not on the "
+                                         "device path");
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -1648,6 +1668,8 @@ extern "C" int gpl_model_vector(gpl_model* m, int32_t which, double* out) {
 
 extern "C" int gpl_model_chol(gpl_model* m, double* out) {
     if (!m || !out) return fail(GPL_ERR_ARG, "NULL argument");
+    if (m->split || m->no_posterior)
+        return fail(GPL_ERR_UNSUPPORTED, "a fit with split sites holds no C_chol (the reference keeps C_lu)");
     std::lock_guard<std::mutex> lk(m->ctx->mu);
     HIP_TRY(hipSetDevice(m->ctx->device));
     const int64_t n = m->n, N = m->N;

@@ -202,7 +202,7 @@ static int fitc_ep_loop(gpl_fitc* f, const double* tr, const double* off) {
     int it;
     for (it = 1; it <= maxiter; ++it) {
         HIP_TRY(hipMemsetAsync(flags, 0, 8, s));
-        ep_sites(s, f->lik.kind, f->lik.link, n, f->d(F_FOLD), f->d(F_FVOLD), f->d(F_ZOLD), f->d(F_POLD), f->d(F_Y),
+        ep_sites(s, f->lik.kind, f->lik.link, lik_par(f->lik), n, f->d(F_FOLD), f->d(F_FVOLD), f->d(F_ZOLD), f->d(F_POLD), f->d(F_Y),
                  tr, off, q, f->d(F_GH), f->d(F_GH) + 32, damp, f->d(F_Z), f->d(F_V), f->d(F_SV), f->d(F_LOGZS),
                  f->d(F_CAVM), f->d(F_CAVV), flags + 1);
         HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
@@ -329,12 +329,12 @@ static int fitc_run(gpl_fitc* f) {
             lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, f->lik.sigma, n, nullptr, f->d(F_Y), nullptr, nullptr,
                               f->d(F_Z), f->d(F_V), f->d(F_SV), flags + 1);
         else
-            lik_pseudodata_la(s, f->lik.kind, f->lik.link, 0.0, n, f->d(F_FOLD), f->d(F_Y), tr, off, f->d(F_Z),
+            lik_pseudodata_la(s, f->lik.kind, f->lik.link, lik_par(f->lik), n, f->d(F_FOLD), f->d(F_Y), tr, off, f->d(F_Z),
                               f->d(F_V), f->d(F_SV), flags + 1);
         // inversion lemma for C = K_xz Kz^-1 K_zx + S, S = D + V (R/linear_algebra.R:3-47)
         fitc_lemma_step(f, flags);
         // log evidence (R/laplace.R:143-151): logdet C - log|V| from the lemma
-        lik_loglik(s, f->lik.kind, f->lik.link, gauss ? f->lik.sigma : 0.0, n, f->d(F_FMEAN), f->d(F_Y), tr, off,
+        lik_loglik(s, f->lik.kind, f->lik.link, lik_par(f->lik), n, f->d(F_FMEAN), f->d(F_Y), tr, off,
                    f->d(F_V3));
         double* scd = f->d(F_SCAL);
         reduce_sum(s, f->d(F_V3), n, scd + 0);

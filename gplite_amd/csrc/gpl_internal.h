@@ -160,7 +160,8 @@ void lik_pseudodata_la(hipStream_t s, int lik, int link, double sigma, int64_t n
                        double* V, double* sv, int* bad);
 void lik_loglik(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f,
                 const double* y, const double* trials, const double* offset, double* ll);
-void ep_sites(hipStream_t s, int lik, int link, int64_t n, const double* fmean, const double* fvar,
+// par: the likelihood's parameter (Gaussian sigma, beta-binomial phi)
+void ep_sites(hipStream_t s, int lik, int link, double par, int64_t n, const double* fmean, const double* fvar,
               const double* z_old, const double* P_old, const double* y, const double* trials,
               const double* offset, int q, const double* gh_x, const double* gh_w, double damping,
               double* z, double* V, double* sv, double* logZs, double* cav_m, double* cav_v,
@@ -191,6 +192,23 @@ void fitc_train_var(hipStream_t s, int64_t n, const double* prior, const double*
                     const double* q, const double* S, double* out);
 // B = A^T (dimensions multiples of 64)
 void transpose_f64(hipStream_t s, const double* A, int64_t lda, int64_t nrow, int64_t ncol, double* B, int64_t ldb);
+
+// Non-log-concave branch (split_kernels.hip)
+// out = [rs] K(ridx, cidx) [cs] (+ I) over nr x nc, identity (diag_pad) or zero outside, K
+// symmetric in the lower tiles of ldk
+void gather_sym_f64(hipStream_t s, const double* K, int64_t ldk, const int64_t* ridx, int64_t nr, const int64_t* cidx,
+                    int64_t nc, const double* rs, const double* cs, int add_identity, int diag_pad, double* out,
+                    int64_t ld, int64_t NR, int64_t NC);
+void gather_vec_f64(hipStream_t s, const double* in, const int64_t* idx, int64_t k, int64_t Kp, double* out);
+void scatter_vec_f64(hipStream_t s, const double* in, const int64_t* idx, int64_t k, double* out);
+void gather_rows_f64(hipStream_t s, const double* src, int64_t lds, const int64_t* idx, int64_t k, int64_t Kp,
+                     int64_t ncol, const double* scale, double* dst, int64_t ldd);
+void tril_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
+void recip_n_f64(hipStream_t s, const double* a, int64_t k, int64_t Kp, double* out);
+void rsqrt_n_f64(hipStream_t s, const double* a, int64_t k, int64_t Kp, double* out);
+void ep_split_var(hipStream_t s, int64_t n, const double* V, const double* dinv, double* fvar);
+void ep_split_terms(hipStream_t s, int64_t n, int ratio, const double* V, const double* z, const double* cm,
+                    const double* cv, const double* logZs, double* term);
 
 // ---- batched forms (one problem per grid row / workgroup, stride per problem) ----
 void launch_features_batched(hipStream_t s, const double* x, const int64_t* xoff, const int64_t* nvec,

@@ -57,6 +57,11 @@ using gpl::fail;
 
 inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
 
+// the likelihood's parameter the device kernels take: Gaussian sigma, beta-binomial phi
+inline double lik_par(const gpl_lik& l) {
+    return l.kind == GPL_LIK_GAUSSIAN ? l.sigma : (l.kind == GPL_LIK_BETABINOM ? l.phi : 0.0);
+}
+
 // ---------------------------------------------------------------------------
 // context, profiling
 // ---------------------------------------------------------------------------

@@ -19,22 +19,50 @@ __device__ __forceinline__ double d_response(int lik, int link, double f) {
     return 1.0 / (1.0 + exp(-f));
 }
 
-// d loglik / d f at f (offset already added)
-__device__ __forceinline__ double d_loglik_d(int lik, int link, double f, double y, double tr) {
+// digamma(x) for x > 0: the recurrence psi(x) = psi(x + 1) - 1/x raises the
+// argument to at least 10, then the asymptotic series
+// psi(x) ~ ln x - 1/(2x) - sum_k B_2k / (2k x^2k) (Abramowitz & Stegun 6.3.18)
+// through the x^-12 term; the first omitted term is below 1e-16 at x = 10.
+__device__ double d_digamma(double x) {
+    double acc = 0.0;
+    while (x < 10.0) {
+        acc -= 1.0 / x;
+        x += 1.0;
+    }
+    const double r = 1.0 / x, r2 = r * r;
+    // B_2k / (2k) for k = 1..6: 1/12, -1/120, 1/252, -1/240, 1/132, -691/32760
+    const double series =
+        r2 * (1.0 / 12 - r2 * (1.0 / 120 - r2 * (1.0 / 252 - r2 * (1.0 / 240 - r2 * (1.0 / 132 - r2 * (691.0 / 32760))))));
+    return acc + log(x) - 0.5 * r - series;
+}
+
+// d loglik / d f at f (offset already added); par is the beta-binomial phi
+__device__ __forceinline__ double d_loglik_d(int lik, int link, double par, double f, double y, double tr) {
     const double mu = d_response(lik, link, f);
     if (lik == GPL_LIK_POISSON) return (y / mu - 1.0) * mu;
     const double dmu = (link == GPL_LINK_PROBIT) ? kInvSqrt2Pi * exp(-0.5 * f * f) : mu * (1.0 - mu);
+    if (lik == GPL_LIK_BETABINOM) {  // R/lik.R:400-425
+        const double a = mu / par, b = (1.0 - mu) / par;
+        return (dmu / par) * (d_digamma(y + a) - d_digamma(tr - y + b)) + (dmu / par) * (-d_digamma(a) + d_digamma(b));
+    }
     return dmu * (y / mu - (tr - y) / (1.0 - mu));
 }
 
-__device__ __forceinline__ double d_loglik(int lik, int link, double sigma, double f, double y, double tr) {
+// log p(y | f); par is the Gaussian sigma or the beta-binomial phi
+__device__ __forceinline__ double d_loglik(int lik, int link, double par, double f, double y, double tr) {
     if (lik == GPL_LIK_GAUSSIAN) {
-        const double u = (y - f) / sigma;
-        return -0.5 * kLog2Pi - log(sigma) - 0.5 * u * u;
+        const double u = (y - f) / par;
+        return -0.5 * kLog2Pi - log(par) - 0.5 * u * u;
     }
     const double mu = d_response(lik, link, f);
     if (lik == GPL_LIK_POISSON) return (y > 0.0 ? y * log(mu) : 0.0) - mu - lgamma(y + 1.0);
     const double lc = lgamma(tr + 1.0) - lgamma(y + 1.0) - lgamma(tr - y + 1.0);
+    if (lik == GPL_LIK_BETABINOM) {  // R/lik.R:332-352
+        const double a = mu / par, b = (1.0 - mu) / par;
+        const double t2 = lgamma(y + a) + lgamma(tr - y + b) - lgamma(tr + a + b);
+        const double t3 = lgamma(a + b) - lgamma(a) - lgamma(b);
+        return lc + t2 + t3;
+    }
     return lc + (y > 0.0 ? y * log(mu) : 0.0) + (tr - y > 0.0 ? (tr - y) * log1p(-mu) : 0.0);
 }
 
@@ -54,8 +82,8 @@ __global__ void k_pseudo_la(int lik, int link, double sigma, int64_t n, const do
     const double tr = trials ? trials[i] : 1.0;
     const double eps = 1e-6;
     const double fi = f[i];
-    const double g = d_loglik_d(lik, link, fi + off, y[i], tr);
-    const double ge = d_loglik_d(lik, link, (fi + eps) + off, y[i], tr);
+    const double g = d_loglik_d(lik, link, sigma, fi + off, y[i], tr);
+    const double ge = d_loglik_d(lik, link, sigma, (fi + eps) + off, y[i], tr);
     double g2 = (ge - g) / eps;
     const double sg = (g2 > 0.0) ? 1.0 : ((g2 < 0.0) ? -1.0 : 0.0);
     g2 = sg * fmax(fabs(g2), 1e-12);
@@ -90,7 +118,7 @@ void lik_loglik(hipStream_t s, int lik, int link, double sigma, int64_t n, const
 }
 
 // EP site update (one thread per site, q <= 32 quadrature nodes)
-__global__ void k_ep_sites(int lik, int link, int64_t n, const double* __restrict__ fmean,
+__global__ void k_ep_sites(int lik, int link, double par, int64_t n, const double* __restrict__ fmean,
                            const double* __restrict__ fvar, const double* __restrict__ z_old,
                            const double* __restrict__ P_old, const double* __restrict__ y,
                            const double* __restrict__ trials, const double* __restrict__ offset, int q,
@@ -113,7 +141,7 @@ __global__ void k_ep_sites(int lik, int link, int64_t n, const double* __restric
     double m = -INFINITY;
     for (int k = 0; k < q; ++k) {
         fg[k] = cmean + scale * gx[k];
-        double v = d_loglik(lik, link, 0.0, fg[k] + off, yi, tr);
+        double v = d_loglik(lik, link, par, fg[k] + off, yi, tr);
         if (isnan(v)) { v = -INFINITY; ok = false; }
         ll[k] = v;
         m = fmax(m, v);
@@ -150,11 +178,11 @@ __global__ void k_ep_sites(int lik, int link, int64_t n, const double* __restric
     else if (!(Vi > 0.0)) atomicOr(nan_flag, 4);
 }
 
-void ep_sites(hipStream_t s, int lik, int link, int64_t n, const double* fmean, const double* fvar,
+void ep_sites(hipStream_t s, int lik, int link, double par, int64_t n, const double* fmean, const double* fvar,
               const double* z_old, const double* P_old, const double* y, const double* trials, const double* offset,
               int q, const double* gh_x, const double* gh_w, double damping, double* z, double* V, double* sv,
               double* logZs, double* cav_m, double* cav_v, int* nan_flag) {
-    hipLaunchKernelGGL(k_ep_sites, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, lik, link, n, fmean, fvar,
+    hipLaunchKernelGGL(k_ep_sites, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, lik, link, par, n, fmean, fvar,
                        z_old, P_old, y, trials, offset, q, gh_x, gh_w, damping, z, V, sv, logZs, cav_m, cav_v,
                        nan_flag);
 }

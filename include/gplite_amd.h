@@ -28,7 +28,7 @@ typedef enum {
     GPL_ERR_NOCONV = 4,     /* Newton / EP did not converge (R: warning or -Inf) */
     GPL_ERR_HIP = 5,        /* HIP runtime error */
     GPL_ERR_NAN = 6,        /* NaN in pseudo-data / quadrature (R: log_evidence = -Inf) */
-    GPL_ERR_UNSUPPORTED = 7 /* a branch the reference handles on the host (non-log-concave sites) */
+    GPL_ERR_UNSUPPORTED = 7 /* a case the device path does not cover (named in the message) */
 } gpl_status;
 
 /* ---- covariance program ------------------------------------------------ */
@@ -71,13 +71,14 @@ typedef struct gpl_cf_term {
 int32_t gpl_grad_len(const gpl_cf_term* terms, int32_t nterms, int32_t lik_kind);
 
 /* ---- likelihood and options -------------------------------------------- */
-enum { GPL_LIK_GAUSSIAN = 0, GPL_LIK_BERNOULLI = 1, GPL_LIK_BINOMIAL = 2, GPL_LIK_POISSON = 3 };
+enum { GPL_LIK_GAUSSIAN = 0, GPL_LIK_BERNOULLI = 1, GPL_LIK_BINOMIAL = 2, GPL_LIK_POISSON = 3, GPL_LIK_BETABINOM = 4 };
 enum { GPL_LINK_IDENTITY = 0, GPL_LINK_LOGIT = 1, GPL_LINK_PROBIT = 2, GPL_LINK_LOG = 3 };
 
 typedef struct gpl_lik {
     int32_t kind;   /* GPL_LIK_* (R/lik.R:46-90) */
     int32_t link;   /* GPL_LINK_* */
     double sigma;   /* Gaussian noise sd */
+    double phi;     /* beta-binomial overdispersion (lik_betabinom(phi), R/lik.R:74-80) */
 } gpl_lik;
 
 enum { GPL_APPROX_LAPLACE = 0, GPL_APPROX_EP = 1 };

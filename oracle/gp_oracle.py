@@ -29,7 +29,7 @@ import warnings
 
 import numpy as np
 import scipy.linalg as sla
-from scipy.special import gammaln, ndtr
+from scipy.special import digamma, gammaln, ndtr
 
 __all__ = [
     "JITTER", "cf_sexp_c", "cf_matern32_c", "cf_matern52_c", "cf_nn_c",
@@ -238,6 +238,8 @@ def _own_names(cf):
         return ["period"]
     if kind == "gaussian":
         return ["sigma"]
+    if kind == "betabinom":
+        return ["phi"]
     return []
 
 
@@ -501,7 +503,7 @@ def response(lik, f):
     """R/lik.R:460-498."""
     if lik["kind"] == "gaussian":
         return f
-    if lik["kind"] in ("bernoulli", "binomial"):
+    if lik["kind"] in ("bernoulli", "binomial", "betabinom"):
         if lik.get("link", "logit") == "probit":
             return ndtr(f)
         return 1.0 / (1.0 + np.exp(-f))
@@ -533,6 +535,15 @@ def loglik(lik, f, y, trials=None, offset=None, total=True):
     elif kind == "poisson":
         mu = response(lik, f)
         ll = np.where(y > 0, y * np.log(mu), 0.0) - mu - gammaln(y + 1)
+    elif kind == "betabinom":  # R/lik.R:332-352
+        t = np.asarray(trials, float)
+        mu = response(lik, f)
+        a = mu / lik["phi"]
+        b = (1 - mu) / lik["phi"]
+        t1 = gammaln(t + 1) - gammaln(y + 1) - gammaln(t - y + 1)
+        t2 = gammaln(y + a) + gammaln(t - y + b) - gammaln(t + a + b)
+        t3 = gammaln(a + b) - gammaln(a) - gammaln(b)
+        ll = t1 + t2 + t3
     else:
         raise ValueError(kind)
     return float(np.sum(ll)) if total else ll
@@ -557,6 +568,14 @@ def loglik_d(lik, f, y, trials=None, offset=None):
     if kind == "poisson":
         mu = response(lik, f)
         return (y / mu - 1) * mu
+    if kind == "betabinom":  # R/lik.R:400-425
+        t = np.asarray(trials, float)
+        mu = response(lik, f)
+        dmu = np.exp(_dnorm_log(f)) if lik.get("link", "logit") == "probit" else mu * (1 - mu)
+        phi = lik["phi"]
+        a = mu / phi
+        b = (1 - mu) / phi
+        return (dmu / phi) * (digamma(y + a) - digamma(t - y + b)) + (dmu / phi) * (-digamma(a) + digamma(b))
     raise ValueError(kind)
 
 
@@ -651,11 +670,66 @@ def _chol_B(K, V):
     return np.sqrt(V)[:, None] * LB
 
 
-def laplace_iter(lik, K, y, fhat_old, **kw):
-    """One Newton step, log-concave branch: R/laplace.R:13-39,119."""
-    z, V = pseudodata_la(lik, fhat_old, y, **kw)
+def _sub_kw(kw, idx):
+    return {k: (None if v is None else np.asarray(v, float)[idx]) for k, v in kw.items()}
+
+
+def _split_sites(K, V, alpha1, C1, good, bad):
+    """The bad sites' predictive N(m, P) given the good pseudo-data, P = Lk Lk^T, and
+    A = I + Lk^T V_bad^-1 Lk with its factor (None if A is not positive definite):
+    R/laplace.R:64-79, R/ep.R:79-92."""
+    K21 = K[np.ix_(bad, good)]
+    K22 = K[np.ix_(bad, bad)]
+    pred_mean = K21 @ alpha1
+    aux = sla.solve_triangular(C1, K21.T, lower=True)
+    pred_cov = K22 - aux.T @ aux
+    Lk = sla.cholesky(pred_cov, lower=True)  # R's chol() raises if this fails
+    A = np.eye(len(bad)) + Lk.T @ (Lk / V[bad][:, None])
+    try:
+        A_chol = sla.cholesky(A, lower=True)
+    except sla.LinAlgError:
+        A_chol = None
+    return pred_mean, pred_cov, Lk, A_chol
+
+
+def _solve_via(Lk, A_chol, v):
+    """backsolve(t(Lk), backsolve(t(A_chol), forwardsolve(A_chol, t(Lk) %*% v)))."""
+    u = sla.solve_triangular(A_chol, Lk.T @ v, lower=True)
+    u = sla.solve_triangular(A_chol, u, lower=True, trans="T")
+    return sla.solve_triangular(Lk, u, lower=True, trans="T")
+
+
+def laplace_iter(lik, K, y, fhat_old, pobs=None, **kw):
+    """One Newton step: R/laplace.R:13-117,119 (log-concave branch, and the split
+    branch for sites with pseudo-variance <= 0, with the final solve by LU as
+    Matrix::lu does)."""
+    if pobs is None:
+        z, V = pseudodata_la(lik, fhat_old, y, **kw)
+    else:
+        z, V = pobs
     if not np.all(V > 0):
-        raise NotImplementedError("non-log-concave branch (R/laplace.R:40-117) is a host fallback")
+        good = np.flatnonzero(V > 0)
+        bad = np.flatnonzero(~(V > 0))
+        if len(bad) > 500:
+            warnings.warn("More than 500 data points with pseudo-variance < 0 (outliers), computation getting slow. "
+                          "Check that the model and hyperparameter values are sensible.")
+        fit1 = laplace_iter(lik, K[np.ix_(good, good)], np.asarray(y)[good], None, pobs=(z[good], V[good]),
+                            **_sub_kw(kw, good))
+        pred_mean, pred_cov, Lk, A_chol = _split_sites(K, V, fit1["alpha"], fit1["C_chol"], good, bad)
+        if A_chol is None:
+            return dict(fmean=fhat_old, log_evidence=-np.inf)
+        alpha2 = _solve_via(Lk, A_chol, z[bad] / V[bad])
+        aux = sla.solve_triangular(Lk, sla.solve_triangular(Lk, pred_mean, lower=True), lower=True, trans="T")
+        aux = _solve_via(Lk, A_chol, aux)
+        f2hat = pred_cov @ (alpha2 + aux)
+        invC2_V2_logdet = 2 * np.sum(np.log(np.diag(A_chol)))
+        df2 = float(np.sum(sla.solve_triangular(Lk, f2hat - pred_mean, lower=True) ** 2))
+        ll2 = loglik(lik, f2hat, np.asarray(y)[bad], **_sub_kw(kw, bad))
+        le2 = ll2 - 0.5 * df2 + 0.5 * invC2_V2_logdet
+        lu = sla.lu_factor(K + np.diag(V))
+        alpha = sla.lu_solve(lu, z)
+        fnew = K @ alpha
+        return dict(fmean=fnew, z=z, V=V, C_lu=lu, alpha=alpha, log_evidence=float(fit1["log_evidence"] + le2))
     C = _chol_B(K, V)
     alpha = sla.solve_triangular(C, sla.solve_triangular(C, z, lower=True), lower=True, trans="T")
     fnew = K @ alpha
@@ -687,14 +761,38 @@ def laplace(lik, K, y, maxiter=30, tol=1e-4, **kw):
     return fit
 
 
-def ep_iter(lik, K, y, fmean_old, fvar_old, z_old, P_old, quad_order=11, damping=0.9, **kw):
-    """Parallel EP sweep, log-concave branch: R/ep.R:16-49,140-146."""
-    z, V, logZ_site, cm, cv, ok = pseudodata_ep(lik, fmean_old, 1 / fvar_old, z_old, P_old, y,
-                                                quad_order, damping, **kw)
+def ep_iter(lik, K, y, fmean_old, fvar_old, z_old, P_old, quad_order=11, damping=0.9, pobs=None, **kw):
+    """Parallel EP sweep: R/ep.R:16-146 (log-concave branch, and the split branch for
+    sites with variance <= 0, with the final solves by LU as Matrix::lu does)."""
+    if pobs is None:
+        z, V, logZ_site, cm, cv, ok = pseudodata_ep(lik, fmean_old, 1 / fvar_old, z_old, P_old, y,
+                                                    quad_order, damping, **kw)
+    else:
+        z, V, logZ_site, cm, cv, ok = pobs
     if np.any(np.isnan(V)) or np.any(np.isnan(z)):
         return dict(fmean=fmean_old, log_evidence=-np.inf, nan=True)
     if not np.all(V > 0):
-        raise NotImplementedError("non-log-concave EP branch (R/ep.R:50-138) is a host fallback")
+        good = np.flatnonzero(V > 0)
+        bad = np.flatnonzero(~(V > 0))
+        if len(bad) > 500:
+            warnings.warn("More than 500 data points with pseudo-variance < 0 (outliers), computation getting slow. "
+                          "Check that the model and hyperparameter values are sensible.")
+        fit1 = ep_iter(lik, K[np.ix_(good, good)], np.asarray(y)[good], None, None, None, None,
+                       pobs=(z[good], V[good], logZ_site[good], cm[good], cv[good], ok), **_sub_kw(kw, good))
+        pred_mean, pred_cov, Lk, A_chol = _split_sites(K, V, fit1["alpha"], fit1["C_chol"], good, bad)
+        if A_chol is None:
+            return dict(fmean=np.inf, log_evidence=-np.inf, no_posterior=True)
+        invC2_V2_logdet = 2 * np.sum(np.log(np.diag(A_chol)))
+        d = pred_mean - z[bad]
+        df2 = float(d @ _solve_via(Lk, A_chol, d / V[bad]))
+        le2 = (-0.5 * df2 + 0.5 * invC2_V2_logdet + np.sum(logZ_site[bad])
+               + 0.5 * np.sum(np.log((cv[bad] + V[bad]) / V[bad])) + 0.5 * np.sum((cm[bad] - z[bad]) ** 2 / (cv[bad] + V[bad])))
+        lu = sla.lu_factor(K + np.diag(V))
+        alpha = sla.lu_solve(lu, z)
+        fmean = K @ alpha
+        fvar = V * np.diag(sla.lu_solve(lu, K))
+        return dict(fmean=fmean, fvar=fvar, cavity_mean=cm, cavity_var=cv, z=z, P=1 / V, C_lu=lu, quad_ok=ok,
+                    alpha=alpha, log_evidence=float(fit1["log_evidence"] + le2))
     C = _chol_B(K, V)
     alpha = sla.solve_triangular(C, sla.solve_triangular(C, z, lower=True), lower=True, trans="T")
     fmean = K @ alpha
@@ -723,6 +821,14 @@ def ep(lik, K, y, maxiter=100, tol=1e-4, damping=0.9, damping_min=0.1, quad_orde
             # R/ep.R:26-27 returns list(fmean_new = fmean_old, log_evidence = -Inf); `fit$fmean`
             # then partially matches fmean_new, so diff_new = 0 < tol and the loop ends here
             break
+        if fit.get("no_posterior"):
+            # R/ep.R:90-92 returns fmean = Inf: diff_new = Inf, a rejected sweep after an
+            # accepted one; on the first sweep R carries Inf into the next sweep and fails
+            if np.isinf(diff_old):
+                raise FloatingPointError("EP: the posterior is not proper on the first sweep")
+            diff_new = np.inf
+            damp = damp * damping
+            continue
         diff_new = float(np.max(np.abs(fmean_old - fit["fmean"])))
         if diff_new > diff_old:
             damp = damp * damping
@@ -817,6 +923,9 @@ def gp_pred(fit, xt, var=False, jitter=None, prior=False, cfind=None):
     mean = Kt @ fit["alpha"]
     if not var:
         return mean, None
+    if "C_lu" in fit:  # R/gp_pred.R:220-223: var_reduction = Kt (K + V)^-1 Kt^T
+        red = np.sum(Kt.T * sla.lu_solve(fit["C_lu"], Kt.T), axis=0)
+        return mean, eval_cf(cfs, xt, xt, diag_only=True, cfind=cfind) - red
     aux = sla.solve_triangular(fit["C_chol"], Kt.T, lower=True)
     v = eval_cf(cfs, xt, xt, diag_only=True, cfind=cfind) - np.sum(aux * aux, axis=0)
     return mean, v

@@ -611,3 +611,118 @@ def test_oracle_loo_gaussian_closed_form():
         return np.log(np.sum(wg[None, :] * np.exp(ll), axis=1))
     assert np.max(np.abs(gh_loos(200) - ref)) < 1e-9
     assert np.max(np.abs(out["loos"] - gh_loos(11))) < 1e-10
+
+
+# ---------------------------------------------------------------- beta-binomial and split sites
+
+def test_betabinom_loglik_and_gradient():
+    """R/lik.R:332-352 against scipy's beta-binomial pmf; R/lik.R:400-425 against a
+    central difference of it."""
+    r = _rng(30)
+    t = r.integers(1, 30, 40).astype(float)
+    y = np.floor(r.uniform(0, 1, 40) * (t + 1))
+    f = r.normal(0, 2, 40)
+    for link in ("logit", "probit"):
+        for phi in (0.05, 1.0, 4.0):
+            lik = {"kind": "betabinom", "link": link, "phi": phi}
+            mu = O.response(lik, f)
+            ref = st.betabinom.logpmf(y, t, mu / phi, (1 - mu) / phi)
+            np.testing.assert_allclose(O.loglik(lik, f, y, trials=t, total=False), ref, rtol=1e-11, atol=1e-13)
+            h = 1e-6
+            fd = (O.loglik(lik, f + h, y, trials=t, total=False) - O.loglik(lik, f - h, y, trials=t, total=False)) / (2 * h)
+            np.testing.assert_allclose(O.loglik_d(lik, f, y, trials=t), fd, rtol=1e-5, atol=1e-7)
+
+
+def _split_case(n=40, nbad=3, seed=31):
+    r = _rng(seed)
+    x = r.uniform(-2, 2, (n, 1))
+    K = O.eval_cf([{"kind": "sexp", "lscale": 0.7, "magn": 1.0}], x, x) + 1e-6 * np.eye(n)
+    z = r.normal(size=n)
+    V = r.uniform(0.2, 1.0, n)
+    bad = r.choice(n, nbad, replace=False)
+    V[bad] = -r.uniform(4.0, 8.0, nbad)
+    t = np.full(n, 10.0)
+    y = np.floor(r.uniform(0, 11, n))
+    return K, z, V, y, t, np.flatnonzero(V > 0), np.sort(bad)
+
+
+def _dense_split(K, z, V, good, bad):
+    """The split branch's Gaussian pieces by dense solves: m, P = the bad latents'
+    predictive given the good pseudo-data, f2 = their posterior mean under the bad
+    pseudo-data, 1/2 log|I + V_bad^-1 P|."""
+    M11 = K[np.ix_(good, good)] + np.diag(V[good])
+    K21 = K[np.ix_(bad, good)]
+    m = K21 @ np.linalg.solve(M11, z[good])
+    P = K[np.ix_(bad, bad)] - K21 @ np.linalg.solve(M11, K21.T)
+    f2 = m + P @ np.linalg.solve(P + np.diag(V[bad]), z[bad] - m)
+    sgn, ld = np.linalg.slogdet(np.eye(len(bad)) + P / V[bad][:, None])
+    assert sgn > 0
+    return m, P, f2, 0.5 * ld
+
+
+def test_split_laplace_step_matches_dense_formulas():
+    """R/laplace.R:40-117 on sites with three negative pseudo-variances: the evidence
+    p(y_good) p(y_bad | y_good) and alpha from dense solves and determinants (no
+    Cholesky chain, no LU), and alpha = (K + V)^-1 z."""
+    K, z, V, y, t, good, bad = _split_case()
+    lik = {"kind": "betabinom", "link": "logit", "phi": 0.5}
+    fit = O.laplace_iter(lik, K, y, None, pobs=(z, V), trials=t)
+    assert "C_lu" in fit
+    K11 = K[np.ix_(good, good)]
+    f1 = K11 @ np.linalg.solve(K11 + np.diag(V[good]), z[good])
+    le1 = (O.loglik(lik, f1, y[good], trials=t[good]) - 0.5 * f1 @ np.linalg.solve(K11, f1)
+           - 0.5 * np.linalg.slogdet(np.eye(len(good)) + K11 / V[good][None, :])[1])
+    m, P, f2, half_logdet = _dense_split(K, z, V, good, bad)
+    le2 = (O.loglik(lik, f2, y[bad], trials=t[bad]) - 0.5 * (f2 - m) @ np.linalg.solve(P, f2 - m) + half_logdet)
+    assert fit["log_evidence"] == pytest.approx(le1 + le2, rel=1e-9)
+    alpha = np.linalg.solve(K + np.diag(V), z)
+    np.testing.assert_allclose(fit["alpha"], alpha, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(fit["fmean"], K @ alpha, rtol=1e-9, atol=1e-12)
+
+
+def test_split_ep_sweep_matches_dense_formulas():
+    """R/ep.R:50-138 with the same split: the bad sites' evidence terms with
+    (m - z)^T (P + V_bad)^-1 (m - z) and 1/2 log|I + V_bad^-1 P| from dense
+    algebra, and fvar = diag(V (K + V)^-1 K)."""
+    K, z, V, y, t, good, bad = _split_case(seed=32)
+    r = _rng(33)
+    n = len(z)
+    logZs = r.normal(-1, 0.3, n)
+    cm = r.normal(size=n)
+    cv = r.uniform(0.5, 2.0, n)  # below |V| at the bad sites, so (c + V) / V > 0 as in an EP state
+    fit = O.ep_iter({"kind": "betabinom", "link": "logit", "phi": 0.5}, K, y, None, None, None, None,
+                    pobs=(z, V, logZs, cm, cv, True), trials=t)
+    assert "C_lu" in fit
+    K11 = K[np.ix_(good, good)]
+    M11 = K11 + np.diag(V[good])
+    zg = z[good]
+    le1 = (-0.5 * np.linalg.slogdet(M11)[1] - 0.5 * zg @ np.linalg.solve(M11, zg) + np.sum(logZs[good])
+           + 0.5 * np.sum(np.log(cv[good] + V[good])) + 0.5 * np.sum((cm[good] - zg) ** 2 / (cv[good] + V[good])))
+    m, P, _, half_logdet = _dense_split(K, z, V, good, bad)
+    d = m - z[bad]
+    le2 = (-0.5 * d @ np.linalg.solve(P + np.diag(V[bad]), d) + half_logdet + np.sum(logZs[bad])
+           + 0.5 * np.sum(np.log((cv[bad] + V[bad]) / V[bad])) + 0.5 * np.sum((cm[bad] - z[bad]) ** 2 / (cv[bad] + V[bad])))
+    assert fit["log_evidence"] == pytest.approx(le1 + le2, rel=1e-9)
+    np.testing.assert_allclose(fit["fvar"], V * np.diag(np.linalg.solve(K + np.diag(V), K)), rtol=1e-9, atol=1e-12)
+
+
+def _betabinom_outliers(seed=3, n=100, phi=0.2):
+    r = _rng(seed)
+    x = np.sort(r.uniform(-3, 3, n))
+    f = 3.0 + 0.5 * np.sin(x)
+    mu = 1 / (1 + np.exp(-f))
+    y = r.binomial(10, r.beta(mu / phi, (1 - mu) / phi)).astype(float)
+    y[[10, 50, 90]] = 0.0
+    return x[:, None], y, np.full(n, 10.0)
+
+
+def test_split_laplace_mode_is_the_fixed_point():
+    """With beta-binomial outliers the converged Newton iterate splits its sites
+    (some pseudo-variances are negative at the mode), and it is still the mode:
+    f = K grad log p(y | f), which holds for any V."""
+    x, y, t = _betabinom_outliers()
+    lik = {"kind": "betabinom", "link": "logit", "phi": 0.2}
+    fit = O.gp_fit([{"kind": "sexp", "lscale": 1.0, "magn": 1.0}], lik, x, y, trials=t, maxiter=100, tol=1e-10)
+    assert "C_lu" in fit and np.sum(fit["V"] < 0) >= 1
+    f = fit["fmean"]
+    np.testing.assert_allclose(f, fit["K"] @ O.loglik_d(lik, f, y, trials=t), atol=1e-8)
