@@ -56,6 +56,7 @@ def main():
     ap.add_argument("--fitc-m", type=int, default=1024)
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
     ap.add_argument("--serve-workers", type=int, default=4, help="c5 server workers (one context each)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="c5 requests per arrival rate")
     ap.add_argument("--serve-precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="c5 fit precision: auto = mixed for Gaussian fits, FP64 for Laplace fits")
     args = ap.parse_args()
@@ -435,7 +436,7 @@ def bench_c5(args, G, L, api, synth):
         srv = Server(workers=args.serve_workers, mem_gb=128.0)
         poisson_load(srv, 100.0, 12, seed=99, precision=prec)  # warm the workers
         for rate in [float(v) for v in args.rates.split(",")]:
-            nreq = int(min(max(2 * rate, 60), 400))
+            nreq = args.serve_requests
             t = time.perf_counter()
             r = poisson_load(srv, rate, nreq, seed=int(rate), precision=prec)
             r["precision"] = prec

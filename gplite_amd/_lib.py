@@ -94,6 +94,9 @@ def _sig(lib):
         "gpl_batch_grad_len": (i32, [_vp]),
         "gpl_batch_info": (C.c_int, [_vp, C.POINTER(i32), _dp]),
         "gpl_batch_status": (C.c_int, [_vp, C.POINTER(i32), _dp]),
+        "gpl_fit_batch": (C.c_int, [_vp, i32, C.POINTER(CfTerm), i32, C.POINTER(_dp), C.POINTER(i64),
+                                    C.POINTER(i64), C.POINTER(_dp), _dp, C.POINTER(Opts), C.POINTER(_vp), _dp,
+                                    C.POINTER(i32)]),
         "gpl_batch_tune": (C.c_int, [_vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), _dp,
                                      C.POINTER(i32)]),
         "gpl_batch_set_tuning": (C.c_int, [_vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),

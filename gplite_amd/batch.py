@@ -137,3 +137,57 @@ class GPBatch:
         if h and h.value and L._lib is not None:
             L._lib.gpl_batch_free(h)
             self.handle = None
+
+
+def gp_fit_batch(gps, xs, ys, refine_iters=6, jitter=1e-6):
+    """gp_fit for many Gaussian-likelihood GPs of the same covariance structure in
+    one batched device pass (gpl_fit_batch): returns fitted GP objects whose
+    resident mixed-precision models predict, draw and differentiate like those of
+    gp_fit(..., precision="mixed").  A problem whose FP32 factor or refinement
+    fails is fitted again on its own by gp_fit (which redoes it in FP64)."""
+    gps = [g.copy() for g in gps]
+    for g in gps:
+        if not isinstance(g.lik, api.lik_gaussian):
+            raise NotImplementedError("the batched fit covers the Gaussian likelihood")
+    xs = [api._as_matrix(x) for x in xs]
+    ys = [np.ascontiguousarray(np.asarray(y, dtype=np.float64).ravel()) for y in ys]
+    for g, x in zip(gps, xs):
+        for c in g.cfs:
+            c.learn_scales(x)
+    progs = [api._program(g.cfs, x) for g, x in zip(gps, xs)]
+    nt = progs[0][1]
+    flat = (L.CfTerm * (nt * len(gps)))()
+    for p, (t, ntp, _, _) in enumerate(progs):
+        if ntp != nt:
+            raise ValueError("all problems need the same number of covariance terms")
+        for k in range(nt):
+            flat[p * nt + k] = t[k]
+    xps = [pr[2] for pr in progs]
+    npb = len(gps)
+    n = (C.c_int64 * npb)(*[x.shape[0] for x in xps])
+    nc = (C.c_int64 * npb)(*[x.shape[1] for x in xps])
+    xptr = (L._dp * npb)(*[L.dptr(x) for x in xps])
+    yptr = (L._dp * npb)(*[L.dptr(y) for y in ys])
+    sig = np.array([g.lik.sigma for g in gps])
+    o = L.Opts()
+    L.lib().gpl_opts_default(C.byref(o))
+    o.refine_iters = refine_iters
+    o.jitter = jitter
+    o.precision = L.PREC_MIXED
+    handles = (L._vp * npb)()
+    lz = np.empty(npb)
+    st = np.empty(npb, dtype=np.int32)
+    L.check(L.lib().gpl_fit_batch(L.ctx(), npb, flat, nt, xptr, n, nc, yptr, L.dptr(sig), C.byref(o), handles,
+                                  L.dptr(lz), st.ctypes.data_as(C.POINTER(C.c_int32))))
+    out = []
+    for p, g in enumerate(gps):
+        if st[p] != L.GPL_OK:
+            out.append(api.gp_fit(g, xs[p], ys[p], jitter=jitter, precision="mixed"))
+            continue
+        g.fit = api._Fit(L._vp(handles[p]), ys[p].shape[0], float(lz[p]), 1, L.APPROX_LAPLACE)
+        g.x = xs[p]
+        g.fitted = True
+        g.precision = "mixed"
+        g._trials, g._offset = None, None
+        out.append(g)
+    return out

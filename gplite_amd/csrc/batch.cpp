@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "host_common.h"
+#include "model.h"
 
 using namespace gpl;
 
@@ -68,6 +69,7 @@ struct gpl_batch {
     std::vector<int32_t> status;       // per problem, from the last pass
     std::vector<double> refine_rel;    // per problem, last refinement correction of the last pass
     hipEvent_t fork = nullptr;
+    std::vector<const double*> xs;  // gpl_fit_batch: the callers' inputs, for the exported models
     ~gpl_batch() {
         for (auto* b : buckets) delete b;
         res_logz.free_();
@@ -627,12 +629,12 @@ static int run_once(gpl_batch* B) {
     return GPL_OK;
 }
 
-extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double* grad) {
-    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
-    std::lock_guard<std::mutex> lk(B->ctx->mu);
+// One pass (the graph, or eager launches when `eager` or profiling), then the
+// results and the per-problem statuses.  Caller holds the context's lock.
+static int batch_pass(gpl_batch* B, bool eager, double* log_evidence, double* grad) {
     HIP_TRY(hipSetDevice(B->ctx->device));
     hipStream_t s = B->ctx->stream;
-    if (B->ctx->prof) {
+    if (eager || B->ctx->prof) {
         // profiled pass: eager launches with per-class events (no graph)
         reset_flags(B, s);
         run_buckets(B, s);
@@ -687,6 +689,107 @@ extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double*
                                         "correction " + fmt_e(B->refine_rel[first_nc]) +
                                         "); refit it in FP64 or raise refine_iters");
     return GPL_OK;
+}
+
+extern "C" int gpl_batch_energy_grad(gpl_batch* B, double* log_evidence, double* grad) {
+    if (!B) return fail(GPL_ERR_ARG, "NULL batch");
+    std::lock_guard<std::mutex> lk(B->ctx->mu);
+    return batch_pass(B, false, log_evidence, grad);
+}
+
+// Copies problem p's fit out of its bucket into the model (allocated or recycled).
+static int export_model(gpl_batch* B, int p, const gpl_cf_term* terms, int64_t n, int64_t ncols, const double* y,
+                        double sigma, double logz, gpl_model* m) {
+    gpl_ctx* c = B->ctx;
+    hipStream_t s = c->stream;
+    const Bucket& k = *B->buckets[B->bucket_of[p]];
+    const int64_t q = B->pos_in[p], N = k.N, N2 = N * N;
+    DevProg P;
+    int r = prepare_prog(terms + (size_t)p * B->nterms, B->nterms, ncols, P);
+    if (r) return r;
+    r = model_alloc(m, n, ncols, true);
+    if (r) return r;
+    ENSURE(m, B_XINV, (size_t)N2 * 4);
+    m->prog = P;
+    m->lik = gpl_lik{GPL_LIK_GAUSSIAN, GPL_LINK_IDENTITY, sigma, 0.0};
+    m->opts = B->opts;
+    m->opts.precision = GPL_PREC_MIXED;
+    m->opts.approx = GPL_APPROX_LAPLACE;
+    m->has_trials = m->has_offset = false;
+    m->have_xinv = true;
+    m->have_binv = false;
+    m->split = m->no_posterior = m->refit_fp64 = false;
+    m->logZ = logz;
+    m->iters = 1;
+    m->refine_rel = B->refine_rel[p];
+    m->refine_iters_done = B->opts.refine_iters;
+    auto cp = [&](int dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(m->b[dst].p, src, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    HIP_TRY(cp(B_PROG, k.b[BB_PROG].as<DevProg>() + q, sizeof(DevProg)));
+    HIP_TRY(cp(B_F, k.d(BB_F) + q * N * MAX_FEAT, (size_t)N * MAX_FEAT * 8));
+    HIP_TRY(cp(B_L, k.f(BB_L) + q * N2, (size_t)N2 * 4));
+    HIP_TRY(cp(B_XINV, k.f(BB_X32) + q * N2, (size_t)N2 * 4));
+    HIP_TRY(cp(B_DINV, k.d(BB_DINV) + q * N * 64, (size_t)N * 64 * 8));
+    HIP_TRY(cp(B_LDIAG, k.d(BB_LDIAG) + q * N, (size_t)N * 8));
+    HIP_TRY(cp(B_SV, k.d(BB_SV) + q * N, (size_t)N * 8));
+    HIP_TRY(cp(B_ALPHA, k.d(BB_ALPHA) + q * N, (size_t)N * 8));
+    HIP_TRY(cp(B_Y, k.d(BB_Y) + q * N, (size_t)N * 8));
+    HIP_TRY(cp(B_Z, k.d(BB_Y) + q * N, (size_t)N * 8));  // Gaussian pseudo-data z = y (R/lik.R:188-192)
+    if (n * ncols > 0) HIP_TRY(hipMemcpyAsync(m->d(B_X), B->xs[p], n * ncols * 8, hipMemcpyHostToDevice, s));
+    for (int i : {B_TR, B_OFF, B_FVAR, B_CAVM, B_CAVV}) fill_f64(s, m->d(i), N, 0.0);
+    fill_f64(s, m->d(B_V), N, 0.0);
+    fill_f64(s, m->d(B_V), n, sigma * sigma);
+    // f = (K + jitter I) alpha = sigma (B u - u), B u from the last refinement sweep
+    fill_f64(s, m->d(B_FMEAN), N, 0.0);
+    vec_axpby(s, n, sigma, k.d(BB_KU) + q * N, -sigma, k.d(BB_U) + q * N, m->d(B_FMEAN));
+    (void)y;
+    return GPL_OK;
+}
+
+extern "C" int gpl_fit_batch(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
+                             const double* const* x, const int64_t* n, const int64_t* ncols, const double* const* y,
+                             const double* sigma, const gpl_opts* opts, gpl_model** models, double* log_evidence,
+                             int32_t* status) {
+    if (!c || !models || !status) return fail(GPL_ERR_ARG, "NULL argument");
+    for (int p = 0; p < nprob; ++p)
+        if (models[p] && model_ctx(models[p]) != c)
+            return fail(GPL_ERR_ARG, "a model to refit belongs to another context");
+    gpl_opts o;
+    if (opts) o = *opts;
+    else {
+        gpl_opts_default(&o);
+        o.refine_iters = 6;
+    }
+    gpl_batch* B = nullptr;
+    int r = gpl_batch_create(c, nprob, terms, nterms, x, n, ncols, y, sigma, &o, &B);
+    if (r) return r;
+    B->xs.assign(x, x + nprob);
+    // GEMM tiles by problem size alone, so that a problem's result does not depend
+    // on the other problems of the call
+    for (auto* k : B->buckets) k->bm = k->N >= 1024 ? 128 : 64;
+    std::vector<double> lz(nprob);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        r = batch_pass(B, true, lz.data(), nullptr);
+        if (r == GPL_OK || r == GPL_ERR_NOT_PD || r == GPL_ERR_NOCONV) {
+            r = GPL_OK;
+            for (int p = 0; p < nprob && !r; ++p) {
+                status[p] = B->status[p];
+                if (log_evidence) log_evidence[p] = lz[p];
+                if (status[p] != GPL_OK) continue;
+                gpl_model* m = models[p];
+                const bool fresh = !m;
+                if (fresh) m = model_new(c);
+                r = export_model(B, p, terms, n[p], ncols[p], y[p], sigma[p], lz[p], m);
+                if (r && fresh) gpl_model_free(m);
+                else models[p] = m;
+            }
+            if (r == GPL_OK) HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+    }
+    gpl_batch_free(B);
+    return r;
 }
 
 extern "C" int gpl_batch_status(gpl_batch* B, int32_t* status, double* refine_rel) {

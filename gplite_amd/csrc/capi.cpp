@@ -476,6 +476,10 @@ static int check_info(gpl_model* m) {
     return GPL_OK;
 }
 
+static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool mixed);
+
+int gpl::model_alloc(gpl_model* m, int64_t n, int64_t ncols, bool mixed) { return alloc_model(m, n, ncols, false, mixed); }
+
 static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool mixed) {
     const int64_t N = pad64(n);
     m->n = n;
@@ -1399,6 +1403,12 @@ void gpl::model_dims(const gpl_model* m, int64_t* N, int64_t* ncols) {
 }
 
 gpl_ctx* gpl::model_ctx(const gpl_model* m) { return m->ctx; }
+
+gpl_model* gpl::model_new(gpl_ctx* c) {
+    gpl_model* m = new gpl_model();
+    m->ctx = c;
+    return m;
+}
 
 // Prediction on the streams of context c (which may differ from the context
 // that fitted the model: the model's device buffers are read-only here).

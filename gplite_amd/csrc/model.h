@@ -3,6 +3,8 @@
 
 #include "host_common.h"
 
+extern "C" void gpl_model_free(gpl_model* m);
+
 enum {
     B_X, B_F, B_Y, B_TR, B_OFF, B_L, B_DINV, B_LDIAG, B_K, B_XINV, B_BINV, B_PROG,
     B_Z, B_V, B_SV, B_ALPHA, B_FMEAN, B_FVAR, B_CAVM, B_CAVV, B_LOGZS, B_TERM,
@@ -53,4 +55,7 @@ namespace gpl {
 // on split sites; and the predictive variance reduction of a split fit.
 int split_step(gpl_model* m, bool ep, double* logZ);
 int split_var_reduction(gpl_ctx* c, gpl_model* m, const double* KtT, int64_t M, double* red);
+// the buffers of a Gaussian-likelihood model of n points (factor FP32 when mixed)
+int model_alloc(gpl_model* m, int64_t n, int64_t ncols, bool mixed);
+gpl_model* model_new(gpl_ctx* c);
 }  // namespace gpl

@@ -30,6 +30,10 @@ using namespace gpl;
 extern "C" void gpl_model_free(gpl_model* m);
 extern "C" int64_t gpl_model_bytes(gpl_model* m);
 extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence);
+extern "C" int gpl_fit_batch(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
+                             const double* const* x, const int64_t* n, const int64_t* ncols, const double* const* y,
+                             const double* sigma, const gpl_opts* opts, gpl_model** models, double* log_evidence,
+                             int32_t* status);
 
 namespace {
 
@@ -114,6 +118,7 @@ struct gpl_server {
     size_t budget = 0, in_use = 0;  // in_use: resident models + work in flight
     size_t inflight = 0;             // footprints of the jobs running now
     int64_t max_coalesce = 4096;
+    int max_fit_batch = 16;  // Gaussian fits per batched pass
     std::vector<gpl_ctx*> ctxs;
     std::vector<std::thread> workers;
     std::mutex mu;
@@ -208,6 +213,27 @@ static gpl_model* take_spare(gpl_server* s, size_t need, int w) {
     return m;
 }
 
+// A Gaussian mixed-precision fit without offset goes through gpl_fit_batch,
+// together with the other such fits queued at the time (one batched pass).
+static bool batchable(const FitSpec& f) {
+    return f.lik.kind == GPL_LIK_GAUSSIAN && f.opts.precision == GPL_PREC_MIXED && f.opts.approx == GPL_APPROX_LAPLACE &&
+           f.offset.empty();
+}
+
+// Same covariance structure and options: the fits can share a batched pass.
+static bool same_structure(const FitSpec& a, const FitSpec& b) {
+    if (a.terms.size() != b.terms.size() || a.ncols != b.ncols || a.opts.jitter != b.opts.jitter ||
+        a.opts.refine_iters != b.opts.refine_iters)
+        return false;
+    for (size_t k = 0; k < a.terms.size(); ++k) {
+        const gpl_cf_term &u = a.terms[k], &v = b.terms[k];
+        if (u.kind != v.kind || (k > 0 && u.op != v.op) || u.col0 != v.col0 || u.ncols != v.ncols ||
+            u.n_lscale != v.n_lscale || u.base_kind != v.base_kind)
+            return false;
+    }
+    return true;
+}
+
 static void complete_with(gpl_server* s, Req* r, int status) {
     r->status = status;
     r->state = ST_DONE;
@@ -249,6 +275,20 @@ static std::vector<Req*> take_runnable(gpl_server* s) {
         }
         out.push_back(r);
         s->queue.erase(it);
+        if (r->type == REQ_FIT && batchable(*r->spec)) {
+            size_t fp = r->footprint;
+            for (auto jt = s->queue.begin(); jt != s->queue.end() && out.size() < (size_t)s->max_fit_batch;) {
+                Req* q = *jt;
+                if (q->type == REQ_FIT && batchable(*q->spec) && same_structure(*r->spec, *q->spec) &&
+                    s->in_use + fp + q->footprint <= s->budget) {
+                    fp += q->footprint;
+                    out.push_back(q);
+                    jt = s->queue.erase(jt);
+                } else {
+                    ++jt;
+                }
+            }
+        }
         if (r->type == REQ_PRED) {
             int64_t tot = r->m;
             size_t fp = r->footprint;
@@ -276,6 +316,7 @@ static void worker_main(gpl_server* s, int w) {
         std::vector<Req*> job;
         size_t fp = 0;
         gpl_model* m = nullptr;
+        std::vector<gpl_model*> fm;  // fit jobs: one (recycled) model per request
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -294,43 +335,79 @@ static void worker_main(gpl_server* s, int w) {
             s->in_use += fp;
             s->inflight += fp;
             s->launches++;
-            if (job[0]->type == REQ_FIT) m = take_spare(s, job[0]->footprint, w);
-            else m = s->models.find(job[0]->model_id)->second.model;  // refs > 0: resident
+            if (job[0]->type == REQ_FIT)
+                for (Req* r : job) fm.push_back(take_spare(s, r->footprint, w));
+            else
+                m = s->models.find(job[0]->model_id)->second.model;  // refs > 0: resident
         }
         Req* r0 = job[0];
         if (r0->type == REQ_FIT) {
-            const FitSpec& f = *r0->spec;
-            double lz = 0;
-            int st = gpl_fit(c, f.terms.data(), (int32_t)f.terms.size(), f.x.data(), f.n, f.ncols, f.y.data(),
-                             f.trials.empty() ? nullptr : f.trials.data(),
-                             f.offset.empty() ? nullptr : f.offset.data(), &f.lik, &f.opts, &m, &lz);
+            const size_t nj = job.size();
+            std::vector<int> st(nj, GPL_OK);
+            std::vector<double> lz(nj, 0.0);
+            std::vector<char> done(nj, 0);
+            if (batchable(*r0->spec)) {
+                // one batched pass for every Gaussian fit taken (a single one included,
+                // so a result does not depend on whether others arrived with it)
+                std::vector<const double*> xs(nj), ys(nj);
+                std::vector<int64_t> ns(nj), ncs(nj);
+                std::vector<double> sig(nj);
+                std::vector<gpl_cf_term> terms;
+                for (size_t i = 0; i < nj; ++i) {
+                    const FitSpec& f = *job[i]->spec;
+                    xs[i] = f.x.data();
+                    ys[i] = f.y.data();
+                    ns[i] = f.n;
+                    ncs[i] = f.ncols;
+                    sig[i] = f.lik.sigma;
+                    terms.insert(terms.end(), f.terms.begin(), f.terms.end());
+                }
+                std::vector<int32_t> bst(nj, GPL_OK);
+                const int rb = gpl_fit_batch(c, (int32_t)nj, terms.data(), (int32_t)r0->spec->terms.size(), xs.data(),
+                                             ns.data(), ncs.data(), ys.data(), sig.data(), &r0->spec->opts, fm.data(),
+                                             lz.data(), bst.data());
+                for (size_t i = 0; i < nj; ++i)
+                    if (rb == GPL_OK && bst[i] == GPL_OK) done[i] = 1;
+            }
+            for (size_t i = 0; i < nj; ++i) {
+                if (done[i]) continue;  // the others (also a batch member whose FP32 pass failed) fit alone
+                const FitSpec& f = *job[i]->spec;
+                st[i] = gpl_fit(c, f.terms.data(), (int32_t)f.terms.size(), f.x.data(), f.n, f.ncols, f.y.data(),
+                                f.trials.empty() ? nullptr : f.trials.data(),
+                                f.offset.empty() ? nullptr : f.offset.data(), &f.lik, &f.opts, &fm[i], &lz[i]);
+            }
             std::unique_lock<std::mutex> lk(s->mu);
-            r0->status = st;
-            r0->log_evidence = lz;
             release_bytes(s, fp);
             s->inflight -= std::min(s->inflight, fp);
-            Resident& md = s->models.find(r0->model_id)->second;  // release waits while fitting
-            md.fitting = false;
-            if (m && (st == GPL_OK || st == GPL_ERR_NOCONV)) {
-                gpl_model_info(m, nullptr, &r0->iters, nullptr);
-                md.model = m;
-                md.ready = true;
-                md.evicted = false;
-                md.bytes = (size_t)gpl_model_bytes(m);
-                s->in_use += md.bytes;
-                // the resident model may be larger than the admission estimate
-                evict_until(s, 0, r0->model_id);
-            } else {
-                retire_model(s, m, m ? (size_t)gpl_model_bytes(m) : 0, w);
-                if (m) s->in_use += (size_t)gpl_model_bytes(m);
-                md.failed = true;
-                md.status = st;
+            for (size_t i = 0; i < nj; ++i) {
+                Req* r = job[i];
+                gpl_model* mi = fm[i];
+                r->status = st[i];
+                r->log_evidence = lz[i];
+                Resident& md = s->models.find(r->model_id)->second;  // release waits while fitting
+                md.fitting = false;
+                if (mi && (st[i] == GPL_OK || st[i] == GPL_ERR_NOCONV)) {
+                    gpl_model_info(mi, nullptr, &r->iters, nullptr);
+                    md.model = mi;
+                    md.ready = true;
+                    md.evicted = false;
+                    md.bytes = (size_t)gpl_model_bytes(mi);
+                    s->in_use += md.bytes;
+                } else {
+                    retire_model(s, mi, mi ? (size_t)gpl_model_bytes(mi) : 0, w);
+                    if (mi) s->in_use += (size_t)gpl_model_bytes(mi);
+                    md.failed = true;
+                    md.status = st[i];
+                }
+                md.last_use = now_ms();
+                r->state = ST_DONE;
+                r->t_done = now_ms();
+                s->done_fits++;
             }
-            md.last_use = now_ms();
-            r0->state = ST_DONE;
-            r0->t_done = now_ms();
-            s->done_fits++;
-            if (r0->internal) s->reqs.erase(r0->ticket);
+            // the resident models may be larger than the admission estimates
+            for (size_t i = 0; i < nj; ++i) evict_until(s, 0, job[i]->model_id);
+            for (size_t i = 0; i < nj; ++i)
+                if (job[i]->internal) s->reqs.erase(job[i]->ticket);
         } else {
             int64_t tot = 0;
             for (Req* r : job) tot += r->m;

@@ -276,6 +276,20 @@ int gpl_batch_set_tuning(gpl_batch* batch, int32_t count, const int32_t* bucket_
                          const int32_t* nbo);
 void gpl_batch_free(gpl_batch* batch);
 
+/* gp_fit for many Gaussian-likelihood problems at once (new: the reference fits
+ * one GP per call): the batched mixed-precision pass of gpl_batch_create (the
+ * same covariance structure for all problems; x, n, ncols, y, sigma, opts as
+ * there, opts NULL = 6 refinement sweeps) runs once, without a graph, and each
+ * problem's factor, L^-1, alpha and fit vectors become a resident model, as
+ * gpl_fit with GPL_PREC_MIXED leaves them (models[p] NULL: a new model; else an
+ * existing model of ctx, refitted in place).  status[p] is GPL_OK, or
+ * GPL_ERR_NOT_PD / GPL_ERR_NOCONV for a problem whose FP32 factor or refinement
+ * failed (its models[p] is left as given: refit it with gpl_fit).  Returns
+ * GPL_OK unless an argument or the device failed. */
+int gpl_fit_batch(gpl_ctx* ctx, int32_t nprob, const gpl_cf_term* terms, int32_t nterms, const double* const* x,
+                  const int64_t* n, const int64_t* ncols, const double* const* y, const double* sigma,
+                  const gpl_opts* opts, gpl_model** models, double* log_evidence, int32_t* status);
+
 /* ---- serving form: continuous batching of fit / predict requests -------- */
 /* New (the reference answers one R call at a time).  Worker threads, each with
  * its own streams, take the oldest request that can run: predictions wait for
