@@ -86,6 +86,7 @@ def _sig(lib):
         "gpl_model_info": (C.c_int, [_vp, C.POINTER(i64), C.POINTER(i32), _dp]),
         "gpl_model_precision": (C.c_int, [_vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), _dp]),
         "gpl_pred_noalloc": (C.c_int, [_vp, _dp, i64, i32, _dp, _dp]),
+        "gpl_model_trim": (C.c_int, [_vp]),
         "gpl_model_free": (None, [_vp]),
         "gpl_batch_create": (C.c_int, [_vp, i32, C.POINTER(CfTerm), i32, C.POINTER(_dp), C.POINTER(i64),
                                        C.POINTER(i64), C.POINTER(_dp), _dp, C.POINTER(Opts), C.POINTER(_vp)]),

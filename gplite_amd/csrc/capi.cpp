@@ -1777,6 +1777,19 @@ extern "C" int gpl_model_precision(gpl_model* m, int32_t* precision, int32_t* re
     return GPL_OK;
 }
 
+// Frees what only the fit needed (K, the FP64 copy of B, B^-1, workspaces); the
+// model keeps what prediction, draws and gp_save read.  A later gradient or
+// refit allocates again what it needs.
+extern "C" int gpl_model_trim(gpl_model* m) {
+    if (!m) return fail(GPL_ERR_ARG, "NULL model");
+    std::lock_guard<std::mutex> lk(m->ctx->mu);
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    for (int i : {B_K, B_B64, B_BINV, B_DACC, B_PART, B_SPG, B_SPH, B_SPQ}) m->b[i].free_();
+    m->have_binv = false;
+    return GPL_OK;
+}
+
 extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence) {
     if (!m) return fail(GPL_ERR_ARG, "NULL model");
     if (n) *n = m->n;

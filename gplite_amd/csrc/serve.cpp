@@ -29,6 +29,7 @@ using namespace gpl;
 
 extern "C" void gpl_model_free(gpl_model* m);
 extern "C" int64_t gpl_model_bytes(gpl_model* m);
+extern "C" int gpl_model_trim(gpl_model* m);
 extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence);
 extern "C" int gpl_fit_batch(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
                              const double* const* x, const int64_t* n, const int64_t* ncols, const double* const* y,
@@ -376,6 +377,9 @@ static void worker_main(gpl_server* s, int w) {
                                 f.trials.empty() ? nullptr : f.trials.data(),
                                 f.offset.empty() ? nullptr : f.offset.data(), &f.lik, &f.opts, &fm[i], &lz[i]);
             }
+            // resident models keep what prediction reads: K, the FP64 copy of B and B^-1 go
+            for (size_t i = 0; i < nj; ++i)
+                if (fm[i] && (st[i] == GPL_OK || st[i] == GPL_ERR_NOCONV)) gpl_model_trim(fm[i]);
             std::unique_lock<std::mutex> lk(s->mu);
             release_bytes(s, fp);
             s->inflight -= std::min(s->inflight, fp);

@@ -347,6 +347,10 @@ int gpl_draw_prior(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms, const
                    int64_t ncols, double jitter, const double* z, int64_t draws, double* out);
 /* Device bytes held by a fitted model. */
 int64_t gpl_model_bytes(gpl_model* model);
+/* Frees the device buffers only the fit needed (K of a Laplace / EP fit, the mixed
+ * form's FP64 copy of B and B^-1, workspaces); prediction, draws and gp_save read
+ * what remains, and a later gradient or refit allocates again what it needs. */
+int gpl_model_trim(gpl_model* model);
 
 /* No counterpart in the reference: gpl_pred for callers that cannot allocate
  * device memory (serving loops that must not stall on hipMalloc / hipFree,

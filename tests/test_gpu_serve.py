@@ -215,3 +215,34 @@ def test_release_is_final(gpl):
     mid2, tk2 = s.submit_fit(gp, x, y)
     assert np.isfinite(s.wait(tk2)["log_evidence"])
     s.close()
+
+
+@pytest.mark.parametrize("kind", ["laplace", "mixed", "ep"])
+def test_trimmed_model_predicts_and_differentiates_as_before(gpl, kind):
+    """gpl_model_trim (what the server does to every resident model) keeps what
+    prediction, draws and the gradient read: the same results, bit for bit, with
+    fewer device bytes."""
+    from gplite_amd import _lib as L
+    G = gpl
+    r = np.random.default_rng(10)
+    if kind == "mixed":
+        gp, x, y = _c5_request(G, r, 900, False)
+        gf = G.gp_fit(gp, x, y, precision="mixed")
+    else:
+        gp, x, y = _c5_request(G, r, 900, True)
+        if kind == "ep":
+            gp = G.gp_init(gp.cfs, gp.lik, approx=G.approx_ep())
+        gf = G.gp_fit(gp, x, y)
+    xt = r.uniform(0, 1, (200, 6))
+    before = G.gp_pred(gf, xt, var=True)
+    d0 = G.gp_draw(gf, xt[:20], draws=3, transform=False, seed=4) if kind != "ep" else None
+    g0 = G.gp_energy_grad(gf) if kind == "mixed" else None
+    b0 = L.lib().gpl_model_bytes(gf.fit.handle)
+    L.check(L.lib().gpl_model_trim(gf.fit.handle))
+    assert L.lib().gpl_model_bytes(gf.fit.handle) < b0
+    after = G.gp_pred(gf, xt, var=True)
+    assert np.array_equal(before["mean"], after["mean"]) and np.array_equal(before["var"], after["var"])
+    if d0 is not None:
+        assert np.array_equal(d0, G.gp_draw(gf, xt[:20], draws=3, transform=False, seed=4))
+    if g0 is not None:
+        assert np.array_equal(g0, G.gp_energy_grad(gf))
