@@ -632,6 +632,7 @@ static int run_once(gpl_batch* B) {
 // One pass (the graph, or eager launches when `eager` or profiling), then the
 // results and the per-problem statuses.  Caller holds the context's lock.
 static int batch_pass(gpl_batch* B, bool eager, double* log_evidence, double* grad) {
+    Range rg("gpl_batch_pass");
     HIP_TRY(hipSetDevice(B->ctx->device));
     hipStream_t s = B->ctx->stream;
     if (eager || B->ctx->prof) {

@@ -21,6 +21,12 @@
 
 #include "host_common.h"
 #include "model.h"
+#include "serve_device.h"
+
+static_assert(gpl::kServeFeatCols == gpl::MAX_FEAT, "the serving footprint counts the device feature columns");
+
+gpl::TraceRange::TraceRange(const char* name) { roctxRangePushA(name); }
+gpl::TraceRange::~TraceRange() { roctxRangePop(); }
 
 using namespace gpl;
 
@@ -1216,6 +1222,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     if (!c || !x || !y || !lik || !inout) return fail(GPL_ERR_ARG, "NULL argument");
     if (n < 1 || ncols < 0) return fail(GPL_ERR_ARG, "n must be >= 1");
     std::lock_guard<std::mutex> lk(c->mu);
+    Range rg("gpl_fit");
     HIP_TRY(hipSetDevice(c->device));
     gpl_opts o;
     if (opts) o = *opts;
@@ -1302,6 +1309,7 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
         return fail(GPL_ERR_UNSUPPORTED, "the energy gradient is provided for the Gaussian likelihood");
     gpl_ctx* c = m->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
+    Range rg("gpl_energy_grad");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = m->n, N = m->N;
@@ -1418,6 +1426,7 @@ int gpl::pred_on(gpl_ctx* c, gpl_model* m, const double* xt, int64_t mt, int32_t
     if (m->no_posterior) return fail(GPL_ERR_ARG, "the fit has no posterior (its log evidence is -Inf)");
     if (mt <= 0) return GPL_OK;
     std::lock_guard<std::mutex> lk(c->mu);
+    Range rg("gpl_pred");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = m->n, N = m->N, M = pad64(mt);

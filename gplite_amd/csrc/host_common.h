@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <map>
 #include <mutex>
@@ -162,6 +163,15 @@ inline hipError_t download_2d(gpl_ctx* c, hipStream_t s, double* dst, const doub
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(dst, st, (size_t)rows * cols * 8, hipMemcpyDeviceToHost, s);
 }
+
+// A ROCTX range over one phase of a call (fit, gradient, prediction, batched
+// pass, serving job), shown by rocprofv3 --marker-trace beside the kernels.
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 // Times one launch group on the library stream when profiling is on.
 struct Prof {

@@ -23,18 +23,10 @@
 #include <thread>
 #include <vector>
 
-#include "host_common.h"
+#include "serve_device.h"
 
 using namespace gpl;
 
-extern "C" void gpl_model_free(gpl_model* m);
-extern "C" int64_t gpl_model_bytes(gpl_model* m);
-extern "C" int gpl_model_trim(gpl_model* m);
-extern "C" int gpl_model_info(gpl_model* m, int64_t* n, int32_t* iters, double* log_evidence);
-extern "C" int gpl_fit_batch(gpl_ctx* c, int32_t nprob, const gpl_cf_term* terms, int32_t nterms,
-                             const double* const* x, const int64_t* n, const int64_t* ncols, const double* const* y,
-                             const double* sigma, const gpl_opts* opts, gpl_model** models, double* log_evidence,
-                             int32_t* status);
 
 namespace {
 
@@ -95,7 +87,7 @@ struct Resident {
 };
 
 size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
-    const size_t N = (size_t)pad64(n), NN = N * N;
+    const size_t N = (size_t)serve_pad64(n), NN = N * N;
     const bool mixed = o.precision == GPL_PREC_MIXED;
     const bool gauss = lik.kind == GPL_LIK_GAUSSIAN;
     size_t b = NN * (mixed ? 4 : 8);                              // factor of B
@@ -104,12 +96,12 @@ size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
     if (mixed) b += NN * 4 + NN * 8 + (gauss ? NN * 4 : 0);       // L^-1, FP64 B, B^-1 (Gaussian)
     else if (o.flags & GPL_FLAG_INVERSE) b += NN * 8;             // L^-1
     const size_t T = N / 64;
-    return b + 2 * N * 64 * 8 + N * (MAX_FEAT + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + N * 8 * 32 + 4096;
+    return b + 2 * N * 64 * 8 + N * (kServeFeatCols + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + N * 8 * 32 + 4096;
 }
 
 size_t pred_footprint(int64_t n, int64_t m) {
-    const size_t N = (size_t)pad64(n), M = (size_t)pad64(m);
-    return 2 * N * M * 8 + M * (MAX_FEAT + 4) * 8;
+    const size_t N = (size_t)serve_pad64(n), M = (size_t)serve_pad64(m);
+    return 2 * N * M * 8 + M * (kServeFeatCols + 4) * 8;
 }
 
 }  // namespace
@@ -342,6 +334,7 @@ static void worker_main(gpl_server* s, int w) {
                 m = s->models.find(job[0]->model_id)->second.model;  // refs > 0: resident
         }
         Req* r0 = job[0];
+        TraceRange rg(r0->type == REQ_FIT ? "serve_fit_job" : "serve_pred_job");
         if (r0->type == REQ_FIT) {
             const size_t nj = job.size();
             std::vector<int> st(nj, GPL_OK);
@@ -423,7 +416,7 @@ static void worker_main(gpl_server* s, int w) {
                     memcpy(xt.data() + off + c2 * tot, r->xt.data() + c2 * r->m, r->m * 8);
                 off += r->m;
             }
-            int st = pred_on(c, m, xt.data(), tot, r0->want_var, mean.data(), r0->want_var ? var.data() : nullptr);
+            int st = gpl_pred_ctx(c, m, xt.data(), tot, r0->want_var, mean.data(), r0->want_var ? var.data() : nullptr);
             off = 0;
             for (Req* r : job) {
                 if (st == GPL_OK) {
