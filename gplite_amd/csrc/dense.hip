@@ -686,56 +686,120 @@ void symv_lower_f64(hipStream_t s, const double* K, int64_t ld, int64_t N, const
 }
 
 // y = A v for a symmetric FP64 matrix held as its lower 64x64 tiles (column-
-// major, leading dimension ld; diagonal tiles complete).  Grid (T, nchunk,
-// batch): block (ti, ch) sums the tiles of block row ti whose column tile lies
-// in chunk ch; tiles above the diagonal are read as the transpose of the
-// stored tile (tj, ti), staged through LDS so every global read is coalesced.
-// A second pass adds the chunk partials in a fixed order.
-__global__ __launch_bounds__(256) void k_symv_lt(const double* __restrict__ A, int64_t ld, int64_t T, int nchunk,
-                                                 const double* __restrict__ v, double* __restrict__ part, int64_t sA,
-                                                 int64_t sV) {
-    const int64_t b = blockIdx.z;
+// major, leading dimension ld; diagonal tiles complete), reading every stored
+// tile once.  The lower tiles are grouped into SG x SG super-tiles; block
+// (RI, CJ) of the lower super-tile grid (RI >= CJ) takes tiles (ti, tj) of its
+// super-tile with tj <= ti.  An off-diagonal tile serves both triangles: A_ij v_j
+// goes to output block i (the "row" role) and A_ij^T v_i to output block j (the
+// "column" role); a diagonal tile serves the row role alone.  Thread (r, p)
+// reads rows r of columns 16p..16p+15 of each tile, so every load is a 512-byte
+// column segment per wave.  Row-role sums are kept per row tile (reduced over
+// the four waves at the end), column-role sums per column tile and column
+// (reduced over the 64 rows at the end), and both are written to their own
+// partial slots: part[CJ][i] (row role of column group CJ) and part[S + RI][j]
+// (column role of row group RI), S = number of super-tile rows.  A second pass
+// adds the slots that exist for each output block in a fixed order, so the
+// result does not depend on scheduling.
+constexpr int kSymvSG = 4;
+
+int symv_slots(int64_t N) {
+    const int64_t T = N / 64, S = (T + kSymvSG - 1) / kSymvSG;
+    return (int)(2 * (S > 0 ? S : 1));
+}
+
+__global__ __launch_bounds__(256) void k_symv2(const double* __restrict__ A, int64_t ld, int64_t T,
+                                               const double* __restrict__ v, double* __restrict__ part, int64_t sA,
+                                               int64_t sV) {
+    constexpr int G = kSymvSG;
+    const int64_t b = blockIdx.y, S = (T + G - 1) / G;
     A += b * sA;
     v += b * sV;
-    part += b * (int64_t)nchunk * T * 64;
-    __shared__ double tile[64][65];
-    __shared__ double vc[64];
-    __shared__ double red[256];
-    const int64_t ti = blockIdx.x;
-    const int ch = blockIdx.y;
-    const int t = threadIdx.x, e = t & 63, p = t >> 6;
-    const int64_t per = (T + nchunk - 1) / nchunk, tb = ch * per, te = (tb + per < T) ? tb + per : T;
-    double s = 0.0;
-    for (int64_t tj = tb; tj < te; ++tj) {
-        __syncthreads();
-        if (t < 64) vc[t] = v[tj * 64 + t];
-        if (tj > ti) {
-            const double* At = A + tj * 64 + ti * 64 * ld;  // stored tile (tj, ti)
-#pragma unroll 4
-            for (int u = 0; u < 16; ++u) {
-                const int c = p * 16 + u;
-                tile[c][e] = At[e + (int64_t)c * ld];
-            }
-        }
-        __syncthreads();
-        if (tj <= ti) {
-            const double* At = A + ti * 64 + tj * 64 * ld;
-#pragma unroll 4
-            for (int u = 0; u < 16; ++u) {
-                const int c = p * 16 + u;
-                s += At[e + (int64_t)c * ld] * vc[c];
-            }
-        } else {
-#pragma unroll 4
-            for (int u = 0; u < 16; ++u) {
-                const int r = p * 16 + u;
-                s += tile[e][r] * vc[r];
+    part += b * 2 * S * T * 64;
+    int64_t RI, CJ;
+    lower_tile(blockIdx.x, RI, CJ);
+    const int64_t ti0 = RI * G, tj0 = CJ * G;
+    const int ni = (int)((ti0 + G <= T) ? G : T - ti0), nj = (int)((tj0 + G <= T) ? G : T - tj0);
+    const int t = threadIdx.x, r = t & 63, p = t >> 6;
+    __shared__ double vc[G][64];   // v over the column tiles
+    __shared__ double red[4][16][65];
+    if (t < 64) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) vc[q][t] = (q < nj) ? v[(tj0 + q) * 64 + t] : 0.0;
+    }
+    double vr[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) vr[q] = (q < ni) ? v[(ti0 + q) * 64 + r] : 0.0;
+    __syncthreads();
+    double rowacc[G], colacc[G][16];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        rowacc[q] = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) colacc[q][u] = 0.0;
+    }
+#pragma unroll
+    for (int qi = 0; qi < G; ++qi) {
+#pragma unroll
+        for (int qj = 0; qj < G; ++qj) {
+            const int64_t ti = ti0 + qi, tj = tj0 + qj;
+            if (qi >= ni || qj >= nj || tj > ti) continue;  // block-uniform
+            const double* At = A + ti * 64 + r + (tj * 64 + 16 * p) * ld;
+            double a[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a[u] = At[(int64_t)u * ld];
+            double s = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += a[u] * vc[qj][16 * p + u];
+            rowacc[qi] += s;
+            if (tj < ti) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) colacc[qj][u] += a[u] * vr[qi];
             }
         }
     }
-    red[t] = s;
-    __syncthreads();
-    if (t < 64) part[(int64_t)ch * T * 64 + ti * 64 + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+    // row role: sum over the four waves, per row tile
+    double* prow = part + CJ * T * 64;
+#pragma unroll
+    for (int qi = 0; qi < G; ++qi) {
+        red[p][0][r] = rowacc[qi];
+        __syncthreads();
+        if (t < 64 && qi < ni)
+            prow[(ti0 + qi) * 64 + t] = red[0][0][t] + red[1][0][t] + red[2][0][t] + red[3][0][t];
+        __syncthreads();
+    }
+    // column role: sum over the 64 rows, per column tile and column (four threads
+    // per column, 16 rows each, then combined in a fixed order)
+    double* pcol = part + (S + RI) * T * 64;
+#pragma unroll
+    for (int qj = 0; qj < G; ++qj) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) red[p][u][r] = colacc[qj][u];
+        __syncthreads();
+        const int c = t >> 2, k4 = t & 3;  // column c = 16 * (c / 16) + c % 16 of wave c / 16
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s += red[c >> 4][c & 15][16 * k4 + e];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (k4 == 0 && qj < nj) pcol[(tj0 + qj) * 64 + c] = s;
+        __syncthreads();
+    }
+}
+
+// y[i] = sum of the row-role slots CJ <= I and the column-role slots RI >= I of
+// output block i (I = its super-tile row), in that order
+__global__ void k_symv2_sum(const double* __restrict__ part, int64_t T, double* __restrict__ y, int64_t sV) {
+    constexpr int G = kSymvSG;
+    const int64_t b = blockIdx.y, S = (T + G - 1) / G, N = T * 64;
+    part += b * 2 * S * N;
+    y += b * sV;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t I = (i / 64) / G;
+    double s = 0.0;
+    for (int64_t cg = 0; cg <= I; ++cg) s += part[cg * N + i];
+    for (int64_t rg = I; rg < S; ++rg) s += part[(S + rg) * N + i];
+    y[i] = s;
 }
 
 __global__ void k_chunk_sum(const double* __restrict__ part, int64_t N, int nchunk, double* __restrict__ y,
@@ -750,14 +814,16 @@ __global__ void k_chunk_sum(const double* __restrict__ part, int64_t N, int nchu
     y[i] = s;
 }
 
-// part: batch * nchunk * N doubles
+// part: batch * symv_slots(N) * N doubles (nchunk is not used)
 void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV);
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV) {
-    const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_symv_lt, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, A, N, T,
-                       nchunk, v, part, sA, sV);
-    chunk_sum_f64(s, part, N, nchunk, y, batch, sV);
+    (void)nchunk;
+    const int64_t T = N / 64, S = (T + kSymvSG - 1) / kSymvSG;
+    hipLaunchKernelGGL(k_symv2, dim3((unsigned)(S * (S + 1) / 2), (unsigned)batch), dim3(256), 0, s, A, N, T, v, part,
+                       sA, sV);
+    hipLaunchKernelGGL(k_symv2_sum, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, T, y,
+                       sV);
 }
 
 // y[b][i] = sum_c part[b][c][i] (fixed order)

@@ -96,7 +96,9 @@ size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
     if (mixed) b += NN * 4 + NN * 8 + (gauss ? NN * 4 : 0);       // L^-1, FP64 B, B^-1 (Gaussian)
     else if (o.flags & GPL_FLAG_INVERSE) b += NN * 8;             // L^-1
     const size_t T = N / 64;
-    return b + 2 * N * 64 * 8 + N * (kServeFeatCols + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + N * 8 * 32 + 4096;
+    // matrix-vector partials: the larger of 32 chunks and two slots per four tile rows
+    const size_t parts = std::max<size_t>(32, 2 * ((T + 3) / 4));
+    return b + 2 * N * 64 * 8 + N * (kServeFeatCols + 40) * 8 + T * (T + 1) / 2 * 8 * 8 + N * 8 * parts + 4096;
 }
 
 size_t pred_footprint(int64_t n, int64_t m) {
