@@ -489,7 +489,7 @@ extern "C" int gpl_batch_create(gpl_ctx* c, int32_t nprob, const gpl_cf_term* te
                   k->b[BB_DINV].ensure((size_t)nb * N * 64 * 8) && k->b[BB_DACC].ensure((size_t)nb * N * 64 * 8) &&
                   k->b[BB_LDIAG].ensure(vec) &&
                   k->b[BB_INFO].ensure(nb * 4 + 64) && k->b[BB_PART].ensure((size_t)nb * T * (T + 1) / 2 * 8 * std::max(1, B->hp.ngrad)) &&
-                  k->b[BB_KMV].ensure((size_t)nb * std::max(k->nchunk, symv_slots(N)) * N * 8) &&
+                  k->b[BB_KMV].ensure((size_t)nb * std::max(k->nchunk, symv_slots(N, nb)) * N * 8) &&
                   k->b[BB_RED].ensure((size_t)nb * red_stride(B) * 8) &&
                   k->b[BB_CORR].ensure((size_t)nb * T * (T + 1) / 2 * 8);
         for (int i : {BB_Y, BB_SV, BB_RHS, BB_U, BB_R, BB_D, BB_T1, BB_KU, BB_ALPHA}) ok = ok && k->b[i].ensure(vec);

@@ -508,7 +508,7 @@ static int alloc_model(gpl_model* m, int64_t n, int64_t ncols, bool need_k, bool
                   B_POLD, B_W1, B_W2, B_W3, B_W4})
         ENSURE(m, i, vec);
     ENSURE(m, B_SCAL, 256 * 8);
-    ENSURE(m, B_KMV, (size_t)N * std::max(kmatvec_chunks(N), symv_slots(N)) * 8);
+    ENSURE(m, B_KMV, (size_t)N * std::max(kmatvec_chunks(N), symv_slots(N, 1)) * 8);
     ENSURE(m, B_GH, 64 * 8);
     ENSURE(m, B_INFO, 64);
     return GPL_OK;
@@ -608,7 +608,7 @@ static int fit_gaussian_mixed(gpl_model* m) {
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     ENSURE(m, B_B64, (size_t)N * N * 8);
     const int nch = kmatvec_chunks(N);
-    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N)) * 8);
+    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N, 1)) * 8);
     float* X = m->b[B_XINV].as<float>();
     double* B64 = m->d(B_B64);
     {
@@ -851,7 +851,7 @@ static int fit_laplace_mixed(gpl_model* m) {
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     ENSURE(m, B_B64, (size_t)N * N * 8);
     const int nch = kmatvec_chunks(N);
-    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N)) * 8);
+    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N, 1)) * 8);
     float* L = m->b[B_L].as<float>();
     float* X = m->b[B_XINV].as<float>();
     double* B64 = m->d(B_B64);
@@ -1119,7 +1119,7 @@ static int fit_ep_mixed(gpl_model* m) {
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     ENSURE(m, B_B64, (size_t)N * N * 8);
     const int nch = kmatvec_chunks(N);
-    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N)) * 8);
+    ENSURE(m, B_KMV, (size_t)N * std::max(nch, symv_slots(N, 1)) * 8);
     const int maxref = m->opts.refine_iters > 0 ? m->opts.refine_iters : 8;
     float* L32 = m->b[B_L].as<float>();
     float* X32 = m->b[B_XINV].as<float>();

@@ -687,30 +687,65 @@ void symv_lower_f64(hipStream_t s, const double* K, int64_t ld, int64_t N, const
 
 // y = A v for a symmetric FP64 matrix held as its lower 64x64 tiles (column-
 // major, leading dimension ld; diagonal tiles complete), reading every stored
-// tile once.  The lower tiles are grouped into SG x SG super-tiles; block
+// tile once.  The lower tiles are grouped into G x G super-tiles; block
 // (RI, CJ) of the lower super-tile grid (RI >= CJ) takes tiles (ti, tj) of its
 // super-tile with tj <= ti.  An off-diagonal tile serves both triangles: A_ij v_j
 // goes to output block i (the "row" role) and A_ij^T v_i to output block j (the
 // "column" role); a diagonal tile serves the row role alone.  Thread (r, p)
-// reads rows r of columns 16p..16p+15 of each tile, so every load is a 512-byte
-// column segment per wave.  Row-role sums are kept per row tile (reduced over
-// the four waves at the end), column-role sums per column tile and column
-// (reduced over the 64 rows at the end), and both are written to their own
-// partial slots: part[CJ][i] (row role of column group CJ) and part[S + RI][j]
-// (column role of row group RI), S = number of super-tile rows.  A second pass
-// adds the slots that exist for each output block in a fixed order, so the
-// result does not depend on scheduling.
-constexpr int kSymvSG = 4;
+// reads row r of columns 16p..16p+15 of each tile, so every load is a 512-byte
+// column segment per wave.  Row-role sums are kept per row tile and reduced over
+// the four waves at the end; column-role sums are kept for one column tile at a
+// time (16 per thread) and reduced over the 64 rows by a butterfly that halves
+// the values held per lane at each exchange.  Both go to partial slots of their
+// own: part[CJ][i] (row role of column group CJ) and part[S + RI][j] (column role
+// of row group RI), S = number of super-tile rows.  A second pass adds the slots
+// that exist for each output block in a fixed order, so the result does not
+// depend on scheduling.
+static int symv_group(int batch) { return batch >= 4 ? 4 : 2; }
 
-int symv_slots(int64_t N) {
-    const int64_t T = N / 64, S = (T + kSymvSG - 1) / kSymvSG;
+int symv_slots(int64_t N, int batch) {
+    const int64_t T = N / 64, G = symv_group(batch), S = (T + G - 1) / G;
     return (int)(2 * (S > 0 ? S : 1));
 }
 
+// Sum of v over the 64 lanes for 16 values per lane: after the exchanges with
+// lanes l^32, l^16, l^8, l^4 a lane holds one value, the partial of value
+// u(l) = 8 b5 + 4 b4 + 2 b3 + b2 (bk = bit k of l) over the 16 lanes that agree
+// with it in bits 2..5; the exchanges with l^2 and l^1 complete the sum.
+__device__ __forceinline__ double butterfly16(double (&v)[16], int lane) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const bool hi = (lane & 32) != 0;
+        const double keep = hi ? v[u + 8] : v[u], give = hi ? v[u] : v[u + 8];
+        v[u] = keep + __shfl_xor(give, 32, 64);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const bool hi = (lane & 16) != 0;
+        const double keep = hi ? v[u + 4] : v[u], give = hi ? v[u] : v[u + 4];
+        v[u] = keep + __shfl_xor(give, 16, 64);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const bool hi = (lane & 8) != 0;
+        const double keep = hi ? v[u + 2] : v[u], give = hi ? v[u] : v[u + 2];
+        v[u] = keep + __shfl_xor(give, 8, 64);
+    }
+    {
+        const bool hi = (lane & 4) != 0;
+        const double keep = hi ? v[1] : v[0], give = hi ? v[0] : v[1];
+        v[0] = keep + __shfl_xor(give, 4, 64);
+    }
+    double x = v[0];
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;
+}
+
+template <int G>
 __global__ __launch_bounds__(256) void k_symv2(const double* __restrict__ A, int64_t ld, int64_t T,
                                                const double* __restrict__ v, double* __restrict__ part, int64_t sA,
                                                int64_t sV) {
-    constexpr int G = kSymvSG;
     const int64_t b = blockIdx.y, S = (T + G - 1) / G;
     A += b * sA;
     v += b * sV;
@@ -720,29 +755,33 @@ __global__ __launch_bounds__(256) void k_symv2(const double* __restrict__ A, int
     const int64_t ti0 = RI * G, tj0 = CJ * G;
     const int ni = (int)((ti0 + G <= T) ? G : T - ti0), nj = (int)((tj0 + G <= T) ? G : T - tj0);
     const int t = threadIdx.x, r = t & 63, p = t >> 6;
-    __shared__ double vc[G][64];   // v over the column tiles
-    __shared__ double red[4][16][65];
+    __shared__ double vc[G][64];  // v over the column tiles
+    __shared__ double red[4][64];
     if (t < 64) {
 #pragma unroll
         for (int q = 0; q < G; ++q) vc[q][t] = (q < nj) ? v[(tj0 + q) * 64 + t] : 0.0;
     }
-    double vr[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) vr[q] = (q < ni) ? v[(ti0 + q) * 64 + r] : 0.0;
-    __syncthreads();
-    double rowacc[G], colacc[G][16];
+    double vr[G], rowacc[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
+        vr[q] = (q < ni) ? v[(ti0 + q) * 64 + r] : 0.0;
         rowacc[q] = 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) colacc[q][u] = 0.0;
     }
+    __syncthreads();
+    // the column role's output: lane l of wave p ends with column 16p + u(l)
+    const int uo = ((r >> 5) & 1) * 8 + ((r >> 4) & 1) * 4 + ((r >> 3) & 1) * 2 + ((r >> 2) & 1);
+    double* pcol = part + (S + RI) * T * 64;
 #pragma unroll
-    for (int qi = 0; qi < G; ++qi) {
+    for (int qj = 0; qj < G; ++qj) {
+        if (qj >= nj) break;  // block-uniform
+        const int64_t tj = tj0 + qj;
+        double colacc[16];
 #pragma unroll
-        for (int qj = 0; qj < G; ++qj) {
-            const int64_t ti = ti0 + qi, tj = tj0 + qj;
-            if (qi >= ni || qj >= nj || tj > ti) continue;  // block-uniform
+        for (int u = 0; u < 16; ++u) colacc[u] = 0.0;
+#pragma unroll
+        for (int qi = 0; qi < G; ++qi) {
+            const int64_t ti = ti0 + qi;
+            if (qi >= ni || tj > ti) continue;  // block-uniform
             const double* At = A + ti * 64 + r + (tj * 64 + 16 * p) * ld;
             double a[16];
 #pragma unroll
@@ -753,43 +792,26 @@ __global__ __launch_bounds__(256) void k_symv2(const double* __restrict__ A, int
             rowacc[qi] += s;
             if (tj < ti) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) colacc[qj][u] += a[u] * vr[qi];
+                for (int u = 0; u < 16; ++u) colacc[u] += a[u] * vr[qi];
             }
         }
+        const double cs = butterfly16(colacc, r);
+        if ((r & 3) == 0) pcol[tj * 64 + 16 * p + uo] = cs;
     }
     // row role: sum over the four waves, per row tile
     double* prow = part + CJ * T * 64;
 #pragma unroll
     for (int qi = 0; qi < G; ++qi) {
-        red[p][0][r] = rowacc[qi];
+        red[p][r] = rowacc[qi];
         __syncthreads();
-        if (t < 64 && qi < ni)
-            prow[(ti0 + qi) * 64 + t] = red[0][0][t] + red[1][0][t] + red[2][0][t] + red[3][0][t];
-        __syncthreads();
-    }
-    // column role: sum over the 64 rows, per column tile and column (four threads
-    // per column, 16 rows each, then combined in a fixed order)
-    double* pcol = part + (S + RI) * T * 64;
-#pragma unroll
-    for (int qj = 0; qj < G; ++qj) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) red[p][u][r] = colacc[qj][u];
-        __syncthreads();
-        const int c = t >> 2, k4 = t & 3;  // column c = 16 * (c / 16) + c % 16 of wave c / 16
-        double s = 0.0;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) s += red[c >> 4][c & 15][16 * k4 + e];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        if (k4 == 0 && qj < nj) pcol[(tj0 + qj) * 64 + c] = s;
+        if (t < 64 && qi < ni) prow[(ti0 + qi) * 64 + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
         __syncthreads();
     }
 }
 
 // y[i] = sum of the row-role slots CJ <= I and the column-role slots RI >= I of
 // output block i (I = its super-tile row), in that order
-__global__ void k_symv2_sum(const double* __restrict__ part, int64_t T, double* __restrict__ y, int64_t sV) {
-    constexpr int G = kSymvSG;
+__global__ void k_symv2_sum(const double* __restrict__ part, int64_t T, int G, double* __restrict__ y, int64_t sV) {
     const int64_t b = blockIdx.y, S = (T + G - 1) / G, N = T * 64;
     part += b * 2 * S * N;
     y += b * sV;
@@ -814,15 +836,18 @@ __global__ void k_chunk_sum(const double* __restrict__ part, int64_t N, int nchu
     y[i] = s;
 }
 
-// part: batch * symv_slots(N) * N doubles (nchunk is not used)
+// part: batch * symv_slots(N, batch) * N doubles (nchunk is not used)
 void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV);
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV) {
     (void)nchunk;
-    const int64_t T = N / 64, S = (T + kSymvSG - 1) / kSymvSG;
-    hipLaunchKernelGGL(k_symv2, dim3((unsigned)(S * (S + 1) / 2), (unsigned)batch), dim3(256), 0, s, A, N, T, v, part,
-                       sA, sV);
-    hipLaunchKernelGGL(k_symv2_sum, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, T, y,
+    const int64_t T = N / 64;
+    const int G = symv_group(batch);
+    const int64_t S = (T + G - 1) / G;
+    const dim3 grid((unsigned)(S * (S + 1) / 2), (unsigned)batch);
+    if (G == 4) hipLaunchKernelGGL(k_symv2<4>, grid, dim3(256), 0, s, A, N, T, v, part, sA, sV);
+    else hipLaunchKernelGGL(k_symv2<2>, grid, dim3(256), 0, s, A, N, T, v, part, sA, sV);
+    hipLaunchKernelGGL(k_symv2_sum, dim3((unsigned)((N + 255) / 256), (unsigned)batch), dim3(256), 0, s, part, T, G, y,
                        sV);
 }
 

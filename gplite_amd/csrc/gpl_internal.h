@@ -131,8 +131,8 @@ void colsumsq_f64(hipStream_t s, const double* A, int64_t ld, int64_t nrow, int6
 // y[b][i] = sum over c of part[b][c][i] (fixed order)
 void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, double* y, int batch, int64_t sV);
 // y = A v, A symmetric FP64 held as lower 64x64 tiles, each read once (part:
-// batch * symv_slots(N) * N doubles; nchunk is not used)
-int symv_slots(int64_t N);
+// batch * symv_slots(N, batch) * N doubles; nchunk is not used)
+int symv_slots(int64_t N, int batch);
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV);
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
