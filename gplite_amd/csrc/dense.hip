@@ -43,14 +43,14 @@ template <> struct vec2_of<float> { typedef float2 type; };
 
 // LDS tiles are [k][BM + 1]: the odd row stride keeps the eight lanes that
 // write one column's k values on distinct banks.
-template <int BM, int MCONT, typename T>
+template <int BM, int MCONT, typename T, int NT = 256>
 __device__ __forceinline__ void lds_gemm_load(const T* P, int64_t ld, int64_t base, int64_t lim, int64_t k0,
-                                              double2 (&r)[BM / 64 * 2]) {
+                                              double2 (&r)[BM * 8 / NT]) {
     typedef typename vec2_of<T>::type V2;
-    constexpr int AL = BM / 64 * 2;
+    constexpr int AL = BM * 8 / NT;
     const int t = threadIdx.x;
     if (MCONT) {  // element (m, k) at P[m + k*ld]: pairs along m, 128-byte rows per 8 lanes
-        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = 256 / (BM / 2);
+        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = NT / (BM / 2);
         const bool ok = base + mp < lim;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
@@ -66,7 +66,7 @@ __device__ __forceinline__ void lds_gemm_load(const T* P, int64_t ld, int64_t ba
         const int kq = t & 7, m = t >> 3;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
-            const int64_t mm = base + m + 32 * j;
+            const int64_t mm = base + m + (NT / 8) * j;
             if (mm < lim) {
                 const V2 v = *reinterpret_cast<const V2*>(P + (k0 + 2 * kq) + mm * ld);
                 r[j] = double2{(double)v.x, (double)v.y};
@@ -77,12 +77,12 @@ __device__ __forceinline__ void lds_gemm_load(const T* P, int64_t ld, int64_t ba
     }
 }
 
-template <int BM, int MCONT>
-__device__ __forceinline__ void lds_gemm_store(double (*S)[BM + 1], const double2 (&r)[BM / 64 * 2]) {
-    constexpr int AL = BM / 64 * 2;
+template <int BM, int MCONT, int NT = 256>
+__device__ __forceinline__ void lds_gemm_store(double (*S)[BM + 1], const double2 (&r)[BM * 8 / NT]) {
+    constexpr int AL = BM * 8 / NT;
     const int t = threadIdx.x;
     if (MCONT) {
-        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = 256 / (BM / 2);
+        const int mp = (t % (BM / 2)) * 2, kq = t / (BM / 2), ks = NT / (BM / 2);
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
             S[kq + ks * j][mp] = r[j].x;
@@ -92,19 +92,22 @@ __device__ __forceinline__ void lds_gemm_store(double (*S)[BM + 1], const double
         const int kq = t & 7, m = t >> 3;
 #pragma unroll
         for (int j = 0; j < AL; ++j) {
-            S[2 * kq][m + 32 * j] = r[j].x;
-            S[2 * kq + 1][m + 32 * j] = r[j].y;
+            S[2 * kq][m + (NT / 8) * j] = r[j].x;
+            S[2 * kq + 1][m + (NT / 8) * j] = r[j].y;
         }
     }
 }
 
 // C = beta*C + alpha*op(A) op(B), FP64 MFMA; A, B and C may be stored as float or double.
-template <int TA, int TB, int BM, typename TAs, typename TBs, typename TCs>
-__global__ __launch_bounds__(256) void k_gemm_lds(int64_t M, int64_t N, int64_t K, int64_t tilesM, double alpha,
-                                                  const TAs* A, int64_t lda, const TBs* B, int64_t ldb,
-                                                  double beta, TCs* C, int64_t ldc, int tiles, int64_t sA,
-                                                  int64_t sB, int64_t sC) {
-    constexpr int WT = BM / 2, NBK = WT / 16, AL = BM / 64 * 2;
+// NW waves in a 2 x NW/2 grid each own a (BM/2) x (2BM/NW) block of 16x16 MFMA tiles.
+template <int TA, int TB, int BM, typename TAs, typename TBs, typename TCs, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_gemm_lds(int64_t M, int64_t N, int64_t K, int64_t tilesM, double alpha,
+                                                      const TAs* A, int64_t lda, const TBs* B, int64_t ldb,
+                                                      double beta, TCs* C, int64_t ldc, int tiles, int64_t sA,
+                                                      int64_t sB, int64_t sC) {
+    constexpr int NT = 64 * NW, WCn = NW / 2, WTR = BM / 2, WTC = BM / WCn;
+    constexpr int NBR = WTR / 16, NBC = WTC / 16, AL = BM * 8 / NT;
+    static_assert(NBR >= 1 && NBC >= 1 && AL >= 1, "tile shape");
     A += blockIdx.y * sA;  // batch: one problem per grid row
     B += blockIdx.y * sB;
     C += blockIdx.y * sC;
@@ -127,55 +130,58 @@ __global__ __launch_bounds__(256) void k_gemm_lds(int64_t M, int64_t N, int64_t 
     const int64_t nch = kend > kbeg ? (kend - kbeg) / 16 : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, g = lane >> 4;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WCn, wc = wave % WCn;
 
-    d4 acc[NBK][NBK];
+    d4 acc[NBR][NBC];
 #pragma unroll
-    for (int i = 0; i < NBK; ++i)
+    for (int i = 0; i < NBR; ++i)
 #pragma unroll
-        for (int j = 0; j < NBK; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < NBC; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
+    auto chunk = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kk = 4 * g + s;
+            double a[NBR], b[NBC];
+#pragma unroll
+            for (int i = 0; i < NBR; ++i) a[i] = As[buf][kk][wr * WTR + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < NBC; ++j) b[j] = Bs[buf][kk][wc * WTC + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < NBR; ++i)
+#pragma unroll
+                for (int j = 0; j < NBC; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    };
     double2 ra[AL], rb[AL];
     if (nch > 0) {
-        lds_gemm_load<BM, !TA>(A, lda, row0, M, kbeg, ra);
-        lds_gemm_load<BM, TB>(B, ldb, col0, N, kbeg, rb);
-        lds_gemm_store<BM, !TA>(As[0], ra);
-        lds_gemm_store<BM, TB>(Bs[0], rb);
+        lds_gemm_load<BM, !TA, TAs, NT>(A, lda, row0, M, kbeg, ra);
+        lds_gemm_load<BM, TB, TBs, NT>(B, ldb, col0, N, kbeg, rb);
+        lds_gemm_store<BM, !TA, NT>(As[0], ra);
+        lds_gemm_store<BM, TB, NT>(Bs[0], rb);
     }
     __syncthreads();
     for (int64_t c = 0; c < nch; ++c) {
         const int buf = (int)(c & 1);
         const bool more = c + 1 < nch;
         if (more) {
-            lds_gemm_load<BM, !TA>(A, lda, row0, M, kbeg + (c + 1) * 16, ra);
-            lds_gemm_load<BM, TB>(B, ldb, col0, N, kbeg + (c + 1) * 16, rb);
+            lds_gemm_load<BM, !TA, TAs, NT>(A, lda, row0, M, kbeg + (c + 1) * 16, ra);
+            lds_gemm_load<BM, TB, TBs, NT>(B, ldb, col0, N, kbeg + (c + 1) * 16, rb);
         }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int kk = 4 * g + s;
-            double a[NBK], b[NBK];
-#pragma unroll
-            for (int i = 0; i < NBK; ++i) a[i] = As[buf][kk][wr * WT + i * 16 + li];
-#pragma unroll
-            for (int j = 0; j < NBK; ++j) b[j] = Bs[buf][kk][wc * WT + j * 16 + li];
-#pragma unroll
-            for (int i = 0; i < NBK; ++i)
-#pragma unroll
-                for (int j = 0; j < NBK; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+        chunk(buf);
         if (more) {
-            lds_gemm_store<BM, !TA>(As[buf ^ 1], ra);
-            lds_gemm_store<BM, TB>(Bs[buf ^ 1], rb);
+            lds_gemm_store<BM, !TA, NT>(As[buf ^ 1], ra);
+            lds_gemm_store<BM, TB, NT>(Bs[buf ^ 1], rb);
         }
         __syncthreads();
     }
     // v_mfma_f64_16x16x4 D layout: lane holds rows g, g+4, g+8, g+12 of column li
 #pragma unroll
-    for (int i = 0; i < NBK; ++i)
+    for (int i = 0; i < NBR; ++i)
 #pragma unroll
-        for (int j = 0; j < NBK; ++j) {
-            const int64_t rb0 = row0 + wr * WT + i * 16, cb0 = col0 + wc * WT + j * 16;
+        for (int j = 0; j < NBC; ++j) {
+            const int64_t rb0 = row0 + wr * WTR + i * 16, cb0 = col0 + wc * WTC + j * 16;
             if (rb0 >= M || cb0 >= N) continue;
             TCs* p = C + (rb0 + g) + (cb0 + li) * ldc;
 #pragma unroll
@@ -186,17 +192,17 @@ __global__ __launch_bounds__(256) void k_gemm_lds(int64_t M, int64_t N, int64_t 
         }
 }
 
-template <int BM, typename TAs, typename TBs, typename TCs>
+template <int BM, typename TAs, typename TBs, typename TCs, int NW = 4>
 static void gemm_lds_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                             const TAs* A, int64_t lda, const TBs* B, int64_t ldb, double beta, TCs* C,
                             int64_t ldc, int tiles, int batch = 1, int64_t sA = 0, int64_t sB = 0,
                             int64_t sC = 0) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
     const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
-    dim3 grid((unsigned)nblk, (unsigned)batch), block(256);
-#define GPL_GL(TAv, TBv)                                                                                        \
-    hipLaunchKernelGGL((k_gemm_lds<TAv, TBv, BM, TAs, TBs, TCs>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, \
-                       B, ldb, beta, C, ldc, tiles, sA, sB, sC)
+    dim3 grid((unsigned)nblk, (unsigned)batch), block(64 * NW);
+#define GPL_GL(TAv, TBv)                                                                                      \
+    hipLaunchKernelGGL((k_gemm_lds<TAv, TBv, BM, TAs, TBs, TCs, NW>), grid, block, 0, s, M, N, K, tm, alpha, A, \
+                       lda, B, ldb, beta, C, ldc, tiles, sA, sB, sC)
     if (!ta && !tb) GPL_GL(0, 0);
     else if (!ta && tb) GPL_GL(0, 1);
     else if (ta && !tb) GPL_GL(1, 0);
@@ -204,20 +210,24 @@ static void gemm_lds_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N,
 #undef GPL_GL
 }
 
-// 64x64 tiles measured faster than 128x128 for every shape on the path (the
-// 128 variant holds one wave per SIMD); the 128 variant stays selectable.
-static int pick_bm(int64_t M, int64_t N, int tiles) {
-    (void)M;
-    (void)N;
-    (void)tiles;
-    return 64;
+// Tile choice for FP64 storage.  Short inner dimensions (the panel and trailing
+// updates, k of 64 to 256) are bound by the chunk loop's load latency and want
+// the most workgroups: 64x64 tiles on four waves.  Long inner dimensions (B^-1 =
+// X^T X, prediction's L^-1 K_t^T, k = n) run faster on 128x128 tiles with eight
+// waves (half the operand reads per flop at the same occupancy): 57 against 46
+// TFLOP/s for X^T X at n = 8192.  bm = 64 or 128 forces the 64/4 or 128/8 form.
+static int pick_bm(int64_t M, int64_t N, int64_t K, int tiles) {
+    const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
+    const int64_t n128 = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
+    return (K >= 1024 && n128 >= 256) ? 128 : 64;
 }
 
 void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles, int bm) {
     if (M <= 0 || N <= 0) return;
-    if (bm == 0) bm = pick_bm(M, N, tiles);
-    if (bm == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    if (bm == 0) bm = pick_bm(M, N, K, tiles);
+    if (bm == 128)
+        gemm_lds_launch<128, double, double, double, 8>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
     else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
@@ -247,8 +257,7 @@ void sum_parts_f64(hipStream_t s, const double* W, int64_t stride, int parts, in
 void gemm_mixed_f32f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
                        int64_t lda, const double* B, int64_t ldb, double beta, float* C, int64_t ldc, int tiles) {
     if (M <= 0 || N <= 0) return;
-    if (pick_bm(M, N, tiles) == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
-    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
 // FP64 B on the left: C(float) = alpha * A(double) op B(float)
@@ -256,8 +265,7 @@ void gemm_mixed_f64f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int6
                        const double* A, int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc,
                        int tiles) {
     if (M <= 0 || N <= 0) return;
-    if (pick_bm(M, N, tiles) == 128) gemm_lds_launch<128>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
-    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
 void gemm_mixed_f32f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
@@ -278,7 +286,9 @@ void gemm_mixed_f32f64f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
                           const float* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
                           int64_t ldc, int tiles) {
     if (M <= 0 || N <= 0) return;
-    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    if (pick_bm(M, N, K, tiles) == 128)
+        gemm_lds_launch<128, float, double, double, 8>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
+    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles);
 }
 
 // ---------------------------------------------------------------------------
