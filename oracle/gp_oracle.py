@@ -35,7 +35,7 @@ __all__ = [
     "JITTER", "cf_sexp_c", "cf_matern32_c", "cf_matern52_c", "cf_nn_c",
     "learn_scales", "prepare_inputmat", "eval_cf", "eval_cf_grads",
     "get_param", "set_param", "param_names", "lpdf_prior", "lpdf_prior_grad",
-    "loglik", "loglik_d", "loglik_d2", "response", "pseudodata_la",
+    "loglik", "loglik_d", "loglik_d2", "loglik_d2_exact", "loglik_d3", "response", "pseudodata_la",
     "hermite_polynomials", "gauss_hermite_points", "logsumexp_w",
     "tilted_moments", "pseudodata_ep",
     "laplace_iter", "laplace", "ep_iter", "ep", "gp_fit", "gp_energy",
@@ -586,12 +586,71 @@ def loglik_d2(lik, f, y, eps=1e-6, **kw):
     return g, (ge - g) / eps
 
 
-def pseudodata_la(lik, f, y, min_abs_curvature=1e-12, **kw):
-    """Laplace pseudo-data: R/lik.R:188-201."""
+def _binom_parts(lik, f, y, trials, offset):
+    f = np.asarray(f, float)
+    if offset is not None:
+        f = f + offset
+    y = np.asarray(y, float)
+    t = np.ones_like(y) if lik["kind"] == "bernoulli" or trials is None else np.asarray(trials, float)
+    return f, y, t
+
+
+def loglik_d2_exact(lik, f, y, trials=None, offset=None):
+    """Analytic second derivative of log p(y|f) in f (not in the reference, whose
+    curvature is the forward difference of R/lik.R:449-454).  logit:
+    -t mu(1-mu); probit with r1 = phi/Phi, r0 = phi/(1-Phi):
+    -y r1 (f + r1) - (t-y) r0 (r0 - f); Poisson (log link): -exp(f)."""
+    kind = lik["kind"]
+    if kind == "gaussian":
+        return -np.ones(len(y)) / lik["sigma"] ** 2
+    if kind in ("bernoulli", "binomial"):
+        f, y, t = _binom_parts(lik, f, y, trials, offset)
+        if lik.get("link", "logit") == "probit":
+            phi = np.exp(_dnorm_log(f))
+            r1 = phi / ndtr(f)
+            r0 = phi / ndtr(-f)
+            return -y * r1 * (f + r1) - (t - y) * r0 * (r0 - f)
+        mu = response(lik, f)
+        return -t * mu * (1 - mu)
+    if kind == "poisson":
+        f = np.asarray(f, float) + (0.0 if offset is None else offset)
+        return -np.exp(f)
+    raise NotImplementedError(f"analytic curvature for {kind}")
+
+
+def loglik_d3(lik, f, y, trials=None, offset=None):
+    """Third derivative of log p(y|f) in f (the implicit term of the Laplace
+    gradient, Rasmussen & Williams 2006 sec. 5.5.1).  logit: -t mu(1-mu)(1-2mu);
+    probit: -y r1 [1 - (f + r1)(f + 2 r1)] - (t-y) r0 [(r0 - f)(2 r0 - f) - 1];
+    Poisson: -exp(f); Gaussian: 0."""
+    kind = lik["kind"]
+    if kind == "gaussian":
+        return np.zeros(len(y))
+    if kind in ("bernoulli", "binomial"):
+        f, y, t = _binom_parts(lik, f, y, trials, offset)
+        if lik.get("link", "logit") == "probit":
+            phi = np.exp(_dnorm_log(f))
+            r1 = phi / ndtr(f)
+            r0 = phi / ndtr(-f)
+            return -y * r1 * (1 - (f + r1) * (f + 2 * r1)) - (t - y) * r0 * ((r0 - f) * (2 * r0 - f) - 1)
+        mu = response(lik, f)
+        return -t * mu * (1 - mu) * (1 - 2 * mu)
+    if kind == "poisson":
+        f = np.asarray(f, float) + (0.0 if offset is None else offset)
+        return -np.exp(f)
+    raise NotImplementedError(f"third derivative for {kind}")
+
+
+def pseudodata_la(lik, f, y, min_abs_curvature=1e-12, analytic_w=False, **kw):
+    """Laplace pseudo-data: R/lik.R:188-201.  analytic_w replaces the reference's
+    forward-difference curvature by loglik_d2_exact."""
     if lik["kind"] == "gaussian":
         n = len(y)
         return np.asarray(y, float).copy(), lik["sigma"] ** 2 * np.ones(n)
-    g, g2 = loglik_d2(lik, f, y, **kw)
+    if analytic_w:
+        g, g2 = loglik_d(lik, f, y, **kw), loglik_d2_exact(lik, f, y, **kw)
+    else:
+        g, g2 = loglik_d2(lik, f, y, **kw)
     g2 = np.sign(g2) * np.maximum(np.abs(g2), min_abs_curvature)
     return f - g / g2, -1.0 / g2
 
@@ -699,12 +758,12 @@ def _solve_via(Lk, A_chol, v):
     return sla.solve_triangular(Lk, u, lower=True, trans="T")
 
 
-def laplace_iter(lik, K, y, fhat_old, pobs=None, **kw):
+def laplace_iter(lik, K, y, fhat_old, pobs=None, analytic_w=False, **kw):
     """One Newton step: R/laplace.R:13-117,119 (log-concave branch, and the split
     branch for sites with pseudo-variance <= 0, with the final solve by LU as
     Matrix::lu does)."""
     if pobs is None:
-        z, V = pseudodata_la(lik, fhat_old, y, **kw)
+        z, V = pseudodata_la(lik, fhat_old, y, analytic_w=analytic_w, **kw)
     else:
         z, V = pobs
     if not np.all(V > 0):
@@ -740,7 +799,7 @@ def laplace_iter(lik, K, y, fhat_old, pobs=None, **kw):
     return dict(fmean=fnew, z=z, V=V, C_chol=C, alpha=alpha, log_evidence=logZ)
 
 
-def laplace(lik, K, y, maxiter=30, tol=1e-4, **kw):
+def laplace(lik, K, y, maxiter=30, tol=1e-4, analytic_w=False, **kw):
     """Newton loop: R/laplace.R:274-297 (Gaussian forced to one step)."""
     n = len(y)
     fhat = np.zeros(n)
@@ -750,7 +809,7 @@ def laplace(lik, K, y, maxiter=30, tol=1e-4, **kw):
     diff = np.inf
     fit = None
     for it in range(1, maxiter + 1):
-        fit = laplace_iter(lik, K, y, fhat, **kw)
+        fit = laplace_iter(lik, K, y, fhat, analytic_w=analytic_w, **kw)
         diff = float(np.max(np.abs(fhat - fit["fmean"])))
         fhat = fit["fmean"]
         if diff < tol:
@@ -852,8 +911,10 @@ def ep(lik, K, y, maxiter=100, tol=1e-4, damping=0.9, damping_min=0.1, quad_orde
 # ---------------------------------------------------------------------------
 
 def gp_fit(cfs, lik, x, y, approx="laplace", maxiter=None, tol=1e-4, jitter=None,
-           damping=0.9, quad_order=11, normalize_learn=True, **kw):
-    """fit_laplace/fit_ep.method_full: R/gp_fit.R:54-60,91-99,150-161."""
+           damping=0.9, quad_order=11, normalize_learn=True, analytic_w=False, ep_tol=1e-4, **kw):
+    """fit_laplace/fit_ep.method_full: R/gp_fit.R:54-60,91-99,150-161.  analytic_w:
+    Laplace with the analytic curvature; ep_tol: EP's stopping tolerance (the
+    reference fixes 1e-4, R/ep.R:286)."""
     x = np.atleast_2d(np.asarray(x, float))
     if x.shape[0] == 1 and len(y) != 1:
         x = x.T
@@ -863,16 +924,17 @@ def gp_fit(cfs, lik, x, y, approx="laplace", maxiter=None, tol=1e-4, jitter=None
     n = len(y)
     K = eval_cf(cfs, x, x) + jitter * np.eye(n)
     if approx == "laplace":
-        fit = laplace(lik, K, y, maxiter=30 if maxiter is None else maxiter, tol=tol, **kw)
+        fit = laplace(lik, K, y, maxiter=30 if maxiter is None else maxiter, tol=tol, analytic_w=analytic_w, **kw)
     elif approx == "ep":
         if lik["kind"] == "gaussian":
             raise ValueError("approx_ep disabled for lik_gaussian")  # R/gp.R:54-56
-        fit = ep(lik, K, y, maxiter=100 if maxiter is None else maxiter, tol=1e-4,
+        fit = ep(lik, K, y, maxiter=100 if maxiter is None else maxiter, tol=ep_tol,
                  damping=damping, damping_min=0.1, quad_order=quad_order, **kw)
     else:
         raise ValueError(approx)
     fit["cfs"] = cfs; fit["lik"] = lik; fit["x"] = x; fit["K"] = K; fit["jitter"] = jitter
-    fit["approx"] = approx
+    fit["approx"] = approx; fit["analytic_w"] = bool(analytic_w); fit["y"] = np.asarray(y, float)
+    fit["trials"] = kw.get("trials"); fit["offset"] = kw.get("offset")
     return fit
 
 
@@ -885,12 +947,23 @@ def gp_energy(fit, include_prior=True):
 
 
 def gp_energy_grad(fit, include_prior=True):
-    """d energy / d log(theta) for the Gaussian likelihood (new work, the
-    reference has none): -1/2 tr((a a^T - Ky^-1) dKy/dtheta), Rasmussen &
-    Williams (2006) eq. 5.9, with Ky = K_cf + (jitter + sigma^2) I."""
+    """d energy / d log(theta) (new work, the reference has none).
+
+    Gaussian: -1/2 tr((a a^T - Ky^-1) dKy/dtheta), Rasmussen & Williams (2006)
+    eq. 5.9, with Ky = K_cf + (jitter + sigma^2) I.
+
+    EP: the same form with the site variances in place of the noise,
+    -1/2 [a^T C_j a - tr((K + V)^-1 C_j)], a = (K + V)^-1 z (R&W eq. 5.27: at
+    the fixed point the site parameters do not move the evidence to first order).
+
+    Laplace with the analytic curvature (R&W Algorithm 5.1): the explicit term
+    with R = (K + W^-1)^-1, plus s2^T (I - K R) C_j grad log p, where
+    s2 = d logZ / d fhat = 1/2 diag((K^-1 + W)^-1) d3 log p (the -1/2 log|B|
+    term through W = -d2 log p, whose derivative in f is -d3 log p).  Only the covariance parameters enter; the
+    likelihood's parameters are not differentiated for these two."""
     lik = fit["lik"]
     if lik["kind"] != "gaussian":
-        raise NotImplementedError("gradient is defined for the Gaussian likelihood")
+        return _grad_latent(fit, include_prior)
     x = fit["x"]; n = x.shape[0]
     _, dK = eval_cf_grads(fit["cfs"], x, x)
     Ky = fit["K"] + lik["sigma"] ** 2 * np.eye(n)
@@ -904,6 +977,38 @@ def gp_energy_grad(fit, include_prior=True):
     g = np.array(g)
     if include_prior:
         g = g - lpdf_prior_grad(fit["cfs"], lik)
+    return g
+
+
+def _grad_latent(fit, include_prior):
+    if "C_chol" not in fit:
+        raise NotImplementedError("the gradient needs a fit without split sites")
+    if fit["approx"] == "laplace" and not fit.get("analytic_w"):
+        raise NotImplementedError("the Laplace gradient is defined for the analytic curvature")
+    x = fit["x"]
+    _, dK = eval_cf_grads(fit["cfs"], x, x)
+    K = fit["K"]
+    V = fit["V"] if "V" in fit else 1.0 / fit["P"]
+    Kv = K + np.diag(V)
+    Lc = sla.cholesky(Kv, lower=True)
+    R = sla.cho_solve((Lc, True), np.eye(len(V)))  # (K + V)^-1 = (K + W^-1)^-1
+    a = fit["alpha"]
+    g = np.array([0.5 * (float(a @ (D @ a)) - float(np.sum(R * D))) for D in dK])
+    if fit["approx"] == "laplace":
+        # implicit term: the mode moves with theta, d fhat = (I + K W)^-1 C_j grad log p
+        kw = dict(trials=fit.get("trials"), offset=fit.get("offset"))
+        f = fit["fmean"]
+        u = loglik_d(fit["lik"], f, fit["y"], **kw)
+        d3 = loglik_d3(fit["lik"], f, fit["y"], **kw)
+        dg = np.diag(K) - np.einsum("ij,ji->i", K @ R, K)  # diag((K^-1 + W)^-1) = diag(K - K R K)
+        # d logZ / d fhat_i = -1/2 [(K^-1 + W)^-1]_ii dW_ii/dfhat_i, with dW/df = -d3
+        s2 = 0.5 * dg * d3
+        for j, D in enumerate(dK):
+            b = D @ u
+            g[j] += float(s2 @ (b - K @ (R @ b)))
+    g = -g
+    if include_prior:
+        g = g - lpdf_prior_grad(fit["cfs"], fit["lik"])
     return g
 
 

@@ -324,6 +324,96 @@ def test_laplace_evidence_uses_the_forward_difference_curvature(lik):
     assert observed == pytest.approx(predicted, rel=2e-2, abs=1e-11)
 
 
+_LATENT_LIKS = [({"kind": "bernoulli", "link": "logit"}, "bernoulli"), ({"kind": "bernoulli", "link": "probit"}, "bernoulli"),
+                ({"kind": "poisson", "link": "log"}, "poisson"), ({"kind": "binomial", "link": "logit"}, "binomial"),
+                ({"kind": "binomial", "link": "probit"}, "binomial")]
+
+
+def _latent_data(kind, n=40, seed=0):
+    r = _rng(seed)
+    x = r.uniform(-2, 2, (n, 2))
+    f = np.sin(x[:, 0]) + 0.5 * x[:, 1]
+    tr = None
+    if kind == "bernoulli":
+        y = (r.uniform(size=n) < 1 / (1 + np.exp(-2 * f))).astype(float)
+    elif kind == "poisson":
+        y = r.poisson(np.exp(f)).astype(float)
+    else:
+        tr = r.integers(1, 8, n).astype(float)
+        y = r.binomial(tr.astype(int), 1 / (1 + np.exp(-f))).astype(float)
+    return x, y, tr
+
+
+@pytest.mark.parametrize("lik,kind", _LATENT_LIKS)
+def test_exact_second_and_third_derivatives_match_central_differences(lik, kind):
+    """loglik_d2_exact and loglik_d3 against central differences of loglik_d and
+    loglik_d2_exact (step 1e-5, error ~1e-10), over f in [-4, 4] and the data's y
+    (beyond, the reference's probit gradient forms 1 - Phi(f) by subtraction and its
+    differences lose the digits the exact forms keep)."""
+    x, y, tr = _latent_data(kind, n=200, seed=3)
+    f = np.linspace(-4, 4, len(y))
+    h = 1e-5
+    d2 = O.loglik_d2_exact(lik, f, y, trials=tr)
+    d2c = (O.loglik_d(lik, f + h, y, trials=tr) - O.loglik_d(lik, f - h, y, trials=tr)) / (2 * h)
+    np.testing.assert_allclose(d2, d2c, rtol=1e-6, atol=1e-9)
+    d3 = O.loglik_d3(lik, f, y, trials=tr)
+    d3c = (O.loglik_d2_exact(lik, f + h, y, trials=tr) - O.loglik_d2_exact(lik, f - h, y, trials=tr)) / (2 * h)
+    np.testing.assert_allclose(d3, d3c, rtol=1e-5, atol=1e-8)
+
+
+def _grad_fd(cfs_of, th, fit_of, h=1e-5):
+    fd = []
+    for j in range(len(th)):
+        tp = th.copy(); tp[j] *= np.exp(h)
+        tm = th.copy(); tm[j] *= np.exp(-h)
+        fd.append(-(fit_of(tp)["log_evidence"] - fit_of(tm)["log_evidence"]) / (2 * h))
+    return np.array(fd)
+
+
+@pytest.mark.parametrize("lik,kind", _LATENT_LIKS)
+def test_laplace_gradient_with_exact_curvature_matches_finite_differences(lik, kind):
+    """R&W Algorithm 5.1 (explicit term plus the mode's implicit dependence through
+    d3 log p) against central differences of the Laplace evidence with the exact
+    curvature, Newton run to 1e-13: a dropped implicit term or a wrong sign of it
+    moves the result by ~10%."""
+    x, y, tr = _latent_data(kind)
+    cfs_of = lambda t: [{"kind": "sexp", "lscale": np.array(t[:2]), "magn": t[2]}]
+    fit_of = lambda t: O.gp_fit(cfs_of(t), lik, x, y, maxiter=200, tol=1e-13, analytic_w=True, trials=tr)
+    th = np.array([0.8, 1.3, 1.1])
+    g = O.gp_energy_grad(fit_of(th), include_prior=False)
+    fd = _grad_fd(cfs_of, th, fit_of)
+    np.testing.assert_allclose(g, fd, rtol=1e-7, atol=1e-8 * np.max(np.abs(fd)))
+    # the same evidence by R&W Algorithm 3.2 at the mode
+    fit = fit_of(th)
+    f, K = fit["fmean"], fit["K"]
+    sW = np.sqrt(-O.loglik_d2_exact(lik, f, y, trials=tr))
+    L = sla.cholesky(np.eye(len(y)) + sW[:, None] * K * sW[None, :], lower=True)
+    a = O.loglik_d(lik, f, y, trials=tr)
+    lz = -0.5 * a @ f + O.loglik(lik, f, y, trials=tr) - np.sum(np.log(np.diag(L)))
+    assert fit["log_evidence"] == pytest.approx(lz, rel=1e-12)
+
+
+@pytest.mark.parametrize("lik,kind,fine,coarse_tol", [(_LATENT_LIKS[0] + (31, 1e-4)), (_LATENT_LIKS[1] + (31, 1e-4)),
+                                                       (_LATENT_LIKS[3] + (31, 1e-3)), (_LATENT_LIKS[2] + (61, 5e-2))])
+def test_ep_gradient_matches_finite_differences(lik, kind, fine, coarse_tol):
+    """The EP gradient at the fixed point (site parameters held: R&W eq. 5.27)
+    against central differences of log Z_EP with EP run to 1e-12.  With a fine
+    Gauss-Hermite rule (31 nodes; 61 for Poisson counts up to 8, whose tilted
+    densities are narrow against the cavity) the tilted moments are exact and the
+    two agree to 1e-6.  With the reference's 11 nodes the moments and log Z_site
+    disagree at the quadrature error: 1e-5 to 1e-4 for the binary and binomial data, 2.5%
+    for the counts."""
+    x, y, tr = _latent_data(kind)
+    cfs_of = lambda t: [{"kind": "sexp", "lscale": np.array(t[:2]), "magn": t[2]}]
+    th = np.array([0.8, 1.3, 1.1])
+    for qo, tol in ((fine, 1e-6), (11, coarse_tol)):
+        fit_of = lambda t: O.gp_fit(cfs_of(t), lik, x, y, approx="ep", maxiter=1000, ep_tol=1e-12, quad_order=qo,
+                                    trials=tr)
+        g = O.gp_energy_grad(fit_of(th), include_prior=False)
+        fd = _grad_fd(cfs_of, th, fit_of)
+        np.testing.assert_allclose(g, fd, rtol=tol, atol=tol * np.max(np.abs(fd)))
+
+
 def test_ep_single_site_is_exact():
     """n=1: the EP evidence equals log int N(f|0,k) p(y|f) df."""
     for lik, y in (({"kind": "bernoulli", "link": "probit"}, 1.0), ({"kind": "bernoulli", "link": "logit"}, 0.0),
