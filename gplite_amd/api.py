@@ -381,10 +381,13 @@ class lik_poisson(Lik):
 
 
 class approx_laplace:
+    """approx_laplace(maxiter, tol) of R/approx.R.  analytic_w (new, default off):
+    the Newton curvature in closed form instead of the reference's forward
+    difference (R/lik.R:449-454); gp_energy_grad of a Laplace fit needs it."""
     name = "laplace"
 
-    def __init__(self, maxiter=30, tol=1e-4):
-        self.maxiter, self.tol = maxiter, tol
+    def __init__(self, maxiter=30, tol=1e-4, analytic_w=False):
+        self.maxiter, self.tol, self.analytic_w = maxiter, tol, bool(analytic_w)
 
 
 class approx_ep:
@@ -903,7 +906,7 @@ def gp_fit(gp, x, y, trials=None, offset=None, jitter=None, precision=None, want
         o.maxiter = g.approx.maxiter
         o.tol = g.approx.tol
     o.jitter = 1e-6 if jitter is None else float(jitter)
-    o.flags = 1 if want_grad else 0
+    o.flags = (1 if want_grad else 0) | (2 if getattr(g.approx, "analytic_w", False) else 0)
     if refine_iters is not None:
         o.refine_iters = int(refine_iters)
     prec = precision or g.precision
@@ -946,16 +949,23 @@ def gp_energy(gp, include_prior=True):
 
 
 def gp_energy_grad(gp, include_prior=True):
-    """d gp_energy / d get_param(gp) (Gaussian likelihood), computed on the device."""
+    """d gp_energy / d get_param(gp), computed on the device: Gaussian likelihood
+    (any precision); EP and Laplace with approx_laplace(analytic_w=True) in FP64,
+    whose likelihood parameters (the beta-binomial phi) must be fixed."""
     if not is_fitted(gp):
         raise RuntimeError("The GP must be fitted. Call gp_fit first.")
     if isinstance(gp.method, method_fitc):
         raise NotImplementedError("the energy gradient is provided for method_full")
     terms, nt, _, mask = _program(gp.cfs, gp.x)
     ng = L.lib().gpl_grad_len(terms, nt, _LIKMAP[gp.lik.kind])
+    latent = gp.lik.kind != "gaussian"
+    if latent and any(not _fixed(gp.lik.priors.get(nm)) for nm in gp.lik.param_names):
+        raise NotImplementedError("the gradient in the likelihood's parameters is provided for lik_gaussian only; "
+                                  "fix them with prior_fixed()")
     full = np.empty(ng)
     L.check(L.lib().gpl_energy_grad(gp.fit.handle, L.dptr(full)))
-    mask = mask + [0 if _fixed(gp.lik.priors.get(nm)) else 1 for nm in gp.lik.param_names]
+    if not latent:
+        mask = mask + [0 if _fixed(gp.lik.priors.get(nm)) else 1 for nm in gp.lik.param_names]
     g = full[np.asarray(mask, dtype=bool)]
     if include_prior:
         g = g - _lpdf_prior_grad(gp)
@@ -1187,6 +1197,7 @@ def gp_load(filename):
     o = L.Opts()
     L.lib().gpl_opts_default(C_byref(o))
     o.approx = L.APPROX_EP if isinstance(g.approx, approx_ep) else L.APPROX_LAPLACE
+    o.flags = 2 if getattr(g.approx, "analytic_w", False) else 0
     lk = _lik_struct(g.lik)
     vecs = [np.ascontiguousarray(arrays[nm], dtype=np.float64) for nm in
             ("fmean", "alpha", "z", "V", "fvar", "cavity_mean", "cavity_var")]

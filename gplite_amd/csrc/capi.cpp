@@ -719,7 +719,7 @@ static int fit_laplace(gpl_model* m) {
         int* flags = m->b[B_INFO].as<int>();
         hipMemsetAsync(flags, 0, 8, s);
         lik_pseudodata_la(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
-                          m->d(B_V), m->d(B_SV), flags + 1);
+                          m->d(B_V), m->d(B_SV), flags + 1, (m->opts.flags & GPL_FLAG_ANALYTIC_W) ? 1 : 0);
         {
             Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
             form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
@@ -868,7 +868,7 @@ static int fit_laplace_mixed(gpl_model* m) {
         int* flags = m->b[B_INFO].as<int>();
         hipMemsetAsync(flags, 0, 8, s);
         lik_pseudodata_la(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FOLD), m->d(B_Y), tr, off, m->d(B_Z),
-                          m->d(B_V), m->d(B_SV), flags + 1);
+                          m->d(B_V), m->d(B_SV), flags + 1, (m->opts.flags & GPL_FLAG_ANALYTIC_W) ? 1 : 0);
         {
             Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
             form_B_mixed(s, m->d(B_K), B64, L, m->d(B_DACC), N, n, N, m->d(B_SV));
@@ -1236,6 +1236,9 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     if ((lik->kind == GPL_LIK_BINOMIAL || lik->kind == GPL_LIK_BETABINOM) && !trials)
         return fail(GPL_ERR_ARG, "trials must be provided for the binomial and beta binomial likelihoods");
     if (o.precision != GPL_PREC_FP64 && o.precision != GPL_PREC_MIXED) return fail(GPL_ERR_ARG, "bad precision");
+    if ((o.flags & GPL_FLAG_ANALYTIC_W) && lik->kind == GPL_LIK_BETABINOM)
+        return fail(GPL_ERR_UNSUPPORTED, "the exact curvature is provided for the logit, probit and log links "
+                                         "(not the beta-binomial likelihood)");
     const bool mixed = o.precision == GPL_PREC_MIXED;
     DevProg P;
     int r = prepare_prog(terms, nterms, ncols, P);
@@ -1303,10 +1306,79 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     return r;
 }
 
+// d(-log evidence)/d(log theta) of a Laplace (exact curvature) or EP fit, FP64:
+// -1/2 sum_ik C_ik (a_i a_k - W_ik) with W = (K + V)^-1 - (t u^T + u t^T), where
+// for Laplace (R&W Algorithm 5.1) u = grad log p at the mode and
+// t = (I - R K) s2, s2 = d logZ / d fhat, R = (K + V)^-1; for EP t = u = 0.
+static int energy_grad_latent(gpl_model* m, double* grad) {
+    gpl_ctx* c = m->ctx;
+    const bool ep = m->opts.approx == GPL_APPROX_EP;
+    if (m->opts.precision != GPL_PREC_FP64)
+        return fail(GPL_ERR_UNSUPPORTED, "the gradient of a Laplace or EP fit is provided in FP64");
+    if (m->split)
+        return fail(GPL_ERR_UNSUPPORTED, "the fit split its sites (non-log-concave branch): no gradient");
+    if (!ep && !(m->opts.flags & GPL_FLAG_ANALYTIC_W))
+        return fail(GPL_ERR_UNSUPPORTED, "the Laplace gradient is that of the exact curvature: fit with "
+                                         "GPL_FLAG_ANALYTIC_W");
+    std::lock_guard<std::mutex> lk(c->mu);
+    Range rg("gpl_energy_grad");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = m->n, N = m->N;
+    ENSURE(m, B_XINV, (size_t)N * N * 8);
+    ENSURE(m, B_BINV, (size_t)N * N * 8);
+    const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
+    const int ng = m->prog.ngrad;
+    ENSURE(m, B_PART, (size_t)ntiles * (ng > 0 ? ng : 1) * 8);
+    ENSURE(m, B_KMV, (size_t)N * std::max(kmatvec_chunks(N), symv_slots(N, 1)) * 8);
+    for (int w : {B_W1, B_W2, B_W3, B_W4}) ENSURE(m, w, (size_t)N * 8);
+    if (!m->have_xinv) {
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        m->have_xinv = true;
+    }
+    double* Bi = m->d(B_BINV);
+    {
+        Prof p(c, "lauum", (double)N * N * N / 3, 8.0 * N * N);
+        gemm2_f64(s, 1, 0, N, N, N, 1.0, m->d(B_XINV), N, m->d(B_XINV), N, 0.0, Bi, N, 2, 0);
+    }
+    const double* sv = m->d(B_SV);
+    double *u = nullptr, *t = nullptr;
+    if (!ep) {
+        u = m->d(B_W1);
+        double* s2 = m->d(B_W2);
+        t = m->d(B_W3);
+        double* w = m->d(B_W4);
+        const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
+        const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+        lik_la_grad_vectors(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, N, m->d(B_FMEAN), m->d(B_Y), tr, off,
+                            m->d(B_V), Bi, N, u, s2);
+        // t = s2 - S B^-1 S K s2 (padding of every vector stays zero)
+        symv_lt_f64(s, m->d(B_K), N, s2, w, m->d(B_KMV), 0, 1, 0, 0);
+        fill_f64(s, t, N, 0.0);
+        vec_mul(s, n, sv, w, t);
+        symv_lt_f64(s, Bi, N, t, w, m->d(B_KMV), 0, 1, 0, 0);
+        vec_mul(s, n, sv, w, w);
+        vec_axpby(s, n, 1.0, s2, -1.0, w, t);
+    }
+    form_wgrad_f64(s, Bi, N, n, sv, t, u);
+    m->have_binv = false;  // B^-1 now holds the weights
+    {
+        Prof p(c, "grad", 0, 8.0 * N * (N + 64) / 2);
+        launch_grad_tiles(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, Bi, N, 1.0, m->d(B_ALPHA),
+                          m->d(B_PART));
+    }
+    double* sc = m->d(B_SCAL);
+    reduce_cols(s, m->d(B_PART), ntiles, ng, ng, sc);
+    std::vector<double> h(ng > 0 ? ng : 1);
+    HIP_TRY(hipMemcpyAsync(h.data(), sc, ng * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int p = 0; p < ng; ++p) grad[p] = -0.5 * h[p];
+    return GPL_OK;
+}
+
 extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
     if (!m || !grad) return fail(GPL_ERR_ARG, "NULL argument");
-    if (m->lik.kind != GPL_LIK_GAUSSIAN)
-        return fail(GPL_ERR_UNSUPPORTED, "the energy gradient is provided for the Gaussian likelihood");
+    if (m->lik.kind != GPL_LIK_GAUSSIAN) return energy_grad_latent(m, grad);
     gpl_ctx* c = m->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     Range rg("gpl_energy_grad");

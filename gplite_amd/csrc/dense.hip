@@ -867,6 +867,40 @@ void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, dou
                        nchunk, y, sV);
 }
 
+// The weight matrix of the latent-likelihood gradient, in place over the lower
+// tiles of B^-1 (every element of a lower tile, diagonal tiles complete):
+// W_ik = s_i B^-1_ik s_k - (t_i u_k + u_i t_k) for i, k < n (t, u may be null:
+// no rank-two term), zero where i or k is padding.  With s = V^-1/2,
+// s B^-1 s = (K + V)^-1.
+__global__ __launch_bounds__(256) void k_form_wgrad(double* __restrict__ B, int64_t ld, int64_t n,
+                                                    const double* __restrict__ sv, const double* __restrict__ t,
+                                                    const double* __restrict__ u) {
+    int64_t ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const int e = threadIdx.x & 63, p = threadIdx.x >> 6;
+    const int64_t i = ti * 64 + e;
+    const double si = (i < n) ? sv[i] : 0.0;
+    const double tii = (t && i < n) ? t[i] : 0.0, ui = (u && i < n) ? u[i] : 0.0;
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+        const int64_t k = tj * 64 + 16 * p + q;
+        double* b = B + i + k * ld;
+        if (i < n && k < n) {
+            double w = si * (*b) * sv[k];
+            if (t) w -= tii * u[k] + ui * t[k];
+            *b = w;
+        } else {
+            *b = 0.0;
+        }
+    }
+}
+
+void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double* sv, const double* t,
+                    const double* u) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_form_wgrad, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, B, N, n, sv, t, u);
+}
+
 // one wave per column: out[j] = sum_i A[i + j*ld]^2; lower != 0 starts at the
 // column's diagonal tile (the upper tiles of a lower-triangular buffer are never written)
 __global__ __launch_bounds__(256) void k_colsumsq(const double* __restrict__ A, int64_t ld, int64_t nrow,

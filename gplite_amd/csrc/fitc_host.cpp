@@ -330,7 +330,7 @@ static int fitc_run(gpl_fitc* f) {
                               f->d(F_Z), f->d(F_V), f->d(F_SV), flags + 1);
         else
             lik_pseudodata_la(s, f->lik.kind, f->lik.link, lik_par(f->lik), n, f->d(F_FOLD), f->d(F_Y), tr, off, f->d(F_Z),
-                              f->d(F_V), f->d(F_SV), flags + 1);
+                              f->d(F_V), f->d(F_SV), flags + 1, (f->opts.flags & GPL_FLAG_ANALYTIC_W) ? 1 : 0);
         // inversion lemma for C = K_xz Kz^-1 K_zx + S, S = D + V (R/linear_algebra.R:3-47)
         fitc_lemma_step(f, flags);
         // log evidence (R/laplace.R:143-151): logdet C - log|V| from the lemma
@@ -398,6 +398,9 @@ extern "C" int gpl_fitc_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms
         return fail(GPL_ERR_ARG, "approx_ep disabled for lik_gaussian");  // R/gp.R:54-56
     if (o.precision != GPL_PREC_FP64) return fail(GPL_ERR_UNSUPPORTED, "FITC runs in FP64");
     if (lik->kind == GPL_LIK_GAUSSIAN && !(lik->sigma > 0)) return fail(GPL_ERR_ARG, "sigma must be positive");
+    if ((o.flags & GPL_FLAG_ANALYTIC_W) && lik->kind == GPL_LIK_BETABINOM)
+        return fail(GPL_ERR_UNSUPPORTED, "the exact curvature is provided for the logit, probit and log links "
+                                         "(not the beta-binomial likelihood)");
     DevProg P;
     int r = prepare_prog(terms, nterms, ncols, P);
     if (r) return r;

@@ -135,6 +135,9 @@ void chunk_sum_f64(hipStream_t s, const double* part, int64_t N, int nchunk, dou
 int symv_slots(int64_t N, int batch);
 void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, double* y, double* part, int nchunk,
                  int batch, int64_t sA, int64_t sV);
+// W = s B^-1 s - (t u^T + u t^T) in place over the lower tiles of B^-1 (t, u may be null)
+void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double* sv, const double* t,
+                    const double* u);
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
@@ -157,9 +160,14 @@ void reduce_cols(hipStream_t s, const double* part, int64_t rows, int64_t ld, in
                  double* out);  // out[c] = sum_r part[r*ld + c]
 
 // Likelihood kernels (lik.hip)
+// exact != 0: the exact curvature in place of the reference's forward difference
 void lik_pseudodata_la(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f,
                        const double* y, const double* trials, const double* offset, double* z,
-                       double* V, double* sv, int* bad);
+                       double* V, double* sv, int* bad, int exact = 0);
+// Laplace gradient vectors at the mode: u = d log p / d f, s2 = 1/2 V_i (1 - B^-1_ii) d3_i
+void lik_la_grad_vectors(hipStream_t s, int lik, int link, double par, int64_t n, int64_t N, const double* f,
+                         const double* y, const double* trials, const double* offset, const double* V,
+                         const double* Binv, int64_t ldb, double* u, double* s2);
 void lik_loglik(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f,
                 const double* y, const double* trials, const double* offset, double* ll);
 // par: the likelihood's parameter (Gaussian sigma, beta-binomial phi)

@@ -66,10 +66,48 @@ __device__ __forceinline__ double d_loglik(int lik, int link, double par, double
     return lc + (y > 0.0 ? y * log(mu) : 0.0) + (tr - y > 0.0 ? (tr - y) * log1p(-mu) : 0.0);
 }
 
+// Exact second and third derivatives of log p(y | f) in f (offset added) for the
+// log-concave likelihoods (not in the reference, whose curvature is the forward
+// difference of R/lik.R:449-454).  logit: d2 = -t mu (1-mu), d3 = d2 (1 - 2 mu);
+// probit, with r1 = phi/Phi and r0 = phi/(1-Phi) (both tails from erfc):
+// d2 = -y r1 (f + r1) - (t-y) r0 (r0 - f),
+// d3 = -y r1 [1 - (f + r1)(f + 2 r1)] - (t-y) r0 [(r0 - f)(2 r0 - f) - 1];
+// Poisson (log link): d2 = d3 = -exp(f); Gaussian: -1/sigma^2 and 0.
+__device__ __forceinline__ void d_loglik_d23(int lik, int link, double par, double f, double y, double tr, double& d2,
+                                             double& d3) {
+    if (lik == GPL_LIK_GAUSSIAN) {
+        d2 = -1.0 / (par * par);
+        d3 = 0.0;
+        return;
+    }
+    if (lik == GPL_LIK_POISSON) {
+        d2 = d3 = -exp(f);
+        return;
+    }
+    if (link == GPL_LINK_PROBIT) {
+        const double phi = kInvSqrt2Pi * exp(-0.5 * f * f);
+        d2 = d3 = 0.0;
+        if (y > 0.0) {
+            const double r1 = phi / (0.5 * erfc(-f * 0.70710678118654752440));
+            d2 -= y * r1 * (f + r1);
+            d3 -= y * r1 * (1.0 - (f + r1) * (f + 2.0 * r1));
+        }
+        if (tr - y > 0.0) {
+            const double r0 = phi / (0.5 * erfc(f * 0.70710678118654752440));
+            d2 -= (tr - y) * r0 * (r0 - f);
+            d3 -= (tr - y) * r0 * ((r0 - f) * (2.0 * r0 - f) - 1.0);
+        }
+        return;
+    }
+    const double mu = 1.0 / (1.0 + exp(-f)), nu = 1.0 / (1.0 + exp(f));  // nu = 1 - mu without cancellation
+    d2 = -tr * mu * nu;
+    d3 = d2 * (nu - mu);
+}
+
 __global__ void k_pseudo_la(int lik, int link, double sigma, int64_t n, const double* __restrict__ f,
                             const double* __restrict__ y, const double* __restrict__ trials,
                             const double* __restrict__ offset, double* __restrict__ z, double* __restrict__ V,
-                            double* __restrict__ sv, int* __restrict__ bad) {
+                            double* __restrict__ sv, int* __restrict__ bad, int exact) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (lik == GPL_LIK_GAUSSIAN) {  // R/lik.R:188-192: z = y (offset ignored), V = sigma^2
@@ -83,8 +121,14 @@ __global__ void k_pseudo_la(int lik, int link, double sigma, int64_t n, const do
     const double eps = 1e-6;
     const double fi = f[i];
     const double g = d_loglik_d(lik, link, sigma, fi + off, y[i], tr);
-    const double ge = d_loglik_d(lik, link, sigma, (fi + eps) + off, y[i], tr);
-    double g2 = (ge - g) / eps;
+    double g2;
+    if (exact) {
+        double d3;
+        d_loglik_d23(lik, link, sigma, fi + off, y[i], tr, g2, d3);
+    } else {
+        const double ge = d_loglik_d(lik, link, sigma, (fi + eps) + off, y[i], tr);
+        g2 = (ge - g) / eps;
+    }
     const double sg = (g2 > 0.0) ? 1.0 : ((g2 < 0.0) ? -1.0 : 0.0);
     g2 = sg * fmax(fabs(g2), 1e-12);
     const double Vi = -1.0 / g2;
@@ -96,9 +140,41 @@ __global__ void k_pseudo_la(int lik, int link, double sigma, int64_t n, const do
 }
 
 void lik_pseudodata_la(hipStream_t s, int lik, int link, double sigma, int64_t n, const double* f, const double* y,
-                       const double* trials, const double* offset, double* z, double* V, double* sv, int* bad) {
+                       const double* trials, const double* offset, double* z, double* V, double* sv, int* bad,
+                       int exact) {
     hipLaunchKernelGGL(k_pseudo_la, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lik, link, sigma, n, f, y,
-                       trials, offset, z, V, sv, bad);
+                       trials, offset, z, V, sv, bad, exact);
+}
+
+// Per-point vectors of the Laplace gradient at the mode f (R&W Algorithm 5.1):
+// u = d log p / d f, and s2 = d logZ / d f = 1/2 [(K^-1 + W)^-1]_ii d3_i with
+// (K^-1 + W)^-1 = W^-1/2 (I - B^-1) W^-1/2, so its diagonal is V_i (1 - B^-1_ii)
+// (B^-1 lower tiles, leading dimension ldb).  Padding (i >= n) gets zeros.
+__global__ void k_la_grad_vec(int lik, int link, double par, int64_t n, int64_t N, const double* __restrict__ f,
+                              const double* __restrict__ y, const double* __restrict__ trials,
+                              const double* __restrict__ offset, const double* __restrict__ V,
+                              const double* __restrict__ Binv, int64_t ldb, double* __restrict__ u,
+                              double* __restrict__ s2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (i >= n) {
+        u[i] = 0.0;
+        s2[i] = 0.0;
+        return;
+    }
+    const double fi = f[i] + (offset ? offset[i] : 0.0);
+    const double tr = trials ? trials[i] : 1.0;
+    double d2, d3;
+    d_loglik_d23(lik, link, par, fi, y[i], tr, d2, d3);
+    u[i] = d_loglik_d(lik, link, par, fi, y[i], tr);
+    s2[i] = 0.5 * V[i] * (1.0 - Binv[i + i * ldb]) * d3;
+}
+
+void lik_la_grad_vectors(hipStream_t s, int lik, int link, double par, int64_t n, int64_t N, const double* f,
+                         const double* y, const double* trials, const double* offset, const double* V,
+                         const double* Binv, int64_t ldb, double* u, double* s2) {
+    hipLaunchKernelGGL(k_la_grad_vec, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, lik, link, par, n, N, f, y,
+                       trials, offset, V, Binv, ldb, u, s2);
 }
 
 __global__ void k_loglik(int lik, int link, double sigma, int64_t n, const double* __restrict__ f,

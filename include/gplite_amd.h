@@ -102,6 +102,13 @@ typedef struct gpl_opts {
  * triangular matrix-vector products, and a following gpl_energy_grad or
  * gpl_pred reuses it).  Set it when the gradient will be requested. */
 #define GPL_FLAG_INVERSE 1
+/* gpl_opts.flags: the Laplace curvature W = -d2 log p(y|f) in closed form
+ * (logit, probit, log links; not the beta-binomial) in place of the reference's
+ * forward difference with step 1e-6 (R/lik.R:449-454).  The evidence then
+ * differs from the reference's by the O(1e-6) curvature error; gpl_energy_grad
+ * needs it for a Laplace fit, because the exact gradient is that of the exact
+ * Laplace approximation.  New (the reference has no such option). */
+#define GPL_FLAG_ANALYTIC_W 2
 
 void gpl_opts_default(gpl_opts* o);
 
@@ -159,9 +166,15 @@ int gpl_fit(gpl_ctx* ctx, const gpl_cf_term* terms, int32_t nterms,
             const double* trials, const double* offset, const gpl_lik* lik,
             const gpl_opts* opts, gpl_model** inout, double* log_evidence);
 
-/* d(-log evidence)/d(log theta) for a Gaussian-likelihood fit (new: the
- * reference optimises without gradients, R/gp_optim.R:94).  grad has
- * gpl_grad_len() entries (full layout, fixed parameters included). */
+/* d(-log evidence)/d(log theta) (new: the reference optimises without
+ * gradients, R/gp_optim.R:94).  grad has gpl_grad_len() entries (full layout,
+ * fixed parameters included).  Gaussian likelihood: any precision.  EP (FP64):
+ * the fixed-point gradient -1/2 [a^T C_j a - tr((K + V)^-1 C_j)], a = (K + V)^-1 z
+ * (Rasmussen & Williams 2006, eq. 5.27).  Laplace (FP64, GPL_FLAG_ANALYTIC_W):
+ * R&W Algorithm 5.1, the explicit term plus the mode's implicit dependence
+ * through d3 log p.  Latent likelihoods contribute no parameter of their own.
+ * GPL_ERR_UNSUPPORTED for mixed-precision latent fits, split (non-log-concave)
+ * fits and a Laplace fit without GPL_FLAG_ANALYTIC_W. */
 int gpl_energy_grad(gpl_model* model, double* grad);
 
 /* gp_pred posterior for method_full (R/gp_pred.R:202-232): latent mean
