@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="c5 arrival rates (req/s)")
     ap.add_argument("--serve-workers", type=int, default=4, help="c5 server workers (one context each)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="c5 requests per arrival rate")
+    ap.add_argument("--serve-mem-gb", type=float, default=220.0,
+                    help="c5 server HBM budget (GiB) for resident models and work in flight; the 600 or so fits "
+                         "of one rate hold ~140 GB of factors, so 128 evicts (and refits) models under load")
     ap.add_argument("--serve-precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="c5 fit precision: auto = mixed for Gaussian fits, FP64 for Laplace fits")
     args = ap.parse_args()
@@ -433,7 +436,7 @@ def bench_c5(args, G, L, api, synth):
     from gplite_amd.serve import Server, poisson_load
     rows = []
     for prec in (args.serve_precision,):
-        srv = Server(workers=args.serve_workers, mem_gb=128.0)
+        srv = Server(workers=args.serve_workers, mem_gb=args.serve_mem_gb)
         poisson_load(srv, 100.0, 12, seed=99, precision=prec)  # warm the workers
         for rate in [float(v) for v in args.rates.split(",")]:
             nreq = args.serve_requests
@@ -454,7 +457,7 @@ def bench_c5(args, G, L, api, synth):
         "dtype": "f32+f64", "data": "synthetic (seeded, numpy PCG64)",
         "config": {"workload": "C5 serving", "workers": args.serve_workers, "fit_fraction": 0.3, "m": 1000,
                    "fit_n": "logU[128,16384]", "liks": "Gaussian(0.1) / Poisson-Laplace 50/50",
-                   "fit_precision": args.serve_precision},
+                   "fit_precision": args.serve_precision, "mem_budget_gib": args.serve_mem_gb},
         "sweep": rows,
     }
 
