@@ -1350,6 +1350,11 @@ static int energy_grad_latent(gpl_model* m, double* grad) {
         double* w = m->d(B_W4);
         const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
         const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
+        if (!m->b[B_K].p) {  // a trimmed model: K is built again from the resident features
+            ENSURE(m, B_K, (size_t)N * N * 8);
+            launch_build_sym_f64(s, m->b[B_PROG].as<DevProg>(), m->prog, m->d(B_F), n, N, m->opts.jitter, 0, 1.0,
+                                 nullptr, m->d(B_K), N);
+        }
         lik_la_grad_vectors(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, N, m->d(B_FMEAN), m->d(B_Y), tr, off,
                             m->d(B_V), Bi, N, u, s2);
         // t = s2 - S B^-1 S K s2 (padding of every vector stays zero)
