@@ -441,7 +441,8 @@ def bench_c5(args, G, L, api, synth):
         for rate in [float(v) for v in args.rates.split(",")]:
             nreq = args.serve_requests
             t = time.perf_counter()
-            r = poisson_load(srv, rate, nreq, seed=int(rate), precision=prec)
+            # the same 2000 requests at every rate (content seed 7); arrivals seeded by the rate
+            r = poisson_load(srv, rate, nreq, seed=int(rate), precision=prec, content_seed=7)
             r["precision"] = prec
             st = srv.stats()
             r["launches"] = st["launches"]

@@ -111,14 +111,22 @@ def c5_fit_request(rng, nmin=128, nmax=16384, d=6):
     return gp, x, y
 
 
-def poisson_load(server, rate, nreq, frac_fit=0.3, m=1000, seed=0, nmin=128, nmax=16384, precision="fp64"):
+def poisson_load(server, rate, nreq, frac_fit=0.3, m=1000, seed=0, nmin=128, nmax=16384, precision="fp64",
+                 content_seed=None):
     """Submit nreq requests with exponential inter-arrival times (rate per second);
-    returns per-type latency percentiles and throughput."""
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, nreq)
+    returns per-type latency percentiles and throughput.  The arrival times come
+    from `seed`; the requests themselves (fit or predict, sizes, data) from
+    `content_seed` (default: `seed`), so one content seed serves the same request
+    set at every rate and the throughputs of a sweep compare the same work."""
+    rng_gap = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed if content_seed is None else content_seed)
+    gaps = rng_gap.exponential(1.0 / rate, nreq)
+    if content_seed is None:
+        rng = rng_gap
     kinds = ["fit"] + ["fit" if u < frac_fit else "pred" for u in rng.uniform(size=nreq - 1)]
     models = []
     tickets = []
+    fit_n = []
     t0 = time.perf_counter()
     t_next = 0.0
     for k, g in zip(kinds, gaps):
@@ -128,6 +136,7 @@ def poisson_load(server, rate, nreq, frac_fit=0.3, m=1000, seed=0, nmin=128, nma
             time.sleep(t_next - now)
         if k == "fit" or not models:
             gp, x, y = c5_fit_request(rng, nmin, nmax)
+            fit_n.append(len(y))
             mid, tk = server.submit_fit(gp, x, y, precision=precision)
             models.append(mid)
             tickets.append(("fit", tk))
@@ -144,7 +153,8 @@ def poisson_load(server, rate, nreq, frac_fit=0.3, m=1000, seed=0, nmin=128, nma
             co.append(r["coalesced"])
     t_end = time.perf_counter() - t0
     out = {"rate_req_s": rate, "requests": nreq, "wall_s": t_end, "throughput_req_s": nreq / t_end,
-           "offered_s": float(np.sum(gaps)), "mean_pred_coalesced": float(np.mean(co)) if co else None}
+           "offered_s": float(np.sum(gaps)), "mean_pred_coalesced": float(np.mean(co)) if co else None,
+           "fit_sum_n3": float(sum(float(n) ** 3 for n in fit_n))}
     allv = res["fit"] + res["pred"]
     for k, v in (("all", allv), ("fit", res["fit"]), ("pred", res["pred"])):
         if v:
