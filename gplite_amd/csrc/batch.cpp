@@ -571,11 +571,21 @@ extern "C" int gpl_batch_set_params(gpl_batch* B, const gpl_cf_term* terms, cons
     if (!B || !terms || !sigma) return fail(GPL_ERR_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(B->ctx->mu);
     HIP_TRY(hipSetDevice(B->ctx->device));
+    bool periodic = false;
     for (int p = 0; p < B->nprob; ++p)
-        for (int t = 0; t < B->nterms; ++t)
-            if (terms[(size_t)p * B->nterms + t].kind == GPL_CF_PERIODIC)
-                return fail(GPL_ERR_UNSUPPORTED, "batched periodic terms need a feature rebuild");
-    return upload_params(B, terms, sigma);
+        for (int t = 0; t < B->nterms; ++t) periodic = periodic || terms[(size_t)p * B->nterms + t].kind == GPL_CF_PERIODIC;
+    int r = upload_params(B, terms, sigma);
+    if (r || !periodic) return r;
+    // periodic features [sin(w x), cos(w x)] and their period derivatives depend on
+    // the period: rebuild them from the resident inputs (O(n d), on the context's
+    // stream, ahead of the next pass)
+    hipStream_t s = B->ctx->stream;
+    for (auto* k : B->buckets) {
+        fill_f64(s, k->d(BB_F), (int64_t)k->idx.size() * k->N * MAX_FEAT, 0.0);
+        launch_features_batched(s, k->d(BB_X), k->b[BB_XOFF].as<int64_t>(), k->b[BB_N].as<int64_t>(),
+                                k->b[BB_PROG].as<DevProg>(), k->d(BB_F), k->N, k->N * MAX_FEAT, (int)k->idx.size());
+    }
+    return GPL_OK;
 }
 
 static void reset_flags(gpl_batch* B, hipStream_t s) {

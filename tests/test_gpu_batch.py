@@ -91,6 +91,40 @@ def test_batch_reports_failed_factor_per_problem(gpl):
     assert "problem 1" in str(ei.value)
 
 
+def test_batch_set_params_rebuilds_periodic_features(gpl):
+    """A periodic term's features [sin(2 pi x / period), cos(...)] depend on the
+    period, so a parameter update rebuilds them from the resident inputs: after
+    set_params with new periods and length-scales every problem's energy and
+    gradient match the oracle at the new parameters, and equal a batch created
+    directly at them."""
+    G = gpl
+    from gplite_amd.batch import GPBatch
+    r = np.random.default_rng(77)
+    gps, xs, ys = [], [], []
+    for p, n in enumerate((60, 200, 333)):
+        x = r.normal(size=(n, 2))
+        ys.append(np.sin(2.0 * x[:, 0]) + 0.3 * x[:, 1] + 0.1 * r.normal(size=n))
+        xs.append(x)
+        cfs = [G.cf_periodic(vars=[0], period=2.0 + 0.3 * p, cf_base=G.cf_sexp(lscale=0.8)), G.cf_lin(vars=[1])]
+        gps.append(G.gp_init(cfs, G.lik_gaussian(sigma=0.2)))
+    B = GPBatch(gps, xs, ys, refine_iters=6)
+    e0, _ = B.energy_grad()
+    newp = [np.array(list(G.get_param(g).values())) + 0.15 * r.normal(size=len(G.get_param(g))) for g in B.gps]
+    B.set_params(newp)
+    e, g = B.energy_grad()
+    assert not np.allclose(e, e0)
+    B2 = GPBatch([G.set_param(gp, pp) for gp, pp in zip(gps, newp)], xs, ys, refine_iters=6)
+    e2, g2 = B2.energy_grad()
+    np.testing.assert_array_equal(e, e2)
+    np.testing.assert_array_equal(g, g2)
+    for p in range(3):
+        fo = O.gp_fit([c.spec() for c in B.gps[p].cfs], B.gps[p].lik.spec(), xs[p], ys[p])
+        eo = O.gp_energy(fo)
+        assert abs(e[p] - eo) <= logz_tol(len(ys[p]), B.gps[p].lik.sigma, eo), (p, e[p], eo)
+        go = O.gp_energy_grad(fo)
+        assert np.max(np.abs(g[p] - go)) <= 1e-3 * np.max(np.abs(go)), p
+
+
 @pytest.mark.parametrize("ci", range(len(GRAD_CASES)))
 def test_batch_covariance_structures(gpl, ci):
     """Every covariance structure of the FP64 gradient tests (sums, products,
