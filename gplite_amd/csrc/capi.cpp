@@ -1263,7 +1263,7 @@ extern "C" int gpl_fit(gpl_ctx* c, const gpl_cf_term* terms, int32_t nterms, con
     m->has_trials = trials != nullptr;
     m->has_offset = offset != nullptr;
     m->have_xinv = m->have_binv = false;
-    m->split = m->no_posterior = false;
+    m->split = m->no_posterior = m->restored = false;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(m->b[B_PROG].p, &m->prog, sizeof(DevProg), hipMemcpyHostToDevice, s));
     if (n * ncols > 0) HIP_TRY(hipMemcpyAsync(m->d(B_X), x, n * ncols * 8, hipMemcpyHostToDevice, s));
@@ -1320,6 +1320,9 @@ static int energy_grad_latent(gpl_model* m, double* grad) {
     if (!ep && !(m->opts.flags & GPL_FLAG_ANALYTIC_W))
         return fail(GPL_ERR_UNSUPPORTED, "the Laplace gradient is that of the exact curvature: fit with "
                                          "GPL_FLAG_ANALYTIC_W");
+    if (!ep && m->restored)
+        return fail(GPL_ERR_UNSUPPORTED, "a restored model holds the fit fields but not the targets the Laplace "
+                                         "gradient reads: refit it");
     std::lock_guard<std::mutex> lk(c->mu);
     Range rg("gpl_energy_grad");
     HIP_TRY(hipSetDevice(c->device));
@@ -1821,6 +1824,7 @@ extern "C" int gpl_model_restore(gpl_ctx* c, const gpl_cf_term* terms, int32_t n
     m->opts = o;
     m->logZ = log_evidence;
     m->iters = iters;
+    m->restored = true;
     const int64_t N = m->N;
     hipStream_t s = c->stream;
     int info = 0;

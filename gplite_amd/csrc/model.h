@@ -37,6 +37,7 @@ struct gpl_model {
     bool split = false;
     int64_t ng = 0, nb = 0, Ng = 0, Nb = 0;
     bool no_posterior = false;  // the split's A was not positive definite: log evidence -Inf
+    bool restored = false;      // built by gpl_model_restore: the fit fields only, no targets
     DevBuf b[B_COUNT];
     ~gpl_model() {
         for (auto& x : b) x.free_();

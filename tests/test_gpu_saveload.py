@@ -107,3 +107,29 @@ def test_load_rejects_bad_fit_fields(gpl, tmp_path):
         G.gp_load(str(tmp_path / "badc.model"))
     assert ei.value.code == L.GPL_ERR_NOT_PD
     assert ei.value.minor == 71
+
+
+def test_loaded_latent_models_and_the_gradient(gpl, tmp_path):
+    """An EP fit's gradient needs only the fit fields: the loaded model gives the
+    saved model's gradient (1e-8).  The Laplace gradient (exact curvature) also
+    reads the targets, which a saved fit does not hold: the loaded model refuses
+    it, and a refit on the same data gives it again."""
+    G = gpl
+    from gplite_amd import _lib as L
+    r = np.random.default_rng(8)
+    x = r.normal(size=(150, 2))
+    f = np.sin(x.sum(1))
+    y = (r.uniform(size=150) < 1 / (1 + np.exp(-2 * f))).astype(float)
+    ge = G.gp_fit(G.gp_init(G.cf_sexp(lscale=[0.9, 1.1], magn=1.3), G.lik_bernoulli(), approx=G.approx_ep()), x, y)
+    G.gp_save(ge, str(tmp_path / "ep.model"))
+    ge2 = G.gp_load(str(tmp_path / "ep.model"))
+    assert _rel(G.gp_energy_grad(ge2), G.gp_energy_grad(ge)) < 1e-8
+    gl = G.gp_fit(G.gp_init(G.cf_sexp(lscale=[0.9, 1.1], magn=1.3), G.lik_bernoulli(),
+                            approx=G.approx_laplace(analytic_w=True)), x, y)
+    G.gp_save(gl, str(tmp_path / "la.model"))
+    gl2 = G.gp_load(str(tmp_path / "la.model"))
+    with pytest.raises(L.GplError) as ei:
+        G.gp_energy_grad(gl2)
+    assert ei.value.code == L.GPL_ERR_UNSUPPORTED
+    gl3 = G.gp_fit(gl2, x, y)
+    assert _rel(G.gp_energy_grad(gl3), G.gp_energy_grad(gl)) < 1e-12
