@@ -47,6 +47,29 @@ void set_minor(int64_t minor) { g_minor = minor; }
 extern "C" const char* gpl_last_error(void) { return g_err.c_str(); }
 extern "C" int64_t gpl_last_error_minor(void) { return g_minor; }
 
+// The stream for the trailing updates of the blocked factorisations on a
+// 256-CU device: every CU but 32, taken as 4 per XCD.  Mask bit b is chosen as
+// 32x + 8j + x (x = 0..7, j = 0..3), which spreads the 32 over the eight XCDs
+// whether the mask numbers CUs XCD by XCD or round-robin over the XCDs.  The
+// look-ahead panel chain (one-workgroup diagonal factors and short panel
+// solves) otherwise waits for each update's workgroups to drain; measured at
+// n = 8192 in FP64, 4 CUs per XCD beat 1, 2 and 8.  *out stays null on
+// other devices (the caller then uses its main stream).
+static hipError_t create_factor_stream(int device, hipStream_t* out) {
+    *out = nullptr;
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess || ncu != 256) return e;
+    uint32_t mask[8];
+    for (auto& m : mask) m = 0xffffffffu;
+    for (int x = 0; x < 8; ++x)
+        for (int j = 0; j < 4; ++j) {
+            const int b = 32 * x + 8 * j + x;
+            mask[b / 32] &= ~(1u << (b % 32));
+        }
+    return hipExtStreamCreateWithCUMask(out, 8, mask);
+}
+
 extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (!out) return fail(GPL_ERR_ARG, "out is NULL");
     int count = 0;
@@ -63,10 +86,12 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
+    if (e == hipSuccess) e = create_factor_stream(device, &c->fac);
     if (e != hipSuccess) {
         delete c;
         return fail(GPL_ERR_HIP, hipGetErrorString(e));
     }
+    if (!c->fac) c->fac = c->stream;
     *out = c;
     return GPL_OK;
 }
@@ -80,6 +105,10 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     for (auto p : c->scratch)
         if (p) hipFree(p);
     hipStreamSynchronize(c->side);
+    if (c->fac != c->stream) {
+        hipStreamSynchronize(c->fac);
+        hipStreamDestroy(c->fac);
+    }
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_p);
     hipStreamDestroy(c->side);
@@ -231,6 +260,24 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
     return GPL_OK;
 }
 
+// The blocked drivers run their trailing updates on the context's factor
+// stream (CU-masked on a 256-CU device): fac_begin makes it wait for the work
+// queued so far on the main stream, fac_end makes the main stream wait for it.
+static hipStream_t fac_begin(gpl_ctx* c) {
+    if (c->fac != c->stream) {
+        hipEventRecord(c->ev_a, c->stream);
+        hipStreamWaitEvent(c->fac, c->ev_a, 0);
+    }
+    return c->fac;
+}
+
+static void fac_end(gpl_ctx* c) {
+    if (c->fac != c->stream) {
+        hipEventRecord(c->ev_p, c->fac);
+        hipStreamWaitEvent(c->stream, c->ev_p, 0);
+    }
+}
+
 // Two-level blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
 // Outer panels of NB columns are factored with 64-column inner steps (diagonal
 // block in LDS/registers, panel solve against its inverse on MFMA, update of the
@@ -265,7 +312,7 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
 // main stream applies the rank-NB update to the trailing matrix, the side
 // stream factors the next panel (whose columns are updated first).
 void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
-    hipStream_t s = c->stream, s2 = c->side;
+    hipStream_t s = fac_begin(c), s2 = c->side;
     potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
@@ -274,7 +321,7 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
         const int64_t W1 = std::min<int64_t>(kOuterNB, R);
         const double* P = L + K1 + K0 * N;  // rows >= K1 of the factored panel
         {
-            Prof p(c, "syrk_update", 2.0 * R * W1 * W, 8.0 * (2.0 * R * W1 + R * W));
+            Prof p(c, "syrk_update", 2.0 * R * W1 * W, 8.0 * (2.0 * R * W1 + R * W), s);
             gemm2_f64(s, 0, 1, R, W1, W, -1.0, P, N, P, N, 1.0, L + K1 * (N + 1), N, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
@@ -284,11 +331,12 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             const double* P2 = L + (K1 + W1) + K0 * N;
-            Prof p(c, "syrk_update", (double)R2 * R2 * W, 8.0 * ((double)R2 * R2 + 2.0 * R2 * W));
+            Prof p(c, "syrk_update", (double)R2 * R2 * W, 8.0 * ((double)R2 * R2 + 2.0 * R2 * W), s);
             gemm2_f64(s, 0, 1, R2, R2, W, -1.0, P2, N, P2, N, 1.0, L + (K1 + W1) * (N + 1), N, 1, 0);
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
+    fac_end(c);
 }
 
 // Rows [I0, I0+H) of X = L^-1: 64-row steps inside the panel, on stream st.
@@ -308,11 +356,49 @@ static void trtri_panel(gpl_ctx* c, hipStream_t st, const double* L, const doubl
     }
 }
 
+// X = L^-1 by recursive doubling from the 64x64 diagonal-block inverses Dinv.
+// At level b (b = 64, 128, ...) the b-blocks pair up: [[X11, 0], [L21, X22]]
+// with X11 and X22 already inverted has the inverse [[X11, 0], [X21, X22]],
+// X21 = -X22 (L21 X11).  The pairs of one level are independent, so each level
+// is two batched GEMMs (plus one more pair when the last is shorter): with
+// N/64 diagonal blocks the dependent chain is 2 log2(N/64) launches, against
+// two launches per 64 rows for forward substitution.  Both triangular operands
+// skip their zero tiles (tile modes 4 and 3).  T (N x N, ld N) holds the
+// products L21 X11 at the positions of X21.
+static void trtri_doubling(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T) {
+    hipStream_t s = c->stream;
+    {
+        Prof p(c, "trtri_leaves", 0, 8.0 * N * N / 2);
+        trtri_leaves_f64(s, X, N, Dinv);
+    }
+    for (int64_t b = 64; b < N; b *= 2) {
+        const int64_t full = N / (2 * b);  // pairs whose right block has b rows
+        const int64_t r = std::min<int64_t>(b, N - full * 2 * b - b);  // rows of a last, shorter right block
+        const int64_t step = 2 * b * (N + 1);
+        Prof p(c, "trtri_level", 2.0 * (double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) : 0.0), 0);
+        if (full > 0) {
+            gemm2_f64_batched(s, 0, 0, b, b, b, 1.0, L + b, N, X, N, 0.0, T + b, N, 4, (int)full, step, step, step);
+            gemm2_f64_batched(s, 0, 0, b, b, b, -1.0, X + b * (N + 1), N, T + b, N, 0.0, X + b, N, 3, (int)full,
+                              step, step, step);
+        }
+        if (r > 0) {
+            const int64_t o = full * step;  // the last pair's top-left corner
+            gemm2_f64(s, 0, 0, r, b, b, 1.0, L + o + b, N, X + o, N, 0.0, T + o + b, N, 4, 0);
+            gemm2_f64(s, 0, 0, r, b, r, -1.0, X + o + b * (N + 1), N, T + o + b, N, 0.0, X + o + b, N, 3, 0);
+        }
+    }
+}
+
 // X = L^-1 by blocked forward substitution against the identity, two levels with
 // look-ahead: the next row panel is updated first and solved on the side stream
-// while the main stream updates the remaining rows.
-void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X) {
-    hipStream_t s = c->stream, s2 = c->side;
+// while the main stream updates the remaining rows.  With scratch T (N x N) the
+// recursive doubling above runs instead.
+void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T) {
+    if (T) {
+        trtri_doubling(c, L, Dinv, N, X, T);
+        return;
+    }
+    hipStream_t s = fac_begin(c), s2 = c->side;
     set_identity_f64(s, X, N, N);
     trtri_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
@@ -322,7 +408,7 @@ void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t
         const int64_t H1 = std::min<int64_t>(kOuterNB, R);
         const int64_t Wc = I0 + H;
         {
-            Prof p(c, "trtri_update", 2.0 * H1 * Wc * H, 8.0 * (2.0 * H1 * Wc + Wc * H));
+            Prof p(c, "trtri_update", 2.0 * H1 * Wc * H, 8.0 * (2.0 * H1 * Wc + Wc * H), s);
             gemm2_f64(s, 0, 0, H1, Wc, H, -1.0, L + I1 + I0 * N, N, X + I0, N, 1.0, X + I1, N, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
@@ -331,11 +417,12 @@ void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t
         hipEventRecord(c->ev_p, s2);
         const int64_t R2 = R - H1;
         if (R2 > 0) {
-            Prof p(c, "trtri_update", 2.0 * R2 * Wc * H, 8.0 * (2.0 * R2 * Wc + Wc * H));
+            Prof p(c, "trtri_update", 2.0 * R2 * Wc * H, 8.0 * (2.0 * R2 * Wc + Wc * H), s);
             gemm2_f64(s, 0, 0, R2, Wc, H, -1.0, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.0, X + I1 + H1, N, 0, 0);
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
+    fac_end(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -375,7 +462,7 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
 
 // Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build
 static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc) {
-    hipStream_t s = c->stream, s2 = c->side;
+    hipStream_t s = fac_begin(c), s2 = c->side;
     potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info, Dacc);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
@@ -384,7 +471,7 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         const int64_t W1 = std::min<int64_t>(kOuterNB, R);
         const float* P = L + K1 + K0 * N;
         {
-            Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W));
+            Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W), s);
             gemm_f32(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, 0);
         }
         {
@@ -399,7 +486,7 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         if (R2 > 0) {
             const float* P2 = L + (K1 + W1) + K0 * N;
             {
-                Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W));
+                Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W), s);
                 gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
             }
             {
@@ -409,6 +496,7 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
+    fac_end(c);
 }
 
 static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const double* Dinv, int64_t N, int64_t I0,
@@ -429,7 +517,7 @@ static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const doub
 }
 
 static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X) {
-    hipStream_t s = c->stream, s2 = c->side;
+    hipStream_t s = fac_begin(c), s2 = c->side;
     set_identity_f32(s, X, N, N);
     trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
@@ -439,7 +527,7 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         const int64_t H1 = std::min<int64_t>(kOuterNB, R);
         const int64_t Wc = I0 + H;
         {
-            Prof p(c, "trtri_update_f32", 2.0 * H1 * Wc * H, 4.0 * (2.0 * H1 * Wc + Wc * H));
+            Prof p(c, "trtri_update_f32", 2.0 * H1 * Wc * H, 4.0 * (2.0 * H1 * Wc + Wc * H), s);
             gemm_f32(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, 0, kFoldInverse);
         }
         hipEventRecord(c->ev_a, s);
@@ -448,12 +536,13 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         hipEventRecord(c->ev_p, s2);
         const int64_t R2 = R - H1;
         if (R2 > 0) {
-            Prof p(c, "trtri_update_f32", 2.0 * R2 * Wc * H, 4.0 * (2.0 * R2 * Wc + Wc * H));
+            Prof p(c, "trtri_update_f32", 2.0 * R2 * Wc * H, 4.0 * (2.0 * R2 * Wc + Wc * H), s);
             gemm_f32(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0, 0,
                      kFoldInverse);
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
+    fac_end(c);
 }
 
 // alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)
@@ -543,9 +632,11 @@ static int fit_gaussian(gpl_model* m) {
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
     potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
     if (m->opts.flags & GPL_FLAG_INVERSE) {
-        // alpha = S X^T X S z with X = L^-1 (S = 1/sigma)
+        // alpha = S X^T X S z with X = L^-1 (S = 1/sigma); B^-1's buffer (which
+        // the gradient fills next) is the inverse's scratch
         ENSURE(m, B_XINV, (size_t)N * N * 8);
-        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        ENSURE(m, B_BINV, (size_t)N * N * 8);
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV), m->d(B_BINV));
         m->have_xinv = true;
         Prof p(c, "trmv", 4.0 * N * N / 2, 8.0 * N * N);
         fill_f64(s, m->d(B_W1), N, 0.0);
@@ -1336,7 +1427,7 @@ static int energy_grad_latent(gpl_model* m, double* grad) {
     ENSURE(m, B_KMV, (size_t)N * std::max(kmatvec_chunks(N), symv_slots(N, 1)) * 8);
     for (int w : {B_W1, B_W2, B_W3, B_W4}) ENSURE(m, w, (size_t)N * 8);
     if (!m->have_xinv) {
-        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV), m->d(B_BINV));
         m->have_xinv = true;
     }
     double* Bi = m->d(B_BINV);
@@ -1414,7 +1505,7 @@ extern "C" int gpl_energy_grad(gpl_model* m, double* grad) {
                               m->d(B_ALPHA), m->d(B_PART));
     } else {
         if (!m->have_xinv) {
-            trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
+            trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV), m->d(B_BINV));
             m->have_xinv = true;
         }
         {

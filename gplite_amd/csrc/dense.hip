@@ -114,18 +114,20 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_lds(int64_t M, int64_t N, int6
     __shared__ double As[2][16][BM + 1];
     __shared__ double Bs[2][16][BM + 1];
     // Contiguous tile ranges per XCD share operand panels in L2; with a
-    // row-dependent k range (tiles == 2) work per tile varies by 100x, so the
-    // hardware's round-robin placement is kept there to balance the XCDs.
-    const int64_t bid = (tiles == 2) ? (int64_t)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
+    // k range that depends on the tile row (tiles == 2) or column (tiles == 4,
+    // whose columns are contiguous in the tile order) work per tile varies
+    // widely, so the hardware's round-robin placement is kept there to balance
+    // the XCDs.
+    const int64_t bid = (tiles == 2 || tiles == 4) ? (int64_t)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
     int64_t ti, tj;
-    if (tiles == 0 || tiles == 3) {
+    if (tiles == 0 || tiles == 3 || tiles == 4) {
         ti = bid % tilesM;
         tj = bid / tilesM;
     } else {
         lower_tile(bid, ti, tj);
     }
     const int64_t row0 = ti * BM, col0 = tj * BM;
-    const int64_t kbeg = (tiles == 2) ? ti * BM : 0;
+    const int64_t kbeg = (tiles == 2) ? ti * BM : (tiles == 4) ? tj * BM : 0;
     const int64_t kend = (tiles == 3 && (ti + 1) * BM < K) ? (ti + 1) * BM : K;
     const int64_t nch = kend > kbeg ? (kend - kbeg) / 16 : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -198,7 +200,7 @@ static void gemm_lds_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N,
                             int64_t ldc, int tiles, int batch = 1, int64_t sA = 0, int64_t sB = 0,
                             int64_t sC = 0) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
-    const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
+    const int64_t nblk = (tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2;
     dim3 grid((unsigned)nblk, (unsigned)batch), block(64 * NW);
 #define GPL_GL(TAv, TBv)                                                                                      \
     hipLaunchKernelGGL((k_gemm_lds<TAv, TBv, BM, TAs, TBs, TCs, NW>), grid, block, 0, s, M, N, K, tm, alpha, A, \
@@ -216,10 +218,10 @@ static void gemm_lds_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N,
 // X^T X, prediction's L^-1 K_t^T, k = n) run faster on 128x128 tiles with eight
 // waves (half the operand reads per flop at the same occupancy): 57 against 46
 // TFLOP/s for X^T X at n = 8192.  bm = 64 or 128 forces the 64/4 or 128/8 form.
-static int pick_bm(int64_t M, int64_t N, int64_t K, int tiles) {
+static int pick_bm(int64_t M, int64_t N, int64_t K, int tiles, int batch = 1) {
     const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
-    const int64_t n128 = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
-    return (K >= 1024 && n128 >= 256) ? 128 : 64;
+    const int64_t n128 = (tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2;
+    return (K >= 1024 && n128 * batch >= 256) ? 128 : 64;
 }
 
 void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
@@ -236,7 +238,10 @@ void gemm2_f64_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int6
                        int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles,
                        int batch, int64_t sA, int64_t sB, int64_t sC) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
+    if (pick_bm(M, N, K, tiles, batch) == 128)
+        gemm_lds_launch<128, double, double, double, 8>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                        tiles, batch, sA, sB, sC);
+    else gemm_lds_launch<64>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
 }
 
 // out[i] = sum over p < parts of W[p * stride + i], in order of p (deterministic)
@@ -960,6 +965,20 @@ __global__ void k_set_identity(double* __restrict__ A, int64_t ld, int64_t T) {
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N) {
     const int64_t T = N / 64;
     hipLaunchKernelGGL(k_set_identity, dim3((unsigned)(T * T)), dim3(256), 0, s, A, ld, T);
+}
+
+// Column j of X (N x N, ld N): zero above its 64x64 diagonal block, the
+// block's column from Dinv (64x64 column-major per block, zero above its
+// diagonal) on it; the rows below are left to the caller.
+__global__ __launch_bounds__(256) void k_trtri_leaves(double* __restrict__ X, int64_t N,
+                                                      const double* __restrict__ Dinv) {
+    const int64_t j = blockIdx.x, kb = j / 64, r0 = kb * 64;
+    const double* D = Dinv + kb * 4096 + (j - r0) * 64;
+    for (int64_t i = threadIdx.x; i < r0 + 64; i += 256) X[i + j * N] = (i < r0) ? 0.0 : D[i - r0];
+}
+
+void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv) {
+    hipLaunchKernelGGL(k_trtri_leaves, dim3((unsigned)N), dim3(256), 0, s, X, N, Dinv);
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {

@@ -77,7 +77,8 @@ void launch_grad_tiles_f32(hipStream_t s, const DevProg* prog, const DevProg& hp
 // C[M x N] = beta*C + alpha * op(A) op(B); all dims multiples of 64.
 // ta/tb: operand stored transposed.  tiles: 0 = full grid, 1 = lower tiles of a square C,
 // 2 = lower tiles and k starts at the row tile (C = X^T X with X lower triangular),
-// 3 = full grid and k stops after the row tile (C = X B with X lower triangular).
+// 3 = full grid and k stops after the row tile (C = X B with X lower triangular),
+// 4 = full grid and k starts at the column tile (C = A X with X lower triangular).
 // bm: output tile edge (64 or 128; 0 = choose by grid size).
 void gemm2_f64(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tiles, int bm);
@@ -139,6 +140,8 @@ void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, dou
 void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double* sv, const double* t,
                     const double* u);
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
+// X (N x N, ld N): zero above the 64x64 diagonal blocks, the blocks from Dinv
+void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
 // gp_load: saved C_chol (in the n x n part of L, ld N) -> device factor and sv = 1/sqrt(V)

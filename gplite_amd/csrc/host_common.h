@@ -27,9 +27,10 @@ void set_minor(int64_t minor);
 int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevProg& P);
 // Blocked FP64 Cholesky of the N x N lower tiles of L in place (Dinv: the 64x64
 // diagonal-block inverses, ldiag: log L_ii, info: first failing row + 1) and the
-// triangular inverse X = L^-1 (all tiles written)
+// triangular inverse X = L^-1 (all tiles written; T: optional N x N scratch,
+// which selects the recursive-doubling form)
 void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info);
-void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X);
+void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T = nullptr);
 // probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
 void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
 // Device buffers of one prediction launch over M = pad64(mt) test points (see
@@ -82,6 +83,11 @@ struct gpl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
+    // trailing updates of the blocked factorisations: the same device minus one
+    // CU in eight (a CU mask), so that the look-ahead panel chain on `side`
+    // always finds free CUs instead of queueing behind the update's workgroups.
+    // Equal to `stream` where the device does not have the 256 CUs the mask is for.
+    hipStream_t fac = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
     bool prof = false;
     std::vector<ProfRec> pending;
