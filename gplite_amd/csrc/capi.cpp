@@ -516,7 +516,42 @@ static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const doub
     }
 }
 
-static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X) {
+// X = L^-1 in FP32 by the recursive doubling of trtri_doubling: the leaves are
+// the FP64 diagonal-block inverses rounded to FP32, the products FP32 MFMA with
+// their sums folded into FP64 every 64 k (the accuracy of the forward
+// substitution's updates).  T: N x N FP32 scratch.
+static void trtri32_doubling(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X, float* T) {
+    hipStream_t s = c->stream;
+    {
+        Prof p(c, "trtri_leaves_f32", 0, 4.0 * N * N / 2);
+        trtri_leaves_f32_batched(s, X, N, Dinv, 1, 0, 0);
+    }
+    for (int64_t b = 64; b < N; b *= 2) {
+        const int64_t full = N / (2 * b);
+        const int64_t r = std::min<int64_t>(b, N - full * 2 * b - b);
+        const int64_t step = 2 * b * (N + 1);
+        Prof p(c, "trtri_level_f32", 2.0 * (double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) : 0.0), 0);
+        if (full > 0) {
+            gemm_f32_batched(s, 0, 0, b, b, b, 1.f, L + b, N, X, N, 0.f, T + b, N, 4, 0, (int)full, step, step, step,
+                             kFoldInverse);
+            gemm_f32_batched(s, 0, 0, b, b, b, -1.f, X + b * (N + 1), N, T + b, N, 0.f, X + b, N, 3, 0, (int)full,
+                             step, step, step, kFoldInverse);
+        }
+        if (r > 0) {
+            const int64_t o = full * step;
+            gemm_f32(s, 0, 0, r, b, b, 1.f, L + o + b, N, X + o, N, 0.f, T + o + b, N, 4, 0, kFoldInverse);
+            gemm_f32(s, 0, 0, r, b, r, -1.f, X + o + b * (N + 1), N, T + o + b, N, 0.f, X + o + b, N, 3, 0,
+                     kFoldInverse);
+        }
+    }
+}
+
+// With scratch T (N x N FP32) the recursive doubling above runs instead.
+static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X, float* T = nullptr) {
+    if (T) {
+        trtri32_doubling(c, L, Dinv, N, X, T);
+        return;
+    }
     hipStream_t s = fac_begin(c), s2 = c->side;
     set_identity_f32(s, X, N, N);
     trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
@@ -709,12 +744,13 @@ static int fit_gaussian_mixed(gpl_model* m) {
     }
     lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
+    // B^-1's buffer (filled below) is the inverse's scratch first
+    ENSURE(m, B_BINV, (size_t)N * N * 4);
     potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), m->d(B_DACC));
-    trtri32_blocked(c, L, m->d(B_DINV), N, X);
+    trtri32_blocked(c, L, m->d(B_DINV), N, X, m->b[B_BINV].as<float>());
     m->have_xinv = true;
     // B^-1 = X^T X (kept for the gradient) and the first-order log|B| correction
     // for having factored fl32(B) rather than B
-    ENSURE(m, B_BINV, (size_t)N * N * 4);
     const int64_t T = N / 64, ntiles = T * (T + 1) / 2;
     ENSURE(m, B_PART, (size_t)ntiles * std::max(1, m->prog.ngrad) * 8);
     float* Bi = m->b[B_BINV].as<float>();
@@ -939,6 +975,7 @@ static int fit_laplace_mixed(gpl_model* m) {
     const double* tr = m->has_trials ? m->d(B_TR) : nullptr;
     const double* off = m->has_offset ? m->d(B_OFF) : nullptr;
     ENSURE(m, B_XINV, (size_t)N * N * 4);
+    ENSURE(m, B_BINV, (size_t)N * N * 4);  // the inverse's scratch
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     ENSURE(m, B_B64, (size_t)N * N * 8);
     const int nch = kmatvec_chunks(N);
@@ -965,7 +1002,7 @@ static int fit_laplace_mixed(gpl_model* m) {
             form_B_mixed(s, m->d(B_K), B64, L, m->d(B_DACC), N, n, N, m->d(B_SV));
         }
         potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
-        trtri32_blocked(c, L, m->d(B_DINV), N, X);
+        trtri32_blocked(c, L, m->d(B_DINV), N, X, m->b[B_BINV].as<float>());
         int rr0 = refine_alpha_mixed(m, maxref, nch);
         if (rr0) return rr0;
         {
@@ -1207,6 +1244,7 @@ static int fit_ep_mixed(gpl_model* m) {
     HIP_TRY(hipMemcpyAsync(m->d(B_GH), gx.data(), q * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(m->d(B_GH) + 32, gw.data(), q * 8, hipMemcpyHostToDevice, s));
     ENSURE(m, B_XINV, (size_t)N * N * 4);
+    ENSURE(m, B_BINV, (size_t)N * N * 4);  // the inverse's scratch
     ENSURE(m, B_DACC, (size_t)N * 64 * 8);
     ENSURE(m, B_B64, (size_t)N * N * 8);
     const int nch = kmatvec_chunks(N);
@@ -1256,7 +1294,7 @@ static int fit_ep_mixed(gpl_model* m) {
             form_B_mixed(s, m->d(B_K), m->d(B_B64), L32, m->d(B_DACC), N, n, N, m->d(B_SV));
         }
         potrf32_blocked(c, L32, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
-        trtri32_blocked(c, L32, m->d(B_DINV), N, X32);
+        trtri32_blocked(c, L32, m->d(B_DINV), N, X32, m->b[B_BINV].as<float>());
         {
             int rr0 = refine_alpha_mixed(m, maxref, nch);
             if (rr0) return rr0;

@@ -95,16 +95,16 @@ __global__ __launch_bounds__(64 * NW) void k_gemm32(int64_t M, int64_t N, int64_
     C += blockIdx.y * sC;
     __shared__ float As[2][KC][BM + 4];
     __shared__ float Bs[2][KC][BM + 4];
-    const int64_t bid = (tiles == 2) ? (int64_t)blockIdx.x : xcd_remap32(blockIdx.x, gridDim.x);
+    const int64_t bid = (tiles == 2 || tiles == 4) ? (int64_t)blockIdx.x : xcd_remap32(blockIdx.x, gridDim.x);
     int64_t ti, tj;
-    if (tiles == 0 || tiles == 3) {
+    if (tiles == 0 || tiles == 3 || tiles == 4) {
         ti = bid % tilesM;
         tj = bid / tilesM;
     } else {
         lower_tile32(bid, ti, tj);
     }
     const int64_t row0 = ti * BM, col0 = tj * BM;
-    const int64_t kbeg = (tiles == 2) ? ti * BM : 0;
+    const int64_t kbeg = (tiles == 2) ? ti * BM : (tiles == 4) ? tj * BM : 0;
     const int64_t kend = (tiles == 3 && (ti + 1) * BM < K) ? (ti + 1) * BM : K;
     const int64_t nch = kend > kbeg ? (kend - kbeg) / KC : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -209,7 +209,7 @@ static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
                           const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
                           int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
-    const int64_t nblk = (tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2;
+    const int64_t nblk = (tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2;
     dim3 grid((unsigned)nblk, (unsigned)batch), block(64 * NW);
 #define GPL_G32(TAv, TBv)                                                                                        \
     hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM, FOLD, NW, KC>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, \
@@ -231,7 +231,7 @@ void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64
     if (M <= 0 || N <= 0 || batch <= 0) return;
     if (bm == 0) {
         const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
-        const int64_t n128 = ((tiles == 0 || tiles == 3) ? tm * tn : tm * (tm + 1) / 2) * batch;
+        const int64_t n128 = ((tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2) * batch;
         bm = (n128 >= 256) ? 128 : 64;
     }
     if (fold) {
@@ -276,6 +276,24 @@ void set_identity_f32_batched(hipStream_t s, float* A, int64_t ld, int64_t N, in
 
 void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
     set_identity_f32_batched(s, A, ld, N, 1, 0);
+}
+
+// Column j of X (FP32, N x N, ld N, batch stride sX): zero above its 64x64
+// diagonal block, the block's column of Dinv (FP64, 64x64 column-major per
+// block, zero above its diagonal, batch stride sD) rounded to FP32 on it; the
+// rows below are left to the caller.
+__global__ __launch_bounds__(256) void k_trtri_leaves32(float* __restrict__ X, int64_t N,
+                                                        const double* __restrict__ Dinv, int64_t sX, int64_t sD) {
+    X += blockIdx.y * sX;
+    Dinv += blockIdx.y * sD;
+    const int64_t j = blockIdx.x, kb = j / 64, r0 = kb * 64;
+    const double* D = Dinv + kb * 4096 + (j - r0) * 64;
+    for (int64_t i = threadIdx.x; i < r0 + 64; i += 256) X[i + j * N] = (i < r0) ? 0.f : (float)D[i - r0];
+}
+
+void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,
+                              int64_t sD) {
+    hipLaunchKernelGGL(k_trtri_leaves32, dim3((unsigned)N, (unsigned)batch), dim3(256), 0, s, X, N, Dinv, sX, sD);
 }
 
 // y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32 (64x64 tiles,

@@ -142,6 +142,8 @@ void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 // X (N x N, ld N): zero above the 64x64 diagonal blocks, the blocks from Dinv
 void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv);
+void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,
+                              int64_t sD);
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
 // gp_load: saved C_chol (in the n x n part of L, ld N) -> device factor and sv = 1/sqrt(V)

@@ -93,7 +93,7 @@ size_t fit_footprint(int64_t n, const gpl_lik& lik, const gpl_opts& o) {
     size_t b = NN * (mixed ? 4 : 8);                              // factor of B
     if (!gauss) b += NN * 8;                                      // stored K (FP64)
     if (o.approx == GPL_APPROX_EP) b += NN * 8;                   // L^-1 (EP)
-    if (mixed) b += NN * 4 + NN * 8 + (gauss ? NN * 4 : 0);       // L^-1, FP64 B, B^-1 (Gaussian)
+    if (mixed) b += NN * 4 + NN * 8 + NN * 4;                     // L^-1, FP64 B, B^-1 (or the inverse's scratch)
     else if (o.flags & GPL_FLAG_INVERSE) b += 2 * NN * 8;         // L^-1, B^-1 (the inverse's scratch)
     const size_t T = N / 64;
     // matrix-vector partials: the larger of 32 chunks and two slots per four tile rows
