@@ -158,27 +158,41 @@ static void potrf_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t K0, in
     potrf_panel_rec(c, s, k, K0 + h, W2);
 }
 
-// rows [I0, I0+H) of X = L^-1, given the updates from all rows above I0
-static void trtri_panel_rec(gpl_ctx* c, hipStream_t s, Bucket& k, int64_t I0, int64_t H) {
+// X = L^-1 for every problem of the bucket by recursive doubling from the FP64
+// diagonal-block inverses: at level b each pair of adjacent b-blocks
+// [[X11, 0], [L21, X22]] gets X21 = -X22 (L21 X11).  One launch per product and
+// level covers all pairs (grid rows) of all problems (grid z); T (the B^-1
+// buffer, free until B^-1 is formed) holds L21 X11.
+static void trtri_doubling_batched(gpl_ctx* c, hipStream_t s, Bucket& k) {
     const int nb = (int)k.idx.size();
     const int64_t N = k.N, N2 = N * N;
     const float* L = k.f(BB_L);
     float* X = k.f(BB_X32);
-    if (H == 64) {
-        const int64_t i = I0 / 64, Wc = I0 + 64;
-        Prof p(c, "batch_trtri_diag_f64", 2.0 * nb * 64 * 64 * Wc, 0, s);
-        gemm_mixed_f64f32_batched(s, 0, 0, 64, Wc, 64, 1.0, k.d(BB_DINV) + i * 64 * 64, 64, X + I0, N, 0.0, X + I0, N,
-                                  0, nb, N * 64, N2, N2);
-        return;
+    float* T = k.f(BB_BI);
+    {
+        Prof p(c, "batch_trtri_leaves", 0, 0, s);
+        trtri_leaves_f32_batched(s, X, N, k.d(BB_DINV), nb, N2, N * 64);
     }
-    const int64_t h = (H / 128) * 64, H2 = H - h, Wc = I0 + h;
-    trtri_panel_rec(c, s, k, I0, h);
-    {  // 64-wide tiles, as in potrf_panel_rec
-        Prof p(c, "batch_trtri_panel_f32", 2.0 * nb * H2 * Wc * h, 0, s);
-        gemm_f32_batched(s, 0, 0, H2, Wc, h, -1.f, L + (I0 + h) + I0 * N, N, X + I0, N, 1.f, X + I0 + h, N, 0, 64, nb,
-                         N2, N2, N2, kFoldInverse);
+    for (int64_t b = 64; b < N; b *= 2) {
+        const int64_t full = N / (2 * b);
+        const int64_t r = std::min<int64_t>(b, N - full * 2 * b - b);
+        const int64_t step = 2 * b * (N + 1);
+        Prof p(c, "batch_trtri_level", 2.0 * nb * ((double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) / 2 : 0.0)),
+               0, s);
+        if (full > 0) {
+            gemm_f32_batched2(s, 0, 0, b, b, b, 1.f, L + b, N, X, N, 0.f, T + b, N, 4, 0, (int)full, step, step, step,
+                              nb, N2, kFoldInverse);
+            gemm_f32_batched2(s, 0, 0, b, b, b, -1.f, X + b * (N + 1), N, T + b, N, 0.f, X + b, N, 3, 0, (int)full,
+                              step, step, step, nb, N2, kFoldInverse);
+        }
+        if (r > 0) {
+            const int64_t o = full * step;
+            gemm_f32_batched(s, 0, 0, r, b, b, 1.f, L + o + b, N, X + o, N, 0.f, T + o + b, N, 4, 0, nb, N2, N2, N2,
+                             kFoldInverse);
+            gemm_f32_batched(s, 0, 0, r, b, r, -1.f, X + o + b * (N + 1), N, T + o + b, N, 0.f, X + o + b, N, 3, 0,
+                             nb, N2, N2, N2, kFoldInverse);
+        }
     }
-    trtri_panel_rec(c, s, k, I0 + h, H2);
 }
 
 // --------------------------------------------------------------------------
@@ -244,31 +258,8 @@ static void bucket_pipeline(gpl_batch* B, Bucket& k, Lane& ln) {
         }
         hipStreamWaitEvent(s, ln.ev_p, 0);
     }
-    // X = L^-1 (FP32, FP64 diagonal-block applications), same look-ahead over row panels
-    set_identity_f32_batched(s, X, N, N, nb, N2);
-    trtri_panel_rec(c, s, k, 0, std::min<int64_t>(NBo, N));
-    for (int64_t I0 = 0; I0 < N; I0 += NBo) {
-        const int64_t H = std::min<int64_t>(NBo, N - I0);
-        const int64_t I1 = I0 + H, R = N - I1;
-        if (R <= 0) break;
-        const int64_t H1 = std::min<int64_t>(NBo, R), Wc = I0 + H;
-        {
-            Prof p(c, "batch_trtri_f32", 2.0 * nb * H1 * Wc * H, 0, s);
-            gemm_f32_batched(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, bm, nb, N2,
-                             N2, N2, kFoldInverse);
-        }
-        hipEventRecord(ln.ev_a, s);
-        hipStreamWaitEvent(s2, ln.ev_a, 0);
-        trtri_panel_rec(c, s2, k, I1, H1);
-        hipEventRecord(ln.ev_p, s2);
-        const int64_t R2 = R - H1;
-        if (R2 > 0) {
-            Prof p(c, "batch_trtri_f32", 2.0 * nb * R2 * Wc * H, 0, s);
-            gemm_f32_batched(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0,
-                             bm, nb, N2, N2, N2, kFoldInverse);
-        }
-        hipStreamWaitEvent(s, ln.ev_p, 0);
-    }
+    // X = L^-1 (FP32, FP64 diagonal-block inverses as leaves)
+    trtri_doubling_batched(c, s, k);
     // alpha = S B^-1 S y by iterative refinement (fixed sweep count: the graph is
     // static) on the side stream, while the main stream forms B^-1 = X^T X
     hipEventRecord(ln.ev_a, s);

@@ -47,29 +47,6 @@ void set_minor(int64_t minor) { g_minor = minor; }
 extern "C" const char* gpl_last_error(void) { return g_err.c_str(); }
 extern "C" int64_t gpl_last_error_minor(void) { return g_minor; }
 
-// The stream for the trailing updates of the blocked factorisations on a
-// 256-CU device: every CU but 32, taken as 4 per XCD.  Mask bit b is chosen as
-// 32x + 8j + x (x = 0..7, j = 0..3), which spreads the 32 over the eight XCDs
-// whether the mask numbers CUs XCD by XCD or round-robin over the XCDs.  The
-// look-ahead panel chain (one-workgroup diagonal factors and short panel
-// solves) otherwise waits for each update's workgroups to drain; measured at
-// n = 8192 in FP64, 4 CUs per XCD beat 1, 2 and 8.  *out stays null on
-// other devices (the caller then uses its main stream).
-static hipError_t create_factor_stream(int device, hipStream_t* out) {
-    *out = nullptr;
-    int ncu = 0;
-    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-    if (e != hipSuccess || ncu != 256) return e;
-    uint32_t mask[8];
-    for (auto& m : mask) m = 0xffffffffu;
-    for (int x = 0; x < 8; ++x)
-        for (int j = 0; j < 4; ++j) {
-            const int b = 32 * x + 8 * j + x;
-            mask[b / 32] &= ~(1u << (b % 32));
-        }
-    return hipExtStreamCreateWithCUMask(out, 8, mask);
-}
-
 extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (!out) return fail(GPL_ERR_ARG, "out is NULL");
     int count = 0;
@@ -86,12 +63,10 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
-    if (e == hipSuccess) e = create_factor_stream(device, &c->fac);
     if (e != hipSuccess) {
         delete c;
         return fail(GPL_ERR_HIP, hipGetErrorString(e));
     }
-    if (!c->fac) c->fac = c->stream;
     *out = c;
     return GPL_OK;
 }
@@ -105,10 +80,6 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     for (auto p : c->scratch)
         if (p) hipFree(p);
     hipStreamSynchronize(c->side);
-    if (c->fac != c->stream) {
-        hipStreamSynchronize(c->fac);
-        hipStreamDestroy(c->fac);
-    }
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_p);
     hipStreamDestroy(c->side);
@@ -260,24 +231,6 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
     return GPL_OK;
 }
 
-// The blocked drivers run their trailing updates on the context's factor
-// stream (CU-masked on a 256-CU device): fac_begin makes it wait for the work
-// queued so far on the main stream, fac_end makes the main stream wait for it.
-static hipStream_t fac_begin(gpl_ctx* c) {
-    if (c->fac != c->stream) {
-        hipEventRecord(c->ev_a, c->stream);
-        hipStreamWaitEvent(c->fac, c->ev_a, 0);
-    }
-    return c->fac;
-}
-
-static void fac_end(gpl_ctx* c) {
-    if (c->fac != c->stream) {
-        hipEventRecord(c->ev_p, c->fac);
-        hipStreamWaitEvent(c->stream, c->ev_p, 0);
-    }
-}
-
 // Two-level blocked right-looking Cholesky of the N x N matrix in L (lower tiles).
 // Outer panels of NB columns are factored with 64-column inner steps (diagonal
 // block in LDS/registers, panel solve against its inverse on MFMA, update of the
@@ -312,7 +265,7 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
 // main stream applies the rank-NB update to the trailing matrix, the side
 // stream factors the next panel (whose columns are updated first).
 void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
-    hipStream_t s = fac_begin(c), s2 = c->side;
+    hipStream_t s = c->stream, s2 = c->side;
     potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
@@ -336,7 +289,6 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    fac_end(c);
 }
 
 // Rows [I0, I0+H) of X = L^-1: 64-row steps inside the panel, on stream st.
@@ -398,7 +350,7 @@ void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t
         trtri_doubling(c, L, Dinv, N, X, T);
         return;
     }
-    hipStream_t s = fac_begin(c), s2 = c->side;
+    hipStream_t s = c->stream, s2 = c->side;
     set_identity_f64(s, X, N, N);
     trtri_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
@@ -422,7 +374,6 @@ void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    fac_end(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,7 +413,7 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
 
 // Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build
 static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc) {
-    hipStream_t s = fac_begin(c), s2 = c->side;
+    hipStream_t s = c->stream, s2 = c->side;
     potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info, Dacc);
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
@@ -496,7 +447,6 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    fac_end(c);
 }
 
 static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const double* Dinv, int64_t N, int64_t I0,
@@ -552,7 +502,7 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         trtri32_doubling(c, L, Dinv, N, X, T);
         return;
     }
-    hipStream_t s = fac_begin(c), s2 = c->side;
+    hipStream_t s = c->stream, s2 = c->side;
     set_identity_f32(s, X, N, N);
     trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
     for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
@@ -577,7 +527,6 @@ static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int6
         }
         hipStreamWaitEvent(s, c->ev_p, 0);
     }
-    fac_end(c);
 }
 
 // alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)

@@ -85,14 +85,16 @@ template <int TA, int TB, int BM, int FOLD, int NW, int KC>
 __global__ __launch_bounds__(64 * NW) void k_gemm32(int64_t M, int64_t N, int64_t K, int64_t tilesM, float alpha,
                                                     const float* A, int64_t lda, const float* B, int64_t ldb,
                                                     float beta, float* C, int64_t ldc, int tiles, int64_t sA,
-                                                    int64_t sB, int64_t sC) {
+                                                    int64_t sB, int64_t sC, int64_t sZ) {
     constexpr int NT = 64 * NW, WCn = NW / 2, WTR = BM / 2, WTC = BM / WCn;
     constexpr int NBR = WTR / 32, NBC = WTC / 32, AL = KC * BM / (4 * NT);
     constexpr int FR = FOLD > 0 ? NBR : 1, FC = FOLD > 0 ? NBC : 1;
     static_assert(NBR >= 1 && NBC >= 1 && AL >= 1, "tile shape");
-    A += blockIdx.y * sA;  // batch: one problem per grid row
-    B += blockIdx.y * sB;
-    C += blockIdx.y * sC;
+    // batch: one problem per grid row; a second batch level (grid z) moves all
+    // three operands by sZ
+    A += blockIdx.y * sA + blockIdx.z * sZ;
+    B += blockIdx.y * sB + blockIdx.z * sZ;
+    C += blockIdx.y * sC + blockIdx.z * sZ;
     __shared__ float As[2][KC][BM + 4];
     __shared__ float Bs[2][KC][BM + 4];
     const int64_t bid = (tiles == 2 || tiles == 4) ? (int64_t)blockIdx.x : xcd_remap32(blockIdx.x, gridDim.x);
@@ -207,13 +209,14 @@ __global__ __launch_bounds__(64 * NW) void k_gemm32(int64_t M, int64_t N, int64_
 template <int BM, int FOLD, int NW, int KC = 16>
 static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha,
                           const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
-                          int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC) {
+                          int64_t ldc, int tiles, int batch, int64_t sA, int64_t sB, int64_t sC, int nz,
+                          int64_t sZ) {
     const int64_t tm = (M + BM - 1) / BM, tn = (N + BM - 1) / BM;
     const int64_t nblk = (tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2;
-    dim3 grid((unsigned)nblk, (unsigned)batch), block(64 * NW);
+    dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)nz), block(64 * NW);
 #define GPL_G32(TAv, TBv)                                                                                        \
     hipLaunchKernelGGL((k_gemm32<TAv, TBv, BM, FOLD, NW, KC>), grid, block, 0, s, M, N, K, tm, alpha, A, lda, B, ldb, \
-                       beta, C, ldc, tiles, sA, sB, sC)
+                       beta, C, ldc, tiles, sA, sB, sC, sZ)
     if (!ta && !tb) GPL_G32(0, 0);
     else if (!ta && tb) GPL_G32(0, 1);
     else if (ta && !tb) GPL_G32(1, 0);
@@ -225,28 +228,33 @@ static void gemm32_launch(hipStream_t s, int ta, int tb, int64_t M, int64_t N, i
 // waves there (64x32 per wave keeps the FP64 accumulators at four waves/SIMD).
 // fold == 0: plain FP32 accumulation.  bm == 0 picks 128-wide tiles when the
 // grid holds at least 256 of them.
+void gemm_f32_batched2(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
+                       int batch, int64_t sA, int64_t sB, int64_t sC, int nz, int64_t sZ, int fold) {
+    if (M <= 0 || N <= 0 || batch <= 0 || nz <= 0) return;
+    if (bm == 0) {
+        const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
+        const int64_t n128 = ((tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2) * batch * nz;
+        bm = (n128 >= 256) ? 128 : 64;
+    }
+#define GPL_G32L(BMv, FOLDv, NWv)                                                                                     \
+    gemm32_launch<BMv, FOLDv, NWv>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC, \
+                                   nz, sZ)
+    if (fold) {
+        if (bm == 128) GPL_G32L(128, 4, 8);
+        else GPL_G32L(64, 4, 4);
+    } else {
+        if (bm == 128) GPL_G32L(128, 0, 4);
+        else GPL_G32L(64, 0, 4);
+    }
+#undef GPL_G32L
+}
+
 void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
                       int batch, int64_t sA, int64_t sB, int64_t sC, int fold) {
-    if (M <= 0 || N <= 0 || batch <= 0) return;
-    if (bm == 0) {
-        const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
-        const int64_t n128 = ((tiles == 0 || tiles == 3 || tiles == 4) ? tm * tn : tm * (tm + 1) / 2) * batch;
-        bm = (n128 >= 256) ? 128 : 64;
-    }
-    if (fold) {
-        if (bm == 128)
-            gemm32_launch<128, 4, 8>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB,
-                                     sC);
-        else
-            gemm32_launch<64, 4, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
-    } else {
-        if (bm == 128)
-            gemm32_launch<128, 0, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB,
-                                     sC);
-        else
-            gemm32_launch<64, 0, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, batch, sA, sB, sC);
-    }
+    gemm_f32_batched2(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles, bm, batch, sA, sB, sC, 1, 0,
+                      fold);
 }
 
 void gemm_f32(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,

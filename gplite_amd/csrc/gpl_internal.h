@@ -243,6 +243,10 @@ void launch_grad_tiles_f32_batched(hipStream_t s, const DevProg* progs, const De
 void gemm_f32_batched(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
                       int batch, int64_t sA, int64_t sB, int64_t sC, int fold = 1);
+// two batch levels: grid row y moves A, B, C by sA, sB, sC, grid z (nz) all three by sZ
+void gemm_f32_batched2(hipStream_t s, int ta, int tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+                       int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int tiles, int bm,
+                       int batch, int64_t sA, int64_t sB, int64_t sC, int nz, int64_t sZ, int fold);
 // FP64 folding in the GEMMs of L^-1 and B^-1 = X^T X (they feed the gradient and
 // the predictive variance, not the log evidence)
 constexpr int kFoldInverse = 1;

@@ -83,11 +83,6 @@ struct gpl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
-    // trailing updates of the blocked factorisations: the same device minus one
-    // CU in eight (a CU mask), so that the look-ahead panel chain on `side`
-    // always finds free CUs instead of queueing behind the update's workgroups.
-    // Equal to `stream` where the device does not have the 256 CUs the mask is for.
-    hipStream_t fac = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
     bool prof = false;
     std::vector<ProfRec> pending;

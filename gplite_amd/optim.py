@@ -51,7 +51,7 @@ def check_convergence_batched(gp, x, y, tol_param=1e-1, batch=None):
 
 
 def gp_optim_multistart(gp, x, y, starts=8, spread=1.0, maxiter=200, gtol=1e-5, ftol=1e-10, memory=10, seed=0,
-                        refine_iters=4, return_info=False):
+                        refine_iters=4, polish=20, return_info=False):
     """Minimise the energy (R/gp.R:138-147) from `starts` initial log-parameter
     vectors (the model's own and starts-1 draws p0 + spread N(0, I)) with
     lock-step L-BFGS.
@@ -61,8 +61,15 @@ def gp_optim_multistart(gp, x, y, starts=8, spread=1.0, maxiter=200, gtol=1e-5, 
     and Armijo backtracking (c1 = 1e-4, halving).  A search stops when
     max |grad| / n < gtol or its energy change is below ftol·(1+|E|).
 
-    Returns the GP refitted (FP64) at the best final point, and optionally
-    an info dict (energies and iteration counts per start, number of passes).
+    The batched pass computes in mixed precision, whose energies carry
+    rounding of about 1e-7 relative: near an optimum the ftol test then stops
+    a search wherever that rounding hides the decrease.  The best final point
+    is therefore polished by up to `polish` FP64 L-BFGS-B iterations on single
+    device fits (gradient tolerance gtol on max |grad| / n); polish=0 skips it.
+
+    Returns the GP refitted (FP64) at the final point, and optionally an info
+    dict (energies and iteration counts per start, number of passes, FP64
+    polish iterations).
     """
     x = api._as_matrix(x)
     y = np.asarray(y, dtype=float).ravel()
@@ -139,8 +146,20 @@ def gp_optim_multistart(gp, x, y, starts=8, spread=1.0, maxiter=200, gtol=1e-5, 
         done = (np.max(np.abs(g), axis=1) < gtol) | (df < ftol * (1 + np.abs(f))) | stalled
         active &= ~done
     best = int(np.argmin(f))
-    g_best = api.gp_fit(api.set_param(gp, p[best]), x, y)
+    p_final, polished = p[best], 0
+    if polish > 0:
+        from scipy.optimize import minimize
+
+        def fg(q):
+            gq = api.gp_fit(api.set_param(gp, q), x, y, want_grad=True)
+            return api.gp_energy(gq) / n, api.gp_energy_grad(gq) / n
+
+        res = minimize(fg, p_final, jac=True, method="L-BFGS-B",
+                       options={"maxiter": polish, "gtol": gtol, "ftol": 1e-15})
+        if np.all(np.isfinite(res.x)) and np.isfinite(res.fun):
+            p_final, polished = res.x, int(res.nit)
+    g_best = api.gp_fit(api.set_param(gp, p_final), x, y)
     if return_info:
         return g_best, {"energy_per_n": f, "params": p, "iterations": iters, "passes": passes, "best": best,
-                        "names": _free_names(gp)}
+                        "polish_iterations": polished, "names": _free_names(gp)}
     return g_best
