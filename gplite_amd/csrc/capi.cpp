@@ -63,6 +63,8 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->psync, 32 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->psync, 0, 32 * sizeof(unsigned));
     if (e != hipSuccess) {
         delete c;
         return fail(GPL_ERR_HIP, hipGetErrorString(e));
@@ -82,6 +84,7 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     hipStreamSynchronize(c->side);
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_p);
+    if (c->psync) hipFree(c->psync);
     hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
@@ -237,33 +240,25 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
 // rest of the outer panel); the trailing matrix then receives one rank-NB update.
 static const int kOuterNB = 256;
 
-// Factor the outer panel of columns [K0, K0+W) (all rows >= K0) on stream st.
+// Factor the outer panel of columns [K0, K0+W) (all rows >= K0) on stream st:
+// one launch, a workgroup per 64-row block (k_potrf_panel).
 static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
                         double* ldiag, int* info) {
-    for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
-        const int64_t k = j0 / 64;
-        {
-            Prof p(c, "potrf_diag", 2.0 * 64 * 64 * 64 / 3, 2 * 64.0 * 64 * 8, st);
-            potrf_diag3_f64(st, L, N, k, Dinv, ldiag, info, 1);
-        }
-        const int64_t M = N - (j0 + 64);
-        if (M <= 0) continue;
-        double* L21 = L + (j0 + 64) + j0 * N;
-        {
-            Prof p(c, "trsm_panel", 2.0 * M * 64 * 64, 2.0 * M * 64 * 8, st);
-            gemm2_f64(st, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0, 0);
-        }
-        const int64_t Wr = K0 + W - (j0 + 64);
-        if (Wr > 0) {
-            Prof p(c, "panel_update", 2.0 * M * Wr * 64, 8.0 * (2.0 * M * Wr + M * 64), st);
-            gemm2_f64(st, 0, 1, M, Wr, 64, -1.0, L21, N, L21, N, 1.0, L + (j0 + 64) * (N + 1), N, 0, 0);
-        }
+    const int64_t nrb = (N - K0) / 64;
+    if (++c->pepoch == 0) c->pepoch = 1;
+    {
+        Prof p(c, "potrf_panel", (double)W * W * W / 3 + 2.0 * (N - K0 - W) * W * W, 8.0 * (N - K0) * W * 3, st);
+        potrf_panel_f64(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase);
     }
+    c->pbase += (unsigned)nrb;
 }
 
-// Two-level blocked right-looking Cholesky with one-panel look-ahead: while the
-// main stream applies the rank-NB update to the trailing matrix, the side
-// stream factors the next panel (whose columns are updated first).
+// Two-level blocked right-looking Cholesky with one-panel look-ahead: after the
+// rank-NB update of the next panel's columns, the main stream factors that
+// panel (one launch) while the side stream applies the update to the rest of
+// the trailing matrix.  The panel launch is queued first: its workgroups (one
+// per 64-row block, each needing half a CU's registers) are placed before the
+// update's GEMM fills the CUs, instead of waiting for it to drain.
 void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
     hipStream_t s = c->stream, s2 = c->side;
     potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
@@ -278,16 +273,18 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
             gemm2_f64(s, 0, 1, R, W1, W, -1.0, P, N, P, N, 1.0, L + K1 * (N + 1), N, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
-        hipStreamWaitEvent(s2, c->ev_a, 0);
-        potrf_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info);
-        hipEventRecord(c->ev_p, s2);
+        potrf_panel(c, s, L, N, K1, W1, Dinv, ldiag, info);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
+            hipStreamWaitEvent(s2, c->ev_a, 0);
             const double* P2 = L + (K1 + W1) + K0 * N;
-            Prof p(c, "syrk_update", (double)R2 * R2 * W, 8.0 * ((double)R2 * R2 + 2.0 * R2 * W), s);
-            gemm2_f64(s, 0, 1, R2, R2, W, -1.0, P2, N, P2, N, 1.0, L + (K1 + W1) * (N + 1), N, 1, 0);
+            {
+                Prof p(c, "syrk_update", (double)R2 * R2 * W, 8.0 * ((double)R2 * R2 + 2.0 * R2 * W), s2);
+                gemm2_f64(s2, 0, 1, R2, R2, W, -1.0, P2, N, P2, N, 1.0, L + (K1 + W1) * (N + 1), N, 1, 0);
+            }
+            hipEventRecord(c->ev_p, s2);
+            hipStreamWaitEvent(s, c->ev_p, 0);
         }
-        hipStreamWaitEvent(s, c->ev_p, 0);
     }
 }
 
@@ -544,14 +541,19 @@ static void solve_alpha(gpl_ctx* c, gpl_model* m, const double* z, double* alpha
     vec_mul(s, m->n, m->d(B_SV), w1, alpha);
 }
 
+// A factorisation's info word: 0, the first failing row + 1, or kPanelWaitTimeout.
+static int pivot_fail(int info) {
+    if (info == kPanelWaitTimeout)
+        return fail(GPL_ERR_HIP, "blocked Cholesky: a wait between the workgroups of a panel timed out");
+    g_minor = info;
+    return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(info) + " is not positive");
+}
+
 static int check_info(gpl_model* m) {
     int info = 0;
     HIP_TRY(hipMemcpyAsync(&info, m->b[B_INFO].p, sizeof(int), hipMemcpyDeviceToHost, m->ctx->stream));
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
-    if (info > 0) {
-        g_minor = info;
-        return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(info) + " is not positive");
-    }
+    if (info > 0) return pivot_fail(info);
     return GPL_OK;
 }
 
@@ -835,10 +837,7 @@ static int fit_laplace(gpl_model* m) {
             r = read_scalars(m, h, 4);
             if (r) return r;
         } else {
-            if (fl[0] > 0) {
-                g_minor = fl[0];
-                return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
-            }
+            if (fl[0] > 0) return pivot_fail(fl[0]);
             m->split = false;
             m->logZ = h[0] - 0.5 * h[1] - h[2];
         }
@@ -974,10 +973,7 @@ static int fit_laplace_mixed(gpl_model* m) {
             return fail(GPL_ERR_NAN, "NaN in Laplace pseudo-data");
         }
         if (fl[1] & 4) return kRefineFail;  // some V <= 0: the split step runs in the FP64 refit
-        if (fl[0] > 0) {
-            g_minor = fl[0];
-            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
-        }
+        if (fl[0] > 0) return pivot_fail(fl[0]);
         m->logZ = h[0] - 0.5 * h[1] - h[2];
         diff = h[3];
         HIP_TRY(hipMemcpyAsync(m->d(B_FOLD), m->d(B_FMEAN), N * 8, hipMemcpyDeviceToDevice, s));
@@ -1141,10 +1137,7 @@ static int fit_ep(gpl_model* m) {
         HIP_TRY(hipMemcpyAsync(fl, flags, 8, hipMemcpyDeviceToHost, s));
         int r = read_scalars(m, h, 4);
         if (r) return r;
-        if (fl[0] > 0) {
-            g_minor = fl[0];
-            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
-        }
+        if (fl[0] > 0) return pivot_fail(fl[0]);
         m->split = false;
         m->no_posterior = false;
         m->have_xinv = true;
@@ -1264,10 +1257,7 @@ static int fit_ep_mixed(gpl_model* m) {
         double h[4];
         int r = read_scalars(m, h, 4);
         if (r) return r;
-        if (fl[0] > 0) {
-            g_minor = fl[0];
-            return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(fl[0]) + " is not positive");
-        }
+        if (fl[0] > 0) return pivot_fail(fl[0]);
         m->logZ = h[0] - h[2] - 0.5 * h[1];
         diff_new = h[3];
         if (diff_new > diff_old) {

@@ -333,27 +333,24 @@ __device__ __forceinline__ double quad_bcast_f64(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// One 64x64 diagonal block (block k of the N x N matrix A, leading dimension
+// ld) on one 256-thread workgroup: factor in place, Dinv + 4096k = the block's
+// inverse (when want_inv), ldiag[64k + i] = log L_ii, info = first failing row
+// + 1.  Dsrc (optional): the block's FP64 values (64x64 column-major) to factor
+// in place of A's.  Lp, Xs (64 x 65), rinv, piv (64): workgroup LDS;
+// Lp[col][row] = L(row, col) (strictly lower after the factor), Xs[col][row] =
+// (L_kk^-1)(row, col).
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t ld, int64_t k,
-                                                     double* __restrict__ Dinv, double* __restrict__ ldiag,
-                                                     int* __restrict__ info, int want_inv, int64_t sA, int64_t sD,
-                                                     int64_t sL, const double* __restrict__ Dsrc) {
+__device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int64_t k, double* __restrict__ Dinv,
+                                             double* __restrict__ ldiag, int* __restrict__ info, int want_inv,
+                                             const double* __restrict__ Dsrc, double (*Lp)[65], double (*Xs)[65],
+                                             double* rinv, double* piv) {
     // One 64x64 diagonal block per workgroup, FP64 arithmetic.  Wave w holds
     // columns 16w..16w+15 of every row (lane = row).  Factor: four phases; in
     // phase p wave p factors its 16 columns alone (pivots and multipliers are
     // exchanged with readlane, no barrier), publishes them to LDS, and the
     // waves to its right apply the rank-16 update.  Inverse: by 16x16 blocks
     // (below).
-    A += blockIdx.x * sA;
-    Dinv += blockIdx.x * sD;
-    ldiag += blockIdx.x * sL;
-    info += blockIdx.x * ((sA != 0) ? 1 : 0);
-    // Dsrc (optional): the FP64 diagonal tiles (64x64 column-major, one per block
-    // row, same batch stride as Dinv) to factor in place of the stored tile of A
-    if (Dsrc) Dsrc += blockIdx.x * sD + k * 64 * 64;
-    __shared__ double Lp[64][65];  // Lp[col][row] = L(row, col); strictly lower after the factor
-    __shared__ double rinv[64];
-    __shared__ double piv[64];
     T* blk = A + k * 64 * (ld + 1);
     const int t = threadIdx.x, i = t & 63, w = t >> 6;
     double a[16];
@@ -419,7 +416,6 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     // v_mfma_f64_16x16x4.  T is left in the D layout (lane (li, g) holds rows
     // g + 4v of column li), which is the A-operand layout of T^T with k = g + 4v,
     // so the product is taken as Y^T = T^T X_{bi,bi}^T without leaving registers.
-    __shared__ double Xs[64][65];  // Xs[col][row] = (L_kk^-1)(row, col)
     const int lane = t & 63, li = lane & 15, g = lane >> 4;
     {
         const int c = lane >> 2, q = lane & 3, base = 16 * w;
@@ -464,6 +460,182 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     }
     double* D = Dinv + k * 64 * 64;
     for (int e = t; e < 64 * 64; e += 256) D[(e & 63) + (e >> 6) * 64] = Xs[e >> 6][e & 63];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t ld, int64_t k,
+                                                     double* __restrict__ Dinv, double* __restrict__ ldiag,
+                                                     int* __restrict__ info, int want_inv, int64_t sA, int64_t sD,
+                                                     int64_t sL, const double* __restrict__ Dsrc) {
+    __shared__ double Lp[64][65];
+    __shared__ double Xs[64][65];
+    __shared__ double rinv[64];
+    __shared__ double piv[64];
+    A += blockIdx.x * sA;
+    Dinv += blockIdx.x * sD;
+    ldiag += blockIdx.x * sL;
+    info += blockIdx.x * ((sA != 0) ? 1 : 0);
+    // Dsrc (optional): the FP64 diagonal tiles (64x64 column-major, one per block
+    // row, same batch stride as Dinv) to factor in place of the stored tile of A
+    if (Dsrc) Dsrc += blockIdx.x * sD + k * 64 * 64;
+    diag_block64<T>(A, ld, k, Dinv, ldiag, info, want_inv, Dsrc, Lp, Xs, rinv, piv);
+}
+
+// ---------------------------------------------------------------------------
+// One outer panel of the FP64 blocked Cholesky in a single launch: columns
+// [K0, K0 + 64nc), nc <= 4 column blocks, all rows from K0.  Workgroup rb owns
+// the 64-row block rb (rows K0 + 64rb ..) for the whole panel; for each column
+// block c it (1) waits for diagonal block c, (2) solves its block against the
+// block's inverse, L_{rb,c} = A_{rb,c} Dinv_c^T, and (3) subtracts
+// L_{rb,c} L_{c2,c}^T from its blocks in the later panel columns c2 <= rb,
+// reading L_{c2,c} once workgroup c2 has published it.  Workgroup c factors
+// diagonal block c (diag_block64) when its own updates are done.  The chain
+// per column block is then one diagonal factor, one 64x64 solve and one 64x64
+// update inside the workgroups, without the kernel boundaries of the separate
+// factor / panel-solve / panel-update launches.
+//
+// Ordering: each workgroup takes a ticket (atomic counter) as its logical row
+// block, so block rb is held by a workgroup that started after those of every
+// lower block; all waits are on lower blocks, which have started and do not
+// wait on higher ones, so every wait ends.  Flags carry the launch's epoch
+// (release after the block's stores, acquire before its loads, agent scope).
+// A wait that outlasts 2^20 polls (a second or more; far beyond any legitimate wait)
+// stores kPanelWaitTimeout in info and leaves the kernel; the callers report
+// it as a device error.
+//
+// sync: [0, 4) diagonal-block flags, [4, 20) flags of L_{c2,c} (4 c2 + c),
+// [31] the ticket counter; base: tickets handed out before this launch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool panel_wait(unsigned* f, unsigned epoch, int* info, int* s_ok) {
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        unsigned n = 0;
+        // poll with plain agent-scope loads, then one acquire fence (its cache
+        // invalidation once per wait rather than once per poll)
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++n > (1u << 20)) {
+                ok = 0;
+                atomicCAS(info, 0, kPanelWaitTimeout);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// Every wave makes its own stores visible at agent scope (the workgroup barrier
+// alone does not wait for other waves' stores), then one thread sets the flag.
+__device__ __forceinline__ void panel_publish(unsigned* f, unsigned epoch) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(f, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// S[k][r] = M(r, k) for a 64x64 block of a column-major matrix (leading dimension ld)
+__device__ __forceinline__ void blk_load(double (*S)[65], const double* M, int64_t ld) {
+    const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) S[kq + 4 * j][r] = M[r + (int64_t)(kq + 4 * j) * ld];
+}
+
+__device__ __forceinline__ void blk_store(double* M, int64_t ld, double (*S)[65]) {
+    const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) M[r + (int64_t)(kq + 4 * j) * ld] = S[kq + 4 * j][r];
+}
+
+__device__ __forceinline__ void blk_sub(double* M, int64_t ld, double (*S)[65]) {
+    const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) M[r + (int64_t)(kq + 4 * j) * ld] -= S[kq + 4 * j][r];
+}
+
+// C(r, j) = sum_k P(r, k) Q(j, k) with P, Q in LDS as [k][row]; wave w computes
+// the 32x32 quadrant (w / 2, w % 2) as 2 x 2 tiles of v_mfma_f64_16x16x4, then
+// (after a barrier) writes C to S as S[j][r].
+__device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], double (*S)[65]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4, r0 = 32 * (w >> 1), j0 = 32 * (w & 1);
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+        double a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = P[k0 + g][r0 + 16 * i + li];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = Q[k0 + g][j0 + 16 * j + li];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();  // S may be P or Q
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[j0 + 16 * j + li][r0 + 16 * i + g + 4 * v] = acc[i][j][v];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int64_t ld, int64_t K0, int nc,
+                                                     double* __restrict__ Dinv, double* __restrict__ ldiag,
+                                                     int* __restrict__ info, unsigned* __restrict__ sync,
+                                                     unsigned epoch, unsigned base) {
+    __shared__ double Ps[64][65];
+    __shared__ double Qs[64][65];
+    __shared__ double rinv[64];
+    __shared__ double piv[64];
+    __shared__ unsigned s_id;
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) s_id = atomicAdd(&sync[31], 1u) - base;
+    __syncthreads();
+    const int rb = (int)s_id;
+    const int64_t kb0 = K0 / 64;
+    double* Arow = A + K0 + 64 * (int64_t)rb;  // row block rb, column 0
+    for (int c = 0; c < nc && c <= rb; ++c) {
+        const int64_t kc = kb0 + c;
+        if (rb == c) {
+            diag_block64<double>(A, ld, kc, Dinv, ldiag, info, 1, nullptr, Ps, Qs, rinv, piv);
+            panel_publish(&sync[c], epoch);
+            return;
+        }
+        if (!panel_wait(&sync[c], epoch, info, &s_ok)) return;
+        double* Ab = Arow + (K0 + 64 * c) * ld;  // A_{rb,c}
+        blk_load(Ps, Ab, ld);
+        blk_load(Qs, Dinv + kc * 4096, 64);
+        __syncthreads();
+        blk_mm_nt(Ps, Qs, Ps);  // Ps[j][r] = L_{rb,c}(r, j)
+        blk_store(Ab, ld, Ps);
+        if (rb < nc) panel_publish(&sync[4 + 4 * rb + c], epoch);
+        for (int c2 = c + 1; c2 < nc && c2 <= rb; ++c2) {
+            if (c2 == rb) {
+                blk_mm_nt(Ps, Ps, Qs);  // the block's own diagonal block
+            } else {
+                if (!panel_wait(&sync[4 + 4 * c2 + c], epoch, info, &s_ok)) return;
+                blk_load(Qs, A + K0 + 64 * c2 + (K0 + 64 * c) * ld, ld);  // L_{c2,c}
+                __syncthreads();
+                blk_mm_nt(Ps, Qs, Qs);
+            }
+            blk_sub(Arow + (K0 + 64 * c2) * ld, ld, Qs);
+            __syncthreads();
+        }
+    }
+}
+
+void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base) {
+    hipLaunchKernelGGL(k_potrf_panel, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info, sync,
+                       epoch, base);
 }
 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,

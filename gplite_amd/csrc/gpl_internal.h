@@ -140,6 +140,14 @@ void symv_lt_f64(hipStream_t s, const double* A, int64_t N, const double* v, dou
 void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double* sv, const double* t,
                     const double* u);
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
+// info value a panel factorisation stores when one of its device waits timed out
+constexpr int kPanelWaitTimeout = 0x7fffffff;
+// one outer panel (nc <= 4 column blocks from column K0, nrb = (ld - K0) / 64 row
+// blocks) of the FP64 blocked Cholesky in one launch; sync: 32 words of the
+// context (zero at creation), epoch: nonzero and new for every launch, base:
+// tickets handed out before (nrb per launch)
+void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base);
 // X (N x N, ld N): zero above the 64x64 diagonal blocks, the blocks from Dinv
 void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv);
 void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,

@@ -84,6 +84,10 @@ struct gpl_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
+    // flags and ticket counter of the single-launch panel factorisation (32
+    // words, zero at creation), its launch epoch and tickets handed out so far
+    unsigned* psync = nullptr;
+    unsigned pepoch = 0, pbase = 0;
     bool prof = false;
     std::vector<ProfRec> pending;
     std::vector<hipEvent_t> pool;
