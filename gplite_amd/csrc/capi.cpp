@@ -379,36 +379,22 @@ void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t
 // against it runs in FP64 arithmetic; the log-determinant accumulates the FP64
 // diagonal-block pivots.
 // ---------------------------------------------------------------------------
+// Factor the outer panel of columns [K0, K0+W) (all rows >= K0), FP32 storage
+// with the FP64 diagonal tiles Dacc: one launch (k_potrf_panel<float>).
 static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
                           double* ldiag, int* info, double* Dacc) {
-    for (int64_t j0 = K0; j0 < K0 + W; j0 += 64) {
-        const int64_t k = j0 / 64;
-        {
-            Prof p(c, "potrf_diag_mixed", 2.0 * 64 * 64 * 64 / 3, 64.0 * 64 * 12, st);
-            potrf_diag3_f32(st, L, N, k, Dinv, ldiag, info, 1, Dacc);
-        }
-        const int64_t M = N - (j0 + 64);
-        if (M <= 0) continue;
-        float* L21 = L + (j0 + 64) + j0 * N;
-        {
-            Prof p(c, "trsm_panel_f64", 2.0 * M * 64 * 64, 2.0 * M * 64 * 4, st);
-            gemm_mixed_f32f64(st, 0, 1, M, 64, 64, 1.0, L21, N, Dinv + k * 64 * 64, 64, 0.0, L21, N, 0);
-        }
-        const int64_t Wr = K0 + W - (j0 + 64);
-        if (Wr > 0) {
-            {
-                Prof p(c, "panel_update_f32", 2.0 * M * Wr * 64, 4.0 * (2.0 * M * Wr + M * 64), st);
-                gemm_f32(st, 0, 1, M, Wr, 64, -1.f, L21, N, L21, N, 1.f, L + (j0 + 64) * (N + 1), N, 0, 0);
-            }
-            {
-                Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * 64 * (Wr / 64), 0, st);
-                dacc_update(st, Dacc + (k + 1) * 64 * 64, L21, N, Wr / 64, 64, 1, 0, 0);
-            }
-        }
+    const int64_t nrb = (N - K0) / 64;
+    if (++c->pepoch == 0) c->pepoch = 1;
+    {
+        Prof p(c, "potrf_panel_mixed", (double)W * W * W / 3 + 2.0 * (N - K0 - W) * W * W, 4.0 * (N - K0) * W * 3, st);
+        potrf_panel_mixed(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, Dacc);
     }
+    c->pbase += (unsigned)nrb;
 }
 
-// Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build
+// Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build.  The
+// look-ahead as in potrf_blocked: the next panel on the main stream, queued
+// ahead of the rest of the trailing update on the side stream.
 static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc) {
     hipStream_t s = c->stream, s2 = c->side;
     potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info, Dacc);
@@ -427,22 +413,22 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
             dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, 1, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
-        hipStreamWaitEvent(s2, c->ev_a, 0);
-        potrf32_panel(c, s2, L, N, K1, W1, Dinv, ldiag, info, Dacc);
-        hipEventRecord(c->ev_p, s2);
+        potrf32_panel(c, s, L, N, K1, W1, Dinv, ldiag, info, Dacc);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
+            hipStreamWaitEvent(s2, c->ev_a, 0);
             const float* P2 = L + (K1 + W1) + K0 * N;
             {
-                Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W), s);
-                gemm_f32(s, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
+                Prof p(c, "syrk_update_f32", (double)R2 * R2 * W, 4.0 * ((double)R2 * R2 + 2.0 * R2 * W), s2);
+                gemm_f32(s2, 0, 1, R2, R2, W, -1.f, P2, N, P2, N, 1.f, L + (K1 + W1) * (N + 1), N, 1, 0);
             }
             {
-                Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (R2 / 64), 0, s);
-                dacc_update(s, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, 1, 0, 0);
+                Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (R2 / 64), 0, s2);
+                dacc_update(s2, Dacc + ((K1 + W1) / 64) * 64 * 64, P2, N, R2 / 64, W, 1, 0, 0);
             }
+            hipEventRecord(c->ev_p, s2);
+            hipStreamWaitEvent(s, c->ev_p, 0);
         }
-        hipStreamWaitEvent(s, c->ev_p, 0);
     }
 }
 

@@ -536,22 +536,35 @@ __device__ __forceinline__ void panel_publish(unsigned* f, unsigned epoch) {
 }
 
 // S[k][r] = M(r, k) for a 64x64 block of a column-major matrix (leading dimension ld)
-__device__ __forceinline__ void blk_load(double (*S)[65], const double* M, int64_t ld) {
+template <typename T>
+__device__ __forceinline__ void blk_load(double (*S)[65], const T* M, int64_t ld) {
     const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) S[kq + 4 * j][r] = M[r + (int64_t)(kq + 4 * j) * ld];
+    for (int j = 0; j < 16; ++j) S[kq + 4 * j][r] = (double)M[r + (int64_t)(kq + 4 * j) * ld];
 }
 
-__device__ __forceinline__ void blk_store(double* M, int64_t ld, double (*S)[65]) {
+// M(r, k) = S[k][r] rounded to T; S keeps the rounded value (each thread
+// rewrites only the elements it stores)
+template <typename T>
+__device__ __forceinline__ void blk_store(T* M, int64_t ld, double (*S)[65]) {
     const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) M[r + (int64_t)(kq + 4 * j) * ld] = S[kq + 4 * j][r];
+    for (int j = 0; j < 16; ++j) {
+        const T v = (T)S[kq + 4 * j][r];
+        M[r + (int64_t)(kq + 4 * j) * ld] = v;
+        S[kq + 4 * j][r] = (double)v;
+    }
 }
 
-__device__ __forceinline__ void blk_sub(double* M, int64_t ld, double (*S)[65]) {
+// M(r, k) = M(r, k) - S[k][r], formed in FP64 and rounded once to T
+template <typename T>
+__device__ __forceinline__ void blk_sub(T* M, int64_t ld, double (*S)[65]) {
     const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) M[r + (int64_t)(kq + 4 * j) * ld] -= S[kq + 4 * j][r];
+    for (int j = 0; j < 16; ++j) {
+        T* q = M + r + (int64_t)(kq + 4 * j) * ld;
+        *q = (T)((double)*q - S[kq + 4 * j][r]);
+    }
 }
 
 // C(r, j) = sum_k P(r, k) Q(j, k) with P, Q in LDS as [k][row]; wave w computes
@@ -587,10 +600,17 @@ __device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], doub
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int64_t ld, int64_t K0, int nc,
+//
+// T = float (the mixed-precision form): A is FP32; Dacc holds FP64 copies of
+// the diagonal tiles (64x64 column-major, tile j at Dacc + 4096j), which the
+// diagonal factor reads instead of A's tiles and which receive the updates of
+// the diagonal blocks.  Block products are FP64 MFMA on the stored FP32 values
+// (exact products, FP64 sums) with one rounding to FP32 per stored result.
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t ld, int64_t K0, int nc,
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, unsigned* __restrict__ sync,
-                                                     unsigned epoch, unsigned base) {
+                                                     unsigned epoch, unsigned base, double* __restrict__ Dacc) {
     __shared__ double Ps[64][65];
     __shared__ double Qs[64][65];
     __shared__ double rinv[64];
@@ -601,21 +621,22 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
     __syncthreads();
     const int rb = (int)s_id;
     const int64_t kb0 = K0 / 64;
-    double* Arow = A + K0 + 64 * (int64_t)rb;  // row block rb, column 0
+    T* Arow = A + K0 + 64 * (int64_t)rb;  // row block rb, column 0
     for (int c = 0; c < nc && c <= rb; ++c) {
         const int64_t kc = kb0 + c;
         if (rb == c) {
-            diag_block64<double>(A, ld, kc, Dinv, ldiag, info, 1, nullptr, Ps, Qs, rinv, piv);
+            diag_block64<T>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv, piv);
             panel_publish(&sync[c], epoch);
             return;
         }
         if (!panel_wait(&sync[c], epoch, info, &s_ok)) return;
-        double* Ab = Arow + (K0 + 64 * c) * ld;  // A_{rb,c}
+        T* Ab = Arow + (K0 + 64 * c) * ld;  // A_{rb,c}
         blk_load(Ps, Ab, ld);
         blk_load(Qs, Dinv + kc * 4096, 64);
         __syncthreads();
         blk_mm_nt(Ps, Qs, Ps);  // Ps[j][r] = L_{rb,c}(r, j)
-        blk_store(Ab, ld, Ps);
+        blk_store(Ab, ld, Ps);  // (Ps now holds the stored values)
+        __syncthreads();
         if (rb < nc) panel_publish(&sync[4 + 4 * rb + c], epoch);
         for (int c2 = c + 1; c2 < nc && c2 <= rb; ++c2) {
             if (c2 == rb) {
@@ -626,7 +647,10 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
                 __syncthreads();
                 blk_mm_nt(Ps, Qs, Qs);
             }
-            blk_sub(Arow + (K0 + 64 * c2) * ld, ld, Qs);
+            if (Dacc && c2 == rb)
+                blk_sub(Dacc + (kb0 + c2) * 4096, 64, Qs);  // the FP64 copy the diagonal factor reads
+            else
+                blk_sub(Arow + (K0 + 64 * c2) * ld, ld, Qs);
             __syncthreads();
         }
     }
@@ -634,8 +658,14 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
 
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                      double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base) {
-    hipLaunchKernelGGL(k_potrf_panel, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info, sync,
-                       epoch, base);
+    hipLaunchKernelGGL(k_potrf_panel<double>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
+                       sync, epoch, base, (double*)nullptr);
+}
+
+void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
+                       double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc) {
+    hipLaunchKernelGGL(k_potrf_panel<float>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
+                       sync, epoch, base, Dacc);
 }
 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,
