@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -65,6 +66,7 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void**)&c->psync, 32 * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->psync, 0, 32 * sizeof(unsigned));
+    if (const char* pw = getenv("GPL_PANEL_WAIT_POLLS")) c->pwait = (unsigned)strtoul(pw, nullptr, 10);
     if (e != hipSuccess) {
         delete c;
         return fail(GPL_ERR_HIP, hipGetErrorString(e));
@@ -248,7 +250,7 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
     if (++c->pepoch == 0) c->pepoch = 1;
     {
         Prof p(c, "potrf_panel", (double)W * W * W / 3 + 2.0 * (N - K0 - W) * W * W, 8.0 * (N - K0) * W * 3, st);
-        potrf_panel_f64(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase);
+        potrf_panel_f64(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, c->pwait);
     }
     c->pbase += (unsigned)nrb;
 }
@@ -387,7 +389,8 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
     if (++c->pepoch == 0) c->pepoch = 1;
     {
         Prof p(c, "potrf_panel_mixed", (double)W * W * W / 3 + 2.0 * (N - K0 - W) * W * W, 4.0 * (N - K0) * W * 3, st);
-        potrf_panel_mixed(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, Dacc);
+        potrf_panel_mixed(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, Dacc,
+                          c->pwait);
     }
     c->pbase += (unsigned)nrb;
 }

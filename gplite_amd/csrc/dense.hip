@@ -499,26 +499,26 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 // lower block; all waits are on lower blocks, which have started and do not
 // wait on higher ones, so every wait ends.  Flags carry the launch's epoch
 // (release after the block's stores, acquire before its loads, agent scope).
-// A wait that outlasts 2^20 polls (a second or more; far beyond any legitimate wait)
-// stores kPanelWaitTimeout in info and leaves the kernel; the callers report
-// it as a device error.
+// A wait that outlasts maxpoll polls (2^20 by default: a second or more, far
+// beyond any legitimate wait) stores kPanelWaitTimeout in info and leaves the
+// kernel; the callers report it as a device error.
 //
 // sync: [0, 4) diagonal-block flags, [4, 20) flags of L_{c2,c} (4 c2 + c),
 // [31] the ticket counter; base: tickets handed out before this launch.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool panel_wait(unsigned* f, unsigned epoch, int* info, int* s_ok) {
+__device__ __forceinline__ bool panel_wait(unsigned* f, unsigned epoch, int* info, int* s_ok, unsigned maxpoll) {
     if (threadIdx.x == 0) {
         int ok = 1;
         unsigned n = 0;
         // poll with plain agent-scope loads, then one acquire fence (its cache
         // invalidation once per wait rather than once per poll)
         while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++n > (1u << 20)) {
+            if (n++ >= maxpoll) {
                 ok = 0;
                 atomicCAS(info, 0, kPanelWaitTimeout);
                 break;
             }
+            __builtin_amdgcn_s_sleep(2);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_ok = ok;
@@ -610,7 +610,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t ld, int64_t K0, int nc,
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, unsigned* __restrict__ sync,
-                                                     unsigned epoch, unsigned base, double* __restrict__ Dacc) {
+                                                     unsigned epoch, unsigned base, double* __restrict__ Dacc,
+                                                     unsigned maxpoll) {
     __shared__ double Ps[64][65];
     __shared__ double Qs[64][65];
     __shared__ double rinv[64];
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
             panel_publish(&sync[c], epoch);
             return;
         }
-        if (!panel_wait(&sync[c], epoch, info, &s_ok)) return;
+        if (!panel_wait(&sync[c], epoch, info, &s_ok, maxpoll)) return;
         T* Ab = Arow + (K0 + 64 * c) * ld;  // A_{rb,c}
         blk_load(Ps, Ab, ld);
         blk_load(Qs, Dinv + kc * 4096, 64);
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
             if (c2 == rb) {
                 blk_mm_nt(Ps, Ps, Qs);  // the block's own diagonal block
             } else {
-                if (!panel_wait(&sync[4 + 4 * c2 + c], epoch, info, &s_ok)) return;
+                if (!panel_wait(&sync[4 + 4 * c2 + c], epoch, info, &s_ok, maxpoll)) return;
                 blk_load(Qs, A + K0 + 64 * c2 + (K0 + 64 * c) * ld, ld);  // L_{c2,c}
                 __syncthreads();
                 blk_mm_nt(Ps, Qs, Qs);
@@ -657,15 +658,16 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
 }
 
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base) {
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll) {
     hipLaunchKernelGGL(k_potrf_panel<double>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
-                       sync, epoch, base, (double*)nullptr);
+                       sync, epoch, base, (double*)nullptr, maxpoll);
 }
 
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                       double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc) {
+                       double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,
+                       unsigned maxpoll) {
     hipLaunchKernelGGL(k_potrf_panel<float>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
-                       sync, epoch, base, Dacc);
+                       sync, epoch, base, Dacc, maxpoll);
 }
 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,

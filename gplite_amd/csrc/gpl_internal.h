@@ -145,12 +145,14 @@ constexpr int kPanelWaitTimeout = 0x7fffffff;
 // one outer panel (nc <= 4 column blocks from column K0, nrb = (ld - K0) / 64 row
 // blocks) of the FP64 blocked Cholesky in one launch; sync: 32 words of the
 // context (zero at creation), epoch: nonzero and new for every launch, base:
-// tickets handed out before (nrb per launch)
+// tickets handed out before (nrb per launch), maxpoll: polls before a wait
+// between workgroups gives up (kPanelWaitTimeout in info)
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base);
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll);
 // the same on FP32 storage with the FP64 diagonal tiles Dacc (mixed precision)
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                       double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc);
+                       double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,
+                       unsigned maxpoll);
 // X (N x N, ld N): zero above the 64x64 diagonal blocks, the blocks from Dinv
 void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv);
 void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,

@@ -85,9 +85,13 @@ struct gpl_ctx {
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
     // flags and ticket counter of the single-launch panel factorisation (32
-    // words, zero at creation), its launch epoch and tickets handed out so far
+    // words, zero at creation), its launch epoch and tickets handed out so far,
+    // and the polls after which one of its waits gives up (2^20; the
+    // environment variable GPL_PANEL_WAIT_POLLS overrides it, which tests use to
+    // reach the timeout path)
     unsigned* psync = nullptr;
     unsigned pepoch = 0, pbase = 0;
+    unsigned pwait = 1u << 20;
     bool prof = false;
     std::vector<ProfRec> pending;
     std::vector<hipEvent_t> pool;
