@@ -171,7 +171,7 @@ static void trtri_doubling_batched(gpl_ctx* c, hipStream_t s, Bucket& k) {
     float* T = k.f(BB_BI);
     {
         Prof p(c, "batch_trtri_leaves", 0, 0, s);
-        trtri_leaves_f32_batched(s, X, N, k.d(BB_DINV), nb, N2, N * 64);
+        trtri_leaves_f32_batched(s, X, N, k.d(BB_DINV), nb, N2, N * 64, 0, N);
     }
     for (int64_t b = 64; b < N; b *= 2) {
         const int64_t full = N / (2 * b);

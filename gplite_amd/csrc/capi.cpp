@@ -62,6 +62,9 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_h, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_t, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void**)&c->psync, 32 * sizeof(unsigned));
@@ -84,8 +87,12 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     for (auto p : c->scratch)
         if (p) hipFree(p);
     hipStreamSynchronize(c->side);
+    hipStreamSynchronize(c->aux);
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_p);
+    hipEventDestroy(c->ev_h);
+    hipEventDestroy(c->ev_t);
+    hipStreamDestroy(c->aux);
     if (c->psync) hipFree(c->psync);
     hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
@@ -242,6 +249,129 @@ int gpl::prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, D
 // rest of the outer panel); the trailing matrix then receives one rank-NB update.
 static const int kOuterNB = 256;
 
+// X = L^-1 by recursive doubling from the 64x64 diagonal-block inverses Dinv.
+// At level b (b = 64, 128, ...) the b-blocks pair up: [[X11, 0], [L21, X22]]
+// with X11 and X22 already inverted has the inverse [[X11, 0], [X21, X22]],
+// X21 = -X22 (L21 X11).  The pairs of one level are independent, so each level
+// is two batched GEMMs (plus one more pair when the last is shorter): with
+// N/64 diagonal blocks the dependent chain is 2 log2(N/64) launches, against
+// two launches per 64 rows for forward substitution.  Both triangular operands
+// skip their zero tiles (tile modes 4 and 3).  T (N x N, ld N) holds the
+// products L21 X11 at the positions of X21.  FP64 (T = double) or FP32 storage
+// (float: FP32 MFMA with sums folded into FP64 every 64 k, FP64 leaves rounded).
+//
+// The work splits at the top level bt = doubling_top(N): the inverse of the
+// diagonal block [0, bt) and T = L21 X11 below it (dbl_front) need only the
+// factor's first bt columns, so the blocked Cholesky can run them on the
+// context's third stream beside its remaining panels; the inverse of [bt, N)
+// and X21 = -X22 T (dbl_back) follow the factorization.
+static void dbl_leaves(hipStream_t s, double* X, int64_t N, const double* Dinv, int64_t c0, int64_t n) {
+    trtri_leaves_f64(s, X, N, Dinv, c0, n);
+}
+static void dbl_leaves(hipStream_t s, float* X, int64_t N, const double* Dinv, int64_t c0, int64_t n) {
+    trtri_leaves_f32_batched(s, X, N, Dinv, 1, 0, 0, c0, n);
+}
+static void dbl_gemm(hipStream_t s, int64_t M, int64_t Nn, int64_t K, double alpha, const double* A, const double* B,
+                     double* C, int64_t ld, int tiles, int64_t batch, int64_t step) {
+    if (batch == 1) gemm2_f64(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, 0.0, C, ld, tiles, 0);
+    else gemm2_f64_batched(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, 0.0, C, ld, tiles, (int)batch, step, step, step);
+}
+static void dbl_gemm(hipStream_t s, int64_t M, int64_t Nn, int64_t K, double alpha, const float* A, const float* B,
+                     float* C, int64_t ld, int tiles, int64_t batch, int64_t step) {
+    gemm_f32_batched(s, 0, 0, M, Nn, K, (float)alpha, A, ld, B, ld, 0.f, C, ld, tiles, 0, (int)batch, step, step, step,
+                     kFoldInverse);
+}
+
+// the leaves (columns r0 .. r0 + n) and the levels inside the diagonal block
+// [r0, r0 + n); r0 is 0 or bt, a multiple of 2b for every level run
+template <typename T>
+static void dbl_levels(gpl_ctx* c, hipStream_t s, const T* L, const double* Dinv, int64_t N, T* X, T* Tm, int64_t r0,
+                       int64_t n) {
+    const bool f32 = sizeof(T) == 4;
+    {
+        Prof p(c, f32 ? "trtri_leaves_f32" : "trtri_leaves", 0, sizeof(T) * (double)N * n / 2, s);
+        dbl_leaves(s, X, N, Dinv, r0, n);
+    }
+    const int64_t o0 = r0 * (N + 1);
+    L += o0;
+    X += o0;
+    Tm += o0;
+    for (int64_t b = 64; b < n; b *= 2) {
+        const int64_t full = n / (2 * b);  // pairs whose right block has b rows
+        const int64_t r = std::min<int64_t>(b, n - full * 2 * b - b);  // rows of a last, shorter right block
+        const int64_t step = 2 * b * (N + 1);
+        Prof p(c, f32 ? "trtri_level_f32" : "trtri_level",
+               2.0 * (double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) : 0.0), 0, s);
+        if (full > 0) {
+            dbl_gemm(s, b, b, b, 1.0, L + b, X, Tm + b, N, 4, full, step);
+            dbl_gemm(s, b, b, b, -1.0, X + b * (N + 1), Tm + b, X + b, N, 3, full, step);
+        }
+        if (r > 0) {
+            const int64_t o = full * step;  // the last pair's top-left corner
+            dbl_gemm(s, r, b, b, 1.0, L + o + b, X + o, Tm + o + b, N, 4, 1, 0);
+            dbl_gemm(s, r, b, r, -1.0, X + o + b * (N + 1), Tm + o + b, X + o + b, N, 3, 1, 0);
+        }
+    }
+}
+
+template <typename T>
+static void dbl_front(gpl_ctx* c, hipStream_t s, const T* L, const double* Dinv, int64_t N, T* X, T* Tm, int64_t bt) {
+    dbl_levels(c, s, L, Dinv, N, X, Tm, 0, bt);
+    Prof p(c, sizeof(T) == 4 ? "trtri_level_f32" : "trtri_level", (double)(N - bt) * bt * bt, 0, s);
+    dbl_gemm(s, N - bt, bt, bt, 1.0, L + bt, X, Tm + bt, N, 4, 1, 0);
+}
+
+template <typename T>
+static void dbl_back(gpl_ctx* c, hipStream_t s, const T* L, const double* Dinv, int64_t N, T* X, T* Tm, int64_t bt) {
+    dbl_levels(c, s, L, Dinv, N, X, Tm, bt, N - bt);
+    Prof p(c, sizeof(T) == 4 ? "trtri_level_f32" : "trtri_level", (double)(N - bt) * (N - bt) * bt, 0, s);
+    dbl_gemm(s, N - bt, bt, N - bt, -1.0, X + bt * (N + 1), Tm + bt, X + bt, N, 3, 1, 0);
+}
+
+template <typename T>
+static void trtri_doubling(gpl_ctx* c, hipStream_t s, const T* L, const double* Dinv, int64_t N, T* X, T* Tm) {
+    const int64_t bt = doubling_top(N);
+    if (bt == 0) {
+        dbl_levels(c, s, L, Dinv, N, X, Tm, 0, N);  // one 64-block: the leaf alone
+        return;
+    }
+    dbl_front(c, s, L, Dinv, N, X, Tm, bt);
+    dbl_back(c, s, L, Dinv, N, X, Tm, bt);
+}
+
+// The blocked Cholesky's fork and join of the front half of L^-1 (X, Tm given):
+// the front runs on the context's third stream once the panel holding column
+// bt - 1 is factored on the main stream; the back half follows the factor.
+template <typename T>
+struct InverseFork {
+    gpl_ctx* c;
+    const T* L;
+    const double* Dinv;
+    int64_t N, bt;
+    T *X, *Tm;
+    bool forked = false;
+    InverseFork(gpl_ctx* c_, const T* L_, const double* Dinv_, int64_t N_, T* X_, T* Tm_)
+        : c(c_), L(L_), Dinv(Dinv_), N(N_), bt(X_ ? doubling_top(N_) : 0), X(X_), Tm(Tm_) {}
+    // after the panel ending at column `done` was queued on the main stream
+    void after_panel(int64_t done) {
+        if (bt == 0 || forked || done < bt) return;
+        hipEventRecord(c->ev_h, c->stream);
+        hipStreamWaitEvent(c->aux, c->ev_h, 0);
+        dbl_front(c, c->aux, L, Dinv, N, X, Tm, bt);
+        hipEventRecord(c->ev_t, c->aux);
+        forked = true;
+    }
+    void finish() {
+        if (!X) return;
+        if (bt == 0) {
+            dbl_levels(c, c->stream, L, Dinv, N, X, Tm, 0, N);
+            return;
+        }
+        hipStreamWaitEvent(c->stream, c->ev_t, 0);
+        dbl_back(c, c->stream, L, Dinv, N, X, Tm, bt);
+    }
+};
+
 // Factor the outer panel of columns [K0, K0+W) (all rows >= K0) on stream st:
 // one launch, a workgroup per 64-row block (k_potrf_panel).
 static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
@@ -260,10 +390,15 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
 // panel (one launch) while the side stream applies the update to the rest of
 // the trailing matrix.  The panel launch is queued first: its workgroups (one
 // per 64-row block, each needing half a CU's registers) are placed before the
-// update's GEMM fills the CUs, instead of waiting for it to drain.
-void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info) {
+// update's GEMM fills the CUs, instead of waiting for it to drain.  With X and
+// Tm (N x N scratch) it also forms X = L^-1 (recursive doubling, its front half
+// beside the remaining panels).
+void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info, double* X,
+                        double* Tm) {
     hipStream_t s = c->stream, s2 = c->side;
+    InverseFork<double> inv(c, L, Dinv, N, X, Tm);
     potrf_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info);
+    inv.after_panel(std::min<int64_t>(kOuterNB, N));
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
         const int64_t K1 = K0 + W, R = N - K1;
@@ -276,6 +411,7 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
         }
         hipEventRecord(c->ev_a, s);
         potrf_panel(c, s, L, N, K1, W1, Dinv, ldiag, info);
+        inv.after_panel(K1 + W1);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             hipStreamWaitEvent(s2, c->ev_a, 0);
@@ -288,6 +424,7 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
             hipStreamWaitEvent(s, c->ev_p, 0);
         }
     }
+    inv.finish();
 }
 
 // Rows [I0, I0+H) of X = L^-1: 64-row steps inside the panel, on stream st.
@@ -307,46 +444,13 @@ static void trtri_panel(gpl_ctx* c, hipStream_t st, const double* L, const doubl
     }
 }
 
-// X = L^-1 by recursive doubling from the 64x64 diagonal-block inverses Dinv.
-// At level b (b = 64, 128, ...) the b-blocks pair up: [[X11, 0], [L21, X22]]
-// with X11 and X22 already inverted has the inverse [[X11, 0], [X21, X22]],
-// X21 = -X22 (L21 X11).  The pairs of one level are independent, so each level
-// is two batched GEMMs (plus one more pair when the last is shorter): with
-// N/64 diagonal blocks the dependent chain is 2 log2(N/64) launches, against
-// two launches per 64 rows for forward substitution.  Both triangular operands
-// skip their zero tiles (tile modes 4 and 3).  T (N x N, ld N) holds the
-// products L21 X11 at the positions of X21.
-static void trtri_doubling(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T) {
-    hipStream_t s = c->stream;
-    {
-        Prof p(c, "trtri_leaves", 0, 8.0 * N * N / 2);
-        trtri_leaves_f64(s, X, N, Dinv);
-    }
-    for (int64_t b = 64; b < N; b *= 2) {
-        const int64_t full = N / (2 * b);  // pairs whose right block has b rows
-        const int64_t r = std::min<int64_t>(b, N - full * 2 * b - b);  // rows of a last, shorter right block
-        const int64_t step = 2 * b * (N + 1);
-        Prof p(c, "trtri_level", 2.0 * (double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) : 0.0), 0);
-        if (full > 0) {
-            gemm2_f64_batched(s, 0, 0, b, b, b, 1.0, L + b, N, X, N, 0.0, T + b, N, 4, (int)full, step, step, step);
-            gemm2_f64_batched(s, 0, 0, b, b, b, -1.0, X + b * (N + 1), N, T + b, N, 0.0, X + b, N, 3, (int)full,
-                              step, step, step);
-        }
-        if (r > 0) {
-            const int64_t o = full * step;  // the last pair's top-left corner
-            gemm2_f64(s, 0, 0, r, b, b, 1.0, L + o + b, N, X + o, N, 0.0, T + o + b, N, 4, 0);
-            gemm2_f64(s, 0, 0, r, b, r, -1.0, X + o + b * (N + 1), N, T + o + b, N, 0.0, X + o + b, N, 3, 0);
-        }
-    }
-}
-
 // X = L^-1 by blocked forward substitution against the identity, two levels with
 // look-ahead: the next row panel is updated first and solved on the side stream
 // while the main stream updates the remaining rows.  With scratch T (N x N) the
 // recursive doubling above runs instead.
 void gpl::trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T) {
     if (T) {
-        trtri_doubling(c, L, Dinv, N, X, T);
+        trtri_doubling(c, c->stream, L, Dinv, N, X, T);
         return;
     }
     hipStream_t s = c->stream, s2 = c->side;
@@ -398,9 +502,12 @@ static void potrf32_panel(gpl_ctx* c, hipStream_t st, float* L, int64_t N, int64
 // Dacc: FP64 copies of the diagonal tiles (N x 64), filled by the build.  The
 // look-ahead as in potrf_blocked: the next panel on the main stream, queued
 // ahead of the rest of the trailing update on the side stream.
-static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc) {
+static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, double* ldiag, int* info, double* Dacc,
+                            float* X = nullptr, float* Tm = nullptr) {
     hipStream_t s = c->stream, s2 = c->side;
+    InverseFork<float> inv(c, L, Dinv, N, X, Tm);
     potrf32_panel(c, s, L, N, 0, std::min<int64_t>(kOuterNB, N), Dinv, ldiag, info, Dacc);
+    inv.after_panel(std::min<int64_t>(kOuterNB, N));
     for (int64_t K0 = 0; K0 < N; K0 += kOuterNB) {
         const int64_t W = std::min<int64_t>(kOuterNB, N - K0);
         const int64_t K1 = K0 + W, R = N - K1;
@@ -417,6 +524,7 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         }
         hipEventRecord(c->ev_a, s);
         potrf32_panel(c, s, L, N, K1, W1, Dinv, ldiag, info, Dacc);
+        inv.after_panel(K1 + W1);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
             hipStreamWaitEvent(s2, c->ev_a, 0);
@@ -433,86 +541,7 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
             hipStreamWaitEvent(s, c->ev_p, 0);
         }
     }
-}
-
-static void trtri32_panel(gpl_ctx* c, hipStream_t st, const float* L, const double* Dinv, int64_t N, int64_t I0,
-                          int64_t H, float* X) {
-    for (int64_t r0 = I0; r0 < I0 + H; r0 += 64) {
-        const int64_t i = r0 / 64, Wc = r0 + 64;
-        {
-            Prof p(c, "trtri_diag_f64", 2.0 * 64 * 64 * Wc, 8.0 * 64 * Wc, st);
-            gemm_mixed_f64f32(st, 0, 0, 64, Wc, 64, 1.0, Dinv + i * 64 * 64, 64, X + r0, N, 0.0, X + r0, N, 0);
-        }
-        const int64_t Mr = I0 + H - (r0 + 64);
-        if (Mr > 0) {
-            Prof p(c, "trtri_panel_f32", 2.0 * Mr * Wc * 64, 8.0 * Mr * Wc, st);
-            gemm_f32(st, 0, 0, Mr, Wc, 64, -1.f, L + (r0 + 64) + r0 * N, N, X + r0, N, 1.f, X + r0 + 64, N, 0, 0,
-                     kFoldInverse);
-        }
-    }
-}
-
-// X = L^-1 in FP32 by the recursive doubling of trtri_doubling: the leaves are
-// the FP64 diagonal-block inverses rounded to FP32, the products FP32 MFMA with
-// their sums folded into FP64 every 64 k (the accuracy of the forward
-// substitution's updates).  T: N x N FP32 scratch.
-static void trtri32_doubling(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X, float* T) {
-    hipStream_t s = c->stream;
-    {
-        Prof p(c, "trtri_leaves_f32", 0, 4.0 * N * N / 2);
-        trtri_leaves_f32_batched(s, X, N, Dinv, 1, 0, 0);
-    }
-    for (int64_t b = 64; b < N; b *= 2) {
-        const int64_t full = N / (2 * b);
-        const int64_t r = std::min<int64_t>(b, N - full * 2 * b - b);
-        const int64_t step = 2 * b * (N + 1);
-        Prof p(c, "trtri_level_f32", 2.0 * (double)b * b * b * full + (r > 0 ? (double)r * b * (b + r) : 0.0), 0);
-        if (full > 0) {
-            gemm_f32_batched(s, 0, 0, b, b, b, 1.f, L + b, N, X, N, 0.f, T + b, N, 4, 0, (int)full, step, step, step,
-                             kFoldInverse);
-            gemm_f32_batched(s, 0, 0, b, b, b, -1.f, X + b * (N + 1), N, T + b, N, 0.f, X + b, N, 3, 0, (int)full,
-                             step, step, step, kFoldInverse);
-        }
-        if (r > 0) {
-            const int64_t o = full * step;
-            gemm_f32(s, 0, 0, r, b, b, 1.f, L + o + b, N, X + o, N, 0.f, T + o + b, N, 4, 0, kFoldInverse);
-            gemm_f32(s, 0, 0, r, b, r, -1.f, X + o + b * (N + 1), N, T + o + b, N, 0.f, X + o + b, N, 3, 0,
-                     kFoldInverse);
-        }
-    }
-}
-
-// With scratch T (N x N FP32) the recursive doubling above runs instead.
-static void trtri32_blocked(gpl_ctx* c, const float* L, const double* Dinv, int64_t N, float* X, float* T = nullptr) {
-    if (T) {
-        trtri32_doubling(c, L, Dinv, N, X, T);
-        return;
-    }
-    hipStream_t s = c->stream, s2 = c->side;
-    set_identity_f32(s, X, N, N);
-    trtri32_panel(c, s, L, Dinv, N, 0, std::min<int64_t>(kOuterNB, N), X);
-    for (int64_t I0 = 0; I0 < N; I0 += kOuterNB) {
-        const int64_t H = std::min<int64_t>(kOuterNB, N - I0);
-        const int64_t I1 = I0 + H, R = N - I1;
-        if (R <= 0) break;
-        const int64_t H1 = std::min<int64_t>(kOuterNB, R);
-        const int64_t Wc = I0 + H;
-        {
-            Prof p(c, "trtri_update_f32", 2.0 * H1 * Wc * H, 4.0 * (2.0 * H1 * Wc + Wc * H), s);
-            gemm_f32(s, 0, 0, H1, Wc, H, -1.f, L + I1 + I0 * N, N, X + I0, N, 1.f, X + I1, N, 0, 0, kFoldInverse);
-        }
-        hipEventRecord(c->ev_a, s);
-        hipStreamWaitEvent(s2, c->ev_a, 0);
-        trtri32_panel(c, s2, L, Dinv, N, I1, H1, X);
-        hipEventRecord(c->ev_p, s2);
-        const int64_t R2 = R - H1;
-        if (R2 > 0) {
-            Prof p(c, "trtri_update_f32", 2.0 * R2 * Wc * H, 4.0 * (2.0 * R2 * Wc + Wc * H), s);
-            gemm_f32(s, 0, 0, R2, Wc, H, -1.f, L + (I1 + H1) + I0 * N, N, X + I0, N, 1.f, X + I1 + H1, N, 0, 0,
-                     kFoldInverse);
-        }
-        hipStreamWaitEvent(s, c->ev_p, 0);
-    }
+    inv.finish();
 }
 
 // alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)
@@ -605,13 +634,17 @@ static int fit_gaussian(gpl_model* m) {
     }
     lik_pseudodata_la(s, GPL_LIK_GAUSSIAN, 0, sigma, n, nullptr, m->d(B_Y), nullptr, nullptr, m->d(B_Z), m->d(B_V),
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
-    potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>());
-    if (m->opts.flags & GPL_FLAG_INVERSE) {
-        // alpha = S X^T X S z with X = L^-1 (S = 1/sigma); B^-1's buffer (which
-        // the gradient fills next) is the inverse's scratch
+    const bool inv = m->opts.flags & GPL_FLAG_INVERSE;
+    if (inv) {
+        // X = L^-1 with the factor; B^-1's buffer (which the gradient fills next)
+        // is the inverse's scratch
         ENSURE(m, B_XINV, (size_t)N * N * 8);
         ENSURE(m, B_BINV, (size_t)N * N * 8);
-        trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV), m->d(B_BINV));
+    }
+    potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), inv ? m->d(B_XINV) : nullptr,
+                  inv ? m->d(B_BINV) : nullptr);
+    if (inv) {
+        // alpha = S X^T X S z (S = 1/sigma)
         m->have_xinv = true;
         Prof p(c, "trmv", 4.0 * N * N / 2, 8.0 * N * N);
         fill_f64(s, m->d(B_W1), N, 0.0);
@@ -686,8 +719,8 @@ static int fit_gaussian_mixed(gpl_model* m) {
                       m->d(B_SV), m->b[B_INFO].as<int>() + 1);
     // B^-1's buffer (filled below) is the inverse's scratch first
     ENSURE(m, B_BINV, (size_t)N * N * 4);
-    potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), m->d(B_DACC));
-    trtri32_blocked(c, L, m->d(B_DINV), N, X, m->b[B_BINV].as<float>());
+    potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), m->d(B_DACC), X,
+                    m->b[B_BINV].as<float>());
     m->have_xinv = true;
     // B^-1 = X^T X (kept for the gradient) and the first-order log|B| correction
     // for having factored fl32(B) rather than B
@@ -938,8 +971,7 @@ static int fit_laplace_mixed(gpl_model* m) {
             Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
             form_B_mixed(s, m->d(B_K), B64, L, m->d(B_DACC), N, n, N, m->d(B_SV));
         }
-        potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
-        trtri32_blocked(c, L, m->d(B_DINV), N, X, m->b[B_BINV].as<float>());
+        potrf32_blocked(c, L, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC), X, m->b[B_BINV].as<float>());
         int rr0 = refine_alpha_mixed(m, maxref, nch);
         if (rr0) return rr0;
         {
@@ -1224,8 +1256,7 @@ static int fit_ep_mixed(gpl_model* m) {
             Prof p(c, "form_B_mixed", 0, 20.0 * N * (N + 64) / 2);
             form_B_mixed(s, m->d(B_K), m->d(B_B64), L32, m->d(B_DACC), N, n, N, m->d(B_SV));
         }
-        potrf32_blocked(c, L32, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC));
-        trtri32_blocked(c, L32, m->d(B_DINV), N, X32, m->b[B_BINV].as<float>());
+        potrf32_blocked(c, L32, N, m->d(B_DINV), m->d(B_LDIAG), flags, m->d(B_DACC), X32, m->b[B_BINV].as<float>());
         {
             int rr0 = refine_alpha_mixed(m, maxref, nch);
             if (rr0) return rr0;

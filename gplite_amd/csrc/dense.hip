@@ -1171,18 +1171,19 @@ void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N) {
     hipLaunchKernelGGL(k_set_identity, dim3((unsigned)(T * T)), dim3(256), 0, s, A, ld, T);
 }
 
-// Column j of X (N x N, ld N): zero above its 64x64 diagonal block, the
-// block's column from Dinv (64x64 column-major per block, zero above its
-// diagonal) on it; the rows below are left to the caller.
+// Column j = c0 + blockIdx.x of X (N x N, ld N): zero above its 64x64
+// diagonal block, the block's column from Dinv (64x64 column-major per block,
+// zero above its diagonal) on it; the rows below are left to the caller.
 __global__ __launch_bounds__(256) void k_trtri_leaves(double* __restrict__ X, int64_t N,
-                                                      const double* __restrict__ Dinv) {
-    const int64_t j = blockIdx.x, kb = j / 64, r0 = kb * 64;
+                                                      const double* __restrict__ Dinv, int64_t c0) {
+    const int64_t j = c0 + blockIdx.x, kb = j / 64, r0 = kb * 64;
     const double* D = Dinv + kb * 4096 + (j - r0) * 64;
     for (int64_t i = threadIdx.x; i < r0 + 64; i += 256) X[i + j * N] = (i < r0) ? 0.0 : D[i - r0];
 }
 
-void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv) {
-    hipLaunchKernelGGL(k_trtri_leaves, dim3((unsigned)N), dim3(256), 0, s, X, N, Dinv);
+void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv, int64_t c0, int64_t ncols) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_trtri_leaves, dim3((unsigned)ncols), dim3(256), 0, s, X, N, Dinv, c0);
 }
 
 __global__ void k_fill(double* p, int64_t n, double v) {

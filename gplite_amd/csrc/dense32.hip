@@ -286,22 +286,25 @@ void set_identity_f32(hipStream_t s, float* A, int64_t ld, int64_t N) {
     set_identity_f32_batched(s, A, ld, N, 1, 0);
 }
 
-// Column j of X (FP32, N x N, ld N, batch stride sX): zero above its 64x64
-// diagonal block, the block's column of Dinv (FP64, 64x64 column-major per
-// block, zero above its diagonal, batch stride sD) rounded to FP32 on it; the
-// rows below are left to the caller.
+// Column j = c0 + blockIdx.x of X (FP32, N x N, ld N, batch stride sX): zero
+// above its 64x64 diagonal block, the block's column of Dinv (FP64, 64x64
+// column-major per block, zero above its diagonal, batch stride sD) rounded to
+// FP32 on it; the rows below are left to the caller.
 __global__ __launch_bounds__(256) void k_trtri_leaves32(float* __restrict__ X, int64_t N,
-                                                        const double* __restrict__ Dinv, int64_t sX, int64_t sD) {
+                                                        const double* __restrict__ Dinv, int64_t sX, int64_t sD,
+                                                        int64_t c0) {
     X += blockIdx.y * sX;
     Dinv += blockIdx.y * sD;
-    const int64_t j = blockIdx.x, kb = j / 64, r0 = kb * 64;
+    const int64_t j = c0 + blockIdx.x, kb = j / 64, r0 = kb * 64;
     const double* D = Dinv + kb * 4096 + (j - r0) * 64;
     for (int64_t i = threadIdx.x; i < r0 + 64; i += 256) X[i + j * N] = (i < r0) ? 0.f : (float)D[i - r0];
 }
 
 void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,
-                              int64_t sD) {
-    hipLaunchKernelGGL(k_trtri_leaves32, dim3((unsigned)N, (unsigned)batch), dim3(256), 0, s, X, N, Dinv, sX, sD);
+                              int64_t sD, int64_t c0, int64_t ncols) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_trtri_leaves32, dim3((unsigned)ncols, (unsigned)batch), dim3(256), 0, s, X, N, Dinv, sX, sD,
+                       c0);
 }
 
 // y = X b (trans 0) or X^T b (trans 1), X lower triangular FP32 (64x64 tiles,

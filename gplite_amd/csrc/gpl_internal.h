@@ -153,10 +153,17 @@ void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, i
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                        double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,
                        unsigned maxpoll);
-// X (N x N, ld N): zero above the 64x64 diagonal blocks, the blocks from Dinv
-void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv);
+// X (N x N, ld N), columns c0 .. c0 + ncols: zero above the 64x64 diagonal
+// blocks, the blocks from Dinv
+void trtri_leaves_f64(hipStream_t s, double* X, int64_t N, const double* Dinv, int64_t c0, int64_t ncols);
 void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* Dinv, int batch, int64_t sX,
-                              int64_t sD);
+                              int64_t sD, int64_t c0, int64_t ncols);
+// the top level of the recursive-doubling inverse: the largest 64 * 2^k below N (0 for N = 64)
+inline int64_t doubling_top(int64_t N) {
+    int64_t b = 0;
+    for (int64_t t = 64; t < N; t *= 2) b = t;
+    return b;
+}
 void fill_f64(hipStream_t s, double* p, int64_t n, double v);
 void scale_rows_f64(hipStream_t s, double* A, int64_t ld, int64_t nrow, int64_t ncol, const double* sv);
 // gp_load: saved C_chol (in the n x n part of L, ld N) -> device factor and sv = 1/sqrt(V)

@@ -28,8 +28,10 @@ int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevPro
 // Blocked FP64 Cholesky of the N x N lower tiles of L in place (Dinv: the 64x64
 // diagonal-block inverses, ldiag: log L_ii, info: first failing row + 1) and the
 // triangular inverse X = L^-1 (all tiles written; T: optional N x N scratch,
-// which selects the recursive-doubling form)
-void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info);
+// which selects the recursive-doubling form); potrf_blocked with X and Tm forms
+// both, the inverse's front half beside the factor's last panels
+void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info, double* X = nullptr,
+                   double* Tm = nullptr);
 void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T = nullptr);
 // probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
 void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
@@ -83,6 +85,8 @@ struct gpl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
+    hipStream_t aux = nullptr;   // the front half of L^-1 beside the Cholesky's last panels
+    hipEvent_t ev_h = nullptr, ev_t = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr;
     // flags and ticket counter of the single-launch panel factorisation (32
     // words, zero at creation), its launch epoch and tickets handed out so far,
