@@ -483,7 +483,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 
 // ---------------------------------------------------------------------------
 // One outer panel of the FP64 blocked Cholesky in a single launch: columns
-// [K0, K0 + 64nc), nc <= 4 column blocks, all rows from K0.  Workgroup rb owns
+// [K0, K0 + 64nc), nc <= kPanelMaxBlocks column blocks, all rows from K0.  Workgroup rb owns
 // the 64-row block rb (rows K0 + 64rb ..) for the whole panel; for each column
 // block c it (1) waits for diagonal block c, (2) solves its block against the
 // block's inverse, L_{rb,c} = A_{rb,c} Dinv_c^T, and (3) subtracts
@@ -503,8 +503,9 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 // beyond any legitimate wait) stores kPanelWaitTimeout in info and leaves the
 // kernel; the callers report it as a device error.
 //
-// sync: [0, 4) diagonal-block flags, [4, 20) flags of L_{c2,c} (4 c2 + c),
-// [31] the ticket counter; base: tickets handed out before this launch.
+// sync (kPanelSyncWords words): [0, P) diagonal-block flags, [P, P + P^2)
+// flags of L_{c2,c} (P + P c2 + c), P = kPanelMaxBlocks, and the ticket counter
+// in the last word; base: tickets handed out before this launch.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool panel_wait(unsigned* f, unsigned epoch, int* info, int* s_ok, unsigned maxpoll) {
     if (threadIdx.x == 0) {
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
     __shared__ double piv[64];
     __shared__ unsigned s_id;
     __shared__ int s_ok;
-    if (threadIdx.x == 0) s_id = atomicAdd(&sync[31], 1u) - base;
+    if (threadIdx.x == 0) s_id = atomicAdd(&sync[kPanelSyncWords - 1], 1u) - base;
     __syncthreads();
     const int rb = (int)s_id;
     const int64_t kb0 = K0 / 64;
@@ -638,12 +639,12 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
         blk_mm_nt(Ps, Qs, Ps);  // Ps[j][r] = L_{rb,c}(r, j)
         blk_store(Ab, ld, Ps);  // (Ps now holds the stored values)
         __syncthreads();
-        if (rb < nc) panel_publish(&sync[4 + 4 * rb + c], epoch);
+        if (rb < nc) panel_publish(&sync[kPanelMaxBlocks + kPanelMaxBlocks * rb + c], epoch);
         for (int c2 = c + 1; c2 < nc && c2 <= rb; ++c2) {
             if (c2 == rb) {
                 blk_mm_nt(Ps, Ps, Qs);  // the block's own diagonal block
             } else {
-                if (!panel_wait(&sync[4 + 4 * c2 + c], epoch, info, &s_ok, maxpoll)) return;
+                if (!panel_wait(&sync[kPanelMaxBlocks + kPanelMaxBlocks * c2 + c], epoch, info, &s_ok, maxpoll)) return;
                 blk_load(Qs, A + K0 + 64 * c2 + (K0 + 64 * c) * ld, ld);  // L_{c2,c}
                 __syncthreads();
                 blk_mm_nt(Ps, Qs, Qs);

@@ -142,9 +142,12 @@ void form_wgrad_f64(hipStream_t s, double* B, int64_t N, int64_t n, const double
 void set_identity_f64(hipStream_t s, double* A, int64_t ld, int64_t N);
 // info value a panel factorisation stores when one of its device waits timed out
 constexpr int kPanelWaitTimeout = 0x7fffffff;
-// one outer panel (nc <= 4 column blocks from column K0, nrb = (ld - K0) / 64 row
-// blocks) of the FP64 blocked Cholesky in one launch; sync: 32 words of the
-// context (zero at creation), epoch: nonzero and new for every launch, base:
+// the single-launch panel: at most this many 64-column blocks, and its flag words
+constexpr int kPanelMaxBlocks = 16;
+constexpr int kPanelSyncWords = kPanelMaxBlocks + kPanelMaxBlocks * kPanelMaxBlocks + 1;
+// one outer panel (nc <= kPanelMaxBlocks column blocks from column K0, nrb = (ld - K0) / 64
+// row blocks) of the FP64 blocked Cholesky in one launch; sync: kPanelSyncWords words
+// of the context (zero at creation), epoch: nonzero and new for every launch, base:
 // tickets handed out before (nrb per launch), maxpoll: polls before a wait
 // between workgroups gives up (kPanelWaitTimeout in info)
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
