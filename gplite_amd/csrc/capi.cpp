@@ -653,8 +653,8 @@ static int fit_gaussian(gpl_model* m) {
         Prof p(c, "trmv", 4.0 * N * N / 2, 8.0 * N * N);
         fill_f64(s, m->d(B_W1), N, 0.0);
         vec_mul(s, n, m->d(B_SV), m->d(B_Z), m->d(B_W1));
-        trmv_lower_f64(s, m->d(B_XINV), N, N, m->d(B_W1), m->d(B_W2), 0, nullptr);
-        trmv_lower_f64(s, m->d(B_XINV), N, N, m->d(B_W2), m->d(B_W1), 1, nullptr);
+        trmv_lower_f64_chunked(s, m->d(B_XINV), N, N, m->d(B_W1), m->d(B_W2), 0, m->d(B_KMV), kmatvec_chunks(N));
+        trmv_lower_f64_chunked(s, m->d(B_XINV), N, N, m->d(B_W2), m->d(B_W1), 1, m->d(B_KMV), kmatvec_chunks(N));
         fill_f64(s, m->d(B_ALPHA), N, 0.0);
         vec_mul(s, n, m->d(B_SV), m->d(B_W1), m->d(B_ALPHA));
     } else {

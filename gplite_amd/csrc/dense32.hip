@@ -312,10 +312,11 @@ void trtri_leaves_f32_batched(hipStream_t s, float* X, int64_t N, const double* 
 // the tiles of output block ti that fall in chunk ch; for X^T the stored tile
 // (r, ti) is staged through LDS so every global read is coalesced.  The chunk
 // partials (part: batch * nchunk * N) are added in a fixed order afterwards.
-__global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int64_t ld, int64_t T, int nchunk,
+template <typename TX>
+__global__ __launch_bounds__(256) void k_trmv32(const TX* __restrict__ X, int64_t ld, int64_t T, int nchunk,
                                                 const double* __restrict__ b, double* __restrict__ part, int trans,
                                                 int64_t sX, int64_t sv) {
-    __shared__ float tile[64][65];
+    __shared__ TX tile[64][65];
     __shared__ double bc[64];
     __shared__ double red[256];
     X += blockIdx.z * sX;
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int
         __syncthreads();
         if (t < 64) bc[t] = b[q * 64 + t];
         if (trans) {
-            const float* Xt = X + q * 64 + ti * 64 * ld;  // stored tile (q, ti)
+            const TX* Xt = X + q * 64 + ti * 64 * ld;  // stored tile (q, ti)
 #pragma unroll 4
             for (int u = 0; u < 16; ++u) {
                 const int c = p * 16 + u;
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int
         }
         __syncthreads();
         if (!trans) {
-            const float* Xt = X + ti * 64 + q * 64 * ld;
+            const TX* Xt = X + ti * 64 + q * 64 * ld;
 #pragma unroll 4
             for (int u = 0; u < 16; ++u) {
                 const int c = p * 16 + u;
@@ -363,9 +364,18 @@ __global__ __launch_bounds__(256) void k_trmv32(const float* __restrict__ X, int
 void trmv_lower_f32_batched(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y,
                             int trans, int batch, int64_t sX, int64_t sv, double* part, int nchunk) {
     const int64_t T = N / 64;
-    hipLaunchKernelGGL(k_trmv32, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, X, ld, T,
-                       nchunk, b, part, trans, sX, sv);
+    hipLaunchKernelGGL(k_trmv32<float>, dim3((unsigned)T, (unsigned)nchunk, (unsigned)batch), dim3(256), 0, s, X, ld,
+                       T, nchunk, b, part, trans, sX, sv);
     chunk_sum_f64(s, part, N, nchunk, y, batch, sv);
+}
+
+// the same over an FP64 matrix (one problem)
+void trmv_lower_f64_chunked(hipStream_t s, const double* X, int64_t ld, int64_t N, const double* b, double* y,
+                            int trans, double* part, int nchunk) {
+    const int64_t T = N / 64;
+    hipLaunchKernelGGL(k_trmv32<double>, dim3((unsigned)T, (unsigned)nchunk, 1u), dim3(256), 0, s, X, ld, T, nchunk,
+                       b, part, trans, (int64_t)0, (int64_t)0);
+    chunk_sum_f64(s, part, N, nchunk, y, 1, 0);
 }
 
 void trmv_lower_f32(hipStream_t s, const float* X, int64_t ld, int64_t N, const double* b, double* y, int trans,

@@ -120,6 +120,11 @@ void trsv_lower_f64(hipStream_t s, const double* L, int64_t ld, const double* Di
 // y = L^T? x with a dense lower-triangular X (GEMV on lower tiles): trans=0: y = X b; 1: y = X^T b
 void trmv_lower_f64(hipStream_t s, const double* X, int64_t ld, int64_t N, const double* b,
                     double* y, int trans, double* part);
+// y = X b or X^T b over nchunk column chunks of an FP64 lower-triangular X
+// (partials in part: nchunk * N, summed in a fixed order; transposed tiles
+// staged through LDS so every read is coalesced)
+void trmv_lower_f64_chunked(hipStream_t s, const double* X, int64_t ld, int64_t N, const double* b, double* y,
+                            int trans, double* part, int nchunk);
 // y = K v for a symmetric K stored in its lower tiles.
 void symv_lower_f64(hipStream_t s, const double* K, int64_t ld, int64_t N, const double* v,
                     double* y, double* part);
