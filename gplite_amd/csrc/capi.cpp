@@ -379,12 +379,13 @@ struct InverseFork {
 // Factor the outer panel of columns [K0, K0+W) (all rows >= K0) on stream st:
 // one launch, a workgroup per 64-row block (k_potrf_panel).
 static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_t K0, int64_t W, double* Dinv,
-                        double* ldiag, int* info) {
+                        double* ldiag, int* info, double* rhs) {
     const int64_t nrb = (N - K0) / 64;
     if (++c->pepoch == 0) c->pepoch = 1;
     {
         Prof p(c, "potrf_panel", (double)W * W * W / 3 + 2.0 * (N - K0 - W) * W * W, 8.0 * (N - K0) * W * 3, st);
-        potrf_panel_f64(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, c->pwait);
+        potrf_panel_f64(st, L, N, K0, (int)(W / 64), nrb, Dinv, ldiag, info, c->psync, c->pepoch, c->pbase, c->pwait,
+                        rhs);
     }
     c->pbase += (unsigned)nrb;
 }
@@ -396,12 +397,12 @@ static void potrf_panel(gpl_ctx* c, hipStream_t st, double* L, int64_t N, int64_
 // per 64-row block, each needing half a CU's registers) are placed before the
 // update's GEMM fills the CUs, instead of waiting for it to drain.  With X and
 // Tm (N x N scratch) it also forms X = L^-1 (recursive doubling, its front half
-// beside the remaining panels).
+// beside the remaining panels); with rhs it overwrites that vector with L^-1 rhs.
 void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info, double* X,
-                        double* Tm) {
+                        double* Tm, double* rhs) {
     hipStream_t s = c->stream, s2 = c->side;
     InverseFork<double> inv(c, L, Dinv, N, X, Tm);
-    potrf_panel(c, s, L, N, 0, std::min<int64_t>(kPanelNB, N), Dinv, ldiag, info);
+    potrf_panel(c, s, L, N, 0, std::min<int64_t>(kPanelNB, N), Dinv, ldiag, info, rhs);
     inv.after_panel(std::min<int64_t>(kPanelNB, N));
     for (int64_t K0 = 0; K0 < N; K0 += kPanelNB) {
         const int64_t W = std::min<int64_t>(kPanelNB, N - K0);
@@ -414,7 +415,7 @@ void gpl::potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* 
             gemm2_f64(s, 0, 1, R, W1, W, -1.0, P, N, P, N, 1.0, L + K1 * (N + 1), N, 0, 0);
         }
         hipEventRecord(c->ev_a, s);
-        potrf_panel(c, s, L, N, K1, W1, Dinv, ldiag, info);
+        potrf_panel(c, s, L, N, K1, W1, Dinv, ldiag, info, rhs);
         inv.after_panel(K1 + W1);
         const int64_t R2 = R - W1;
         if (R2 > 0) {
@@ -549,18 +550,26 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
 }
 
 // alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)
-static void solve_alpha(gpl_ctx* c, gpl_model* m, const double* z, double* alpha) {
+// alpha = S B^-1 S z = S L^-T L^-1 S z with the factorization carrying the
+// forward solve: W1 = S z before potrf_blocked(..., rhs = W1), which leaves
+// L^-1 S z in W1; then alpha = S L^-T W1.
+static double* alpha_rhs(gpl_model* m, const double* z) {
+    hipStream_t s = m->ctx->stream;
+    double* w1 = m->d(B_W1);
+    fill_f64(s, w1, m->N, 0.0);
+    vec_mul(s, m->n, m->d(B_SV), z, w1);
+    return w1;
+}
+
+static void solve_alpha_back(gpl_ctx* c, gpl_model* m, double* alpha) {
     hipStream_t s = c->stream;
     const int64_t N = m->N;
     double* w1 = m->d(B_W1);
     double* w2 = m->d(B_W2);
-    Prof p(c, "trsv", 2.0 * N * N, 8.0 * N * N);
-    fill_f64(s, w1, N, 0.0);
-    vec_mul(s, m->n, m->d(B_SV), z, w1);
-    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w1, w2, 0);
-    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w2, w1, 1);
+    Prof p(c, "trsv", 1.0 * N * N, 8.0 * N * N / 2);
+    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w1, w2, 1);
     fill_f64(s, alpha, N, 0.0);
-    vec_mul(s, m->n, m->d(B_SV), w1, alpha);
+    vec_mul(s, m->n, m->d(B_SV), w2, alpha);
 }
 
 // A factorisation's info word: 0, the first failing row + 1, or kPanelWaitTimeout.
@@ -646,7 +655,7 @@ static int fit_gaussian(gpl_model* m) {
         ENSURE(m, B_BINV, (size_t)N * N * 8);
     }
     potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), m->b[B_INFO].as<int>(), inv ? m->d(B_XINV) : nullptr,
-                  inv ? m->d(B_BINV) : nullptr);
+                  inv ? m->d(B_BINV) : nullptr, inv ? nullptr : alpha_rhs(m, m->d(B_Z)));
     if (inv) {
         // alpha = S X^T X S z (S = 1/sigma)
         m->have_xinv = true;
@@ -658,7 +667,7 @@ static int fit_gaussian(gpl_model* m) {
         fill_f64(s, m->d(B_ALPHA), N, 0.0);
         vec_mul(s, n, m->d(B_SV), m->d(B_W1), m->d(B_ALPHA));
     } else {
-        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+        solve_alpha_back(c, m, m->d(B_ALPHA));
     }
     {
         Prof p(c, "kmatvec", 0, 0);
@@ -828,11 +837,11 @@ static int fit_laplace(gpl_model* m) {
             Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
             form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
         }
-        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags);
-        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags, nullptr, nullptr, alpha_rhs(m, m->d(B_Z)));
+        solve_alpha_back(c, m, m->d(B_ALPHA));
         {
             Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
-            symv_lower_f64(s, m->d(B_K), N, N, m->d(B_ALPHA), m->d(B_FMEAN), nullptr);
+            symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), kmatvec_chunks(N), 1, 0, 0);
         }
         lik_loglik(s, m->lik.kind, m->lik.link, lik_par(m->lik), n, m->d(B_FMEAN), m->d(B_Y), tr, off, m->d(B_W3));
         double* sc = m->d(B_SCAL);
@@ -1145,11 +1154,11 @@ static int fit_ep(gpl_model* m) {
             Prof p(c, "form_B", 0, 16.0 * N * (N + 64) / 2);
             form_B_f64(s, m->d(B_K), m->d(B_L), N, n, N, m->d(B_SV));
         }
-        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags);
-        solve_alpha(c, m, m->d(B_Z), m->d(B_ALPHA));
+        potrf_blocked(c, m->d(B_L), N, m->d(B_DINV), m->d(B_LDIAG), flags, nullptr, nullptr, alpha_rhs(m, m->d(B_Z)));
+        solve_alpha_back(c, m, m->d(B_ALPHA));
         {
             Prof p(c, "symv", 2.0 * N * N, 8.0 * N * N / 2);
-            symv_lower_f64(s, m->d(B_K), N, N, m->d(B_ALPHA), m->d(B_FMEAN), nullptr);
+            symv_lt_f64(s, m->d(B_K), N, m->d(B_ALPHA), m->d(B_FMEAN), m->d(B_KMV), kmatvec_chunks(N), 1, 0, 0);
         }
         trtri_blocked(c, m->d(B_L), m->d(B_DINV), N, m->d(B_XINV));
         colsumsq_f64(s, m->d(B_XINV), N, N, n, m->d(B_W3), 1);

@@ -494,6 +494,12 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
 // update inside the workgroups, without the kernel boundaries of the separate
 // factor / panel-solve / panel-update launches.
 //
+// rhs (optional, FP64 storage only): a vector b of N rows that leaves as
+// y = L^-1 b, the forward solve carried along: workgroup c stores
+// y_c = Dinv_c b_c with diagonal block c, and every workgroup below subtracts
+// L_{rb,c} y_c from its block of b after its block solve.  Only the backward
+// solve is then left to the caller.
+//
 // Ordering: each workgroup takes a ticket (atomic counter) as its logical row
 // block, so block rb is held by a workgroup that started after those of every
 // lower block; all waits are on lower blocks, which have started and do not
@@ -612,11 +618,13 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, unsigned* __restrict__ sync,
                                                      unsigned epoch, unsigned base, double* __restrict__ Dacc,
-                                                     unsigned maxpoll) {
+                                                     unsigned maxpoll, double* __restrict__ rhs) {
     __shared__ double Ps[64][65];
     __shared__ double Qs[64][65];
     __shared__ double rinv[64];
     __shared__ double piv[64];
+    __shared__ double red[256];
+    __shared__ double vb[64];
     __shared__ unsigned s_id;
     __shared__ int s_ok;
     if (threadIdx.x == 0) s_id = atomicAdd(&sync[kPanelSyncWords - 1], 1u) - base;
@@ -628,6 +636,18 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
         const int64_t kc = kb0 + c;
         if (rb == c) {
             diag_block64<T>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv, piv);
+            if (rhs) {  // y_c = L_cc^-1 b_c from the inverse left in Qs (Qs[k][e] = X(e, k))
+                double* bc = rhs + K0 + 64 * c;
+                if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
+                __syncthreads();
+                const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
+                double sum = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) sum += Qs[16 * pq + u][e] * vb[16 * pq + u];
+                red[threadIdx.x] = sum;
+                __syncthreads();
+                if (threadIdx.x < 64) bc[threadIdx.x] = red[e] + red[e + 64] + red[e + 128] + red[e + 192];
+            }
             panel_publish(&sync[c], epoch);
             return;
         }
@@ -639,6 +659,18 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
         blk_mm_nt(Ps, Qs, Ps);  // Ps[j][r] = L_{rb,c}(r, j)
         blk_store(Ab, ld, Ps);  // (Ps now holds the stored values)
         __syncthreads();
+        if (rhs) {  // b_rb -= L_{rb,c} y_c (y_c published with diagonal block c)
+            if (threadIdx.x < 64) vb[threadIdx.x] = rhs[K0 + 64 * c + threadIdx.x];
+            __syncthreads();
+            const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
+            double sum = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) sum += Ps[16 * pq + u][e] * vb[16 * pq + u];
+            red[threadIdx.x] = sum;
+            __syncthreads();
+            if (threadIdx.x < 64) rhs[K0 + 64 * rb + e] -= red[e] + red[e + 64] + red[e + 128] + red[e + 192];
+            __syncthreads();
+        }
         if (rb < nc) panel_publish(&sync[kPanelMaxBlocks + kPanelMaxBlocks * rb + c], epoch);
         for (int c2 = c + 1; c2 < nc && c2 <= rb; ++c2) {
             if (c2 == rb) {
@@ -659,16 +691,17 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
 }
 
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll) {
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll,
+                     double* rhs) {
     hipLaunchKernelGGL(k_potrf_panel<double>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
-                       sync, epoch, base, (double*)nullptr, maxpoll);
+                       sync, epoch, base, (double*)nullptr, maxpoll, rhs);
 }
 
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                        double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,
                        unsigned maxpoll) {
     hipLaunchKernelGGL(k_potrf_panel<float>, dim3((unsigned)nrb), dim3(256), 0, s, A, ld, K0, nc, Dinv, ldiag, info,
-                       sync, epoch, base, Dacc, maxpoll);
+                       sync, epoch, base, Dacc, maxpoll, (double*)nullptr);
 }
 
 void potrf_diag3_f64(hipStream_t s, double* A, int64_t ld, int64_t k, double* Dinv, double* ldiag, int* info,

@@ -154,9 +154,11 @@ constexpr int kPanelSyncWords = kPanelMaxBlocks + kPanelMaxBlocks * kPanelMaxBlo
 // row blocks) of the FP64 blocked Cholesky in one launch; sync: kPanelSyncWords words
 // of the context (zero at creation), epoch: nonzero and new for every launch, base:
 // tickets handed out before (nrb per launch), maxpoll: polls before a wait
-// between workgroups gives up (kPanelWaitTimeout in info)
+// between workgroups gives up (kPanelWaitTimeout in info), rhs (optional): a
+// vector carried through the forward solve (leaves as L^-1 rhs on the panel's rows)
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
-                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll);
+                     double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll,
+                     double* rhs);
 // the same on FP32 storage with the FP64 diagonal tiles Dacc (mixed precision)
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                        double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,

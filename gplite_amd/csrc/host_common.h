@@ -29,9 +29,10 @@ int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevPro
 // diagonal-block inverses, ldiag: log L_ii, info: first failing row + 1) and the
 // triangular inverse X = L^-1 (all tiles written; T: optional N x N scratch,
 // which selects the recursive-doubling form); potrf_blocked with X and Tm forms
-// both, the inverse's front half beside the factor's last panels
+// both, the inverse's front half beside the factor's last panels, and with rhs
+// overwrites that vector (N rows) with L^-1 rhs
 void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info, double* X = nullptr,
-                   double* Tm = nullptr);
+                   double* Tm = nullptr, double* rhs = nullptr);
 void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T = nullptr);
 // probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
 void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
