@@ -339,8 +339,11 @@ __device__ __forceinline__ double quad_bcast_f64(double v) {
 // + 1.  Dsrc (optional): the block's FP64 values (64x64 column-major) to factor
 // in place of A's.  Lp, Xs (64 x 65), rinv, piv (64): workgroup LDS;
 // Lp[col][row] = L(row, col) (strictly lower after the factor), Xs[col][row] =
-// (L_kk^-1)(row, col).
-template <typename T>
+// (L_kk^-1)(row, col).  MU: the rank-16 updates between the phases on f64 MFMA
+// (through Xs) rather than as scalar FMA sums; measured faster inside
+// k_potrf_panel (Laplace n = 4096: 42.9 -> 44.8 fits/s) but slower as the
+// standalone kernel (25.1 against 20.5 us), whose register allocation it changes.
+template <typename T, bool MU>
 __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int64_t k, double* __restrict__ Dinv,
                                              double* __restrict__ ldiag, int* __restrict__ info, int want_inv,
                                              const double* __restrict__ Dsrc, double (*Lp)[65], double (*Xs)[65],
@@ -382,7 +385,29 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
             for (int q = 0; q < 16; ++q) Lp[16 * p + q][i] = a[q];
         }
         __syncthreads();
-        if (w > p) {
+        if (MU && w > p) {
+            // rank-16 update of the wave's columns: U = L(:, P) L(16w.., P)^T
+            // (P = columns 16p..16p+15) on v_mfma_f64_16x16x4, four 16-row
+            // tiles; U goes through the wave's own columns of Xs (free until
+            // the inverse) from the MFMA layout to lane = row, in program order
+            const int ln = i & 15, lg = i >> 4;
+            d4 u[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0},
+                       d4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+            for (int k0 = 0; k0 < 16; k0 += 4) {
+                const double bv = Lp[16 * p + k0 + lg][16 * w + ln];
+#pragma unroll
+                for (int rt = 0; rt < 4; ++rt)
+                    u[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lp[16 * p + k0 + lg][16 * rt + ln], bv, u[rt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Xs[16 * w + ln][16 * rt + lg + 4 * v] = u[rt][v];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (16 * w + q <= i) a[q] -= Xs[16 * w + q][i];
+        } else if (w > p) {
             double li[16];
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk) li[kk] = Lp[16 * p + kk][i];
@@ -478,7 +503,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag3(T* __restrict__ A, int64_t 
     // Dsrc (optional): the FP64 diagonal tiles (64x64 column-major, one per block
     // row, same batch stride as Dinv) to factor in place of the stored tile of A
     if (Dsrc) Dsrc += blockIdx.x * sD + k * 64 * 64;
-    diag_block64<T>(A, ld, k, Dinv, ldiag, info, want_inv, Dsrc, Lp, Xs, rinv, piv);
+    diag_block64<T, false>(A, ld, k, Dinv, ldiag, info, want_inv, Dsrc, Lp, Xs, rinv, piv);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,7 +639,8 @@ __device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], doub
 // the diagonal blocks.  Block products are FP64 MFMA on the stored FP32 values
 // (exact products, FP64 sums) with one rounding to FP32 per stored result.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t ld, int64_t K0, int nc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_potrf_panel(
+    T* __restrict__ A, int64_t ld, int64_t K0, int nc,
                                                      double* __restrict__ Dinv, double* __restrict__ ldiag,
                                                      int* __restrict__ info, unsigned* __restrict__ sync,
                                                      unsigned epoch, unsigned base, double* __restrict__ Dacc,
@@ -635,7 +661,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ A, int64_t 
     for (int c = 0; c < nc && c <= rb; ++c) {
         const int64_t kc = kb0 + c;
         if (rb == c) {
-            diag_block64<T>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv, piv);
+            diag_block64<T, true>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv, piv);
             if (rhs) {  // y_c = L_cc^-1 b_c from the inverse left in Qs (Qs[k][e] = X(e, k))
                 double* bc = rhs + K0 + 64 * c;
                 if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
