@@ -178,7 +178,26 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_lds(int64_t M, int64_t N, int6
         }
         __syncthreads();
     }
-    // v_mfma_f64_16x16x4 D layout: lane holds rows g, g+4, g+8, g+12 of column li
+    // v_mfma_f64_16x16x4 D layout: lane holds rows g, g+4, g+8, g+12 of column li.
+    // With beta != 0 every C value is loaded before the first store (a load
+    // placed after a store that may alias it waits for that store's round
+    // trip).  The eight-wave 128 tile pays for the registers with one wave per
+    // SIMD less and still comes out ahead (Cholesky trailing updates at
+    // n = 4096 and 8192); the FP32 GEMM, whose 128 tile measured slower with
+    // the same change, keeps the interleaved order.
+    TCs cv[NBR][NBC][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int i = 0; i < NBR; ++i)
+#pragma unroll
+            for (int j = 0; j < NBC; ++j) {
+                const int64_t rb0 = row0 + wr * WTR + i * 16, cb0 = col0 + wc * WTC + j * 16;
+                if (rb0 >= M || cb0 >= N) continue;
+                const TCs* p = C + (rb0 + g) + (cb0 + li) * ldc;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cv[i][j][v] = p[4 * v];
+            }
+    }
 #pragma unroll
     for (int i = 0; i < NBR; ++i)
 #pragma unroll
@@ -189,7 +208,7 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_lds(int64_t M, int64_t N, int6
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const double o = alpha * acc[i][j][v];
-                p[4 * v] = (TCs)((beta == 0.0) ? o : beta * (double)p[4 * v] + o);
+                p[4 * v] = (TCs)((beta == 0.0) ? o : beta * (double)cv[i][j][v] + o);
             }
         }
 }
@@ -337,13 +356,14 @@ __device__ __forceinline__ double quad_bcast_f64(double v) {
 // ld) on one 256-thread workgroup: factor in place, Dinv + 4096k = the block's
 // inverse (when want_inv), ldiag[64k + i] = log L_ii, info = first failing row
 // + 1.  Dsrc (optional): the block's FP64 values (64x64 column-major) to factor
-// in place of A's.  Lp, Xs (64 x 65), rinv, piv (64): workgroup LDS;
+// in place of A's.  IX: the block's values (FP64) are already in Xs
+// (Xs[col][row]), in place of either.  Lp, Xs (64 x 65), rinv, piv (64): workgroup LDS;
 // Lp[col][row] = L(row, col) (strictly lower after the factor), Xs[col][row] =
 // (L_kk^-1)(row, col).  MU: the rank-16 updates between the phases on f64 MFMA
 // (through Xs) rather than as scalar FMA sums; measured faster inside
 // k_potrf_panel (Laplace n = 4096: 42.9 -> 44.8 fits/s) but slower as the
 // standalone kernel (25.1 against 20.5 us), whose register allocation it changes.
-template <typename T, bool MU>
+template <typename T, bool MU, bool IX = false>
 __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int64_t k, double* __restrict__ Dinv,
                                              double* __restrict__ ldiag, int* __restrict__ info, int want_inv,
                                              const double* __restrict__ Dsrc, double (*Lp)[65], double (*Xs)[65],
@@ -360,11 +380,12 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int c = 16 * w + q;
-        a[q] = (c <= i) ? (Dsrc ? Dsrc[i + c * 64] : (double)blk[i + (int64_t)c * ld]) : 0.0;
+        a[q] = (c <= i) ? (IX ? Xs[c][i] : Dsrc ? Dsrc[i + c * 64] : (double)blk[i + (int64_t)c * ld]) : 0.0;
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         if (w == p) {
+            double myinv = 0.0, mypiv = 0.0;
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const int j = 16 * p + jj;
@@ -373,23 +394,30 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
                 const double rt = d * inv;
                 const double l = (i > j) ? a[jj] * inv : ((i == j) ? rt : 0.0);
                 a[jj] = l;
-                if (i == j) {
-                    rinv[j] = inv;
-                    piv[j] = d;
-                }
-                // rows above a column are cleared when that column is reached
+                myinv = (i == j) ? inv : myinv;
+                mypiv = (i == j) ? d : mypiv;
+                // rows above a column are cleared when that column is reached.
+                // Column j goes to LDS, which is also where the multipliers
+                // L(16p + cc, j) of the later columns are read from (one
+                // address for the whole wave); only the next pivot's column
+                // takes its multiplier by readlane, off the LDS round trip.
+                Lp[j][i] = l;
+                if (jj + 1 < 16) a[jj + 1] -= l * readlane_f64(l, j + 1);
 #pragma unroll
-                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= l * readlane_f64(l, 16 * p + cc);
+                for (int cc = jj + 2; cc < 16; ++cc) a[cc] -= l * Lp[j][16 * p + cc];
             }
-#pragma unroll
-            for (int q = 0; q < 16; ++q) Lp[16 * p + q][i] = a[q];
+            if (i >= 16 * p && i < 16 * p + 16) {
+                rinv[i] = myinv;
+                piv[i] = mypiv;
+            }
         }
         __syncthreads();
         if (MU && w > p) {
             // rank-16 update of the wave's columns: U = L(:, P) L(16w.., P)^T
-            // (P = columns 16p..16p+15) on v_mfma_f64_16x16x4, four 16-row
-            // tiles; U goes through the wave's own columns of Xs (free until
-            // the inverse) from the MFMA layout to lane = row, in program order
+            // (P = columns 16p..16p+15) on v_mfma_f64_16x16x4, the 16-row tiles
+            // rt >= w (those above are upper-triangle positions); U goes through
+            // the wave's own columns of Xs (free until the inverse) from the
+            // MFMA layout to lane = row, in program order
             const int ln = i & 15, lg = i >> 4;
             d4 u[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0},
                        d4{0.0, 0.0, 0.0, 0.0}};
@@ -398,12 +426,15 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
                 const double bv = Lp[16 * p + k0 + lg][16 * w + ln];
 #pragma unroll
                 for (int rt = 0; rt < 4; ++rt)
-                    u[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lp[16 * p + k0 + lg][16 * rt + ln], bv, u[rt], 0, 0, 0);
+                    if (rt >= w)
+                        u[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lp[16 * p + k0 + lg][16 * rt + ln], bv, u[rt], 0, 0,
+                                                                     0);
             }
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt)
+                if (rt >= w)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Xs[16 * w + ln][16 * rt + lg + 4 * v] = u[rt][v];
+                    for (int v = 0; v < 4; ++v) Xs[16 * w + ln][16 * rt + lg + 4 * v] = u[rt][v];
 #pragma unroll
             for (int q = 0; q < 16; ++q)
                 if (16 * w + q <= i) a[q] -= Xs[16 * w + q][i];
@@ -575,6 +606,22 @@ __device__ __forceinline__ void blk_load(double (*S)[65], const T* M, int64_t ld
     for (int j = 0; j < 16; ++j) S[kq + 4 * j][r] = (double)M[r + (int64_t)(kq + 4 * j) * ld];
 }
 
+// the same block in two steps: its values into registers (a load that can be
+// issued early), then into S
+template <typename T>
+__device__ __forceinline__ void blk_fetch(T (&v)[16], const T* M, int64_t ld) {
+    const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = M[r + (int64_t)(kq + 4 * j) * ld];
+}
+
+template <typename T>
+__device__ __forceinline__ void blk_put(double (*S)[65], const T (&v)[16]) {
+    const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) S[kq + 4 * j][r] = (double)v[j];
+}
+
 // M(r, k) = S[k][r] rounded to T; S keeps the rounded value (each thread
 // rewrites only the elements it stores)
 template <typename T>
@@ -588,21 +635,28 @@ __device__ __forceinline__ void blk_store(T* M, int64_t ld, double (*S)[65]) {
     }
 }
 
-// M(r, k) = M(r, k) - S[k][r], formed in FP64 and rounded once to T
+// M(r, k) = M(r, k) - S[k][r], formed in FP64 and rounded once to T.  All
+// loads are issued before the first store: interleaved, each load would wait
+// for the previous store (the compiler cannot rule out that two columns alias
+// for an arbitrary ld), sixteen global round trips in a row.
 template <typename T>
 __device__ __forceinline__ void blk_sub(T* M, int64_t ld, double (*S)[65]) {
     const int r = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    T v[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        T* q = M + r + (int64_t)(kq + 4 * j) * ld;
-        *q = (T)((double)*q - S[kq + 4 * j][r]);
-    }
+    for (int j = 0; j < 16; ++j) v[j] = M[r + (int64_t)(kq + 4 * j) * ld];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) M[r + (int64_t)(kq + 4 * j) * ld] = (T)((double)v[j] - S[kq + 4 * j][r]);
 }
 
 // C(r, j) = sum_k P(r, k) Q(j, k) with P, Q in LDS as [k][row]; wave w computes
 // the 32x32 quadrant (w / 2, w % 2) as 2 x 2 tiles of v_mfma_f64_16x16x4, then
-// (after a barrier) writes C to S as S[j][r].
-__device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], double (*S)[65]) {
+// (after a barrier) writes C to S as S[j][r] (SUB: subtracts it from S).
+// QG: Q is read from global memory instead, Qg (64x64 column-major, Q(j, k)
+// at Qg[j + 64k]).
+template <bool QG = false, bool SUB = false>
+__device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], double (*S)[65],
+                                          const double* __restrict__ Qg = nullptr) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, g = lane >> 4, r0 = 32 * (w >> 1), j0 = 32 * (w & 1);
     d4 acc[2][2];
@@ -616,7 +670,7 @@ __device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], doub
 #pragma unroll
         for (int i = 0; i < 2; ++i) a[i] = P[k0 + g][r0 + 16 * i + li];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = Q[k0 + g][j0 + 16 * j + li];
+        for (int j = 0; j < 2; ++j) b[j] = QG ? Qg[(j0 + 16 * j + li) + 64 * (k0 + g)] : Q[k0 + g][j0 + 16 * j + li];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -628,7 +682,25 @@ __device__ __forceinline__ void blk_mm_nt(double (*P)[65], double (*Q)[65], doub
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) S[j0 + 16 * j + li][r0 + 16 * i + g + 4 * v] = acc[i][j][v];
+            for (int v = 0; v < 4; ++v) {
+                double& d = S[j0 + 16 * j + li][r0 + 16 * i + g + 4 * v];
+                d = SUB ? d - acc[i][j][v] : acc[i][j][v];
+            }
+    __syncthreads();
+}
+
+// b_rb -= L_{rb,c} y_c for the 64-row blocks bc (y_c) and brb, L_{rb,c} in S as S[k][r]
+__device__ __forceinline__ void rhs_update(const double* bc, double* brb, double (*S)[65], double* vb,
+                                           double* red) {
+    if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
+    __syncthreads();
+    const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sum += S[16 * pq + u][e] * vb[16 * pq + u];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x < 64) brb[e] -= red[e] + red[e + 64] + red[e + 128] + red[e + 192];
     __syncthreads();
 }
 
@@ -658,45 +730,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int rb = (int)s_id;
     const int64_t kb0 = K0 / 64;
     T* Arow = A + K0 + 64 * (int64_t)rb;  // row block rb, column 0
-    for (int c = 0; c < nc && c <= rb; ++c) {
+    // kNext: the owner of the next diagonal block keeps that block's last
+    // update in LDS (below).  FP64 storage only: with FP32 storage the
+    // registers it takes cost more than it saves (mixed-precision fits
+    // measured 10 % slower).
+    constexpr bool kNext = sizeof(T) == 8;
+    for (int c = 0; c < nc && c < rb; ++c) {
         const int64_t kc = kb0 + c;
-        if (rb == c) {
-            diag_block64<T, true>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv, piv);
-            if (rhs) {  // y_c = L_cc^-1 b_c from the inverse left in Qs (Qs[k][e] = X(e, k))
-                double* bc = rhs + K0 + 64 * c;
-                if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
-                __syncthreads();
-                const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
-                double sum = 0.0;
-#pragma unroll
-                for (int u = 0; u < 16; ++u) sum += Qs[16 * pq + u][e] * vb[16 * pq + u];
-                red[threadIdx.x] = sum;
-                __syncthreads();
-                if (threadIdx.x < 64) bc[threadIdx.x] = red[e] + red[e + 64] + red[e + 128] + red[e + 192];
-            }
-            panel_publish(&sync[c], epoch);
-            return;
-        }
-        if (!panel_wait(&sync[c], epoch, info, &s_ok, maxpoll)) return;
         T* Ab = Arow + (K0 + 64 * c) * ld;  // A_{rb,c}
-        blk_load(Ps, Ab, ld);
+        if (kNext && rb == c + 1 && rb < nc) {
+            // This workgroup factors diagonal block c + 1 next, and its only update
+            // left is that block's own.  A_{rb,c} and the diagonal block are
+            // final before the wait (their updates are this workgroup's own
+            // stores, made before the barriers since) and are staged into Ps
+            // and Qs while it waits; the solve reads Dinv_c straight from
+            // global memory, the update is subtracted in LDS, and the factor
+            // starts from Qs (FP64).  L_{rb,c} is published after the update
+            // (its stores have landed by then).
+            blk_load(Ps, Ab, ld);
+            blk_load(Qs, A + (kb0 + rb) * 64 * (ld + 1), ld);
+            if (!panel_wait(&sync[c], epoch, info, &s_ok, maxpoll)) return;
+            blk_mm_nt<true, false>(Ps, Ps, Ps, Dinv + kc * 4096);  // Ps[j][r] = L_{rb,c}(r, j)
+            blk_store(Ab, ld, Ps);
+            __syncthreads();
+            if (rhs) rhs_update(rhs + K0 + 64 * c, rhs + K0 + 64 * rb, Ps, vb, red);
+            blk_mm_nt<false, true>(Ps, Ps, Qs);  // Qs -= L_{rb,c} L_{rb,c}^T
+            panel_publish(&sync[kPanelMaxBlocks + kPanelMaxBlocks * rb + c], epoch);
+            continue;
+        }
+        // A_{rb,c} is final before the wait: its updates are this workgroup's own
+        // stores, made by the same threads (the blk_sub layout)
+        T pre[16];
+        blk_fetch(pre, Ab, ld);
+        if (!panel_wait(&sync[c], epoch, info, &s_ok, maxpoll)) return;
         blk_load(Qs, Dinv + kc * 4096, 64);
+        blk_put(Ps, pre);
         __syncthreads();
         blk_mm_nt(Ps, Qs, Ps);  // Ps[j][r] = L_{rb,c}(r, j)
         blk_store(Ab, ld, Ps);  // (Ps now holds the stored values)
         __syncthreads();
-        if (rhs) {  // b_rb -= L_{rb,c} y_c (y_c published with diagonal block c)
-            if (threadIdx.x < 64) vb[threadIdx.x] = rhs[K0 + 64 * c + threadIdx.x];
-            __syncthreads();
-            const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
-            double sum = 0.0;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) sum += Ps[16 * pq + u][e] * vb[16 * pq + u];
-            red[threadIdx.x] = sum;
-            __syncthreads();
-            if (threadIdx.x < 64) rhs[K0 + 64 * rb + e] -= red[e] + red[e + 64] + red[e + 128] + red[e + 192];
-            __syncthreads();
-        }
+        if (rhs) rhs_update(rhs + K0 + 64 * c, rhs + K0 + 64 * rb, Ps, vb, red);
         if (rb < nc) panel_publish(&sync[kPanelMaxBlocks + kPanelMaxBlocks * rb + c], epoch);
         for (int c2 = c + 1; c2 < nc && c2 <= rb; ++c2) {
             if (c2 == rb) {
@@ -713,6 +786,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 blk_sub(Arow + (K0 + 64 * c2) * ld, ld, Qs);
             __syncthreads();
         }
+    }
+    if (rb < nc) {
+        // diagonal block rb with all its updates; with kNext in Qs: left there
+        // by the last column (rb > 0), or the stored block (no updates inside
+        // the panel)
+        const int c = rb;
+        const int64_t kc = kb0 + c;
+        if (kNext && rb == 0) {
+            blk_load(Qs, A + kc * 64 * (ld + 1), ld);
+            __syncthreads();
+        }
+        diag_block64<T, true, kNext>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv,
+                                     piv);
+        if (rhs) {  // y_c = L_cc^-1 b_c from the inverse left in Qs (Qs[k][e] = X(e, k))
+            double* bc = rhs + K0 + 64 * c;
+            if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
+            __syncthreads();
+            const int e = threadIdx.x & 63, pq = threadIdx.x >> 6;
+            double sum = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) sum += Qs[16 * pq + u][e] * vb[16 * pq + u];
+            red[threadIdx.x] = sum;
+            __syncthreads();
+            if (threadIdx.x < 64) bc[threadIdx.x] = red[e] + red[e + 64] + red[e + 128] + red[e + 192];
+        }
+        panel_publish(&sync[c], epoch);
     }
 }
 
