@@ -69,6 +69,8 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void**)&c->psync, kPanelSyncWords * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->psync, 0, kPanelSyncWords * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->tsync, kSolveSyncWords * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->tsync, 0, kSolveSyncWords * sizeof(unsigned));
     if (const char* pw = getenv("GPL_PANEL_WAIT_POLLS")) c->pwait = (unsigned)strtoul(pw, nullptr, 10);
     if (e != hipSuccess) {
         delete c;
@@ -94,6 +96,7 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     hipEventDestroy(c->ev_t);
     hipStreamDestroy(c->aux);
     if (c->psync) hipFree(c->psync);
+    if (c->tsync) hipFree(c->tsync);
     hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
@@ -552,6 +555,18 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
 // alpha = sv .* L^-T L^-1 (sv .* z)   (R/laplace.R:29 with C = diag(sqrt V) L)
 // alpha = S B^-1 S z = S L^-T L^-1 S z with the factorization carrying the
 // forward solve: W1 = S z before potrf_blocked(..., rhs = W1), which leaves
+void gpl::trsv_lower(gpl_ctx* c, hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N, double* b,
+                     double* x, int trans, int* info) {
+    const int64_t T = N / 64;
+    if (T > kSolveMaxBlocks) {
+        trsv_lower_f64(s, L, ld, Dinv, N, b, x, trans);
+        return;
+    }
+    if (++c->pepoch == 0) c->pepoch = 1;
+    trsv_persist_f64(s, L, ld, Dinv, N, b, x, trans, c->tsync, c->pepoch, c->tbase, info, c->pwait);
+    c->tbase += (unsigned)T;
+}
+
 // L^-1 S z in W1; then alpha = S L^-T W1.
 static double* alpha_rhs(gpl_model* m, const double* z) {
     hipStream_t s = m->ctx->stream;
@@ -566,16 +581,21 @@ static void solve_alpha_back(gpl_ctx* c, gpl_model* m, double* alpha) {
     const int64_t N = m->N;
     double* w1 = m->d(B_W1);
     double* w2 = m->d(B_W2);
-    Prof p(c, "trsv", 1.0 * N * N, 8.0 * N * N / 2);
-    trsv_lower_f64(s, m->d(B_L), N, m->d(B_DINV), N, w1, w2, 1);
+    {
+        Prof p(c, "trsv", 1.0 * N * N, 8.0 * N * N / 2);
+        trsv_lower(c, s, m->d(B_L), N, m->d(B_DINV), N, w1, w2, 1, m->b[B_INFO].as<int>());
+    }
     fill_f64(s, alpha, N, 0.0);
     vec_mul(s, m->n, m->d(B_SV), w2, alpha);
 }
 
+int gpl::wait_timeout_fail() {
+    return fail(GPL_ERR_HIP, "blocked Cholesky or triangular solve: a wait between the workgroups of a launch timed out");
+}
+
 // A factorisation's info word: 0, the first failing row + 1, or kPanelWaitTimeout.
 static int pivot_fail(int info) {
-    if (info == kPanelWaitTimeout)
-        return fail(GPL_ERR_HIP, "blocked Cholesky: a wait between the workgroups of a panel timed out");
+    if (info == kPanelWaitTimeout) return gpl::wait_timeout_fail();
     g_minor = info;
     return fail(GPL_ERR_NOT_PD, "the leading minor of order " + std::to_string(info) + " is not positive");
 }

@@ -987,6 +987,75 @@ void trsv_lower_f64(hipStream_t s, const double* L, int64_t ld, const double* Di
 }
 
 // ---------------------------------------------------------------------------
+// The same solves in one launch: workgroup k (ticket order) owns block i = k
+// (forward) or T - 1 - k (transposed).  It subtracts the products with the
+// blocks it depends on as each is published (the tile values are loaded
+// before the wait), forms x_i = Dinv_i b_i (Dinv_i^T b_i) and publishes x_i.
+// Every wait is on a block of a lower ticket, whose workgroup has started and
+// waits only on lower tickets, so every wait ends; the summation order is
+// fixed (deterministic).  sync: flags [0, T), the ticket counter at
+// sync[kSolveMaxBlocks].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trsv_persist(const double* __restrict__ L, int64_t ld,
+                                                      const double* __restrict__ Dinv, int64_t T,
+                                                      const double* __restrict__ b, double* __restrict__ x, int trans,
+                                                      unsigned* __restrict__ sync, unsigned epoch, unsigned base,
+                                                      int* __restrict__ info, unsigned maxpoll) {
+    __shared__ double red[256];
+    __shared__ double sv[64];
+    __shared__ unsigned s_id;
+    __shared__ int s_ok;
+    const int t = threadIdx.x, e = t & 63, p = t >> 6;
+    if (t == 0) s_id = atomicAdd(&sync[kSolveMaxBlocks], 1u) - base;
+    __syncthreads();
+    const int64_t k = s_id;
+    const int64_t i = trans ? T - 1 - k : k;
+    // Dinv_i and b_i do not depend on the waits: loaded first, off the chain
+    const double* D = Dinv + i * 64 * 64;
+    double dv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int c = 16 * p + u;
+        dv[u] = trans ? D[c + e * 64] : D[e + c * 64];
+    }
+    const double bi = (t < 64) ? b[64 * i + t] : 0.0;
+    double acc = 0.0;
+    for (int64_t n = 0; n < k; ++n) {
+        const int64_t j = trans ? T - 1 - n : n;  // the blocks solved before i, in their order
+        double lt[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int r = 16 * p + u;
+            // forward: L(64i + e, 64j + r); transposed: L(64j + r, 64i + e)
+            lt[u] = trans ? L[(64 * j + r) + (64 * i + e) * ld] : L[(64 * i + e) + (64 * j + r) * ld];
+        }
+        if (!panel_wait(&sync[j], epoch, info, &s_ok, maxpoll)) return;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += lt[u] * x[64 * j + 16 * p + u];
+    }
+    red[t] = acc;
+    __syncthreads();
+    if (t < 64) sv[t] = bi - (red[t] + red[t + 64] + red[t + 128] + red[t + 192]);
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += dv[u] * sv[16 * p + u];
+    red[t] = s;
+    __syncthreads();
+    if (t < 64) x[64 * i + t] = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+    panel_publish(&sync[i], epoch);
+}
+
+void trsv_persist_f64(hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N, const double* b,
+                      double* x, int trans, unsigned* sync, unsigned epoch, unsigned base, int* info,
+                      unsigned maxpoll) {
+    const int64_t T = N / 64;
+    if (T <= 0) return;
+    hipLaunchKernelGGL(k_trsv_persist, dim3((unsigned)T), dim3(256), 0, s, L, ld, Dinv, T, b, x, trans, sync, epoch,
+                       base, info, maxpoll);
+}
+
+// ---------------------------------------------------------------------------
 // Triangular / symmetric matrix-vector products over lower tiles.  One
 // workgroup per 64-row output block, fixed summation order (deterministic).
 // ---------------------------------------------------------------------------

@@ -147,13 +147,13 @@ static void fitc_lemma_step(gpl_fitc* f, int* flags) {
     potrf_blocked(c, f->d(F_G), Mz, f->d(F_DINVG), f->d(F_LDG), flags);
     Prof pv(c, "fitc_gemv", 0, 32.0 * N * Mz);
     gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ZS), f->d(F_Q1));
-    trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q1), f->d(F_Q2), 0);
-    trsv_lower_f64(s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q2), f->d(F_Q1), 1);
+    trsv_lower(c, s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q1), f->d(F_Q2), 0, flags);
+    trsv_lower(c, s, f->d(F_G), Mz, f->d(F_DINVG), Mz, f->d(F_Q2), f->d(F_Q1), 1, flags);
     gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V1));
     fitc_alpha(s, n, N, f->d(F_ZS), f->d(F_V1), f->d(F_S), f->d(F_ALPHA));
     gemv_t_f64(s, f->d(F_U), N, N, Mz, f->d(F_ALPHA), f->d(F_Q1));
-    trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_Q2), 0);
-    trsv_lower_f64(s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q2), f->d(F_Q1), 1);
+    trsv_lower(c, s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q1), f->d(F_Q2), 0, flags);
+    trsv_lower(c, s, f->d(F_LZ), Mz, f->d(F_DINVZ), Mz, f->d(F_Q2), f->d(F_Q1), 1, flags);
     gemv_t_f64(s, f->d(F_UT), Mz, Mz, N, f->d(F_Q1), f->d(F_V2));
     vec_mul(s, n, f->d(F_D), f->d(F_ALPHA), f->d(F_V3));
     fill_f64(s, f->d(F_FMEAN), N, 0.0);
@@ -235,6 +235,7 @@ static int fitc_ep_loop(gpl_fitc* f, const double* tr, const double* off) {
         reduce_maxabsdiff(s, f->d(F_FOLD), f->d(F_FMEAN), n, scd + 6);
         int r = fitc_scalars(f, sc, 7, fl);
         if (r) return r;
+        if (fl[0] == kPanelWaitTimeout) return wait_timeout_fail();
         if (fl[0] > 0) {
             set_minor(fl[0]);
             return fail(GPL_ERR_NOT_PD, "inversion lemma: the leading minor of order " + std::to_string(fl[0]) +
@@ -284,6 +285,7 @@ static int fitc_run(gpl_fitc* f) {
     double sc[8];
     int r = fitc_scalars(f, sc, 1, fl);
     if (r) return r;
+    if (fl[0] == kPanelWaitTimeout) return wait_timeout_fail();
     if (fl[0] > 0) {
         set_minor(fl[0]);
         return fail(GPL_ERR_NOT_PD, "Kz: the leading minor of order " + std::to_string(fl[0]) + " is not positive");
@@ -354,6 +356,7 @@ static int fitc_run(gpl_fitc* f) {
                                     : "non-log-concave pseudo-variance (V <= 0): R/laplace.R:155-235 is not on "
                                       "the device path");
         }
+        if (fl[0] == kPanelWaitTimeout) return wait_timeout_fail();
         if (fl[0] > 0) {
             set_minor(fl[0]);
             return fail(GPL_ERR_NOT_PD, "inversion lemma: the leading minor of order " + std::to_string(fl[0]) +

@@ -159,6 +159,14 @@ constexpr int kPanelSyncWords = kPanelMaxBlocks + kPanelMaxBlocks * kPanelMaxBlo
 void potrf_panel_f64(hipStream_t s, double* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                      double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, unsigned maxpoll,
                      double* rhs);
+// the blocked solves of trsv_lower_f64 (N / 64 <= kSolveMaxBlocks) in one launch
+// (b is left unchanged); sync: kSolveSyncWords words of the context (zero at
+// creation), epoch, base (N / 64 tickets per launch) and maxpoll as for the panel
+constexpr int kSolveMaxBlocks = 4096;
+constexpr int kSolveSyncWords = kSolveMaxBlocks + 1;
+void trsv_persist_f64(hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N, const double* b,
+                      double* x, int trans, unsigned* sync, unsigned epoch, unsigned base, int* info,
+                      unsigned maxpoll);
 // the same on FP32 storage with the FP64 diagonal tiles Dacc (mixed precision)
 void potrf_panel_mixed(hipStream_t s, float* A, int64_t ld, int64_t K0, int nc, int64_t nrb, double* Dinv,
                        double* ldiag, int* info, unsigned* sync, unsigned epoch, unsigned base, double* Dacc,

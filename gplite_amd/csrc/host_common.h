@@ -34,6 +34,14 @@ int prepare_prog(const gpl_cf_term* terms, int32_t nterms, int64_t ncols, DevPro
 void potrf_blocked(gpl_ctx* c, double* L, int64_t N, double* Dinv, double* ldiag, int* info, double* X = nullptr,
                    double* Tm = nullptr, double* rhs = nullptr);
 void trtri_blocked(gpl_ctx* c, const double* L, const double* Dinv, int64_t N, double* X, double* T = nullptr);
+// Blocked triangular solve with one vector (trsv_lower_f64's arguments, b left
+// unchanged): one launch on the context's solve flags when N / 64 <=
+// kSolveMaxBlocks, else one launch per block.  info receives kPanelWaitTimeout
+// if a device wait times out.
+void trsv_lower(gpl_ctx* c, hipStream_t s, const double* L, int64_t ld, const double* Dinv, int64_t N, double* b,
+                double* x, int trans, int* info);
+// the error reported for an info word of kPanelWaitTimeout
+int wait_timeout_fail();
 // probabilists' Gauss-Hermite nodes and weights normalised to sum 1 (R/quadrature.R:1-44)
 void gauss_hermite(int q, std::vector<double>& x, std::vector<double>& w);
 // Device buffers of one prediction launch over M = pad64(mt) test points (see
@@ -96,6 +104,10 @@ struct gpl_ctx {
     // reach the timeout path)
     unsigned* psync = nullptr;
     unsigned pepoch = 0, pbase = 0;
+    // the same for the single-launch triangular solve (kSolveSyncWords words; its
+    // launches take their epochs from pepoch too)
+    unsigned* tsync = nullptr;
+    unsigned tbase = 0;
     unsigned pwait = 1u << 20;
     bool prof = false;
     std::vector<ProfRec> pending;

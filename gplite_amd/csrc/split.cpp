@@ -186,6 +186,7 @@ int gpl::split_step(gpl_model* m, bool ep, double* logZ) {
     int fl[2];
     HIP_TRY(hipMemcpyAsync(fl, info + 2, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (fl[0] == kPanelWaitTimeout || fl[1] == kPanelWaitTimeout) return wait_timeout_fail();
     if (fl[0] > 0) {
         set_minor(fl[0]);
         return fail(GPL_ERR_NOT_PD, "sites with positive pseudo-variance: the leading minor of order " +
@@ -207,6 +208,7 @@ int gpl::split_step(gpl_model* m, bool ep, double* logZ) {
     tril_f64(s, Ac, Nb, Nb);
     HIP_TRY(hipMemcpyAsync(fl, info + 2, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (fl[0] == kPanelWaitTimeout) return wait_timeout_fail();
     if (fl[0] > 0) {  // the posterior of all the data is not proper
         m->no_posterior = true;
         *logZ = -INFINITY;
