@@ -316,9 +316,9 @@ def bench_c3(args, G, L, api, synth):
     a, b = res["mixed"]["fit"], res["fp64"]["fit"]
     it = b.iters
     flops_step = n ** 3 / 3
-    fp = res["fp64"]
-    top = {k: {"ms": round(v["ms"], 3), "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}
-           for k, v in sorted(fp["prof"].items(), key=lambda kv: -kv[1]["ms"])[:5]}
+    def top(prof, k):
+        return {c: {"ms": round(v["ms"], 3), "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 1)}
+                for c, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:k]}
     best = min(res, key=lambda p: np.median(res[p]["lat"]))
     return {
         "metric": "Laplace fits/s (n=4096, d=4, Bernoulli-logit, tol 1e-8)",
@@ -333,7 +333,8 @@ def bench_c3(args, G, L, api, synth):
         "parity": {"rel_logz_mixed_vs_fp64": abs(a.log_evidence / b.log_evidence - 1),
                    "rel_mode_mixed_vs_fp64": float(np.max(np.abs(a.fmean - b.fmean)) / np.max(np.abs(b.fmean))),
                    "same_step_count": a.iters == b.iters},
-        "roofline_fp64_classes": top,
+        "roofline_fp64_classes": top(res["fp64"]["prof"], 5),
+        "mixed_classes": top(res["mixed"]["prof"], 8),
         "algorithmic_flops_per_newton_step": flops_step,
     }
 

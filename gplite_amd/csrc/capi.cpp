@@ -67,6 +67,7 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_t, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_d, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void**)&c->psync, kPanelSyncWords * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->psync, 0, kPanelSyncWords * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc((void**)&c->tsync, kSolveSyncWords * sizeof(unsigned));
@@ -92,6 +93,7 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     hipStreamSynchronize(c->aux);
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_p);
+    hipEventDestroy(c->ev_d);
     hipEventDestroy(c->ev_h);
     hipEventDestroy(c->ev_t);
     hipStreamDestroy(c->aux);
@@ -522,14 +524,20 @@ static void potrf32_blocked(gpl_ctx* c, float* L, int64_t N, double* Dinv, doubl
         if (R <= 0) break;
         const int64_t W1 = std::min<int64_t>(kPanelNB, R);
         const float* P = L + K1 + K0 * N;
+        // the FP64 diagonal tiles of the next panel on the side stream, beside
+        // its FP32 update (a few workgroups, latency-bound)
+        hipEventRecord(c->ev_d, s);
+        hipStreamWaitEvent(s2, c->ev_d, 0);
+        {
+            Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (W1 / 64), 0, s2);
+            dacc_update(s2, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, 1, 0, 0);
+        }
+        hipEventRecord(c->ev_d, s2);
         {
             Prof p(c, "syrk_update_f32", 2.0 * R * W1 * W, 4.0 * (2.0 * R * W1 + R * W), s);
             gemm_f32(s, 0, 1, R, W1, W, -1.f, P, N, P, N, 1.f, L + K1 * (N + 1), N, 0, 0);
         }
-        {
-            Prof p(c, "dacc_f64", 2.0 * 1 * 4096.0 * W * (W1 / 64), 0, s);
-            dacc_update(s, Dacc + (K1 / 64) * 64 * 64, P, N, W1 / 64, W, 1, 0, 0);
-        }
+        hipStreamWaitEvent(s, c->ev_d, 0);
         hipEventRecord(c->ev_a, s);
         potrf32_panel(c, s, L, N, K1, W1, Dinv, ldiag, info, Dacc);
         inv.after_panel(K1 + W1);
