@@ -359,15 +359,14 @@ __device__ __forceinline__ double quad_bcast_f64(double v) {
 // in place of A's.  IX: the block's values (FP64) are already in Xs
 // (Xs[col][row]), in place of either.  Lp, Xs (64 x 65), rinv, piv (64): workgroup LDS;
 // Lp[col][row] = L(row, col) (strictly lower after the factor), Xs[col][row] =
-// (L_kk^-1)(row, col).  MU: the rank-16 updates between the phases on f64 MFMA
-// (through Xs) rather than as scalar FMA sums; measured faster inside
-// k_potrf_panel (Laplace n = 4096: 42.9 -> 44.8 fits/s) but slower as the
-// standalone kernel (25.1 against 20.5 us), whose register allocation it changes.
+// (L_kk^-1)(row, col).  MU (pcol: one LDS int): the waves to the right of
+// phase p apply its columns as rank-1 updates while wave p factors, following
+// a column counter in LDS, instead of one rank-16 update after a barrier.
 template <typename T, bool MU, bool IX = false>
 __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int64_t k, double* __restrict__ Dinv,
                                              double* __restrict__ ldiag, int* __restrict__ info, int want_inv,
                                              const double* __restrict__ Dsrc, double (*Lp)[65], double (*Xs)[65],
-                                             double* rinv, double* piv) {
+                                             double* rinv, double* piv, int* pcol = nullptr) {
     // One 64x64 diagonal block per workgroup, FP64 arithmetic.  Wave w holds
     // columns 16w..16w+15 of every row (lane = row).  Factor: four phases; in
     // phase p wave p factors its 16 columns alone (pivots and multipliers are
@@ -376,6 +375,10 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
     // (below).
     T* blk = A + k * 64 * (ld + 1);
     const int t = threadIdx.x, i = t & 63, w = t >> 6;
+    if (MU) {
+        if (t == 0) *pcol = 0;
+        __syncthreads();
+    }
     double a[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -405,40 +408,38 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
                 if (jj + 1 < 16) a[jj + 1] -= l * readlane_f64(l, j + 1);
 #pragma unroll
                 for (int cc = jj + 2; cc < 16; ++cc) a[cc] -= l * Lp[j][16 * p + cc];
+                // MU: every four columns, tell the waves to the right how many
+                // columns of L are in LDS (release: the columns land first)
+                if (MU && (jj & 3) == 3)
+                    __hip_atomic_store(pcol, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             if (i >= 16 * p && i < 16 * p + 16) {
                 rinv[i] = myinv;
                 piv[i] = mypiv;
             }
-        }
-        __syncthreads();
-        if (MU && w > p) {
-            // rank-16 update of the wave's columns: U = L(:, P) L(16w.., P)^T
-            // (P = columns 16p..16p+15) on v_mfma_f64_16x16x4, the 16-row tiles
-            // rt >= w (those above are upper-triangle positions); U goes through
-            // the wave's own columns of Xs (free until the inverse) from the
-            // MFMA layout to lane = row, in program order
-            const int ln = i & 15, lg = i >> 4;
-            d4 u[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0},
-                       d4{0.0, 0.0, 0.0, 0.0}};
+        } else if (MU && w > p) {
+            // Pipelined: apply the rank-1 updates of phase p's columns to this
+            // wave's columns as they are published, four at a time, while wave p
+            // factors the next ones; no barrier between the phases, and the wave
+            // of the next phase starts right after the last four.  (Groups
+            // shrinking to single columns at the end of the phase, applied by a
+            // rolled loop, measured slower: 31 against 24 us per 64 columns.)
+#pragma unroll 1
+            for (int g = 0; g < 4; ++g) {
+                const int need = 16 * p + 4 * g + 4;
+                while (__hip_atomic_load(pcol, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need)
+                    __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-            for (int k0 = 0; k0 < 16; k0 += 4) {
-                const double bv = Lp[16 * p + k0 + lg][16 * w + ln];
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int j = need - 4 + jj;
+                    const double li = Lp[j][i];
 #pragma unroll
-                for (int rt = 0; rt < 4; ++rt)
-                    if (rt >= w)
-                        u[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lp[16 * p + k0 + lg][16 * rt + ln], bv, u[rt], 0, 0,
-                                                                     0);
+                    for (int q = 0; q < 16; ++q) a[q] -= li * Lp[j][16 * w + q];
+                }
             }
-#pragma unroll
-            for (int rt = 0; rt < 4; ++rt)
-                if (rt >= w)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) Xs[16 * w + ln][16 * rt + lg + 4 * v] = u[rt][v];
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                if (16 * w + q <= i) a[q] -= Xs[16 * w + q][i];
-        } else if (w > p) {
+        }
+        if (!MU) __syncthreads();
+        if (!MU && w > p) {
             double li[16];
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk) li[kk] = Lp[16 * p + kk][i];
@@ -452,6 +453,7 @@ __device__ __forceinline__ void diag_block64(T* __restrict__ A, int64_t ld, int6
             }
         }
     }
+    if (MU) __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int c = 16 * w + q;
@@ -725,6 +727,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __shared__ double vb[64];
     __shared__ unsigned s_id;
     __shared__ int s_ok;
+    __shared__ int s_col;
     if (threadIdx.x == 0) s_id = atomicAdd(&sync[kPanelSyncWords - 1], 1u) - base;
     __syncthreads();
     const int rb = (int)s_id;
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             __syncthreads();
         }
         diag_block64<T, true, kNext>(A, ld, kc, Dinv, ldiag, info, 1, Dacc ? Dacc + kc * 4096 : nullptr, Ps, Qs, rinv,
-                                     piv);
+                                     piv, &s_col);
         if (rhs) {  // y_c = L_cc^-1 b_c from the inverse left in Qs (Qs[k][e] = X(e, k))
             double* bc = rhs + K0 + 64 * c;
             if (threadIdx.x < 64) vb[threadIdx.x] = bc[threadIdx.x];
