@@ -65,6 +65,7 @@ extern "C" int gpl_ctx_create(int32_t device, gpl_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_h, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_t, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_h2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_p, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_d, hipEventDisableTiming);
@@ -96,6 +97,7 @@ extern "C" void gpl_ctx_destroy(gpl_ctx* c) {
     hipEventDestroy(c->ev_d);
     hipEventDestroy(c->ev_h);
     hipEventDestroy(c->ev_t);
+    hipEventDestroy(c->ev_h2);
     hipStreamDestroy(c->aux);
     if (c->psync) hipFree(c->psync);
     if (c->tsync) hipFree(c->tsync);
@@ -281,14 +283,14 @@ static void dbl_leaves(hipStream_t s, float* X, int64_t N, const double* Dinv, i
     trtri_leaves_f32_batched(s, X, N, Dinv, 1, 0, 0, c0, n);
 }
 static void dbl_gemm(hipStream_t s, int64_t M, int64_t Nn, int64_t K, double alpha, const double* A, const double* B,
-                     double* C, int64_t ld, int tiles, int64_t batch, int64_t step) {
-    if (batch == 1) gemm2_f64(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, 0.0, C, ld, tiles, 0);
-    else gemm2_f64_batched(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, 0.0, C, ld, tiles, (int)batch, step, step, step);
+                     double* C, int64_t ld, int tiles, int64_t batch, int64_t step, double beta = 0.0) {
+    if (batch == 1) gemm2_f64(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, beta, C, ld, tiles, 0);
+    else gemm2_f64_batched(s, 0, 0, M, Nn, K, alpha, A, ld, B, ld, beta, C, ld, tiles, (int)batch, step, step, step);
 }
 static void dbl_gemm(hipStream_t s, int64_t M, int64_t Nn, int64_t K, double alpha, const float* A, const float* B,
-                     float* C, int64_t ld, int tiles, int64_t batch, int64_t step) {
-    gemm_f32_batched(s, 0, 0, M, Nn, K, (float)alpha, A, ld, B, ld, 0.f, C, ld, tiles, 0, (int)batch, step, step, step,
-                     kFoldInverse);
+                     float* C, int64_t ld, int tiles, int64_t batch, int64_t step, double beta = 0.0) {
+    gemm_f32_batched(s, 0, 0, M, Nn, K, (float)alpha, A, ld, B, ld, (float)beta, C, ld, tiles, 0, (int)batch, step,
+                     step, step, kFoldInverse);
 }
 
 // the leaves (columns r0 .. r0 + n) and the levels inside the diagonal block
@@ -348,27 +350,49 @@ static void trtri_doubling(gpl_ctx* c, hipStream_t s, const T* L, const double* 
     dbl_back(c, s, L, Dinv, N, X, Tm, bt);
 }
 
-// The blocked Cholesky's fork and join of the front half of L^-1 (X, Tm given):
-// the front runs on the context's third stream once the panel holding column
-// bt - 1 is factored on the main stream; the back half follows the factor.
+// The blocked Cholesky's fork and join of L^-1 (X, Tm given).  The front half
+// (the inverse of the leading bt x bt block and T = L21 X11) runs on the
+// context's third stream once the panel holding column bt - 1 is factored on
+// the main stream.  The back half X22 = inv(L22) splits the same way at bt2:
+// once column bt2 - 1 is factored, the third stream also forms its leading
+// block A = inv(L(bt:bt2, bt:bt2)), T2 = L(bt2:N, bt:bt2) A and the top rows of
+// X21 = -X22 T, which need only A (X21(bt:bt2, :) = -A T(bt:bt2, :)).  What
+// follows the factor is then C = inv(L(bt2:N, bt2:N)), B = -C T2 and the bottom
+// rows X21(bt2:N, :) = -B T(bt:bt2, :) - C T(bt2:N, :).
 template <typename T>
 struct InverseFork {
     gpl_ctx* c;
     const T* L;
     const double* Dinv;
-    int64_t N, bt;
+    int64_t N, bt, bt2;
     T *X, *Tm;
-    bool forked = false;
+    bool forked = false, forked2 = false;
     InverseFork(gpl_ctx* c_, const T* L_, const double* Dinv_, int64_t N_, T* X_, T* Tm_)
-        : c(c_), L(L_), Dinv(Dinv_), N(N_), bt(X_ ? doubling_top(N_) : 0), X(X_), Tm(Tm_) {}
+        : c(c_), L(L_), Dinv(Dinv_), N(N_), bt(X_ ? doubling_top(N_) : 0), X(X_), Tm(Tm_) {
+        bt2 = (bt > 0 && N - bt > 64) ? bt + doubling_top(N - bt) : 0;
+    }
     // after the panel ending at column `done` was queued on the main stream
     void after_panel(int64_t done) {
-        if (bt == 0 || forked || done < bt) return;
-        hipEventRecord(c->ev_h, c->stream);
-        hipStreamWaitEvent(c->aux, c->ev_h, 0);
-        dbl_front(c, c->aux, L, Dinv, N, X, Tm, bt);
-        hipEventRecord(c->ev_t, c->aux);
-        forked = true;
+        if (bt == 0) return;
+        if (!forked && done >= bt) {
+            hipEventRecord(c->ev_h, c->stream);
+            hipStreamWaitEvent(c->aux, c->ev_h, 0);
+            dbl_front(c, c->aux, L, Dinv, N, X, Tm, bt);
+            hipEventRecord(c->ev_t, c->aux);
+            forked = true;
+        }
+        if (forked && !forked2 && bt2 > 0 && done >= bt2) {
+            hipEventRecord(c->ev_h2, c->stream);
+            hipStreamWaitEvent(c->aux, c->ev_h2, 0);
+            const int64_t h2 = bt2 - bt, r2 = N - bt2;
+            hipStream_t a = c->aux;
+            dbl_levels(c, a, L, Dinv, N, X, Tm, bt, h2);
+            Prof p(c, sizeof(T) == 4 ? "trtri_level_f32" : "trtri_level", (double)r2 * h2 * h2 + (double)h2 * h2 * bt, 0, a);
+            dbl_gemm(a, r2, h2, h2, 1.0, L + bt2 + bt * N, X + bt * (N + 1), Tm + bt2 + bt * N, N, 4, 1, 0);
+            dbl_gemm(a, h2, bt, h2, -1.0, X + bt * (N + 1), Tm + bt, X + bt, N, 3, 1, 0);
+            hipEventRecord(c->ev_t, a);
+            forked2 = true;
+        }
     }
     void finish() {
         if (!X) return;
@@ -377,7 +401,18 @@ struct InverseFork {
             return;
         }
         hipStreamWaitEvent(c->stream, c->ev_t, 0);
-        dbl_back(c, c->stream, L, Dinv, N, X, Tm, bt);
+        if (!forked2) {
+            dbl_back(c, c->stream, L, Dinv, N, X, Tm, bt);
+            return;
+        }
+        hipStream_t s = c->stream;
+        const int64_t h2 = bt2 - bt, r2 = N - bt2;
+        dbl_levels(c, s, L, Dinv, N, X, Tm, bt2, r2);
+        Prof p(c, sizeof(T) == 4 ? "trtri_level_f32" : "trtri_level",
+               (double)r2 * r2 * h2 + 2.0 * r2 * h2 * bt + (double)r2 * r2 * bt, 0, s);
+        dbl_gemm(s, r2, h2, r2, -1.0, X + bt2 * (N + 1), Tm + bt2 + bt * N, X + bt2 + bt * N, N, 3, 1, 0);
+        dbl_gemm(s, r2, bt, h2, -1.0, X + bt2 + bt * N, Tm + bt, X + bt2, N, 0, 1, 0);
+        dbl_gemm(s, r2, bt, r2, -1.0, X + bt2 * (N + 1), Tm + bt2, X + bt2, N, 3, 1, 0, 1.0);
     }
 };
 

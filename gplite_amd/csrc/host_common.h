@@ -95,7 +95,7 @@ struct gpl_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // look-ahead panel work (default priority, like the main stream)
     hipStream_t aux = nullptr;   // the front half of L^-1 beside the Cholesky's last panels
-    hipEvent_t ev_h = nullptr, ev_t = nullptr;
+    hipEvent_t ev_h = nullptr, ev_t = nullptr, ev_h2 = nullptr;
     hipEvent_t ev_a = nullptr, ev_p = nullptr, ev_d = nullptr;
     // flags and ticket counter of the single-launch panel factorisation
     // (kPanelSyncWords words, zero at creation), its launch epoch and tickets handed out so far,
