@@ -165,6 +165,25 @@ def test_energy_gradient(gpl, ci, n):
     assert rel(g, go) < RTOL, (g, go)
 
 
+@pytest.mark.parametrize("n", [600, 1000, 2500])
+def test_energy_gradient_inverse_split_sizes(gpl, n):
+    """Padded sizes 640, 1024 and 2560: L^-1 is formed in pieces beside the
+    Cholesky (the leading half, then the leading part of the back half and the
+    top rows of their cross block, the rest after the last panel), with back
+    halves of 128, 512 and 512 rows. The FP64 gradient matches the oracle at
+    1e-8; the gradient of the mixed form (FP32 L^-1, B^-1) at 1e-3."""
+    G = gpl
+    r = np.random.default_rng(n)
+    x = r.normal(size=(n, 3)); y = np.sin(x.sum(1)) + 0.1 * r.normal(size=n)
+    gp = G.gp_init(G.cf_matern52(lscale=[0.9, 1.1, 1.3], magn=0.8), G.lik_gaussian(sigma=0.2))
+    gf = G.gp_fit(gp, x, y)
+    fo = O.gp_fit(_spec(gf.cfs), gf.lik.spec(), x, y)
+    go = O.gp_energy_grad(fo)
+    assert rel(G.gp_energy_grad(gf), go) < RTOL
+    gm = G.gp_fit(gp, x, y, precision="mixed")
+    assert rel(G.gp_energy_grad(gm), go) < 1e-3
+
+
 @pytest.mark.parametrize("lik_name", ["logit", "probit", "poisson", "binomial"])
 def test_laplace_parity(gpl, lik_name):
     G = gpl
