@@ -761,6 +761,11 @@ static int fit_gaussian(gpl_model* m) {
 // (internal status kRefineFail, never returned to the caller).
 static const double kRefineTol = 1e-9;
 static const int kRefineFail = 1000;
+// A sweep whose correction is below kRefineStop (relative to the solution) is
+// the last: alpha is then good to about that relative size, and the residual
+// against the FP64 B (its own rounding about n * 2^-53) leaves little below it
+// for further sweeps to gain.
+static const double kRefineStop = 1e-12;
 
 static bool refine_diverging(int sweep, double rel, double prev) {
     if (!std::isfinite(rel)) return true;
@@ -844,7 +849,7 @@ static int fit_gaussian_mixed(gpl_model* m) {
         HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         rel = h;
-        if (rel < 1e-14) break;
+        if (rel < kRefineStop) break;
         if (refine_diverging(it, rel, prev)) break;
         prev = rel;
     }
@@ -960,7 +965,7 @@ static int fit_laplace(gpl_model* m) {
 
 // alpha = S u with B u = S z by iterative refinement (mixed form): corrections
 // from the FP32 inverse X (B^-1 ~ X^T X), residuals against the FP64 copy of B.
-// Stops when the correction is below 1e-14 relative or after maxref sweeps.
+// Stops when the correction is below kRefineStop relative or after maxref sweeps.
 static int refine_alpha_mixed(gpl_model* m, int maxref, int nch) {
     gpl_ctx* c = m->ctx;
     hipStream_t s = c->stream;
@@ -995,7 +1000,7 @@ static int refine_alpha_mixed(gpl_model* m, int maxref, int nch) {
         HIP_TRY(hipMemcpyAsync(&h, m->d(B_SCAL) + 8, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         rel = h;
-        if (rel < 1e-14) break;
+        if (rel < kRefineStop) break;
         if (refine_diverging(k, rel, prev)) break;
         prev = rel;
     }
